@@ -1,0 +1,80 @@
+/*
+ * wnd_oracle.h -- CPU FP64 restatement of the WND-CHARM feature-extraction
+ * path and the WND5 classifier.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * MI355X engine and the timed CPU baseline ("port").  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product library never links it and has no CPU fallback through it.
+ *
+ * Every entry point follows the reference's own arithmetic (same expression
+ * order, FP64, built with -ffp-contract=off); the file:line of the reference
+ * function each one restates is given in wnd_oracle.cpp.
+ */
+#ifndef WND_ORACLE_H
+#define WND_ORACLE_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* transform ids (reference names: "Fourier", "Chebyshev", "Wavelet", "Edge") */
+enum { ORC_T_FOURIER = 0, ORC_T_CHEBYSHEV = 1, ORC_T_WAVELET = 2, ORC_T_EDGE = 3 };
+
+/* feature algorithm ids, with n_features in brackets */
+enum {
+  ORC_A_CHEB_FOURIER = 0,  /* Chebyshev-Fourier Coefficients [32] */
+  ORC_A_CHEBYSHEV = 1,     /* Chebyshev Coefficients [32] */
+  ORC_A_ZERNIKE = 2,       /* Zernike Coefficients [72] */
+  ORC_A_HARALICK = 3,      /* Haralick Textures [28] */
+  ORC_A_MSH = 4,           /* Multiscale Histograms [24] */
+  ORC_A_TAMURA = 5,        /* Tamura Textures [6] */
+  ORC_A_COMB = 6,          /* Comb Moments [48] */
+  ORC_A_RADON = 7,         /* Radon Coefficients [12] */
+  ORC_A_FRACTAL = 8,       /* Fractal Features [20] */
+  ORC_A_PIXSTATS = 9,      /* Pixel Intensity Statistics [5] */
+  ORC_A_EDGE = 10,         /* Edge Features [28] */
+  ORC_A_OTSU = 11,         /* Otsu Object Features [34] */
+  ORC_A_INV_OTSU = 12,     /* Inverse-Otsu Object Features [34] */
+  ORC_A_GABOR = 13,        /* Gabor Textures [7] */
+  ORC_A_GINI = 14,         /* Gini Coefficient [1] */
+  ORC_A_COUNT = 15
+};
+
+int orc_alg_n_features(int alg);
+
+/* Image transform: in is row-major H x W FP64.  out must hold the output
+ * (Fourier/Edge: H*W; Chebyshev: N*N, N=min(W,H); Wavelet:
+ * 2*floor((H+9)/2) * 2*floor((W+9)/2)).  Returns 0 on success. */
+int orc_transform(int kind, const double *in, int h, int w, double *out, int *out_h, int *out_w);
+
+/* Feature algorithm on one image; out gets orc_alg_n_features(alg) doubles. */
+int orc_algorithm(int alg, const double *in, int h, int w, double *out);
+
+/* Full plan.  set: 0 = short (1059), 1 = long (2919).  out gets n features. */
+int orc_plan_n_features(int set);
+int orc_features(int set, const double *img, int h, int w, double *out);
+/* column -> feature-name string of a plan ("Zernike Coefficients (Fourier ()) [3]") */
+const char *orc_feature_name(int set, int column);
+
+/* Intermediates used by integer-exact parity tests. */
+double orc_otsu(const double *in, int h, int w);                         /* Otsu(dynamic) */
+int orc_radon_proj(const double *in, int h, int w, double *out4xr, int *rsize);
+int orc_haralick_grays(const double *in, int h, int w, unsigned char *out);
+int orc_cooc(const unsigned char *grays, int h, int w, int angle, double *counts256x256, int *count);
+unsigned long orc_bwlabel(const double *in, int h, int w, double *labels); /* Otsu mask + 8-conn labels */
+int orc_haralick_f14(const double *P, int ng, double *eig_re, double *eig_im);
+
+/* Classifier (reference TrainingSet.cpp / signatures.cpp). */
+void orc_normalize_train(double *data, int n, int f, double *mins, double *maxs);
+void orc_normalize_test(double *vec, int f, const double *mins, const double *maxs);
+void orc_fisher(const double *data, const int *cls, int n, int f, int n_classes,
+                double used, double *weights);
+long orc_wnd5(const double *train, const int *cls, int n, int f, int n_classes,
+              const double *weights, const double *test_normalized,
+              double *probabilities, double *norm_factor);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

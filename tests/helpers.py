@@ -1,0 +1,118 @@
+"""Shared test helpers: oracle / reference-harness loaders and the
+reference's own feature comparator."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, 'tests', 'golden')
+ORACLE_SO = os.path.join(ROOT, 'oracle', 'liboracle.so')
+REF_SO = os.path.join(ROOT, 'oracle', '_ref', 'libwndref.so')
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+_oracle = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        if not os.path.exists(ORACLE_SO):
+            pytest.skip('oracle not built (run __graft_entry__.build())')
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.orc_transform.argtypes = [I, P, I, I, P, P, P]
+        lib.orc_algorithm.argtypes = [I, P, I, I, P]
+        lib.orc_features.argtypes = [I, P, I, I, P]
+        lib.orc_feature_name.restype = ctypes.c_char_p
+        lib.orc_otsu.argtypes = [P, I, I]
+        lib.orc_otsu.restype = D
+        lib.orc_haralick_grays.argtypes = [P, I, I, P]
+        lib.orc_radon_proj.argtypes = [P, I, I, P, P]
+        lib.orc_cooc.argtypes = [P, I, I, I, P, P]
+        lib.orc_bwlabel.argtypes = [P, I, I, P]
+        lib.orc_bwlabel.restype = ctypes.c_ulong
+        lib.orc_haralick_f14.argtypes = [P, I, P, P]
+        lib.orc_normalize_train.argtypes = [P, I, I, P, P]
+        lib.orc_normalize_test.argtypes = [P, I, P, P]
+        lib.orc_fisher.argtypes = [P, P, I, I, I, D, P]
+        lib.orc_wnd5.argtypes = [P, P, I, I, I, P, P, P, P]
+        lib.orc_wnd5.restype = ctypes.c_long
+        _oracle = lib
+    return _oracle
+
+
+def ref_harness():
+    if not os.path.exists(REF_SO):
+        pytest.skip('reference harness oracle/_ref not built (needs /root/reference)')
+    lib = ctypes.CDLL(REF_SO)
+    lib.ref_texture.argtypes = [P, I, I, I, P]
+    lib.ref_eigen.argtypes = [P, I, P, P]
+    lib.ref_radon.argtypes = [P, I, I, I, I, P]
+    lib.ref_wavelet.argtypes = [P, I, I, P, P, P]
+    lib.ref_cheb1d.argtypes = [P, I, I, I, P]
+    return lib
+
+
+ALG_NFEAT = [32, 32, 72, 28, 24, 6, 48, 12, 20, 5, 28, 34, 34, 7, 1]
+T_FOURIER, T_CHEBYSHEV, T_WAVELET, T_EDGE = 0, 1, 2, 3
+
+
+def orc_features(img, long_set=True):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    lib = oracle()
+    n = 2919 if long_set else 1059
+    out = np.zeros(n)
+    assert lib.orc_features(1 if long_set else 0, ptr(img), img.shape[0], img.shape[1], ptr(out)) == 0
+    return out
+
+
+def orc_names(long_set=True):
+    lib = oracle()
+    s = 1 if long_set else 0
+    n = 2919 if long_set else 1059
+    return [lib.orc_feature_name(s, i).decode() for i in range(n)]
+
+
+def orc_transform(kind, img):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    h, w = img.shape
+    out = np.zeros((2 * ((h + 9) // 2) + 2) * (2 * ((w + 9) // 2) + 2))
+    oh, ow = ctypes.c_int(), ctypes.c_int()
+    assert oracle().orc_transform(kind, ptr(img), h, w, ptr(out), ctypes.byref(oh), ctypes.byref(ow)) == 0
+    return out[:oh.value * ow.value].reshape(oh.value, ow.value)
+
+
+def orc_algorithm(alg, img):
+    img = np.ascontiguousarray(img, dtype=np.float64)
+    out = np.zeros(ALG_NFEAT[alg])
+    assert oracle().orc_algorithm(alg, ptr(img), img.shape[0], img.shape[1], ptr(out)) == 0
+    return out
+
+
+def wndcharm_compare(a, b):
+    """The reference's feature comparator (wndcharm/utils.py:276-366): equal,
+    or |a-b| < 1e-7, or the 6-significant-digit mantissas differ by at most
+    one unit in the last digit."""
+    if a == b or abs(a - b) < 1e-7:
+        return True
+    if not (np.isfinite(a) and np.isfinite(b)):
+        return False
+    a6, b6 = float('%.6g' % a), float('%.6g' % b)
+    mag = max(abs(a6), abs(b6))
+    if mag == 0:
+        return True
+    unit = 10.0 ** (np.floor(np.log10(mag)) - 5)
+    return abs(a6 - b6) <= unit * 1.0001
+
+
+def load_npz(name):
+    return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
