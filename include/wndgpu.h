@@ -1,0 +1,120 @@
+/*
+ * wndgpu.h -- C ABI of the MI355X WND-CHARM feature engine.
+ *
+ * Plain pointers and sizes only.  Every function returns WNDGPU_OK (0) or a
+ * negative error code; wndgpu_last_error() gives the message of the last
+ * failure on that context.  One context per host thread; a context owns one
+ * GPU stream, its device buffers and the host-computed basis tables.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   wndgpu_plan_standard        StdFeatureComputationPlans::getFeatureSet / getFeatureSetLong
+ *                               (src/Tasks.cpp:313-342, plans 354-508)
+ *   wndgpu_plan_from_groups     FeatureComputationPlan(name) + add(group name) + finalize()
+ *                               (src/Tasks.cpp:118-158, src/FeatureNames.cpp:309-396)
+ *   wndgpu_plan_n_features / wndgpu_plan_feature_name
+ *                               FeatureComputationPlan::n_features / getFeatureNameByIndex
+ *                               (src/Tasks.h:166-170, src/Tasks.cpp:160-168)
+ *   wndgpu_compute              FeatureComputationPlanExecutor::run(src, feature_mat, row)
+ *                               (src/Tasks.cpp:275-295), for a whole batch of images
+ *   wndgpu_transform            ImageTransform::execute(in, out) (src/ImageTransforms.h:45-51,
+ *                               src/ImageTransforms.cpp:63-109) by transform name
+ *   wndgpu_algorithm            FeatureAlgorithm::execute(in) (src/FeatureAlgorithms.h:280-290,
+ *                               src/FeatureAlgorithms.cpp:54-489) by algorithm name
+ *   wndgpu_normalize_train      TrainingSet::normalize (src/wndchrm_src/TrainingSet.cpp:2090-2123)
+ *   wndgpu_normalize_test       signatures::normalize (src/wndchrm_src/signatures.cpp:196-215)
+ *   wndgpu_fisher               TrainingSet::SetFisherScores (TrainingSet.cpp:2183-2324)
+ *   wndgpu_wnd5                 TrainingSet::classify2 (TrainingSet.cpp:2429-2586)
+ */
+#ifndef WNDGPU_H
+#define WNDGPU_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  WNDGPU_OK = 0,
+  WNDGPU_E_ARG = -1,      /* bad argument (null pointer, size out of range, unknown name) */
+  WNDGPU_E_DEVICE = -2,   /* HIP runtime error */
+  WNDGPU_E_NOMEM = -3,    /* device allocation failed */
+  WNDGPU_E_IMAGE = -4     /* at least one image of the batch was rejected (see status[]) */
+};
+
+enum { WNDGPU_U16 = 0, WNDGPU_F32 = 1, WNDGPU_F64 = 2, WNDGPU_U8 = 3 };
+
+/* per-image status inside a batch */
+enum { WNDGPU_IMG_OK = 0, WNDGPU_IMG_TOO_SMALL = 1, WNDGPU_IMG_TOO_LARGE = 2, WNDGPU_IMG_BAD_DTYPE = 3 };
+
+/* smallest / largest side accepted (the Symlet-5 and Tamura paths need >= 16) */
+#define WNDGPU_MIN_SIDE 16
+#define WNDGPU_MAX_SIDE 4096
+
+typedef struct wndgpu_ctx wndgpu_ctx;
+typedef struct wndgpu_plan wndgpu_plan;
+
+/* one input image: row-major, `stride` elements between rows */
+typedef struct {
+  const void *data;
+  int height, width, stride;
+  int dtype; /* WNDGPU_U16 / F32 / F64 / U8 */
+} wndgpu_image;
+
+int wndgpu_create(int device, wndgpu_ctx **ctx);
+void wndgpu_destroy(wndgpu_ctx *ctx);
+const char *wndgpu_last_error(const wndgpu_ctx *ctx);
+const char *wndgpu_version(void);
+
+/* plans: set 0 = short (1059 features), 1 = long (2919 features) */
+int wndgpu_plan_standard(int set, wndgpu_plan **plan);
+/* custom plan from feature-group names such as "Haralick Textures (Fourier ())" */
+int wndgpu_plan_from_groups(const char *const *group_names, int n_groups, wndgpu_plan **plan);
+void wndgpu_plan_destroy(wndgpu_plan *plan);
+int wndgpu_plan_n_features(const wndgpu_plan *plan);
+const char *wndgpu_plan_feature_name(const wndgpu_plan *plan, int column);
+/* column offset of a feature group in the plan, or -1 */
+int wndgpu_plan_group_offset(const wndgpu_plan *plan, const char *group_name);
+
+/* Features of a batch of host images.  out: host [n_images][n_features]
+ * (row i written for every image whose status is WNDGPU_IMG_OK).  status may
+ * be NULL.  Returns WNDGPU_E_IMAGE if any image was rejected. */
+int wndgpu_compute(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image *images, int n_images,
+                   double *out, int *status);
+/* Same, with image data and out already in device memory of the context's
+ * GPU (pointers from hipMalloc / torch tensors); no host round trip. */
+int wndgpu_compute_device(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image *images, int n_images,
+                          double *out_device, int *status);
+
+/* One transform ("Fourier", "Chebyshev", "Wavelet", "Edge") on one FP64
+ * host plane.  out must hold the output plane: Fourier/Edge h*w, Chebyshev
+ * n*n with n = min(h,w), Wavelet 2*floor((h+9)/2) * 2*floor((w+9)/2). */
+int wndgpu_transform(wndgpu_ctx *ctx, const char *name, const double *in, int h, int w, double *out, int *out_h,
+                     int *out_w);
+/* One feature algorithm by reference name ("Zernike Coefficients", ...) on
+ * one FP64 host plane; out gets n_features doubles (returned in *n_out). */
+int wndgpu_algorithm(wndgpu_ctx *ctx, const char *name, const double *in, int h, int w, double *out, int *n_out);
+/* n_features of an algorithm name, or -1 */
+int wndgpu_algorithm_n_features(const char *name);
+
+/* Classifier (host arrays, FP64, row-major [n_samples][n_features];
+ * classes are 1..n_classes). */
+int wndgpu_normalize_train(wndgpu_ctx *ctx, double *data, int n_samples, int n_features, double *mins,
+                           double *maxs);
+int wndgpu_normalize_test(wndgpu_ctx *ctx, double *vec, int n_features, const double *mins, const double *maxs);
+int wndgpu_fisher(wndgpu_ctx *ctx, const double *data, const int *classes, int n_samples, int n_features,
+                  int n_classes, double used_fraction, double *weights);
+/* WND5 marginal probabilities for n_test already-normalised test vectors:
+ * probabilities [n_test][n_classes], norm_factor [n_test], predicted [n_test]. */
+int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
+                int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
+                double *norm_factor, long *predicted);
+
+/* Per-node timings of the last wndgpu_compute (ms, one entry per DAG node
+ * in execution order), the counterpart of the reference's TIME lines
+ * (src/Tasks.cpp:249-253).  Returns the number of entries written. */
+int wndgpu_last_node_times(wndgpu_ctx *ctx, double *ms, const char **names, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

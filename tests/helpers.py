@@ -116,3 +116,31 @@ def wndcharm_compare(a, b):
 
 def load_npz(name):
     return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+
+
+# Feature indices whose reference values are integer counts or ratios of
+# integer counts (bit-exact bar); the rest are floating point.
+ALG_NAMES = ['Chebyshev-Fourier Coefficients', 'Chebyshev Coefficients', 'Zernike Coefficients', 'Haralick Textures',
+             'Multiscale Histograms', 'Tamura Textures', 'Comb Moments', 'Radon Coefficients', 'Fractal Features',
+             'Pixel Intensity Statistics', 'Edge Features', 'Otsu Object Features', 'Inverse-Otsu Object Features',
+             'Gabor Textures', 'Gini Coefficient']
+EXACT = {
+    0: list(range(32)), 1: list(range(32)), 2: [], 3: [], 4: list(range(24)), 5: [0, 1, 2, 5],
+    6: list(range(48)), 7: list(range(12)), 8: [], 9: [1, 3, 4],
+    10: [0] + list(range(1, 14)) + list(range(17, 25)) + [26],
+    11: list(range(0, 14)) + [15, 16, 17] + list(range(18, 29)) + [30, 31, 33],
+    12: list(range(0, 14)) + [15, 16, 17] + list(range(18, 29)) + [30, 31, 33],
+    13: list(range(7)), 14: [],
+}
+
+
+def exact_mask_for_names(names):
+    """bool mask over plan columns: True where the feature is integer-valued."""
+    idx = {n: i for i, n in enumerate(ALG_NAMES)}
+    m = np.zeros(len(names), bool)
+    for c, nm in enumerate(names):
+        g, k = nm.rsplit(' [', 1)
+        k = int(k[:-1])
+        alg = g[:g.index(' (')]
+        m[c] = k in EXACT[idx[alg]]
+    return m

@@ -1,0 +1,93 @@
+// Shared device-side types and helpers for the MI355X feature engine.
+//
+// Every kernel is batch-generic: it receives a device array of image
+// descriptors (one per image of the batch, sizes may differ) and a
+// grid whose y dimension indexes the image.  Blocks beyond an image's own
+// extent exit immediately, so one launch serves a size-bucketed batch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wnd {
+
+// One FP64 row-major plane on the device (an ImageMatrix pixel plane).
+struct Img {
+  double *p;
+  int h, w;
+};
+
+// Pixel statistics of a plane (Moments2 semantics: NaN and |x| > FLT_MAX
+// values are not counted; var/std are 0 for n <= 2).
+struct Stats {
+  double mn, mx, mean, var, stdv;
+  long long n;
+};
+
+// Output row addressing: feature vector of image i starts at out + i*stride.
+struct OutRef {
+  double *out;
+  int stride;  // features per image
+  int off;     // column offset of the feature group
+};
+
+constexpr double kFltMax = 3.40282346638528859812e+38;
+constexpr double kDblEps = 2.2204460492503131e-16;
+constexpr double kInf = 10E200;  // the reference's INF (src/cmatrix.h:48)
+
+__device__ __forceinline__ bool moments_ok(double x) {
+  return !(x != x) && !(x > kFltMax) && !(x < -kFltMax);
+}
+
+// histogram bin exactly as ImageMatrix::histogram (src/cmatrix.cpp:1069-1103):
+// bin = (unsigned long)((v - min) * scale), clamped to nbins-1.
+__device__ __forceinline__ int hist_bin(double v, double mn, double scale, int nbins) {
+  double t = (v - mn) * scale;
+  // (unsigned long) of a negative or NaN value is undefined in C; the
+  // reference only ever feeds v >= min.  Values >= nbins clamp.
+  unsigned long long b = (t >= 18446744073709551615.0 || t != t) ? (unsigned long long)nbins
+                                                                 : (unsigned long long)t;
+  if (b >= (unsigned long long)nbins) b = nbins - 1;
+  return (int)b;
+}
+
+__device__ __forceinline__ double hist_scale(double mn, double mx, int nbins) {
+  return (mx - mn > 0) ? (double)nbins / (double)(mx - mn) : 0.0;
+}
+
+// Order-preserving map of a double to uint64 (for radix sort / select).
+__device__ __forceinline__ uint64_t dkey(double v) {
+  uint64_t u = __double_as_longlong(v);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_inv(uint64_t k) {
+  uint64_t u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double(u);
+}
+
+// Block-wide reductions (blockDim.x a multiple of 64, <= 1024).
+template <typename T, typename Op>
+__device__ T block_reduce(T v, Op op, T *sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_down(v, o, 64));
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    v = (lane < nw) ? sh[lane] : sh[0];
+    if (lane >= nw) v = sh[0];
+    for (int o = 32; o > 0; o >>= 1) {
+      T u = __shfl_down(v, o, 64);
+      if (lane + o < nw) v = op(v, u);
+    }
+    if (lane == 0) sh[0] = v;
+  }
+  __syncthreads();
+  T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+struct SumOp { template <typename T> __device__ T operator()(T a, T b) const { return a + b; } };
+struct MinOp { template <typename T> __device__ T operator()(T a, T b) const { return a < b ? a : b; } };
+struct MaxOp { template <typename T> __device__ T operator()(T a, T b) const { return a > b ? a : b; } };
+
+}  // namespace wnd

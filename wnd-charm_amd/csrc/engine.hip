@@ -1,0 +1,1134 @@
+// Host orchestration of the feature DAG on one MI355X and the C ABI
+// (include/wndgpu.h).
+//
+// A batch of images runs node by node through the transform DAG of a plan
+// (FeatureComputationPlanExecutor::run, src/Tasks.cpp:185-311): every node
+// is one batched transform launch whose output planes stay resident in HBM
+// for the whole batch, and every feature group on a node is one batched
+// launch writing its columns of the [n_images][n_features] result.  All
+// device memory of a batch comes from one arena; all kernel descriptors are
+// staged in pinned host memory and uploaded with a single copy.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/wndgpu.h"
+#include "kernels.hpp"
+#include "tables.hpp"
+
+namespace wnd {
+void launch_norm_train(double *data, int n, int f, double *mins, double *maxs, hipStream_t s);
+void launch_norm_test(double *vec, int nvec, int f, const double *mins, const double *maxs, hipStream_t s);
+void launch_fisher(const double *data, const int *cls, int n, int f, int C, double used, double *w, hipStream_t s);
+void launch_wnd5(const double *train, const int *cls, const double *w, const double *test, int n, int f, int C,
+                 int ntest, double *dist, double *prob, double *norm, long *pred, hipStream_t s);
+}  // namespace wnd
+
+using namespace wnd;
+
+// ============================================================ names / plans
+enum { T_FOURIER = 0, T_CHEBYSHEV = 1, T_WAVELET = 2, T_EDGE = 3 };
+enum {
+  A_CHEB_FOURIER = 0, A_CHEBYSHEV, A_ZERNIKE, A_HARALICK, A_MSH, A_TAMURA, A_COMB, A_RADON,
+  A_FRACTAL, A_PIXSTATS, A_EDGE, A_OTSU, A_INV_OTSU, A_GABOR, A_GINI, A_COUNT
+};
+static const int kNFeat[A_COUNT] = {32, 32, 72, 28, 24, 6, 48, 12, 20, 5, 28, 34, 34, 7, 1};
+// registered names (src/FeatureAlgorithms.cpp:54-489, src/ImageTransforms.cpp:57-103)
+static const char *kAlgNames[A_COUNT] = {
+    "Chebyshev-Fourier Coefficients", "Chebyshev Coefficients", "Zernike Coefficients", "Haralick Textures",
+    "Multiscale Histograms", "Tamura Textures", "Comb Moments", "Radon Coefficients", "Fractal Features",
+    "Pixel Intensity Statistics", "Edge Features", "Otsu Object Features", "Inverse-Otsu Object Features",
+    "Gabor Textures", "Gini Coefficient"};
+static const char *kTrNames[4] = {"Fourier", "Chebyshev", "Wavelet", "Edge"};
+
+static std::string trim(const std::string &s) {
+  size_t a = s.find_first_not_of(" \t\f\v\n\r"), b = s.find_last_not_of(" \t\f\v\n\r");
+  return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+static int alg_index(const std::string &n) {
+  for (int i = 0; i < A_COUNT; i++) if (n == kAlgNames[i]) return i;
+  return -1;
+}
+static int tr_index(const std::string &n) {
+  for (int i = 0; i < 4; i++) if (n == kTrNames[i]) return i;
+  return -1;
+}
+static std::string node_text(const std::vector<int> &chain) {
+  std::string s = "()";
+  for (int t : chain) s = "(" + std::string(kTrNames[t]) + " " + s + ")";
+  return s;
+}
+
+struct Group {
+  std::vector<int> chain;  // innermost first
+  int alg;
+  int offset;
+  std::string name;
+};
+
+struct wndgpu_plan {
+  std::vector<Group> groups;
+  int n_features = 0;
+  std::vector<std::string> names;
+  std::vector<std::vector<int>> nodes;  // DAG nodes, parents before children
+};
+
+// "Alg (T2 (T1 ()))" -> alg, chain {T1, T2} (FeatureNames::getGroupByName, FeatureNames.cpp:309-396)
+static bool parse_group(const std::string &name, int *alg, std::vector<int> *chain) {
+  size_t lp = name.find('(');
+  if (lp == std::string::npos) return false;
+  *alg = alg_index(trim(name.substr(0, lp)));
+  if (*alg < 0) return false;
+  std::string rest = trim(name.substr(lp));
+  std::vector<int> outer_first;
+  while (true) {
+    if (rest.size() < 2 || rest.front() != '(' || rest.back() != ')') return false;
+    std::string inner = trim(rest.substr(1, rest.size() - 2));
+    if (inner.empty()) break;
+    size_t p = inner.find('(');
+    if (p == std::string::npos) return false;
+    const int t = tr_index(trim(inner.substr(0, p)));
+    if (t < 0) return false;
+    outer_first.push_back(t);
+    rest = trim(inner.substr(p));
+  }
+  chain->assign(outer_first.rbegin(), outer_first.rend());
+  return true;
+}
+
+static void plan_add(wndgpu_plan *p, int alg, const std::vector<int> &chain) {
+  Group g{chain, alg, p->n_features, std::string(kAlgNames[alg]) + " " + node_text(chain)};
+  for (int i = 0; i < kNFeat[alg]; i++) p->names.push_back(g.name + " [" + std::to_string(i) + "]");
+  p->n_features += kNFeat[alg];
+  p->groups.push_back(g);
+  std::vector<int> key;
+  auto has = [&](const std::vector<int> &k) { return std::find(p->nodes.begin(), p->nodes.end(), k) != p->nodes.end(); };
+  if (!has(key)) p->nodes.push_back(key);
+  for (int t : chain) {
+    key.push_back(t);
+    if (!has(key)) p->nodes.push_back(key);
+  }
+}
+
+static void plan_finalize(wndgpu_plan *p) {
+  std::stable_sort(p->nodes.begin(), p->nodes.end(),
+                   [](const std::vector<int> &a, const std::vector<int> &b) { return a.size() < b.size(); });
+}
+
+// StdFeatureComputationPlans (src/Tasks.cpp:354-508)
+static wndgpu_plan *make_standard(int set) {
+  wndgpu_plan *p = new wndgpu_plan;
+  const int F = T_FOURIER, C = T_CHEBYSHEV, W = T_WAVELET, E = T_EDGE;
+  if (set == 1) {
+    auto A = [&](std::vector<int> ch) { for (int a : {A_EDGE, A_OTSU, A_INV_OTSU, A_GABOR}) plan_add(p, a, ch); };
+    auto B = [&](std::vector<int> ch) { for (int a : {A_CHEB_FOURIER, A_CHEBYSHEV, A_ZERNIKE}) plan_add(p, a, ch); };
+    auto Cg = [&](std::vector<int> ch) {
+      for (int a : {A_COMB, A_HARALICK, A_MSH, A_TAMURA, A_RADON, A_FRACTAL, A_PIXSTATS, A_GINI}) plan_add(p, a, ch);
+    };
+    A({}); B({}); Cg({});
+    B({F}); Cg({F});
+    B({W}); Cg({W});
+    B({C}); Cg({C});
+    Cg({F, C});
+    Cg({F, W});
+    B({W, F}); Cg({W, F});
+    Cg({C, F});
+    Cg({W, C});
+    B({E}); Cg({E});
+    B({E, F}); Cg({E, F});
+    B({E, W}); Cg({E, W});
+  } else {
+    plan_add(p, A_CHEB_FOURIER, {}); plan_add(p, A_CHEB_FOURIER, {F});
+    plan_add(p, A_CHEBYSHEV, {}); plan_add(p, A_CHEBYSHEV, {F});
+    plan_add(p, A_COMB, {}); plan_add(p, A_COMB, {C}); plan_add(p, A_COMB, {F, C}); plan_add(p, A_COMB, {F});
+    plan_add(p, A_COMB, {W}); plan_add(p, A_COMB, {F, W});
+    plan_add(p, A_EDGE, {}); plan_add(p, A_OTSU, {}); plan_add(p, A_INV_OTSU, {}); plan_add(p, A_GABOR, {});
+    plan_add(p, A_HARALICK, {}); plan_add(p, A_HARALICK, {C}); plan_add(p, A_HARALICK, {F, C});
+    plan_add(p, A_HARALICK, {F}); plan_add(p, A_HARALICK, {W}); plan_add(p, A_HARALICK, {F, W});
+    plan_add(p, A_MSH, {}); plan_add(p, A_MSH, {C}); plan_add(p, A_MSH, {F, C}); plan_add(p, A_MSH, {F});
+    plan_add(p, A_MSH, {W}); plan_add(p, A_MSH, {F, W});
+    plan_add(p, A_RADON, {}); plan_add(p, A_RADON, {C}); plan_add(p, A_RADON, {F, C}); plan_add(p, A_RADON, {F});
+    plan_add(p, A_TAMURA, {}); plan_add(p, A_TAMURA, {C}); plan_add(p, A_TAMURA, {F, C});
+    plan_add(p, A_TAMURA, {F}); plan_add(p, A_TAMURA, {W}); plan_add(p, A_TAMURA, {F, W});
+    plan_add(p, A_ZERNIKE, {}); plan_add(p, A_ZERNIKE, {F});
+  }
+  plan_finalize(p);
+  return p;
+}
+
+// ============================================================ the context
+struct DevTable {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+struct CfDev {
+  DevTable kk, tn, fc, fs;
+  int np = 0, NN = 0;
+};
+
+struct NodeRun {
+  std::vector<int> chain;
+  std::vector<int> h, w;
+  Img *imgs = nullptr;    // device descriptor array
+  Stats *stats = nullptr; // device stats array
+  std::vector<double *> planes;
+  double ms = 0;
+};
+
+struct wndgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // arena
+  char *dev = nullptr;
+  size_t dev_cap = 0, dev_used = 0;
+  char *host = nullptr;   // pinned descriptor staging
+  char *hdesc = nullptr;  // device copy of the staging
+  size_t host_cap = 0, host_used = 0;
+  // caches (device)
+  std::map<std::pair<int, int>, DevTable> cheb;  // (len, N)
+  std::map<int, std::pair<DevTable, std::vector<int>>> fft;
+  std::map<std::pair<int, int>, CfDev> cf;
+  std::map<std::pair<int, double>, DevTable> dspos;
+  std::map<int, DevTable> comb;
+  DevTable gabor, zern, rtrig;
+  std::vector<std::pair<std::string, double>> times;
+  bool sized = false;
+};
+
+static int fail(wndgpu_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+#define HIPCHK(ctx, x)                                                                              \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) return fail(ctx, WNDGPU_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+static int upload(wndgpu_ctx *c, DevTable &t, const void *src, size_t bytes) {
+  HIPCHK(c, hipMalloc(&t.p, bytes ? bytes : 8));
+  t.bytes = bytes;
+  if (bytes) HIPCHK(c, hipMemcpy(t.p, src, bytes, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// --------------------------------------------------------------- arenas
+// Two passes over the batch: a sizing pass with null bases, then the real
+// pass.  dalloc / halloc are bump allocators over the device arena and the
+// pinned staging buffer; hptr_dev maps a staging address to its device copy.
+struct Arena {
+  wndgpu_ctx *c;
+  bool sizing;
+  size_t d = 0, h = 0;
+  void *dalloc(size_t bytes) {
+    size_t off = (d + 255) & ~(size_t)255;
+    d = off + bytes;
+    return sizing ? (void *)(uintptr_t)(off + 256) : (void *)(c->dev + off);
+  }
+  template <typename T>
+  T *dalloc_n(size_t n) { return (T *)dalloc(n * sizeof(T)); }
+  // stage n descriptors; returns the device address the kernels use
+  template <typename T>
+  T *stage(const std::vector<T> &v) {
+    size_t off = (h + 63) & ~(size_t)63;
+    h = off + v.size() * sizeof(T);
+    if (sizing) return (T *)(uintptr_t)(off + 64);
+    memcpy(c->host + off, v.data(), v.size() * sizeof(T));
+    return (T *)(c->hdesc + off);
+  }
+};
+
+// ============================================================ batch execution
+struct Batch {
+  int n;
+  std::vector<int> h0, w0;
+  std::vector<LoadDesc> load;
+  double *out;  // device [n][nf]
+  int nf;
+};
+
+static int get_cheb(wndgpu_ctx *c, int len, int N, const double **p) {
+  auto key = std::make_pair(len, N);
+  auto it = c->cheb.find(key);
+  if (it == c->cheb.end()) {
+    std::vector<double> b = cheb_basis(len, N);
+    DevTable t;
+    int r = upload(c, t, b.data(), b.size() * sizeof(double));
+    if (r) return r;
+    it = c->cheb.emplace(key, t).first;
+  }
+  *p = (const double *)it->second.p;
+  return 0;
+}
+
+static int get_fft(wndgpu_ctx *c, int n, FftPlan *pl) {
+  auto it = c->fft.find(n);
+  if (it == c->fft.end()) {
+    std::vector<double> tw;
+    std::vector<int> radix;
+    fft_tables(n, tw, radix);
+    DevTable t;
+    int r = upload(c, t, tw.data(), tw.size() * sizeof(double));
+    if (r) return r;
+    it = c->fft.emplace(n, std::make_pair(t, radix)).first;
+  }
+  pl->tw = (const double2 *)it->second.first.p;
+  pl->n = n;
+  pl->nst = (int)it->second.second.size();
+  if (pl->nst > 20) return fail(c, WNDGPU_E_ARG, "FFT length %d has too many factors", n);
+  for (int i = 0; i < pl->nst; i++) pl->radix[i] = it->second.second[i];
+  return 0;
+}
+
+static int get_cf(wndgpu_ctx *c, int w, int h, CfDev **out) {
+  auto key = std::make_pair(w, h);
+  auto it = c->cf.find(key);
+  if (it == c->cf.end()) {
+    CfTable t = cf_table(w, h);
+    CfDev d;
+    d.np = (int)t.kk.size();
+    d.NN = t.NN;
+    int r;
+    if ((r = upload(c, d.kk, t.kk.data(), t.kk.size() * sizeof(int)))) return r;
+    if ((r = upload(c, d.tn, t.tn.data(), t.tn.size() * sizeof(double)))) return r;
+    if ((r = upload(c, d.fc, t.fc.data(), t.fc.size() * sizeof(double)))) return r;
+    if ((r = upload(c, d.fs, t.fs.data(), t.fs.size() * sizeof(double)))) return r;
+    it = c->cf.emplace(key, d).first;
+  }
+  *out = &it->second;
+  return 0;
+}
+
+static int get_dspos(wndgpu_ctx *c, int len, double dx, const double **p, int *count) {
+  auto key = std::make_pair(len, dx);
+  auto it = c->dspos.find(key);
+  if (it == c->dspos.end()) {
+    std::vector<double> v = ds_positions(len, dx);
+    DevTable t;
+    int r = upload(c, t, v.data(), v.size() * sizeof(double));
+    if (r) return r;
+    it = c->dspos.emplace(key, t).first;
+  }
+  *p = (const double *)it->second.p;
+  *count = (int)(it->second.bytes / sizeof(double));
+  return 0;
+}
+
+static int get_comb(wndgpu_ctx *c, int m, const double **p) {
+  auto it = c->comb.find(m);
+  if (it == c->comb.end()) {
+    std::vector<double> v = comb_offsets(m);
+    DevTable t;
+    int r = upload(c, t, v.data(), v.size() * sizeof(double));
+    if (r) return r;
+    it = c->comb.emplace(m, t).first;
+  }
+  *p = (const double *)it->second.p;
+  return 0;
+}
+
+static void out_size(int kind, int h, int w, int *oh, int *ow) {
+  if (kind == T_CHEBYSHEV) { *oh = *ow = std::min(h, w); }
+  else if (kind == T_WAVELET) { *oh = 2 * ((h + 9) / 2); *ow = 2 * ((w + 9) / 2); }
+  else { *oh = h; *ow = w; }
+}
+
+// Build (or size) every launch of a batch; when !A.sizing the launches are
+// also appended to `launches` in execution order.
+static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, Arena &A,
+                       std::vector<std::pair<std::string, std::function<void()>>> &launches) {
+  hipStream_t s = c->stream;
+  const int n = B.n;
+  std::map<std::vector<int>, NodeRun> nodes;
+  auto maxof = [](const std::vector<int> &v) { return *std::max_element(v.begin(), v.end()); };
+  auto push = [&](const std::string &name, std::function<void()> f) {
+    if (!A.sizing) launches.emplace_back(name, std::move(f));
+  };
+  // ---- root: load + stats
+  {
+    NodeRun &R = nodes[{}];
+    R.h = B.h0;
+    R.w = B.w0;
+    std::vector<Img> im(n);
+    long maxp = 0;
+    for (int i = 0; i < n; i++) {
+      im[i] = Img{A.dalloc_n<double>((size_t)R.h[i] * R.w[i]), R.h[i], R.w[i]};
+      maxp = std::max(maxp, (long)R.h[i] * R.w[i]);
+    }
+    R.imgs = A.stage(im);
+    R.stats = A.dalloc_n<Stats>(n);
+    R.planes.resize(n);
+    for (int i = 0; i < n; i++) R.planes[i] = im[i].p;
+    LoadDesc *ld = A.stage(B.load);
+    Img *ri = R.imgs;
+    Stats *rs = R.stats;
+    push("root", [=]() {
+      launch_load(ld, ri, n, maxp, s);
+      launch_stats(ri, rs, n, s);
+    });
+  }
+  // ---- transform nodes (parents first)
+  for (const auto &key : plan->nodes) {
+    if (key.empty()) continue;
+    std::vector<int> pk(key.begin(), key.end() - 1);
+    NodeRun &P = nodes[pk];
+    NodeRun &N = nodes[key];
+    const int kind = key.back();
+    N.chain = key;
+    N.h.resize(n);
+    N.w.resize(n);
+    std::vector<Img> im(n);
+    for (int i = 0; i < n; i++) {
+      out_size(kind, P.h[i], P.w[i], &N.h[i], &N.w[i]);
+      im[i] = Img{A.dalloc_n<double>((size_t)N.h[i] * N.w[i]), N.h[i], N.w[i]};
+    }
+    N.imgs = A.stage(im);
+    N.stats = A.dalloc_n<Stats>(n);
+    const int mph = maxof(P.h), mpw = maxof(P.w);
+    long maxp = 0;
+    for (int i = 0; i < n; i++) maxp = std::max(maxp, (long)P.h[i] * P.w[i]);
+    Img *pin = P.imgs, *pout = N.imgs;
+    Stats *nst = N.stats;
+    const std::string nm = node_text(key);
+    if (kind == T_EDGE) {
+      push(nm, [=]() { launch_edge(pin, pout, n, maxp, s); launch_stats(pout, nst, n, s); });
+    } else if (kind == T_WAVELET) {
+      std::vector<double *> g(n);
+      std::vector<int> gs(n);
+      for (int i = 0; i < n; i++) {
+        const long gsz = (long)((P.h[i] + 9) / 2) * P.w[i];
+        g[i] = A.dalloc_n<double>(2 * gsz);
+        gs[i] = (int)gsz;
+      }
+      double **gd = A.stage(g);
+      int *gsd = A.stage(gs);
+      push(nm, [=]() { launch_wavelet(pin, gd, gsd, pout, n, mph, mpw, s); launch_stats(pout, nst, n, s); });
+    } else if (kind == T_CHEBYSHEV) {
+      // Chebyshev2D (chebyshev.cpp:118-165): rows against the x basis, then
+      // the transposed intermediate against the y basis
+      std::vector<GemmDesc> g1(n), g2(n);
+      int maxN = 0, maxH = 0;
+      for (int i = 0; i < n; i++) {
+        const int h = P.h[i], w = P.w[i], NN = std::min(h, w);
+        const double *bx = nullptr, *by = nullptr;
+        if (!A.sizing) {
+          int r;
+          if ((r = get_cheb(c, w, NN, &bx))) return r;
+          if ((r = get_cheb(c, h, NN, &by))) return r;
+        }
+        double *c1 = A.dalloc_n<double>((size_t)h * NN);
+        g1[i] = GemmDesc{P.planes[i], bx, c1, h, w, NN, (long)w, 1};
+        g2[i] = GemmDesc{c1, by, im[i].p, NN, h, NN, 1, (long)NN};
+        maxN = std::max(maxN, NN);
+        maxH = std::max(maxH, h);
+      }
+      GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
+      push(nm, [=]() {
+        launch_cheb_gemm(d1, n, maxH, maxN, s);
+        launch_cheb_gemm(d2, n, maxN, maxN, s);
+        launch_stats(pout, nst, n, s);
+      });
+    } else {  // Fourier
+      std::vector<double2 *> f1(n);
+      std::vector<FftPlan> pw(n), ph(n);
+      for (int i = 0; i < n; i++) {
+        f1[i] = A.dalloc_n<double2>((size_t)P.h[i] * P.w[i]);
+        if (!A.sizing) {
+          int r;
+          if ((r = get_fft(c, P.w[i], &pw[i]))) return r;
+          if ((r = get_fft(c, P.h[i], &ph[i]))) return r;
+        }
+      }
+      double2 **f1d = A.stage(f1);
+      FftPlan *pwd = A.stage(pw), *phd = A.stage(ph);
+      push(nm, [=]() { launch_fft(pin, f1d, pwd, phd, pout, n, mph, mpw, s); launch_stats(pout, nst, n, s); });
+    }
+    N.planes.resize(n);
+    for (int i = 0; i < n; i++) N.planes[i] = im[i].p;
+  }
+  // ---- feature groups, node by node
+  for (const auto &key : plan->nodes) {
+    NodeRun &N = nodes[key];
+    const std::string nm = node_text(key);
+    Img *ni = N.imgs;
+    Stats *nst = N.stats;
+    const int mh = maxof(N.h), mw = maxof(N.w);
+    long maxp = 0;
+    for (int i = 0; i < n; i++) maxp = std::max(maxp, (long)N.h[i] * N.w[i]);
+    int off_pix = -1, off_gini = -1;
+    for (const Group &g : plan->groups) {
+      if (g.chain != key) continue;
+      const OutRef o{B.out, B.nf, g.offset};
+      const std::string gn = g.name;
+      switch (g.alg) {
+        case A_PIXSTATS: off_pix = g.offset; break;
+        case A_GINI: off_gini = g.offset; break;
+        case A_MSH: push(gn, [=]() { launch_msh(ni, nst, o, n, s); }); break;
+        case A_FRACTAL: push(gn, [=]() { launch_fractal(ni, o, n, s); }); break;
+        case A_COMB: {
+          std::vector<double> iis;
+          for (int i = 0; i < n; i++) {
+            std::vector<double> v = comb_offsets(N.h[i]);
+            iis.insert(iis.end(), v.begin(), v.end());
+          }
+          double *d = A.stage(iis);
+          push(gn, [=]() { launch_comb(ni, d, o, n, s); });
+        } break;
+        case A_RADON: {
+          std::vector<double *> pr(n);
+          int maxbins = 0;
+          for (int i = 0; i < n; i++) {
+            const int W = N.w[i], H = N.h[i];
+            const int a = W - 1 - (W - 1) / 2, b = H - 1 - (H - 1) / 2;
+            const int rl = (int)ceil(sqrt((double)(a * a) + (double)(b * b))) + 1;
+            pr[i] = A.dalloc_n<double>(4 * (size_t)(2 * rl + 1));
+            maxbins = std::max(maxbins, 2 * rl + 1);
+          }
+          double **prd = A.stage(pr);
+          const double *trig = (const double *)c->rtrig.p;
+          push(gn, [=]() { launch_radon(ni, trig, prd, o, n, maxbins, s); });
+        } break;
+        case A_TAMURA: {
+          std::vector<double *> sc(n);
+          for (int i = 0; i < n; i++) sc[i] = A.dalloc_n<double>(5 * (size_t)N.h[i] * N.w[i] + 8);
+          double **scd = A.stage(sc);
+          push(gn, [=]() { launch_tamura(ni, nst, scd, o, n, mh, mw, s); });
+        } break;
+        case A_HARALICK: {
+          std::vector<HarDesc> hd(n);
+          int *lut_all = A.dalloc_n<int>(256 * (size_t)n);
+          for (int i = 0; i < n; i++) {
+            HarDesc &H = hd[i];
+            H.img = nullptr;
+            H.gray = A.dalloc_n<uint8_t>((size_t)N.h[i] * N.w[i]);
+            H.cooc = nullptr;
+            H.P = A.dalloc_n<double>(4 * 256 * 256);
+            H.Q = A.dalloc_n<double>(4 * 256 * 256);
+            H.feat = A.dalloc_n<double>(4 * 14);
+            H.ng = A.dalloc_n<int>(1);
+            H.lut = lut_all + 256 * i;
+            H.count = A.dalloc_n<int>(4);
+          }
+          HarDesc *hdd = A.stage(hd);
+          const long mp = maxp;
+          push(gn, [=]() {
+            hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)n, s);
+            launch_haralick(ni, nst, hdd, o, n, (int)mp, s);
+          });
+        } break;
+        case A_ZERNIKE: {
+          const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
+          double *part = A.dalloc_n<double>((size_t)n * (4 + nchunk * 144));
+          const double *ht = (const double *)c->zern.p;
+          push(gn, [=]() { launch_zernike(ni, part, nchunk, ht, o, n, s); });
+        } break;
+        case A_CHEBYSHEV: {
+          std::vector<GemmDesc> g1(n), g2(n);
+          std::vector<Img> vals(n);
+          int maxN = 0, maxH = 0;
+          for (int i = 0; i < n; i++) {
+            const int h = N.h[i], w = N.w[i], NN = std::min(20, std::min(h, w));
+            const double *bx = nullptr, *by = nullptr;
+            if (!A.sizing) {
+              int r;
+              if ((r = get_cheb(c, w, NN, &bx))) return r;
+              if ((r = get_cheb(c, h, NN, &by))) return r;
+            }
+            double *c1 = A.dalloc_n<double>((size_t)h * NN);
+            double *c2 = A.dalloc_n<double>((size_t)NN * NN);
+            g1[i] = GemmDesc{N.planes[i], bx, c1, h, w, NN, (long)w, 1};
+            g2[i] = GemmDesc{c1, by, c2, NN, h, NN, 1, (long)NN};
+            vals[i] = Img{c2, NN, NN};
+            maxN = std::max(maxN, NN);
+            maxH = std::max(maxH, h);
+          }
+          GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
+          Img *vd = A.stage(vals);
+          push(gn, [=]() {
+            launch_cheb_gemm(d1, n, maxH, maxN, s);
+            launch_cheb_gemm(d2, n, maxN, maxN, s);
+            launch_hist_values(vd, 32, o, n, s);
+          });
+        } break;
+        case A_CHEB_FOURIER: {
+          // downsample planes above 300x300 (ImageMatrix::ChebyshevFourierTransform2D)
+          std::vector<DsDesc> ds;
+          std::vector<CfDesc> cd(n);
+          int maxchunk = 0, moh = 0, mow = 0, mnh = 0, mnw = 0;
+          for (int i = 0; i < n; i++) {
+            int h = N.h[i], w = N.w[i];
+            const double *plane = N.planes[i];
+            if ((long)w * h > 300 * 300) {
+              double rr = std::min(300.0 / (double)w, 300.0 / (double)h);
+              if (rr > 1) rr = 1;
+              const double dx = 1 / rr;
+              const int nw = (int)(unsigned)(rr * w), nh = (int)(unsigned)(rr * h);
+              DsDesc d;
+              d.in = plane;
+              d.tmp = A.dalloc_n<double>((size_t)h * w);
+              d.out = A.dalloc_n<double>((size_t)nh * nw);
+              d.oh = h; d.ow = w; d.nh = nh; d.nw = nw;
+              d.dx = dx; d.dy = dx;
+              d.xs = d.ys = nullptr;
+              d.cx = d.cy = 0;
+              if (!A.sizing) {
+                int r;
+                if ((r = get_dspos(c, w, dx, &d.xs, &d.cx))) return r;
+                if ((r = get_dspos(c, h, dx, &d.ys, &d.cy))) return r;
+              }
+              ds.push_back(d);
+              moh = std::max(moh, h); mow = std::max(mow, w); mnh = std::max(mnh, nh); mnw = std::max(mnw, nw);
+              plane = d.out;
+              h = nh;
+              w = nw;
+            }
+            CfDev *t = nullptr;
+            CfDesc C;
+            C.img = plane;
+            C.w = w;
+            C.h = h;
+            if (!A.sizing) {
+              int r;
+              if ((r = get_cf(c, w, h, &t))) return r;
+              C.kk = (const int *)t->kk.p;
+              C.tn = (const double *)t->tn.p;
+              C.fc = (const double *)t->fc.p;
+              C.fs = (const double *)t->fs.p;
+              C.np = t->np;
+              C.NN = t->NN;
+            } else {
+              C.kk = nullptr; C.tn = C.fc = C.fs = nullptr;
+              C.np = w * h;  // upper bound for sizing
+              C.NN = 23;
+            }
+            const int nch = (C.np + 127) / 128;
+            C.partial = A.dalloc_n<double>((size_t)nch * 2 * 23 * 23 + 8);
+            maxchunk = std::max(maxchunk, nch);
+            cd[i] = C;
+          }
+          CfDesc *cdd = A.stage(cd);
+          DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
+          const int nds = (int)ds.size();
+          push(gn, [=]() {
+            if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
+            launch_cf(cdd, n, maxchunk, o, s);
+          });
+        } break;
+        case A_EDGE: {
+          std::vector<Img> md(2 * n), mg(n), dr(n);
+          for (int i = 0; i < n; i++) {
+            const size_t P = (size_t)N.h[i] * N.w[i];
+            mg[i] = Img{A.dalloc_n<double>(P), N.h[i], N.w[i]};
+            dr[i] = Img{A.dalloc_n<double>(P), N.h[i], N.w[i]};
+            md[2 * i] = mg[i];
+            md[2 * i + 1] = dr[i];
+          }
+          std::vector<uint64_t *> sb(2 * n);
+          std::vector<long> cap(2 * n);
+          for (int i = 0; i < 2 * n; i++) {
+            cap[i] = (long)md[i].h * md[i].w;
+            sb[i] = A.dalloc_n<uint64_t>(2 * (size_t)cap[i]);
+          }
+          Img *mgd = A.stage(mg), *drd = A.stage(dr), *mdd = A.stage(md);
+          uint64_t **sbd = A.stage(sb);
+          long *capd = A.stage(cap);
+          Stats *st2 = A.dalloc_n<Stats>(2 * n);
+          double *med = A.dalloc_n<double>(2 * n);
+          const long mp = maxp;
+          push(gn, [=]() {
+            launch_prewitt(ni, mgd, drd, n, mp, s);
+            launch_stats(mdd, st2, 2 * n, s);
+            launch_sort(mdd, sbd, capd, 2 * n, s);
+            launch_median_gini(mdd, sbd, st2, med, OutRef{nullptr, 0, 0}, OutRef{nullptr, 0, 0}, 2 * n, s);
+            launch_edge_final(mgd, drd, st2, med, o, n, s);
+          });
+        } break;
+        case A_OTSU:
+        case A_INV_OTSU: {
+          std::vector<ObjDesc> od(n);
+          for (int i = 0; i < n; i++) {
+            const size_t P = (size_t)N.h[i] * N.w[i];
+            od[i].label = A.dalloc_n<int>(P);
+            od[i].area = A.dalloc_n<int>(P + 1);
+            od[i].xm = A.dalloc_n<long long>(P + 1);
+            od[i].ym = A.dalloc_n<long long>(P + 1);
+            od[i].keys = A.dalloc_n<uint64_t>(2 * (P + 1));
+            od[i].dist = A.dalloc_n<double>(P + 1);
+            od[i].count = A.dalloc_n<int>(1);
+          }
+          ObjDesc *odd = A.stage(od);
+          const int inv = g.alg == A_INV_OTSU;
+          const long mp = maxp;
+          push(gn, [=]() { launch_objects(ni, nst, inv, odd, o, n, (int)mp, s); });
+        } break;
+        case A_GABOR: {
+          std::vector<double *> en(n);
+          for (int i = 0; i < n; i++) en[i] = A.dalloc_n<double>(8 * (size_t)N.h[i] * N.w[i]);
+          double **end_ = A.stage(en);
+          const double *gk = (const double *)c->gabor.p;
+          push(gn, [=]() { launch_gabor(ni, gk, end_, o, n, mh, mw, s); });
+        } break;
+        default: break;
+      }
+    }
+    if (off_pix >= 0 || off_gini >= 0) {
+      std::vector<uint64_t *> sb(n);
+      std::vector<long> cap(n);
+      for (int i = 0; i < n; i++) {
+        cap[i] = (long)N.h[i] * N.w[i];
+        sb[i] = A.dalloc_n<uint64_t>(2 * (size_t)cap[i]);
+      }
+      uint64_t **sbd = A.stage(sb);
+      long *capd = A.stage(cap);
+      const OutRef op{off_pix >= 0 ? B.out : nullptr, B.nf, off_pix};
+      const OutRef og{off_gini >= 0 ? B.out : nullptr, B.nf, off_gini};
+      push("Pixel Intensity Statistics / Gini " + nm, [=]() {
+        launch_sort(ni, sbd, capd, n, s);
+        launch_median_gini(ni, sbd, nst, nullptr, op, og, n, s);
+      });
+    }
+  }
+  return 0;
+}
+
+static int ensure_tables(wndgpu_ctx *c) {
+  if (c->gabor.p) return 0;
+  int r;
+  std::vector<double> g = gabor_kernels(), z = zernike_h(), t = radon_trig();
+  if ((r = upload(c, c->gabor, g.data(), g.size() * sizeof(double)))) return r;
+  if ((r = upload(c, c->zern, z.data(), z.size() * sizeof(double)))) return r;
+  if ((r = upload(c, c->rtrig, t.data(), t.size() * sizeof(double)))) return r;
+  return 0;
+}
+
+static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timing) {
+  int r;
+  if ((r = ensure_tables(c))) return r;
+  std::vector<std::pair<std::string, std::function<void()>>> launches;
+  // pass 1: sizes
+  Arena A1{c, true};
+  if ((r = build_batch(c, plan, B, A1, launches))) return r;
+  if (A1.d + 4096 > c->dev_cap) {
+    if (c->dev) hipFree(c->dev);
+    c->dev = nullptr;
+    size_t cap = (A1.d + 4096) + (A1.d + 4096) / 4;
+    HIPCHK(c, hipMalloc(&c->dev, cap));
+    c->dev_cap = cap;
+  }
+  if (A1.h + 4096 > c->host_cap) {
+    if (c->host) hipHostFree(c->host);
+    if (c->hdesc) hipFree(c->hdesc);
+    size_t cap = (A1.h + 4096) * 2;
+    HIPCHK(c, hipHostMalloc((void **)&c->host, cap, hipHostMallocDefault));
+    HIPCHK(c, hipMalloc(&c->hdesc, cap));
+    c->host_cap = cap;
+  }
+  // pass 2: real addresses
+  Arena A2{c, false};
+  if ((r = build_batch(c, plan, B, A2, launches))) return r;
+  HIPCHK(c, hipMemcpyAsync(c->hdesc, c->host, A2.h, hipMemcpyHostToDevice, c->stream));
+  c->times.clear();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timing) {
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+  }
+  for (auto &L : launches) {
+    if (timing) hipEventRecord(e0, c->stream);
+    L.second();
+    if (timing) {
+      hipEventRecord(e1, c->stream);
+      hipEventSynchronize(e1);
+      float ms = 0;
+      hipEventElapsedTime(&ms, e0, e1);
+      c->times.emplace_back(L.first, ms);
+    }
+  }
+  if (timing) {
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+  }
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
+// ============================================================ C ABI
+extern "C" {
+
+const char *wndgpu_version(void) { return "wndgpu 0.1 (gfx950)"; }
+
+int wndgpu_create(int device, wndgpu_ctx **out) {
+  if (!out) return WNDGPU_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return WNDGPU_E_DEVICE;
+  wndgpu_ctx *c = new wndgpu_ctx;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return WNDGPU_E_DEVICE;
+  }
+  // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS
+  *out = c;
+  return WNDGPU_OK;
+}
+
+void wndgpu_destroy(wndgpu_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  auto fr = [](DevTable &t) { if (t.p) hipFree(t.p); t.p = nullptr; };
+  for (auto &kv : c->cheb) fr(kv.second);
+  for (auto &kv : c->fft) fr(kv.second.first);
+  for (auto &kv : c->cf) { fr(kv.second.kk); fr(kv.second.tn); fr(kv.second.fc); fr(kv.second.fs); }
+  for (auto &kv : c->dspos) fr(kv.second);
+  for (auto &kv : c->comb) fr(kv.second);
+  fr(c->gabor); fr(c->zern); fr(c->rtrig);
+  if (c->dev) hipFree(c->dev);
+  if (c->hdesc) hipFree(c->hdesc);
+  if (c->host) hipHostFree(c->host);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *wndgpu_last_error(const wndgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int wndgpu_plan_standard(int set, wndgpu_plan **plan) {
+  if (!plan || (set != 0 && set != 1)) return WNDGPU_E_ARG;
+  *plan = make_standard(set);
+  return WNDGPU_OK;
+}
+
+int wndgpu_plan_from_groups(const char *const *names, int ng, wndgpu_plan **plan) {
+  if (!plan || !names || ng <= 0) return WNDGPU_E_ARG;
+  wndgpu_plan *p = new wndgpu_plan;
+  for (int i = 0; i < ng; i++) {
+    int alg;
+    std::vector<int> chain;
+    if (!names[i] || !parse_group(names[i], &alg, &chain)) {
+      delete p;
+      return WNDGPU_E_ARG;
+    }
+    const std::string gname = std::string(kAlgNames[alg]) + " " + node_text(chain);
+    for (const Group &g : p->groups)
+      if (g.name == gname) { delete p; return WNDGPU_E_ARG; }  // duplicate group (Tasks.cpp:125)
+    plan_add(p, alg, chain);
+  }
+  plan_finalize(p);
+  *plan = p;
+  return WNDGPU_OK;
+}
+
+void wndgpu_plan_destroy(wndgpu_plan *p) { delete p; }
+int wndgpu_plan_n_features(const wndgpu_plan *p) { return p ? p->n_features : -1; }
+const char *wndgpu_plan_feature_name(const wndgpu_plan *p, int col) {
+  if (!p || col < 0 || col >= p->n_features) return "";
+  return p->names[col].c_str();
+}
+int wndgpu_plan_group_offset(const wndgpu_plan *p, const char *name) {
+  if (!p || !name) return -1;
+  int alg;
+  std::vector<int> chain;
+  if (!parse_group(name, &alg, &chain)) return -1;
+  for (const Group &g : p->groups)
+    if (g.alg == alg && g.chain == chain) return g.offset;
+  return -1;
+}
+
+static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n, double *out,
+                          int *status, bool device_io) {
+  if (!c || !plan || !images || n <= 0 || !out) return fail(c, WNDGPU_E_ARG, "null argument or empty batch");
+  HIPCHK(c, hipSetDevice(c->device));
+  Batch B;
+  B.nf = plan->n_features;
+  std::vector<int> idx;  // accepted images
+  bool rejected = false;
+  for (int i = 0; i < n; i++) {
+    const wndgpu_image &im = images[i];
+    int st = WNDGPU_IMG_OK;
+    if (im.dtype < 0 || im.dtype > 3 || !im.data || im.stride < im.width) st = WNDGPU_IMG_BAD_DTYPE;
+    else if (im.height < WNDGPU_MIN_SIDE || im.width < WNDGPU_MIN_SIDE) st = WNDGPU_IMG_TOO_SMALL;
+    else if (im.height > WNDGPU_MAX_SIDE || im.width > WNDGPU_MAX_SIDE) st = WNDGPU_IMG_TOO_LARGE;
+    if (status) status[i] = st;
+    if (st == WNDGPU_IMG_OK) idx.push_back(i);
+    else rejected = true;
+  }
+  if (idx.empty()) return fail(c, WNDGPU_E_IMAGE, "no valid image in the batch");
+  B.n = (int)idx.size();
+  // host inputs are staged into one device buffer
+  static const size_t esz[4] = {2, 4, 8, 1};
+  std::vector<void *> tmp;
+  size_t in_bytes = 0;
+  std::vector<size_t> in_off(B.n);
+  for (int k = 0; k < B.n; k++) {
+    const wndgpu_image &im = images[idx[k]];
+    B.h0.push_back(im.height);
+    B.w0.push_back(im.width);
+    in_off[k] = in_bytes;
+    in_bytes += ((size_t)im.height * im.stride * esz[im.dtype] + 255) & ~(size_t)255;
+  }
+  char *din = nullptr;
+  double *dout = nullptr;
+  if (!device_io) {
+    HIPCHK(c, hipMallocAsync((void **)&din, in_bytes, c->stream));
+    for (int k = 0; k < B.n; k++) {
+      const wndgpu_image &im = images[idx[k]];
+      HIPCHK(c, hipMemcpyAsync(din + in_off[k], im.data, (size_t)im.height * im.stride * esz[im.dtype],
+                               hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)B.n * B.nf * sizeof(double), c->stream));
+  } else {
+    HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)B.n * B.nf * sizeof(double), c->stream));
+  }
+  for (int k = 0; k < B.n; k++) {
+    const wndgpu_image &im = images[idx[k]];
+    LoadDesc L;
+    L.src = device_io ? im.data : (const void *)(din + in_off[k]);
+    L.h = im.height;
+    L.w = im.width;
+    L.stride = im.stride;
+    L.dtype = im.dtype;
+    B.load.push_back(L);
+  }
+  B.out = dout;
+  int r = run_batch(c, plan, B, getenv("WNDGPU_TIMING") != nullptr);
+  if (r) return r;
+  // scatter accepted rows
+  if (!device_io) {
+    std::vector<double> h((size_t)B.n * B.nf);
+    HIPCHK(c, hipMemcpyAsync(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < B.n; k++) memcpy(out + (size_t)idx[k] * B.nf, h.data() + (size_t)k * B.nf, B.nf * sizeof(double));
+    hipFreeAsync(din, c->stream);
+  } else {
+    for (int k = 0; k < B.n; k++)
+      HIPCHK(c, hipMemcpyAsync(out + (size_t)idx[k] * B.nf, dout + (size_t)k * B.nf, B.nf * sizeof(double),
+                               hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  hipFreeAsync(dout, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return rejected ? fail(c, WNDGPU_E_IMAGE, "some images were rejected (see status)") : WNDGPU_OK;
+}
+
+int wndgpu_compute(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n, double *out,
+                   int *status) {
+  return compute_common(c, plan, images, n, out, status, false);
+}
+
+int wndgpu_compute_device(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n, double *out,
+                          int *status) {
+  return compute_common(c, plan, images, n, out, status, true);
+}
+
+int wndgpu_algorithm_n_features(const char *name) {
+  if (!name) return -1;
+  const int a = alg_index(trim(name));
+  return a < 0 ? -1 : kNFeat[a];
+}
+
+int wndgpu_algorithm(wndgpu_ctx *c, const char *name, const double *in, int h, int w, double *out, int *n_out) {
+  if (!c || !name || !in || !out) return fail(c, WNDGPU_E_ARG, "null argument");
+  const int a = alg_index(trim(name));
+  if (a < 0) return fail(c, WNDGPU_E_ARG, "unknown feature algorithm '%s'", name);
+  wndgpu_plan *p = new wndgpu_plan;
+  plan_add(p, a, {});
+  plan_finalize(p);
+  wndgpu_image im{in, h, w, w, WNDGPU_F64};
+  int st;
+  int r = wndgpu_compute(c, p, &im, 1, out, &st);
+  if (n_out) *n_out = p->n_features;
+  delete p;
+  return r;
+}
+
+int wndgpu_transform(wndgpu_ctx *c, const char *name, const double *in, int h, int w, double *out, int *oh,
+                     int *ow) {
+  if (!c || !name || !in || !out) return fail(c, WNDGPU_E_ARG, "null argument");
+  const int t = tr_index(trim(name));
+  if (t < 0) return fail(c, WNDGPU_E_ARG, "unknown transform '%s'", name);
+  if (h < WNDGPU_MIN_SIDE || w < WNDGPU_MIN_SIDE || h > WNDGPU_MAX_SIDE || w > WNDGPU_MAX_SIDE)
+    return fail(c, WNDGPU_E_ARG, "image size %dx%d out of range", w, h);
+  HIPCHK(c, hipSetDevice(c->device));
+  int r;
+  if ((r = ensure_tables(c))) return r;
+  int th, tw;
+  out_size(t, h, w, &th, &tw);
+  double *din = nullptr, *dout = nullptr;
+  HIPCHK(c, hipMalloc(&din, (size_t)h * w * 8));
+  HIPCHK(c, hipMalloc(&dout, (size_t)th * tw * 8));
+  HIPCHK(c, hipMemcpy(din, in, (size_t)h * w * 8, hipMemcpyHostToDevice));
+  // single-node plan: the transform is run through the same batched path
+  Img ii{din, h, w}, oo{dout, th, tw};
+  Img *dii = nullptr, *doo = nullptr;
+  HIPCHK(c, hipMalloc(&dii, sizeof(Img)));
+  HIPCHK(c, hipMalloc(&doo, sizeof(Img)));
+  HIPCHK(c, hipMemcpy(dii, &ii, sizeof(Img), hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(doo, &oo, sizeof(Img), hipMemcpyHostToDevice));
+  hipStream_t s = c->stream;
+  std::vector<void *> frees;
+  if (t == T_EDGE) {
+    launch_edge(dii, doo, 1, (long)h * w, s);
+  } else if (t == T_WAVELET) {
+    double *g = nullptr;
+    const int gs = ((h + 9) / 2) * w;
+    HIPCHK(c, hipMalloc(&g, 2 * (size_t)gs * 8));
+    double **gd = nullptr;
+    int *gsd = nullptr;
+    HIPCHK(c, hipMalloc(&gd, sizeof(double *)));
+    HIPCHK(c, hipMalloc(&gsd, sizeof(int)));
+    HIPCHK(c, hipMemcpy(gd, &g, sizeof(double *), hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(gsd, &gs, sizeof(int), hipMemcpyHostToDevice));
+    launch_wavelet(dii, gd, gsd, doo, 1, h, w, s);
+    frees = {g, gd, gsd};
+  } else if (t == T_CHEBYSHEV) {
+    const int N = std::min(h, w);
+    const double *bx, *by;
+    if ((r = get_cheb(c, w, N, &bx))) return r;
+    if ((r = get_cheb(c, h, N, &by))) return r;
+    double *c1 = nullptr;
+    HIPCHK(c, hipMalloc(&c1, (size_t)h * N * 8));
+    GemmDesc g[2] = {GemmDesc{din, bx, c1, h, w, N, (long)w, 1}, GemmDesc{c1, by, dout, N, h, N, 1, (long)N}};
+    GemmDesc *gd = nullptr;
+    HIPCHK(c, hipMalloc(&gd, sizeof g));
+    HIPCHK(c, hipMemcpy(gd, g, sizeof g, hipMemcpyHostToDevice));
+    launch_cheb_gemm(gd, 1, h, N, s);
+    launch_cheb_gemm(gd + 1, 1, N, N, s);
+    frees = {c1, gd};
+  } else {
+    FftPlan pw, ph;
+    if ((r = get_fft(c, w, &pw))) return r;
+    if ((r = get_fft(c, h, &ph))) return r;
+    double2 *f1 = nullptr;
+    HIPCHK(c, hipMalloc(&f1, (size_t)h * w * sizeof(double2)));
+    void *pd = nullptr;
+    HIPCHK(c, hipMalloc(&pd, 2 * sizeof(FftPlan) + sizeof(double2 *)));
+    FftPlan pls[2] = {pw, ph};
+    HIPCHK(c, hipMemcpy(pd, pls, sizeof pls, hipMemcpyHostToDevice));
+    double2 **f1d = (double2 **)((char *)pd + 2 * sizeof(FftPlan));
+    HIPCHK(c, hipMemcpy(f1d, &f1, sizeof(double2 *), hipMemcpyHostToDevice));
+    launch_fft(dii, f1d, (FftPlan *)pd, (FftPlan *)pd + 1, doo, 1, h, w, s);
+    frees = {f1, pd};
+  }
+  HIPCHK(c, hipStreamSynchronize(s));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpy(out, dout, (size_t)th * tw * 8, hipMemcpyDeviceToHost));
+  for (void *p : frees) hipFree(p);
+  hipFree(din); hipFree(dout); hipFree(dii); hipFree(doo);
+  if (oh) *oh = th;
+  if (ow) *ow = tw;
+  return WNDGPU_OK;
+}
+
+// ---- classifier
+struct DBuf {
+  void *p = nullptr;
+  ~DBuf() { if (p) hipFree(p); }
+};
+#define DALLOC(ctx, buf, bytes) HIPCHK(ctx, hipMalloc(&(buf).p, (bytes) ? (bytes) : 8))
+
+int wndgpu_normalize_train(wndgpu_ctx *c, double *data, int n, int f, double *mins, double *maxs) {
+  if (!c || !data || !mins || !maxs || n <= 0 || f <= 0) return fail(c, WNDGPU_E_ARG, "bad argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf d, mn, mx;
+  DALLOC(c, d, (size_t)n * f * 8);
+  DALLOC(c, mn, (size_t)f * 8);
+  DALLOC(c, mx, (size_t)f * 8);
+  HIPCHK(c, hipMemcpy(d.p, data, (size_t)n * f * 8, hipMemcpyHostToDevice));
+  launch_norm_train((double *)d.p, n, f, (double *)mn.p, (double *)mx.p, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(data, d.p, (size_t)n * f * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(mins, mn.p, (size_t)f * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(maxs, mx.p, (size_t)f * 8, hipMemcpyDeviceToHost));
+  return WNDGPU_OK;
+}
+
+int wndgpu_normalize_test(wndgpu_ctx *c, double *vec, int f, const double *mins, const double *maxs) {
+  if (!c || !vec || !mins || !maxs || f <= 0) return fail(c, WNDGPU_E_ARG, "bad argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf v, mn, mx;
+  DALLOC(c, v, (size_t)f * 8);
+  DALLOC(c, mn, (size_t)f * 8);
+  DALLOC(c, mx, (size_t)f * 8);
+  HIPCHK(c, hipMemcpy(v.p, vec, (size_t)f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(mn.p, mins, (size_t)f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(mx.p, maxs, (size_t)f * 8, hipMemcpyHostToDevice));
+  launch_norm_test((double *)v.p, 1, f, (const double *)mn.p, (const double *)mx.p, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(vec, v.p, (size_t)f * 8, hipMemcpyDeviceToHost));
+  return WNDGPU_OK;
+}
+
+int wndgpu_fisher(wndgpu_ctx *c, const double *data, const int *cls, int n, int f, int C, double used, double *w) {
+  if (!c || !data || !cls || !w || n <= 0 || f <= 0 || C <= 0) return fail(c, WNDGPU_E_ARG, "bad argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf d, k, wd;
+  DALLOC(c, d, (size_t)n * f * 8);
+  DALLOC(c, k, (size_t)n * 4);
+  DALLOC(c, wd, (size_t)f * 8);
+  HIPCHK(c, hipMemcpy(d.p, data, (size_t)n * f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(k.p, cls, (size_t)n * 4, hipMemcpyHostToDevice));
+  launch_fisher((const double *)d.p, (const int *)k.p, n, f, C, used, (double *)wd.p, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(w, wd.p, (size_t)f * 8, hipMemcpyDeviceToHost));
+  return WNDGPU_OK;
+}
+
+int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f, int C, const double *w,
+                const double *test, int ntest, double *prob, double *norm, long *pred) {
+  if (!c || !train || !cls || !w || !test || !prob || !norm || !pred || n <= 0 || f <= 0 || C <= 0 || C > 63 ||
+      ntest <= 0)
+    return fail(c, WNDGPU_E_ARG, "bad argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf tr, k, wd, te, dist, pr, nm, pd;
+  DALLOC(c, tr, (size_t)n * f * 8);
+  DALLOC(c, k, (size_t)n * 4);
+  DALLOC(c, wd, (size_t)f * 8);
+  DALLOC(c, te, (size_t)ntest * f * 8);
+  DALLOC(c, dist, (size_t)ntest * n * 8);
+  DALLOC(c, pr, (size_t)ntest * C * 8);
+  DALLOC(c, nm, (size_t)ntest * 8);
+  DALLOC(c, pd, (size_t)ntest * sizeof(long));
+  HIPCHK(c, hipMemcpy(tr.p, train, (size_t)n * f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(k.p, cls, (size_t)n * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(wd.p, w, (size_t)f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(te.p, test, (size_t)ntest * f * 8, hipMemcpyHostToDevice));
+  launch_wnd5((const double *)tr.p, (const int *)k.p, (const double *)wd.p, (const double *)te.p, n, f, C, ntest,
+              (double *)dist.p, (double *)pr.p, (double *)nm.p, (long *)pd.p, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(prob, pr.p, (size_t)ntest * C * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(norm, nm.p, (size_t)ntest * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(pred, pd.p, (size_t)ntest * sizeof(long), hipMemcpyDeviceToHost));
+  return WNDGPU_OK;
+}
+
+int wndgpu_last_node_times(wndgpu_ctx *c, double *ms, const char **names, int cap) {
+  if (!c) return 0;
+  int k = 0;
+  for (auto &t : c->times) {
+    if (k >= cap) break;
+    if (ms) ms[k] = t.second;
+    if (names) names[k] = t.first.c_str();
+    k++;
+  }
+  return k;
+}
+
+}  // extern "C"

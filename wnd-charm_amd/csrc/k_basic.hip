@@ -1,0 +1,469 @@
+// Memory-bound kernels of the feature path: input conversion, plane
+// statistics, the Edge / Symlet-5 wavelet / Chebyshev / Fourier transforms,
+// per-image radix sort (median, Gini), multiscale histograms and fractal
+// features.  All FP64; operations whose order fixes integer-valued outputs
+// (wavelet taps, Chebyshev contractions) follow the reference order exactly.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "blocksort.hpp"
+
+namespace wnd {
+
+// ---------------------------------------------------------------------------
+// input conversion: uint16 / f32 / f64 source with row stride -> FP64 plane
+__global__ void k_load(const LoadDesc *d, Img *dst) {
+  const LoadDesc L = d[blockIdx.y];
+  const Img o = dst[blockIdx.y];
+  const long n = (long)L.h * L.w;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const int y = i / L.w, x = i - (long)y * L.w;
+    double v;
+    if (L.dtype == 0) v = (double)((const uint16_t *)L.src)[(long)y * L.stride + x];
+    else if (L.dtype == 1) v = (double)((const float *)L.src)[(long)y * L.stride + x];
+    else if (L.dtype == 2) v = ((const double *)L.src)[(long)y * L.stride + x];
+    else v = (double)((const uint8_t *)L.src)[(long)y * L.stride + x];
+    o.p[i] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// plane statistics (ImageMatrix::GetStats, src/cmatrix.cpp:815-827; Moments2,
+// src/statistics/Moments.h:68-107).  Two-pass mean / M2 per block of 1024.
+__global__ void __launch_bounds__(1024) k_stats(const Img *imgs, Stats *st) {
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  const Img im = imgs[blockIdx.x];
+  const long n = (long)im.h * im.w;
+  double s = 0, mn = 1e308, mx = -1e308;
+  long long c = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    double v = im.p[i];
+    if (moments_ok(v)) { s += v; c++; mn = fmin(mn, v); mx = fmax(mx, v); }
+  }
+  s = block_reduce(s, SumOp(), shd);
+  c = block_reduce(c, SumOp(), shl);
+  mn = block_reduce(mn, MinOp(), shd);
+  mx = block_reduce(mx, MaxOp(), shd);
+  const double mean = c ? s / (double)c : 0.0;
+  double m2 = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    double v = im.p[i];
+    if (moments_ok(v)) { double d = v - mean; m2 += d * d; }
+  }
+  m2 = block_reduce(m2, SumOp(), shd);
+  if (threadIdx.x == 0) {
+    Stats r;
+    r.n = c;
+    r.mean = mean;
+    r.mn = c ? mn : 1.79769313486231570815e+308;
+    r.mx = c ? mx : -1.79769313486231570815e+308;
+    r.var = c > 2 ? m2 / (double)(c - 1) : 0.0;
+    r.stdv = c > 2 ? sqrt(m2 / (double)(c - 1)) : 0.0;
+    st[blockIdx.x] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Edge transform (ImageMatrix::EdgeTransform, src/cmatrix.cpp:1294-1311)
+__global__ void k_edge(const Img *in, const Img *out) {
+  const Img a = in[blockIdx.y], o = out[blockIdx.y];
+  const long n = (long)a.h * a.w;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int y = i / a.w, x = i - (long)y * a.w;
+  const double c = a.p[i];
+  double my = 0, mx = 0;
+  if (y > 0 && y < a.h - 1) my = fmax(fabs(c - a.p[i - a.w]), fabs(c - a.p[i + a.w]));
+  if (x > 0 && x < a.w - 1) mx = fmax(fabs(c - a.p[i - 1]), fabs(c - a.p[i + 1]));
+  o.p[i] = fmax(mx, my);
+}
+
+// ---------------------------------------------------------------------------
+// Symlet-5 1-level DWT, zero padding (Symlet5.cpp:10-59, Wavelet.cpp:136-434,
+// convolution.cpp:86-326).  Output o of a length-N line is
+//   sum_{j=max(0,i-N+1)}^{min(9,i)} in[i-j]*f[j],  i = 2o+1,
+// accumulated in ascending j from 0.0 (the reference's three loop regions).
+__constant__ double c_sym5_lo[10] = {0.027333068345077982, 0.029519490925774643, -0.039134249302383094,
+  0.1993975339773936, 0.72340769040242059, 0.63397896345821192, 0.016602105764522319,
+  -0.17532808990845047, -0.021101834024758855, 0.019538882735286728};
+__constant__ double c_sym5_hi[10] = {-0.019538882735286728, -0.021101834024758855, 0.17532808990845047,
+  0.016602105764522319, -0.63397896345821192, 0.72340769040242059, -0.1993975339773936,
+  -0.039134249302383094, -0.029519490925774643, 0.027333068345077982};
+
+// pass 1 (along y): g1/g2[oy][x] = approx/detail of column x
+__global__ void k_wavelet_cols(const Img *in, double *const *g, const int *gstride) {
+  const Img a = in[blockIdx.y];
+  const int oy = (a.h + 9) / 2;
+  const long n = (long)oy * a.w;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int o = t / a.w, x = t - (long)o * a.w;
+  const int i = 2 * o + 1, N = a.h;
+  const int jlo = i - (N - 1) > 0 ? i - (N - 1) : 0, jhi = i < 9 ? i : 9;
+  double s1 = 0, s2 = 0;
+  for (int j = jlo; j <= jhi; ++j) {
+    const double v = a.p[(long)(i - j) * a.w + x];
+    s1 += v * c_sym5_lo[j];
+    s2 += v * c_sym5_hi[j];
+  }
+  double *g1 = g[blockIdx.y];
+  g1[t] = s1;
+  g1[(long)gstride[blockIdx.y] + t] = s2;
+}
+
+// pass 2 (along x) + quadrant placement: with OX = floor((W+9)/2),
+// OY = floor((H+9)/2), the output is 2*OY x 2*OX with
+//   (y<OY, x<OX) = dd, (y>=OY, x<OX) = da, (y<OY, x>=OX) = ad, (y>=OY, x>=OX) = aa
+// (first letter: along y, second: along x; Wavelet.cpp:348-428, nsteps = 1).
+__global__ void k_wavelet_rows(const Img *in, double *const *g, const int *gstride, const Img *out) {
+  const Img a = in[blockIdx.y], o = out[blockIdx.y];
+  const int OY = (a.h + 9) / 2, OX = (a.w + 9) / 2, N = a.w;
+  const long n = (long)OY * OX;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int r = t / OX, ox = t - (long)r * OX;
+  const double *g1 = g[blockIdx.y] + (long)r * N;
+  const double *g2 = g1 + gstride[blockIdx.y];
+  const int i = 2 * ox + 1;
+  const int jlo = i - (N - 1) > 0 ? i - (N - 1) : 0, jhi = i < 9 ? i : 9;
+  double a1 = 0, d1 = 0, a2 = 0, d2 = 0;
+  for (int j = jlo; j <= jhi; ++j) {
+    const double v1 = g1[i - j], v2 = g2[i - j];
+    a1 += v1 * c_sym5_lo[j];
+    d1 += v1 * c_sym5_hi[j];
+    a2 += v2 * c_sym5_lo[j];
+    d2 += v2 * c_sym5_hi[j];
+  }
+  const int W = 2 * OX;
+  o.p[(long)r * W + ox] = d2;
+  o.p[(long)(r + OY) * W + ox] = a2;
+  o.p[(long)r * W + ox + OX] = d1;
+  o.p[(long)(r + OY) * W + ox + OX] = a1;
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev contraction (Chebyshev2D / getChCoeff1D, src/transforms/chebyshev.cpp:80-165):
+//   C[r][c] = sum_{a<K} (A(r,a) * B[a][c]) / 2, accumulated in ascending a
+// so the result is bit-identical to the reference's loop.  B holds the
+// host-computed basis tj = T/W (c = 0) or T*2/W.  64x64 output tile per block,
+// 4x4 outputs per thread, K staged through LDS 16 at a time.
+__global__ void __launch_bounds__(256) k_cheb_gemm(const GemmDesc *ds) {
+  const GemmDesc d = ds[blockIdx.z];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  if (r0 >= d.M || c0 >= d.N) return;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < d.K; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, rr = e & 63;
+      const int k = k0 + kk, r = r0 + rr, c = c0 + rr;
+      As[kk][rr] = (k < d.K && r < d.M) ? d.A[(long)r * d.sar + (long)k * d.sak] : 0.0;
+      Bs[kk][rr] = (k < d.K && c < d.N) ? d.B[(long)k * d.N + c] : 0.0;
+    }
+    __syncthreads();
+    const int kn = d.K - k0 < 16 ? d.K - k0 : 16;
+    for (int kk = 0; kk < kn; kk++) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] += (av[i] * bv[j]) / 2;
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
+      if (r < d.M && c < d.N) d.C[(long)r * d.N + c] = acc[i][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fourier magnitude (ImageMatrix::fft2, src/cmatrix.cpp:1107-1160): the
+// unnormalised 2-D DFT, |F(ky,kx)| for ky < H/2+1, the lower rows filled by
+// |F(H-ky, (W-kx) mod W)| as the reference does.  Mixed-radix Stockham FFT
+// in LDS, one work-group per line; any radix is evaluated as a direct
+// R-point DFT with combined twiddles from a length-n table.
+__device__ void fft_lds(double2 *a, double2 *b, const FftPlan &pl, double2 **res) {
+  const int n = pl.n;
+  int Ns = 1;
+  double2 *src = a, *dst = b;
+  for (int s = 0; s < pl.nst; s++) {
+    const int R = pl.radix[s];
+    const int nr = n / R, step = n / (Ns * R);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int j = t % nr, q = t / nr;
+      const int k = j % Ns;
+      const int base = k * step + q * nr;
+      double re = 0, im = 0;
+      long idx = 0;
+      for (int r = 0; r < R; r++) {
+        const double2 v = src[j + r * nr];
+        const double2 w = pl.tw[idx];
+        re += v.x * w.x - v.y * w.y;
+        im += v.x * w.y + v.y * w.x;
+        idx += base;
+        if (idx >= n) idx -= n;
+      }
+      dst[(j / Ns) * Ns * R + k + q * Ns] = make_double2(re, im);
+    }
+    __syncthreads();
+    double2 *tt = src; src = dst; dst = tt;
+    Ns *= R;
+  }
+  *res = src;
+}
+
+// rows: F1[y][x] = DFT_x(img[y][:]), complex, row-major H x W
+__global__ void __launch_bounds__(256) k_fft_rows(const Img *in, double2 *const *f1, const FftPlan *plw) {
+  extern __shared__ double2 sh[];
+  const Img a = in[blockIdx.y];
+  if ((int)blockIdx.x >= a.h) return;
+  const FftPlan pl = plw[blockIdx.y];
+  double2 *b0 = sh, *b1 = sh + pl.n;
+  for (int x = threadIdx.x; x < a.w; x += blockDim.x) b0[x] = make_double2(a.p[(long)blockIdx.x * a.w + x], 0.0);
+  __syncthreads();
+  double2 *r;
+  fft_lds(b0, b1, pl, &r);
+  double2 *o = f1[blockIdx.y] + (long)blockIdx.x * a.w;
+  for (int x = threadIdx.x; x < a.w; x += blockDim.x) o[x] = r[x];
+}
+
+// columns: out(ky, x) = |DFT_y(F1[:][x])| for ky < H/2+1
+__global__ void __launch_bounds__(256) k_fft_cols(const Img *in, double2 *const *f1, const FftPlan *plh,
+                                                  const Img *out) {
+  extern __shared__ double2 sh[];
+  const Img a = in[blockIdx.y], o = out[blockIdx.y];
+  if ((int)blockIdx.x >= a.w) return;
+  const FftPlan pl = plh[blockIdx.y];
+  double2 *b0 = sh, *b1 = sh + pl.n;
+  const double2 *src = f1[blockIdx.y];
+  for (int y = threadIdx.x; y < a.h; y += blockDim.x) b0[y] = src[(long)y * a.w + blockIdx.x];
+  __syncthreads();
+  double2 *r;
+  fft_lds(b0, b1, pl, &r);
+  const int hh = a.h / 2 + 1;
+  for (int y = threadIdx.x; y < hh; y += blockDim.x) {
+    const double2 c = r[y];
+    o.p[(long)y * a.w + blockIdx.x] = sqrt(c.x * c.x + c.y * c.y);
+  }
+}
+
+__global__ void k_fft_fill(const Img *out) {
+  const Img o = out[blockIdx.y];
+  const int hh = o.h / 2 + 1;
+  const long n = (long)(o.h - hh) * o.w;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int y = hh + t / o.w, x = t % o.w;
+  const int xs = x == 0 ? 0 : o.w - x;
+  o.p[(long)y * o.w + x] = o.p[(long)(o.h - y) * o.w + xs];
+}
+
+// ---------------------------------------------------------------------------
+// Per-image radix sort of the plane (median, Gini); one 1024-thread
+// work-group per image, result left in buf[0..n).
+__global__ void __launch_bounds__(1024) k_sort(const Img *imgs, uint64_t *const *buf, const long *cap) {
+  const Img im = imgs[blockIdx.x];
+  const long n = (long)im.h * im.w;
+  uint64_t *k0 = buf[blockIdx.x], *k1 = k0 + cap[blockIdx.x];
+  for (long i = threadIdx.x; i < n; i += 1024) k0[i] = dkey(im.p[i]);
+  __syncthreads();
+  uint64_t *r = block_radix_sort(k0, k1, n);
+  if (r != k0)
+    for (long i = threadIdx.x; i < n; i += 1024) k0[i] = r[i];
+}
+
+// median (ImageMatrix::get_median, cmatrix.cpp:780-800) and Gini
+// (FeatureAlgorithms.cpp:448-489) from the sorted keys; writes the five
+// Pixel Intensity Statistics (FeatureAlgorithms.cpp:272-291) and/or Gini.
+__global__ void __launch_bounds__(1024) k_median_gini(const Img *imgs, uint64_t *const *buf, const Stats *st,
+                                                      double *median_out, OutRef pix, OutRef gini) {
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  __shared__ long long first_pos;
+  const Img im = imgs[blockIdx.x];
+  const long n = (long)im.h * im.w;
+  const uint64_t *k = buf[blockIdx.x];
+  if (threadIdx.x == 0) first_pos = n;
+  __syncthreads();
+  // first index whose value is > 0 (keys are sorted)
+  for (long i = threadIdx.x; i < n; i += 1024) {
+    const double v = dkey_inv(k[i]);
+    const double pv = i ? dkey_inv(k[i - 1]) : -1.0;
+    if (v > 0 && !(pv > 0)) atomicMin((unsigned long long *)&first_pos, (unsigned long long)i);
+  }
+  __syncthreads();
+  const long half = n / 2;
+  double med;
+  if (n % 2 == 0) { med = dkey_inv(k[half]); med += dkey_inv(k[half - 1]); med /= 2.0; }
+  else med = dkey_inv(k[half]);
+  const long fp = first_pos, cnt = n - fp;
+  double s = 0, g = 0;
+  for (long i = fp + threadIdx.x; i < n; i += 1024) {
+    const double v = dkey_inv(k[i]);
+    const long ii = i - fp + 1;
+    s += v;
+    g += (2. * ii - cnt - 1.) * v;
+  }
+  s = block_reduce(s, SumOp(), shd);
+  g = block_reduce(g, SumOp(), shd);
+  if (threadIdx.x == 0) {
+    if (median_out) median_out[blockIdx.x] = med;
+    if (pix.out) {
+      double *o = pix.out + (long)blockIdx.x * pix.stride + pix.off;
+      const Stats S = st[blockIdx.x];
+      o[0] = S.mean; o[1] = med; o[2] = S.stdv; o[3] = S.mn; o[4] = S.mx;
+    }
+    if (gini.out) {
+      const double mean = cnt > 0 ? s / cnt : 0.0;
+      double r;
+      if (cnt <= 1 || mean <= 0.0) r = 0.0;
+      else r = g / (mean * cnt * (cnt - 1));
+      gini.out[(long)blockIdx.x * gini.stride + gini.off] = r;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Multiscale histograms (ImageMatrix::MultiScaleHistogram, cmatrix.cpp:1743-1756)
+__global__ void __launch_bounds__(1024) k_msh(const Img *imgs, const Stats *st, OutRef o) {
+  __shared__ int cnt[24];
+  const Img im = imgs[blockIdx.x];
+  const Stats S = st[blockIdx.x];
+  if (threadIdx.x < 24) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const long n = (long)im.h * im.w;
+  const double s3 = hist_scale(S.mn, S.mx, 3), s5 = hist_scale(S.mn, S.mx, 5), s7 = hist_scale(S.mn, S.mx, 7),
+               s9 = hist_scale(S.mn, S.mx, 9);
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = im.p[i];
+    atomicAdd(&cnt[hist_bin(v, S.mn, s3, 3)], 1);
+    atomicAdd(&cnt[3 + hist_bin(v, S.mn, s5, 5)], 1);
+    atomicAdd(&cnt[8 + hist_bin(v, S.mn, s7, 7)], 1);
+    atomicAdd(&cnt[15 + hist_bin(v, S.mn, s9, 9)], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = 0;
+    for (int a = 0; a < 24; a++) if (cnt[a] > mx) mx = cnt[a];
+    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    for (int a = 0; a < 24; a++) out[a] = (double)cnt[a] / mx;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fractal features (FeatureAlgorithms.cpp:232-264): one block per (k, image)
+__global__ void __launch_bounds__(256) k_fractal(const Img *imgs, OutRef o) {
+  __shared__ double shd[4];
+  const Img im = imgs[blockIdx.y];
+  const int W = im.w, H = im.h;
+  const int K = (W > H ? H : W) / 5;
+  int step = K / 20;
+  if (step < 1) step = 1;
+  const int bin = blockIdx.x;
+  const int k = 1 + bin * step;
+  double *out = o.out + (long)blockIdx.y * o.stride + o.off;
+  if (bin >= 20) return;
+  if (k >= K) {
+    if (threadIdx.x == 0) out[bin] = 0.0;
+    return;
+  }
+  double s = 0;
+  const long nv = (long)(H - k) * W;
+  for (long t = threadIdx.x; t < nv; t += blockDim.x) s += fabs(im.p[t] - im.p[t + (long)k * W]);
+  const long nh = (long)H * (W - k);
+  for (long t = threadIdx.x; t < nh; t += blockDim.x) {
+    const int y = t / (W - k), x = t - (long)y * (W - k);
+    s += fabs(im.p[(long)y * W + x] - im.p[(long)y * W + x + k]);
+  }
+  s = block_reduce(s, SumOp(), shd);
+  if (threadIdx.x == 0) out[bin] = s / (W * (W - k) + H * (H - k));
+}
+
+// ---------------------------------------------------------------------------
+// Small dense histogram of a value list with its own min/max (used for the
+// Chebyshev-20 coefficient block, ImageMatrix::ChebyshevStatistics2D,
+// cmatrix.cpp:1274-1281).
+__global__ void __launch_bounds__(256) k_hist_values(const Img *vals, int nbins, OutRef o) {
+  __shared__ double shd[4];
+  __shared__ int cnt[64];
+  const Img v = vals[blockIdx.x];
+  const long n = (long)v.h * v.w;
+  double mn = 1e308, mx = -1e308;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const double x = v.p[i];
+    if (moments_ok(x)) { mn = fmin(mn, x); mx = fmax(mx, x); }
+  }
+  mn = block_reduce(mn, MinOp(), shd);
+  mx = block_reduce(mx, MaxOp(), shd);
+  if (threadIdx.x < 64) cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const double sc = hist_scale(mn, mx, nbins);
+  for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[hist_bin(v.p[i], mn, sc, nbins)], 1);
+  __syncthreads();
+  if (threadIdx.x < nbins) o.out[(long)blockIdx.x * o.stride + o.off + threadIdx.x] = cnt[threadIdx.x];
+}
+
+// ===========================================================================
+// launchers
+void launch_load(const LoadDesc *d, Img *dst, int nimg, long maxn, hipStream_t s) {
+  int blocks = (int)((maxn + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  k_load<<<dim3(blocks, nimg), 256, 0, s>>>(d, dst);
+}
+void launch_stats(const Img *imgs, Stats *st, int nimg, hipStream_t s) {
+  k_stats<<<nimg, 1024, 0, s>>>(imgs, st);
+}
+void launch_edge(const Img *in, const Img *out, int nimg, long maxn, hipStream_t s) {
+  k_edge<<<dim3((maxn + 255) / 256, nimg), 256, 0, s>>>(in, out);
+}
+void launch_wavelet(const Img *in, double *const *g, const int *gstride, const Img *out, int nimg, int maxh,
+                    int maxw, hipStream_t s) {
+  const long n1 = (long)((maxh + 9) / 2) * maxw;
+  k_wavelet_cols<<<dim3((n1 + 255) / 256, nimg), 256, 0, s>>>(in, g, gstride);
+  const long n2 = (long)((maxh + 9) / 2) * ((maxw + 9) / 2);
+  k_wavelet_rows<<<dim3((n2 + 255) / 256, nimg), 256, 0, s>>>(in, g, gstride, out);
+}
+void launch_cheb_gemm(const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s) {
+  k_cheb_gemm<<<dim3((maxN + 63) / 64, (maxM + 63) / 64, nimg), 256, 0, s>>>(d);
+}
+void launch_fft(const Img *in, double2 *const *f1, const FftPlan *plw, const FftPlan *plh, const Img *out, int nimg,
+                int maxh, int maxw, hipStream_t s) {
+  const int nmax = maxh > maxw ? maxh : maxw;
+  const size_t lds = 2 * (size_t)nmax * sizeof(double2);
+  k_fft_rows<<<dim3(maxh, nimg), 256, lds, s>>>(in, f1, plw);
+  k_fft_cols<<<dim3(maxw, nimg), 256, lds, s>>>(in, f1, plh, out);
+  const long nf = (long)(maxh - (maxh / 2 + 1) + 1) * maxw;
+  k_fft_fill<<<dim3((nf + 255) / 256, nimg), 256, 0, s>>>(out);
+}
+void launch_sort(const Img *imgs, uint64_t *const *buf, const long *cap, int nimg, hipStream_t s) {
+  k_sort<<<nimg, 1024, 0, s>>>(imgs, buf, cap);
+}
+void launch_median_gini(const Img *imgs, uint64_t *const *buf, const Stats *st, double *median_out, OutRef pix,
+                        OutRef gini, int nimg, hipStream_t s) {
+  k_median_gini<<<nimg, 1024, 0, s>>>(imgs, buf, st, median_out, pix, gini);
+}
+void launch_msh(const Img *imgs, const Stats *st, OutRef o, int nimg, hipStream_t s) {
+  k_msh<<<nimg, 1024, 0, s>>>(imgs, st, o);
+}
+void launch_fractal(const Img *imgs, OutRef o, int nimg, hipStream_t s) {
+  k_fractal<<<dim3(20, nimg), 256, 0, s>>>(imgs, o);
+}
+void launch_hist_values(const Img *vals, int nbins, OutRef o, int nimg, hipStream_t s) {
+  k_hist_values<<<nimg, 256, 0, s>>>(vals, nbins, o);
+}
+
+}  // namespace wnd

@@ -1,0 +1,429 @@
+// Otsu / Inverse-Otsu object features, Prewitt edge features and Gabor
+// textures.  FP64.  Integer outputs (thresholds, labels, areas, counts,
+// Euler number) are exact; the connected components are canonicalised to
+// the reference's raster-order labelling.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "blocksort.hpp"
+
+namespace wnd {
+
+// Otsu threshold of a plane (ImageMatrix::Otsu(dynamic), cmatrix.cpp:1520-1570):
+// 1024-bin histogram over [mn, mx], then the reference's sequential
+// omega / myu / sigma recurrence in one thread.  hist must hold 1024 ints.
+template <typename ValF>
+__device__ double block_otsu(long n, double mn, double mx, ValF val, int *hist) {
+  for (int b = threadIdx.x; b < 1024; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  const double sc = hist_scale(mn, mx, 1024);
+  for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[hist_bin(val(i), mn, sc, 1024)], 1);
+  __syncthreads();
+  __shared__ double thr_s;
+  if (threadIdx.x == 0) {
+    const double wh = (double)(unsigned)n;
+    // myu[L-1] first (the recurrence is replayed below with identical operations)
+    double my_last = 0.0;
+    for (int i = 1; i < 1024; i++) my_last = my_last + i * (hist[i] / wh);
+    double om = hist[0] / wh, my = 0.0;
+    int thr = 0;
+    double maxs = 0.0;
+    for (int i = 0; i < 1023; i++) {
+      if (i > 0) {
+        om = om + (hist[i] / wh);
+        my = my + i * (hist[i] / wh);
+      }
+      double sg;
+      if (om != 0.0 && om != 1.0) {
+        const double d = my_last * om - my;
+        sg = (d * d) / (om * (1.0 - om));
+      } else sg = 0.0;
+      if (sg > maxs) { maxs = sg; thr = i; }
+    }
+    thr_s = ((((double)thr / (double)(1023)) * (mx - mn)) + mn);
+  }
+  __syncthreads();
+  return thr_s;
+}
+
+// ------------------------------------------------------------- union-find
+__device__ __forceinline__ int uf_find(volatile int *lab, int x) {
+  while (true) {
+    const int p = lab[x];
+    if (p == x) return x;
+    x = p;
+  }
+}
+__device__ void uf_union(int *lab, int a, int b) {
+  while (true) {
+    a = uf_find(lab, a);
+    b = uf_find(lab, b);
+    if (a == b) return;
+    if (a < b) { const int t = a; a = b; b = t; }
+    const int old = atomicMin(&lab[a], b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+// Object features (ImageMatrix::FeatureStatistics, cmatrix.cpp:1642-1717;
+// bwlabel / GlobalCentroid / FeatureCentroid / EulerNumber,
+// FeatureStatistics.cpp:44-254; output layout FeatureAlgorithms.cpp:370-394).
+// One 1024-thread work-group per image.  invert != 0 evaluates the
+// Inverse-Otsu variant on v' = (max - v) + min (ImageMatrix::invert, cmatrix.cpp:569-572).
+__global__ void __launch_bounds__(1024) k_objects(const Img *imgs, const Stats *st, int invert, const ObjDesc *ods,
+                                                  OutRef o) {
+  __shared__ int hist[1024];
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  __shared__ int scan_w[16];
+  __shared__ int s_base;
+  const Img im = imgs[blockIdx.x];
+  const ObjDesc OD = ods[blockIdx.x];
+  const Stats S = st[blockIdx.x];
+  const int W = im.w, H = im.h;
+  const long n = (long)W * H;
+  const int t = threadIdx.x;
+  const double smx = S.mx, smn = S.mn;
+  auto val = [&](long i) -> double { return invert ? (smx - im.p[i]) + smn : im.p[i]; };
+  // stats of the plane the features are taken on
+  double mn = smn, mx = smx;
+  if (invert) {
+    double a = 1e308, b = -1e308;
+    for (long i = t; i < n; i += 1024) {
+      const double v = val(i);
+      if (moments_ok(v)) { a = fmin(a, v); b = fmax(b, v); }
+    }
+    mn = block_reduce(a, MinOp(), shd);
+    mx = block_reduce(b, MaxOp(), shd);
+  }
+  const double thr = block_otsu(n, mn, mx, val, hist);
+  int *lab = OD.label;
+  // mask, labels, global centroid (exact integer sums)
+  long long xm = 0, ym = 0, mass = 0, c1 = 0, c3 = 0;
+  for (long i = t; i < n; i += 1024) {
+    const bool m = val(i) > thr;
+    lab[i] = m ? (int)i : -1;
+    if (m) {
+      const int y = i / W, x = i - (long)y * W;
+      xm += x + 1; ym += y + 1; mass++;
+    }
+  }
+  __syncthreads();
+  // 8-connected union-find; every root ends at the component's smallest index
+  for (long i = t; i < n; i += 1024) {
+    if (lab[i] < 0) continue;
+    const int y = i / W, x = i - (long)y * W;
+    if (x > 0 && lab[i - 1] >= 0) uf_union(lab, (int)i, (int)(i - 1));
+    if (y > 0) {
+      const long u = i - W;
+      if (x > 0 && lab[u - 1] >= 0) uf_union(lab, (int)i, (int)(u - 1));
+      if (lab[u] >= 0) uf_union(lab, (int)i, (int)u);
+      if (x < W - 1 && lab[u + 1] >= 0) uf_union(lab, (int)i, (int)(u + 1));
+    }
+  }
+  __syncthreads();
+  __threadfence_block();
+  for (long i = t; i < n; i += 1024)
+    if (lab[i] >= 0) lab[i] = uf_find(lab, (int)i);
+  __syncthreads();
+  // Euler number patterns (EulerNumber(8): (C1 - C3) / 4; Cd is never counted)
+  for (long i = t; i < n; i += 1024) {
+    const int y = i / W, x = i - (long)y * W;
+    if (y < 1 || x < 1) continue;
+    const int c = (lab[i - W - 1] >= 0) + (lab[i - W] >= 0) + (lab[i - 1] >= 0) + (lab[i] >= 0);
+    c1 += (c == 1);
+    c3 += (c == 3);
+  }
+  xm = block_reduce(xm, SumOp(), shl);
+  ym = block_reduce(ym, SumOp(), shl);
+  mass = block_reduce(mass, SumOp(), shl);
+  c1 = block_reduce(c1, SumOp(), shl);
+  c3 = block_reduce(c3, SumOp(), shl);
+  // label ids: rank of each root in raster order (1-based)
+  int *rid = (int *)OD.keys;  // reuse: rid[i] for roots, before the keys are needed
+  if (t == 0) s_base = 0;
+  __syncthreads();
+  for (long c0 = 0; c0 < n; c0 += 1024) {
+    const long i = c0 + t;
+    const int isroot = (i < n && lab[i] == (int)i) ? 1 : 0;
+    // block exclusive scan of isroot
+    const int lane = t & 63, wv = t >> 6;
+    const uint64_t bal = __ballot(isroot);
+    const int pre = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+    if (lane == 0) scan_w[wv] = __popcll(bal);
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int w = 0; w < 16; w++) { if (w < wv) woff += scan_w[w]; tot += scan_w[w]; }
+    if (isroot) rid[i] = s_base + woff + pre + 1;
+    __syncthreads();
+    if (t == 0) s_base += tot;
+    __syncthreads();
+  }
+  const int count = s_base;
+  int *area = OD.area;
+  long long *axm = OD.xm, *aym = OD.ym;
+  for (long l = t; l <= count; l += 1024) { area[l] = 0; axm[l] = 0; aym[l] = 0; }
+  __syncthreads();
+  for (long i = t; i < n; i += 1024) {
+    const int r = lab[i];
+    if (r < 0) continue;
+    const int l = rid[r];
+    const int y = i / W, x = i - (long)y * W;
+    atomicAdd(&area[l], 1);
+    atomicAdd((unsigned long long *)&axm[l], (unsigned long long)(x + 1));
+    atomicAdd((unsigned long long *)&aym[l], (unsigned long long)(y + 1));
+  }
+  __syncthreads();
+  // per-object distance to the global centroid
+  double cx = 0, cy = 0;
+  if (mass) { cx = (double)xm / (double)mass; cy = (double)ym / (double)mass; }
+  uint64_t *ka = OD.keys, *kb = OD.keys + (count + 1);
+  double *dist = OD.dist;
+  double sum_d = 0;
+  for (long l = t; l < count; l += 1024) {
+    const long L = l + 1;
+    const double xc = (double)axm[L] / (double)area[L], yc = (double)aym[L] / (double)area[L];
+    const double dx = xc - cx, dy = yc - cy;
+    const double d = sqrt(dx * dx + dy * dy);
+    dist[l] = d;
+    sum_d += d;
+  }
+  sum_d = block_reduce(sum_d, SumOp(), shd);
+  // sort the areas (as integer keys)
+  for (long l = t; l < count; l += 1024) ka[l] = (uint64_t)area[l + 1];
+  __syncthreads();
+  uint64_t *sa = block_radix_sort(ka, kb, count);
+  // sorted areas -> dist buffer tail? keep them in place: copy to area[] (1-based free now)
+  for (long l = t; l < count; l += 1024) area[l] = (int)sa[l];
+  __syncthreads();
+  for (long l = t; l < count; l += 1024) ka[l] = dkey(dist[l]);
+  __syncthreads();
+  uint64_t *sd = block_radix_sort(ka, kb, count);
+  for (long l = t; l < count; l += 1024) dist[l] = dkey_inv(sd[l]);
+  __syncthreads();
+  // statistics (count == 0 reads an empty array in the reference; defined as 0 here)
+  const double sum_a = (double)mass;  // sum of object areas == mask pixel count
+  double AreaMean = 0, DistMean = 0, amin = 0, amax = 0, dmin = 0, dmax = 0;
+  if (count > 0) {
+    amin = area[0]; amax = area[count - 1];
+    dmin = dist[0]; dmax = dist[count - 1];
+    AreaMean = sum_a / count;
+    DistMean = sum_d / count;
+  }
+  if (t < 20) hist[t] = 0;
+  __syncthreads();
+  double va = 0, vd = 0;
+  const double hsa = (amax - amin > 0) ? 10.0 / (double)(amax - amin) : 0.0;
+  const double hsd = (dmax - dmin > 0) ? 10.0 / (double)(dmax - dmin) : 0.0;
+  for (long l = t; l < count; l += 1024) {
+    if (amax - amin > 0) {
+      const double da = area[l] - AreaMean;
+      va += da * da;
+      atomicAdd(&hist[hist_bin((double)area[l], amin, hsa, 10)], 1);
+    }
+    const double dd = dist[l] - DistMean;
+    vd += dd * dd;
+    atomicAdd(&hist[10 + hist_bin(dist[l], dmin, hsd, 10)], 1);
+  }
+  va = block_reduce(va, SumOp(), shd);
+  vd = block_reduce(vd, SumOp(), shd);
+  if (t == 0) {
+    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    int k = 0;
+    for (int j = 0; j < 10; j++) out[k++] = hist[j];
+    out[k++] = amax;
+    out[k++] = AreaMean;
+    out[k++] = count > 0 ? (double)(unsigned)area[count / 2] : 0.0;
+    out[k++] = amin;
+    out[k++] = count > 1 ? va / (count - 1) : va;
+    out[k++] = cx;
+    out[k++] = cy;
+    out[k++] = count;
+    for (int j = 0; j < 10; j++) out[k++] = hist[10 + j];
+    out[k++] = dmax;
+    out[k++] = DistMean;
+    out[k++] = count > 0 ? dist[count / 2] : 0.0;
+    out[k++] = dmin;
+    out[k++] = count > 1 ? vd / (count - 1) : vd;
+    out[k++] = (double)((c1 - c3) / 4);
+  }
+}
+
+void launch_objects(const Img *imgs, const Stats *st, int invert, const ObjDesc *od, OutRef o, int nimg, int maxn,
+                    hipStream_t s) {
+  (void)maxn;
+  k_objects<<<nimg, 1024, 0, s>>>(imgs, st, invert, od, o);
+}
+
+// ================================================================ Edge features
+// Prewitt magnitude / direction (cmatrix.cpp:1317-1384), x outer order of the
+// reference is irrelevant per pixel; the accumulation order inside a pixel is kept.
+__global__ void k_prewitt(const Img *in, const Img *mag, const Img *dir) {
+  const Img a = in[blockIdx.y], om = mag[blockIdx.y], od = dir[blockIdx.y];
+  const long w = a.w, h = a.h;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= w * h) return;
+  const long y = i / w, x = i - y * w;
+  double sumx = 0, sumy = 0;
+  for (long j = y - 1; j <= y + 1; j++) if (j >= 0 && j < h && x - 1 >= 0) sumx += a.p[j * w + x - 1] * 1;
+  for (long j = y - 1; j <= y + 1; j++) if (j >= 0 && j < h && x + 1 < w) sumx += a.p[j * w + x + 1] * -1;
+  for (long k = x - 1; k <= x + 1; k++) if (k >= 0 && k < w && y - 1 >= 0) sumy += a.p[(y - 1) * w + k] * 1;
+  for (long k = x - 1; k <= x + 1; k++) if (k >= 0 && k < w && y + 1 < h) sumy += a.p[(y + 1) * w + k] * -1;
+  om.p[i] = sqrt(sumx * sumx + sumy * sumy);
+  double d;
+  if (sumy == 0 || fabs(sumy) < 1 / kInf) d = 3.1415926 * (sumx < 0 ? 1 : 0);
+  else d = atan2(sumy, sumx);
+  od.p[i] = d;
+}
+
+// EdgeStatistics (cmatrix.cpp:1402-1458) + output layout (FeatureAlgorithms.cpp:314-335).
+// st2[2*i] / st2[2*i+1] and medians[2*i] / [2*i+1] belong to magnitude / direction.
+__global__ void __launch_bounds__(256) k_edge_final(const Img *mag, const Img *dir, const Stats *st2,
+                                                    const double *medians, OutRef o) {
+  __shared__ long long shl[4];
+  __shared__ int hm[8], hd[8];
+  const Img M = mag[blockIdx.x], D = dir[blockIdx.x];
+  const Stats SM = st2[2 * blockIdx.x], SD = st2[2 * blockIdx.x + 1];
+  const long n = (long)M.h * M.w;
+  if (threadIdx.x < 8) { hm[threadIdx.x] = 0; hd[threadIdx.x] = 0; }
+  __syncthreads();
+  const double level = SM.mean;
+  const double sm = hist_scale(SM.mn, SM.mx, 8), sd = hist_scale(SD.mn, SD.mx, 8);
+  long long area = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const double m = M.p[i];
+    if (m > level) area++;
+    atomicAdd(&hm[hist_bin(m, SM.mn, sm, 8)], 1);
+    atomicAdd(&hd[hist_bin(D.p[i], SD.mn, sd, 8)], 1);
+  }
+  area = block_reduce(area, SumOp(), shl);
+  if (threadIdx.x == 0) {
+    double DH[8], MH[8], diff[4] = {0, 0, 0, 0}, homog = 0;
+    for (int b = 0; b < 8; b++) { DH[b] = hd[b]; MH[b] = hm[b]; }
+    for (int b = 0; b < 4; b++) diff[b] = fabs(DH[b] - DH[b + 4]);
+    double sum = 0;
+    for (int b = 0; b < 4; b++) {
+      if (DH[b] + DH[b + 4] != 0) diff[b] = diff[b] / (DH[b] + DH[b + 4]);
+      sum += (DH[b] + DH[b + 4]);
+    }
+    if (sum > 0) homog = (DH[0] + DH[1]) / sum;
+    double *c = o.out + (long)blockIdx.x * o.stride + o.off;
+    int h = 0;
+    c[h++] = (double)area;
+    for (int j = 0; j < 4; j++) c[h++] = diff[j];
+    for (int j = 0; j < 8; j++) c[h++] = DH[j];
+    c[h++] = homog;
+    c[h++] = SD.mean;
+    c[h++] = medians[2 * blockIdx.x + 1];
+    c[h++] = SD.var;
+    for (int j = 0; j < 8; j++) c[h++] = MH[j];
+    c[h++] = SM.mean;
+    c[h++] = medians[2 * blockIdx.x];
+    c[h++] = SM.var;
+  }
+}
+
+void launch_prewitt(const Img *imgs, const Img *mag, const Img *dir, int nimg, long maxn, hipStream_t s) {
+  k_prewitt<<<dim3((maxn + 255) / 256, nimg), 256, 0, s>>>(imgs, mag, dir);
+}
+
+void launch_edge_final(const Img *mag, const Img *dir, const Stats *st2, const double *medians, OutRef o, int nimg,
+                       hipStream_t s) {
+  k_edge_final<<<nimg, 256, 0, s>>>(mag, dir, st2, medians, o);
+}
+
+// ======================================================================= Gabor
+// GaborTextureFilters2D (gabor.cpp:201-231) with GaborEnergy / conv2comp
+// (gabor.cpp:19-52,166-195).  kernels: 8 host-computed normalised complex
+// 38x38 filters (index 0 = low-pass f0 = 0.1, 1..7 = f0 1..7).  Each output
+// pixel accumulates the 1444 taps in the reference's (row, column) tap order
+// with separate multiply and add; taps that fall outside the image add
+// +-0, which leaves the sum unchanged.
+#define GB_N 38
+#define GB_TR 16
+#define GB_TC 64
+__global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const double *kernels, double *const *energy) {
+  __shared__ double tile[GB_TR + GB_N - 1][GB_TC + GB_N - 1 + 1];
+  __shared__ double2 wk[GB_N * GB_N];
+  const Img im = imgs[blockIdx.z];
+  const int f = blockIdx.y % 8;
+  const int tilesx = (im.w + GB_TC - 1) / GB_TC, tilesy = (im.h + GB_TR - 1) / GB_TR;
+  if ((int)blockIdx.x >= tilesx * tilesy) return;
+  const int b0 = (blockIdx.x / tilesx) * GB_TR, a0 = (blockIdx.x % tilesx) * GB_TC;
+  const int t = threadIdx.x;
+  const double *K = kernels + (long)f * GB_N * GB_N * 2;
+  for (int e = t; e < GB_N * GB_N; e += 256) wk[e] = make_double2(K[2 * e], K[2 * e + 1]);
+  // input rows b0-18 .. b0+GB_TR+18, columns a0-18 .. a0+GB_TC+18
+  for (int e = t; e < (GB_TR + GB_N - 1) * (GB_TC + GB_N - 1); e += 256) {
+    const int r = e / (GB_TC + GB_N - 1), c = e - r * (GB_TC + GB_N - 1);
+    const int y = b0 - 18 + r, x = a0 - 18 + c;
+    tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+  }
+  __syncthreads();
+  const int ty = t / 16, tx = t % 16;
+  const int b = b0 + ty, a = a0 + tx * 4;
+  double re[4] = {0, 0, 0, 0}, imv[4] = {0, 0, 0, 0};
+  const int rb = ty + 37, cb = tx * 4 + 37;
+  for (int j = 0; j < GB_N; j++) {
+    const double *row = &tile[rb - j][0];
+    for (int i = 0; i < GB_N; i++) {
+      const double2 w = wk[j * GB_N + i];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const double v = row[cb + q - i];
+        re[q] += v * w.x;
+        imv[q] += v * w.y;
+      }
+    }
+  }
+  double *E = energy[blockIdx.z] + (long)f * im.h * im.w;
+  if (b < im.h)
+    for (int q = 0; q < 4; q++)
+      if (a + q < im.w) E[(long)b * im.w + a + q] = sqrt(re[q] * re[q] + imv[q] * imv[q]);
+}
+
+__global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *const *energy, OutRef o) {
+  __shared__ int hist[1024];
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  const Img im = imgs[blockIdx.x];
+  const long n = (long)im.h * im.w;
+  const double *E0 = energy[blockIdx.x];
+  double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+  long long orig = 0;
+  for (int f = 0; f < 8; f++) {
+    const double *E = E0 + (long)f * n;
+    double mx = -1e308;
+    for (long i = threadIdx.x; i < n; i += 1024) mx = fmax(mx, E[i]);
+    mx = block_reduce(mx, MaxOp(), shd);
+    if (f == 0) {
+      long long c = 0;
+      const double lvl = mx * 0.4;
+      for (long i = threadIdx.x; i < n; i += 1024) c += E[i] > lvl;
+      orig = block_reduce(c, SumOp(), shl);
+      continue;
+    }
+    // e / max, its min / max, Otsu threshold, count above
+    double a = 1e308, b = -1e308;
+    for (long i = threadIdx.x; i < n; i += 1024) {
+      const double v = E[i] / mx;
+      if (moments_ok(v)) { a = fmin(a, v); b = fmax(b, v); }
+    }
+    a = block_reduce(a, MinOp(), shd);
+    b = block_reduce(b, MaxOp(), shd);
+    const double thr = block_otsu(n, a, b, [&](long i) { return E[i] / mx; }, hist);
+    long long c = 0;
+    for (long i = threadIdx.x; i < n; i += 1024) c += (E[i] / mx) > thr;
+    c = block_reduce(c, SumOp(), shl);
+    if (threadIdx.x == 0) out[f - 1] = (double)c / (double)orig;
+  }
+}
+
+void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
+                  int maxw, hipStream_t s) {
+  const int tiles = ((maxw + GB_TC - 1) / GB_TC) * ((maxh + GB_TR - 1) / GB_TR);
+  k_gabor_conv<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, kernels, energy);
+  k_gabor_post<<<nimg, 1024, 0, s>>>(imgs, energy, o);
+}
+
+}  // namespace wnd

@@ -1,0 +1,245 @@
+// Moment / basis contractions: Zernike coefficients, Chebyshev-Fourier
+// coefficients (with the reference's area-weighted downsampling for planes
+// above 300x300 pixels).  FP64.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace wnd {
+
+// ===================================================================== Zernike
+// mb_zernike2D (src/textures/zernike/zernike.cpp:207-341), order 15, radius
+// min(W,H), unit circle centred on the intensity centroid.  Per-pixel values
+// (radial powers, angle-addition cos/sin, the H1/H2/H3 radial recurrence)
+// use the reference's operations in the reference's order; the sums over
+// pixels are a fixed-order tree.
+constexpr int ZL = 15;
+constexpr int ZPAIRS = 72;  // (n, m), 0 <= m <= n <= 15, n - m even
+__device__ __forceinline__ int zoff(int n) {  // pairs before order n
+  int s = 0;
+  for (int k = 0; k < n; k++) s += k / 2 + 1;
+  return s;
+}
+
+// header per image in partial[]: [0] m10/m00, [1] m01/m00, [2] sum
+__global__ void __launch_bounds__(1024) k_zern_moments(const Img *imgs, double *partial, int nchunk) {
+  __shared__ double shd[16];
+  const Img im = imgs[blockIdx.x];
+  const long n = (long)im.h * im.w;
+  double s = 0, m10 = 0, m01 = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const int y = i / im.w, x = i - (long)y * im.w;
+    const double v = im.p[i];
+    s += v;
+    m10 += (x + 1) * v;
+    m01 += (y + 1) * v;
+  }
+  s = block_reduce(s, SumOp(), shd);
+  m10 = block_reduce(m10, SumOp(), shd);
+  m01 = block_reduce(m01, SumOp(), shd);
+  if (threadIdx.x == 0) {
+    double *hdr = partial + (long)blockIdx.x * (4 + (long)nchunk * 2 * ZPAIRS);
+    hdr[0] = m10 / s;
+    hdr[1] = m01 / s;
+    hdr[2] = s;
+  }
+}
+
+// 256 threads = 16 pixel streams x 16 orders n; block = one pixel chunk
+__global__ void __launch_bounds__(256) k_zern_accum(const Img *imgs, double *partial, int nchunk, const double *htab) {
+  __shared__ double red[16][2 * ZPAIRS];
+  const Img im = imgs[blockIdx.y];
+  const long P = (long)im.h * im.w;
+  const long csz = (P + nchunk - 1) / nchunk;
+  const long p0 = blockIdx.x * csz, p1 = (p0 + csz < P) ? p0 + csz : P;
+  double *hdr = partial + (long)blockIdx.y * (4 + (long)nchunk * 2 * ZPAIRS);
+  const double cxm = hdr[0], cym = hdr[1], sum = hdr[2];
+  const int t = threadIdx.x, n = t & 15, stream = t >> 4;
+  const double rad = (double)(im.w < im.h ? im.w : im.h);
+  const double PI = 3.14159265358979323846264338328;
+  const double *H1 = htab, *H2 = htab + 32 * 32, *H3 = htab + 2 * 32 * 32;
+  double AR[8], AI[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) AR[k] = AI[k] = 0.0;
+  for (long p = p0 + stream; p < p1; p += 16) {
+    const int j = p / im.w, i = p - (long)j * im.w;  // i: column, j: row
+    const double x = (i + 1 - cxm) / rad;
+    const double y = (j + 1 - cym) / rad;
+    const double r2 = x * x + y * y;
+    const double r = sqrt(r2);
+    if (r < kDblEps || r > 1.0) continue;
+    double R[ZL + 1];
+    R[0] = 1;
+    for (int k = 1; k <= n; k++) R[k] = r * R[k - 1];
+    double COST[ZL + 1], SINT[ZL + 1];
+    const double a = COST[0] = x / r;
+    const double b = SINT[0] = y / r;
+    for (int m = 1; m <= n; m++) {
+      COST[m] = a * COST[m - 1] - b * SINT[m - 1];
+      SINT[m] = a * SINT[m - 1] + b * COST[m - 1];
+    }
+    const double f = im.p[p] / sum;
+    const double const_t = (n + 1) * f / PI;
+    const double Rn = R[n];
+    const double Rnm2 = n >= 2 ? R[n - 2] : 0.0;
+    double Rnmp2 = 0, Rnmp4 = 0, Rnm;
+    int slot = n / 2;  // m = n sits at the last slot of this order
+    for (int m = n; m >= 0; m -= 2, slot--) {
+      if (m == n) { Rnm = Rn; Rnmp4 = Rn; }
+      else if (m == n - 2) { const double Rnnm2 = n * Rn - (n - 1) * Rnm2; Rnm = Rnnm2; Rnmp2 = Rnnm2; }
+      else {
+        Rnm = H1[n * 32 + m] * Rnmp4 + (H2[n * 32 + m] + (H3[n * 32 + m] / r2)) * Rnmp2;
+        Rnmp4 = Rnmp2;
+        Rnmp2 = Rnm;
+      }
+      AR[slot] += const_t * Rnm * COST[m];
+      AI[slot] -= const_t * Rnm * SINT[m];
+    }
+  }
+  const int base = zoff(n), cnt = n / 2 + 1;
+  for (int k = 0; k < cnt; k++) { red[stream][2 * (base + k)] = AR[k]; red[stream][2 * (base + k) + 1] = AI[k]; }
+  __syncthreads();
+  double *dst = hdr + 4 + (long)blockIdx.x * 2 * ZPAIRS;
+  for (int e = t; e < 2 * ZPAIRS; e += 256) {
+    double s = 0;
+    for (int q = 0; q < 16; q++) s += red[q][e];
+    dst[e] = s;
+  }
+}
+
+__global__ void k_zern_final(const Img *imgs, const double *partial, int nchunk, OutRef o) {
+  const int e = threadIdx.x;
+  if (e >= ZPAIRS) return;
+  const double *hdr = partial + (long)blockIdx.x * (4 + (long)nchunk * 2 * ZPAIRS);
+  double ar = 0, ai = 0;
+  for (int c = 0; c < nchunk; c++) {
+    ar += hdr[4 + (long)c * 2 * ZPAIRS + 2 * e];
+    ai += hdr[4 + (long)c * 2 * ZPAIRS + 2 * e + 1];
+  }
+  ar *= ar;
+  ai *= ai;
+  o.out[(long)blockIdx.x * o.stride + o.off + e] = fabs(sqrt(ar + ai));
+}
+
+void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
+                    hipStream_t s) {
+  k_zern_moments<<<nimg, 1024, 0, s>>>(imgs, partial, nchunk);
+  k_zern_accum<<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
+  k_zern_final<<<nimg, 128, 0, s>>>(imgs, partial, nchunk, o);
+}
+
+// ============================================================ Chebyshev-Fourier
+// ChebyshevFourier2D (src/transforms/ChebyshevFourier.cpp:59-160), N = 11,
+// NN = 23: sum_r[im][a] += (Ftrm*cos(mf*phi) * T_a(2r-1)) * I  over in-disk
+// pixels; the geometry tables are host-computed with the reference's libm
+// calls.  Block = 128-pixel chunk staged in LDS; every thread owns <= 5 of
+// the 2*NN*NN outputs and walks the chunk in the reference's pixel order.
+constexpr int CF_CHUNK = 128;
+constexpr int CF_NN = 23;
+
+__global__ void __launch_bounds__(256) k_cf_partial(const CfDesc *cds) {
+  __shared__ double tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN], iv[CF_CHUNK];
+  const CfDesc C = cds[blockIdx.y];
+  const int nchunk = (C.np + CF_CHUNK - 1) / CF_CHUNK;
+  if ((int)blockIdx.x >= nchunk) return;
+  const int NN = C.NN, NN2 = NN * NN;
+  const int p0 = blockIdx.x * CF_CHUNK;
+  const int np = (C.np - p0 < CF_CHUNK) ? C.np - p0 : CF_CHUNK;
+  for (int e = threadIdx.x; e < np * NN; e += 256) {
+    const int p = e / NN, a = e - p * NN;
+    tn[p][a] = C.tn[(long)(p0 + p) * NN + a];
+    fc[p][a] = C.fc[(long)(p0 + p) * NN + a];
+    fs[p][a] = C.fs[(long)(p0 + p) * NN + a];
+  }
+  for (int p = threadIdx.x; p < np; p += 256) iv[p] = C.img[C.kk[p0 + p]];
+  __syncthreads();
+  double *dst = C.partial + (long)blockIdx.x * 2 * NN2;
+  for (int o = threadIdx.x; o < 2 * NN2; o += 256) {
+    const int part = o / NN2, k = o - part * NN2;
+    const int im = k / NN, a = k - im * NN;
+    double acc = 0;
+    if (part == 0)
+      for (int p = 0; p < np; p++) acc += (fc[p][im] * tn[p][a]) * iv[p];
+    else
+      for (int p = 0; p < np; p++) acc += (fs[p][im] * tn[p][a]) * iv[p];
+    dst[o] = acc;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) {
+  __shared__ double shd[16];
+  __shared__ double coeff[CF_NN * CF_NN];
+  __shared__ int bins[32];
+  const CfDesc C = cds[blockIdx.x];
+  const int NN = C.NN, NN2 = NN * NN;
+  const int nchunk = (C.np + CF_CHUNK - 1) / CF_CHUNK;
+  double mn = kInf, mx = -kInf;
+  for (int k = threadIdx.x; k < NN2; k += blockDim.x) {
+    double sr = 0, si = 0;
+    for (int c = 0; c < nchunk; c++) {
+      sr += C.partial[(long)c * 2 * NN2 + k];
+      si += C.partial[(long)c * 2 * NN2 + NN2 + k];
+    }
+    const double v = sqrt(sr * sr + si * si);
+    coeff[k] = v;
+    mn = fmin(mn, v);
+    mx = fmax(mx, v);
+  }
+  mn = block_reduce(mn, MinOp(), shd);
+  mx = block_reduce(mx, MaxOp(), shd);
+  if (threadIdx.x < 32) bins[threadIdx.x] = 0;
+  __syncthreads();
+  if (mx - mn > 0) {
+    const double hs = (double)32 / (double)(mx - mn);
+    for (int k = threadIdx.x; k < NN2; k += blockDim.x) atomicAdd(&bins[hist_bin(coeff[k], mn, hs, 32)], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < 32) o.out[(long)blockIdx.x * o.stride + o.off + threadIdx.x] = bins[threadIdx.x];
+}
+
+void launch_cf(const CfDesc *cd, int nimg, int maxchunk, OutRef o, hipStream_t s) {
+  k_cf_partial<<<dim3(maxchunk, nimg), 256, 0, s>>>(cd);
+  k_cf_final<<<nimg, 1024, 0, s>>>(cd, o);
+}
+
+// ================================================================== Downsample
+// ImageMatrix::Downsample (cmatrix.cpp:579-720), grayscale: each output is
+// the reference's left-fraction + whole pixels + right-fraction sum / dx, with
+// the step positions xs[k] (ys[k]) precomputed by the same sequential
+// x += dx accumulation.
+__device__ __forceinline__ double ds_line(const double *v, long stride, unsigned len, double x, double dx) {
+  double sum_i = 0;
+  unsigned a = (unsigned)(floor(x));
+  double frac = ceil(x) - x;
+  if (frac > 0 && a < len) sum_i += v[(long)a * stride] * frac;
+  for (a = (unsigned)(ceil(x)); a < floor(x + dx); a++)
+    if (a < len) sum_i += v[(long)a * stride];
+  frac = x + dx - floor(x + dx);
+  if (frac > 0 && a < len) sum_i += v[(long)a * stride] * frac;
+  return sum_i;
+}
+
+__global__ void k_ds_x(const DsDesc *dds) {
+  const DsDesc D = dds[blockIdx.y];
+  const long n = (long)D.oh * D.cx;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int y = t / D.cx, k = t - (long)y * D.cx;
+  D.tmp[(long)y * D.ow + k] = ds_line(D.in + (long)y * D.ow, 1, D.ow, D.xs[k], D.dx) / (D.dx);
+}
+
+__global__ void k_ds_y(const DsDesc *dds) {
+  const DsDesc D = dds[blockIdx.y];
+  const long n = (long)D.nh * D.nw;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int y = t / D.nw, x = t - (long)y * D.nw;
+  D.out[(long)y * D.nw + x] = ds_line(D.tmp + x, D.ow, D.oh, D.ys[y], D.dy) / D.dy;
+}
+
+void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s) {
+  const long n1 = (long)maxoh * maxow, n2 = (long)maxnh * maxnw;
+  k_ds_x<<<dim3((n1 + 255) / 256, nimg), 256, 0, s>>>(dd);
+  k_ds_y<<<dim3((n2 + 255) / 256, nimg), 256, 0, s>>>(dd);
+}
+
+}  // namespace wnd

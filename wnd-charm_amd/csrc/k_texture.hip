@@ -1,0 +1,948 @@
+// Texture feature kernels: Haralick (co-occurrence statistics f1..f14),
+// Tamura, Comb moments and Radon.  FP64.
+//
+// Haralick f14 needs the eigenvalue at index 2 of the real Schur form that
+// the reference's Eigen 3.1.2 EigenSolver produces (CVIPtexture.cpp:993-1023),
+// so Q, the Householder Hessenberg reduction and the Francis double-shift QR
+// below perform the same floating-point operations in the same order as the
+// CPU oracle's restatement of that algorithm: every reduction that feeds a
+// decision is a per-thread sequential loop, parallelism is across the
+// independent rows / columns a reflector touches.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace wnd {
+
+// ===================================================================== Haralick
+// quantisation (haralick.cpp:29-37) and tone presence
+__global__ void k_har_quant(const Img *imgs, const Stats *st, const HarDesc *hd) {
+  const Img im = imgs[blockIdx.y];
+  const HarDesc H = hd[blockIdx.y];
+  const Stats S = st[blockIdx.y];
+  const long n = (long)im.h * im.w;
+  const double scale255 = (255.0 / (S.mx - S.mn));
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double t = (im.p[i] - S.mn) * scale255;
+    const int g = (t >= 0 && t < 256) ? (int)t : (t >= 256 ? 255 : 0);
+    H.gray[i] = (uint8_t)g;
+    H.lut[g] = 1;  // benign race: every writer stores 1
+  }
+}
+
+// tone LUT (CVIPtexture.cpp:400-414): rank of each present gray level
+__global__ void k_har_lut(const HarDesc *hd) {
+  const HarDesc H = hd[blockIdx.x];
+  if (threadIdx.x == 0) {
+    int it = 0;
+    for (int g = 0; g < 256; g++) H.lut[g] = H.lut[g] ? it++ : -1;
+    *H.ng = it;
+  }
+}
+
+// co-occurrence counts (CoOcMat_Angle_*, CVIPtexture.cpp:455-614) in LDS as the
+// upper triangle of the symmetric matrix, then P = count / total
+__device__ __forceinline__ int tri(int i, int j) { return i * (2 * 256 - i + 1) / 2 + (j - i); }
+
+__global__ void __launch_bounds__(1024) k_har_cooc(const Img *imgs, const HarDesc *hd) {
+  extern __shared__ uint32_t cnt[];  // 256*257/2
+  __shared__ int lut[256];
+  __shared__ int total;
+  const int angle = blockIdx.x;
+  const Img im = imgs[blockIdx.y];
+  const HarDesc H = hd[blockIdx.y];
+  const int ng = *H.ng;
+  for (int e = threadIdx.x; e < 256 * 257 / 2; e += blockDim.x) cnt[e] = 0;
+  if (threadIdx.x < 256) lut[threadIdx.x] = H.lut[threadIdx.x];
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
+  const int rows = im.h, cols = im.w;
+  const long n = (long)rows * cols;
+  int local = 0;
+  for (long i = threadIdx.x; i < n; i += blockDim.x) {
+    const int row = i / cols, col = i - (long)row * cols;
+    const int g = H.gray[i];
+    if (!g) continue;
+    int r2 = row, c2 = col;
+    if (angle == 0) { c2 = col + 1; if (!(c2 < cols)) continue; }
+    else if (angle == 1) { r2 = row + 1; c2 = col - 1; if (!(r2 < rows && c2 >= 0)) continue; }
+    else if (angle == 2) { r2 = row + 1; if (!(r2 < rows)) continue; }
+    else { r2 = row + 1; c2 = col + 1; if (!(r2 < rows && c2 < cols)) continue; }
+    const int g2 = H.gray[(long)r2 * cols + c2];
+    if (!g2) continue;
+    int x = lut[g], y = lut[g2];
+    if (x == y) atomicAdd(&cnt[tri(x, x)], 2u);
+    else atomicAdd(&cnt[x < y ? tri(x, y) : tri(y, x)], 1u);
+    local += 2;
+  }
+  atomicAdd(&total, local);
+  __syncthreads();
+  const int count = total;
+  double *P = H.P + (long)angle * ng * ng;
+  for (int e = threadIdx.x; e < ng * ng; e += blockDim.x) {
+    const int i = e / ng, j = e - i * ng;
+    const uint32_t c = cnt[i <= j ? tri(i, j) : tri(j, i)];
+    P[e] = count == 0 ? 0.0 : (double)c / count;
+  }
+  if (threadIdx.x == 0) H.count[angle] = count;
+}
+
+// f1..f13 (CVIPtexture.cpp:631-958) and Q (_Q_calc_eigen, 961-987) per angle
+__global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
+  __shared__ double px[256], py[256], pxpy_s[512], pxpy_d[256];
+  __shared__ double shd[4];
+  const int angle = blockIdx.x;
+  const HarDesc H = hd[blockIdx.y];
+  const int ng = *H.ng;
+  const double *P = H.P + (long)angle * ng * ng;
+  const int t = threadIdx.x;
+  const double LOG2 = log10(2.0), EPS = 0.000000001;
+  // exact per-row / per-column marginals (sequential, as the reference loops)
+  if (t < ng) {
+    double a = 0, b = 0;
+    for (int j = 0; j < ng; j++) a += P[t * ng + j];
+    for (int i = 0; i < ng; i++) b += P[i * ng + t];
+    px[t] = a;
+    py[t] = b;
+  }
+  // Pxpy[i+j] (anti-diagonals, raster order) and Pxpy[|i-j|] (f2, f10, f11)
+  for (int k = t; k < 2 * ng - 1; k += blockDim.x) {
+    double s = 0;
+    const int i0 = k - (ng - 1) > 0 ? k - (ng - 1) : 0, i1 = k < ng - 1 ? k : ng - 1;
+    for (int i = i0; i <= i1; i++) s += P[i * ng + (k - i)];
+    pxpy_s[k] = s;
+  }
+  if (t < ng) {
+    double s = 0;
+    for (int i = 0; i < ng; i++) {
+      if (i - t >= 0) s += P[i * ng + i - t];
+      if (t > 0 && i + t < ng) s += P[i * ng + i + t];
+    }
+    pxpy_d[t] = s;
+  }
+  __syncthreads();
+  // raster double sums (parallel)
+  double f1 = 0, tmp3 = 0, mean4 = 0, idm = 0, ent = 0, hxy1 = 0, hxy2 = 0;
+  for (int e = t; e < ng * ng; e += blockDim.x) {
+    const int i = e / ng, j = e - i * ng;
+    const double p = P[e];
+    f1 += p * p;
+    tmp3 += i * j * p;
+    mean4 += i * p;
+    idm += p / (1 + (i - j) * (i - j));
+    ent += p * log10(p + EPS) / LOG2;
+    hxy1 -= p * log10(px[i] * py[j] + EPS) / LOG2;
+    hxy2 -= px[i] * py[j] * log10(px[i] * py[j] + EPS) / LOG2;
+  }
+  f1 = block_reduce(f1, SumOp(), shd);
+  tmp3 = block_reduce(tmp3, SumOp(), shd);
+  mean4 = block_reduce(mean4, SumOp(), shd);
+  idm = block_reduce(idm, SumOp(), shd);
+  ent = block_reduce(ent, SumOp(), shd);
+  hxy1 = block_reduce(hxy1, SumOp(), shd);
+  hxy2 = block_reduce(hxy2, SumOp(), shd);
+  double var4 = 0;
+  for (int e = t; e < ng * ng; e += blockDim.x) {
+    const int i = e / ng;
+    var4 += (i - mean4) * (i - mean4) * P[e];
+  }
+  var4 = block_reduce(var4, SumOp(), shd);
+  if (t == 0) {
+    double *F = H.feat + angle * 14;
+    F[0] = f1;
+    double bigsum = 0;
+    for (int n = 0; n < ng; n++) bigsum += n * n * pxpy_d[n];
+    F[1] = bigsum;
+    double meanx = 0, sum_sqrx = 0;
+    for (int i = 0; i < ng; i++) { meanx += px[i] * i; sum_sqrx += px[i] * i * i; }
+    const double sd = sqrt(sum_sqrx - (meanx * meanx));
+    F[2] = (sd * sd == 0) ? 1.0 : (tmp3 - meanx * meanx) / (sd * sd);
+    F[3] = var4;
+    F[4] = idm;
+    double savg = 0;
+    for (int i = 0; i <= 2 * ng - 2; i++) savg += i * pxpy_s[i];
+    F[5] = savg;
+    double se = 0;
+    for (int i = 2; i <= 2 * ng; i++) {
+      const double v = (i - 2 <= 2 * ng - 2) ? pxpy_s[i - 2] : 0.0;
+      se -= v * log10(v + EPS) / LOG2;
+    }
+    F[7] = se;
+    double sv = 0;
+    for (int i = 0; i <= 2 * ng - 2; i++) sv += (i - se) * (i - se) * pxpy_s[i];
+    F[6] = sv;
+    F[8] = -ent;
+    double s = 0, ss = 0, de = 0;
+    for (int i = 0; i < ng; i++) { s += i * pxpy_d[i]; ss += i * i * pxpy_d[i]; }
+    F[9] = ss - s * s;
+    for (int i = 0; i < ng; i++) de += pxpy_d[i] * log10(pxpy_d[i] + EPS) / LOG2;
+    F[10] = -de;
+    double hx = 0, hy = 0;
+    for (int i = 0; i < ng; i++) {
+      hx -= px[i] * log10(px[i] + EPS) / LOG2;
+      hy -= py[i] * log10(py[i] + EPS) / LOG2;
+    }
+    const double hxy = -ent;
+    const double mxh = hx > hy ? hx : hy;
+    F[11] = (mxh == 0) ? 1.0 : (hxy - hxy1) / mxh;
+    F[12] = sqrt(fabs(1 - exp(-2.0 * (hxy2 - hxy))));
+  }
+  // Q(i,j) = sum_k (P[i][k]*P[j][k]) / (px[i]*py[k] + 1e-13), sequential in k
+  double *Q = H.Q + (long)angle * ng * ng;
+  for (int e = t; e < ng * ng; e += blockDim.x) {
+    const int i = e / ng, j = e - i * ng;
+    double s = 0;
+    const double pxi = px[i];
+    for (int k = 0; k < ng; k++) s += (P[i * ng + k] * P[j * ng + k]) / (pxi * py[k] + 0.0000000000001);
+    Q[e] = s;
+  }
+}
+
+// --------------------------------------------------------- eigenvalues of Q
+// Householder vector (Eigen makeHouseholder) of x[0..m-1], one thread.
+__device__ void make_hh(const double *x, int m, double *ess, double *tau, double *beta) {
+  double tail = 0;
+  for (int i = 1; i < m; i++) tail += x[i] * x[i];
+  const double c0 = x[0];
+  if (tail == 0.0) {
+    *tau = 0; *beta = c0;
+    for (int i = 1; i < m; i++) ess[i - 1] = 0;
+    return;
+  }
+  double b = sqrt(c0 * c0 + tail);
+  if (c0 >= 0) b = -b;
+  for (int i = 1; i < m; i++) ess[i - 1] = x[i] / (c0 - b);
+  *tau = (b - c0) / b;
+  *beta = b;
+}
+
+// One block per (angle, image): Hessenberg reduction of Q (n x n, row-major,
+// in place) followed by the Francis QR on the active windows.  Only the
+// values of index 0..2 of the Schur diagonal are consumed (f14 = sqrt of
+// index 2 when that eigenvalue is real).
+__global__ void __launch_bounds__(256) k_har_eig(const HarDesc *hd) {
+  __shared__ double ess[256];
+  __shared__ double sc[8];
+  __shared__ int si[8];
+  const int angle = blockIdx.x;
+  const HarDesc H = hd[blockIdx.y];
+  const int n = *H.ng;
+  double *A = H.Q + (long)angle * n * n;
+  const int t = threadIdx.x;
+#define AT(r, c) A[(long)(r) * n + (c)]
+  // ---- Hessenberg (HessenbergDecomposition::_compute)
+  for (int i = 0; i < n - 1; ++i) {
+    const int rem = n - i - 1;
+    if (t == 0) {
+      // column i below the diagonal -> reflector
+      double tail = 0;
+      for (int k = 1; k < rem; k++) { const double v = AT(i + 1 + k, i); tail += v * v; }
+      const double c0 = AT(i + 1, i);
+      double tau, beta;
+      if (tail == 0.0) {
+        tau = 0; beta = c0;
+        for (int k = 1; k < rem; k++) ess[k - 1] = 0;
+      } else {
+        double b = sqrt(c0 * c0 + tail);
+        if (c0 >= 0) b = -b;
+        for (int k = 1; k < rem; k++) ess[k - 1] = AT(i + 1 + k, i) / (c0 - b);
+        tau = (b - c0) / b;
+        beta = b;
+      }
+      AT(i + 1, i) = beta;
+      for (int k = 1; k < rem; k++) AT(i + 1 + k, i) = 0.0;  // matrixH() keeps only the Hessenberg part
+      sc[0] = tau;
+    }
+    __syncthreads();
+    const double tau = sc[0];
+    // A = H A on rows i+1..n-1, columns i+1..n-1
+    for (int c = i + 1 + t; c < n; c += blockDim.x) {
+      if (rem == 1) { AT(i + 1, c) *= (1 - tau); continue; }
+      double tt = 0;
+      for (int r = 1; r < rem; r++) tt += ess[r - 1] * AT(i + 1 + r, c);
+      tt += AT(i + 1, c);
+      AT(i + 1, c) -= tau * tt;
+      for (int r = 1; r < rem; r++) AT(i + 1 + r, c) -= (tau * ess[r - 1]) * tt;
+    }
+    __syncthreads();
+    // A = A H on all rows, columns i+1..n-1
+    for (int r = t; r < n; r += blockDim.x) {
+      if (rem == 1) { AT(r, i + 1) *= (1 - tau); continue; }
+      double tt = 0;
+      for (int c = 1; c < rem; c++) tt += AT(r, i + 1 + c) * ess[c - 1];
+      tt += AT(r, i + 1);
+      AT(r, i + 1) -= tau * tt;
+      for (int c = 1; c < rem; c++) AT(r, i + 1 + c) -= (tau * tt) * ess[c - 1];
+    }
+    __syncthreads();
+  }
+  // ---- real Schur form (RealSchur::compute without U); left updates are
+  // restricted to columns <= iu (entries right of the active window are
+  // never read again), right updates cover rows 0..min(iu,k+3) as in Eigen.
+  __shared__ double normv;
+  {
+    double s = 0;
+    for (int e = t; e < n * n; e += blockDim.x) {
+      const int r = e / n, c = e - r * n;
+      if (c >= (r - 1 > 0 ? r - 1 : 0)) s += fabs(A[e]);
+    }
+    s = block_reduce(s, SumOp(), sc);
+    if (t == 0) normv = s;
+  }
+  __syncthreads();
+  const double norm = normv;
+  const double eps = kDblEps;
+  // decision state lives in thread 0; the block follows through si / sc
+  __shared__ int s_iu;
+  int iu = n - 1, iter = 0, totalIter = 0;
+  double exshift = 0;
+  if (t == 0) s_iu = n - 1;
+  __syncthreads();
+  while (norm != 0) {
+    __syncthreads();  // every thread has consumed the previous decision
+    if (t == 0) {
+      int action = 3;
+      if (iu >= 0) {
+        int il = iu;
+        while (il > 0) {
+          double s = fabs(AT(il - 1, il - 1)) + fabs(AT(il, il));
+          if (s == 0.0) s = norm;
+          if (fabs(AT(il, il - 1)) < eps * s) break;
+          il--;
+        }
+        if (il == iu) {  // one root
+          AT(iu, iu) = AT(iu, iu) + exshift;
+          if (iu > 0) AT(iu, iu - 1) = 0;
+          iu--; iter = 0;
+          action = 0;
+        } else if (il == iu - 1) {  // two roots (RealSchur::splitOffTwoRows)
+          action = 1;
+          const double p = 0.5 * (AT(iu - 1, iu - 1) - AT(iu, iu));
+          const double q = p * p + AT(iu, iu - 1) * AT(iu - 1, iu);
+          AT(iu, iu) += exshift;
+          AT(iu - 1, iu - 1) += exshift;
+          si[3] = q >= 0;
+          if (q >= 0) {
+            const double z = sqrt(fabs(q));
+            const double pp = (p >= 0) ? p + z : p - z, qq = AT(iu, iu - 1);
+            double c, s;  // JacobiRotation::makeGivens, real case
+            if (qq == 0) { c = pp < 0 ? -1 : 1; s = 0; }
+            else if (pp == 0) { c = 0; s = qq < 0 ? 1 : -1; }
+            else if (fabs(pp) > fabs(qq)) {
+              const double tt = qq / pp;
+              double u = sqrt(1 + tt * tt);
+              if (pp < 0) u = -u;
+              c = 1 / u; s = -tt * c;
+            } else {
+              const double tt = pp / qq;
+              double u = sqrt(1 + tt * tt);
+              if (qq < 0) u = -u;
+              s = -1 / u; c = -tt * s;
+            }
+            sc[1] = c; sc[2] = s;
+            for (int j = iu - 1; j <= iu; j++) {  // rot.adjoint() on the left, columns <= iu
+              const double xv = AT(iu - 1, j), yv = AT(iu, j);
+              AT(iu - 1, j) = c * xv - s * yv;
+              AT(iu, j) = s * xv + c * yv;
+            }
+          }
+          si[1] = iu;  // rows 0..iu get the right rotation
+          iu -= 2; iter = 0;
+        } else {  // Francis QR step (RealSchur::computeShift / initFrancisQRStep)
+          double sh0 = AT(iu, iu), sh1 = AT(iu - 1, iu - 1), sh2 = AT(iu, iu - 1) * AT(iu - 1, iu);
+          double shift_amt = 0;
+          int shift_kind = 0;
+          if (iter == 10) {
+            exshift += sh0;
+            shift_kind = 1; shift_amt = sh0;
+            const double s = fabs(AT(iu, iu - 1)) + fabs(AT(iu - 1, iu - 2));
+            sh0 = 0.75 * s; sh1 = 0.75 * s; sh2 = -0.4375 * s * s;
+          }
+          if (iter == 30) {
+            double s = (sh1 - sh0) / 2.0;
+            s = s * s + sh2;
+            if (s > 0) {
+              s = sqrt(s);
+              if (sh1 < sh0) s = -s;
+              s = s + (sh1 - sh0) / 2.0;
+              s = sh0 - sh2 / s;
+              exshift += s;
+              shift_kind = 2; shift_amt = s;
+              sh0 = sh1 = sh2 = 0.964;
+            }
+          }
+          iter++; totalIter++;
+          if (totalIter > 40 * n) action = 3;
+          else {
+            action = 2;
+            si[4] = shift_kind;
+            sc[3] = shift_amt;
+            sc[4] = sh0; sc[5] = sh1; sc[6] = sh2;
+            si[1] = il;
+            si[5] = iu;
+          }
+        }
+      }
+      si[0] = action;
+    }
+    __syncthreads();
+    const int action = si[0];
+    if (action == 3) break;
+    if (action == 0) continue;
+    if (action == 1) {
+      const int iu2 = si[1];
+      if (si[3]) {
+        const double c = sc[1], s = sc[2];
+        for (int r = t; r <= iu2; r += blockDim.x) {  // rot on the right, rows 0..iu
+          const double xv = AT(r, iu2 - 1), yv = AT(r, iu2);
+          AT(r, iu2 - 1) = c * xv - s * yv;
+          AT(r, iu2) = s * xv + c * yv;
+        }
+        __syncthreads();
+        if (t == 0) AT(iu2, iu2 - 1) = 0;
+      }
+      if (t == 0 && iu2 > 1) AT(iu2 - 1, iu2 - 2) = 0;
+      __syncthreads();
+      continue;
+    }
+    // ---- Francis QR step on the window [il, iuq]
+    const int il = si[1], iuq = si[5];
+    if (si[4]) {
+      const double amt = sc[3];
+      for (int i = t; i <= iuq; i += blockDim.x) AT(i, i) -= amt;
+      __syncthreads();
+    }
+    if (t == 0) {
+      const double sh0 = sc[4], sh1 = sc[5], sh2 = sc[6];
+      int im;
+      double v0 = 0, v1 = 0, v2 = 0;
+      for (im = iuq - 2; im >= il; --im) {
+        const double Tmm = AT(im, im);
+        const double r = sh0 - Tmm, s = sh1 - Tmm;
+        v0 = (r * s - sh2) / AT(im + 1, im) + AT(im, im + 1);
+        v1 = AT(im + 1, im + 1) - Tmm - r - s;
+        v2 = AT(im + 2, im + 1);
+        if (im == il) break;
+        const double lhs = AT(im, im - 1) * (fabs(v1) + fabs(v2));
+        const double rhs = v0 * (fabs(AT(im - 1, im - 1)) + fabs(Tmm) + fabs(AT(im + 1, im + 1)));
+        if (fabs(lhs) < eps * rhs) break;
+      }
+      si[2] = im;
+      sc[0] = v0; sc[1] = v1; sc[2] = v2;
+    }
+    __syncthreads();
+    const int im = si[2];
+    const double fv0 = sc[0], fv1 = sc[1], fv2 = sc[2];
+    __syncthreads();
+    for (int k = im; k <= iuq - 2; ++k) {
+      if (t == 0) {
+        const bool first = (k == im);
+        double x[3];
+        if (first) { x[0] = fv0; x[1] = fv1; x[2] = fv2; }
+        else { x[0] = AT(k, k - 1); x[1] = AT(k + 1, k - 1); x[2] = AT(k + 2, k - 1); }
+        double e2[2], tau, beta;
+        make_hh(x, 3, e2, &tau, &beta);
+        si[6] = beta != 0;
+        if (beta != 0) {
+          if (first && k > il) AT(k, k - 1) = -AT(k, k - 1);
+          else if (!first) AT(k, k - 1) = beta;
+        }
+        sc[3] = tau; ess[0] = e2[0]; ess[1] = e2[1];
+      }
+      __syncthreads();
+      if (si[6]) {
+        const double tau = sc[3], e0 = ess[0], e1 = ess[1];
+        for (int c = k + t; c <= iuq; c += blockDim.x) {
+          double tt = e0 * AT(k + 1, c);
+          tt += e1 * AT(k + 2, c);
+          tt += AT(k, c);
+          AT(k, c) -= tau * tt;
+          AT(k + 1, c) -= (tau * e0) * tt;
+          AT(k + 2, c) -= (tau * e1) * tt;
+        }
+        __syncthreads();
+        const int rlast = (iuq < k + 3 ? iuq : k + 3);
+        for (int r = t; r <= rlast; r += blockDim.x) {
+          double tt = AT(r, k + 1) * e0;
+          tt += AT(r, k + 2) * e1;
+          tt += AT(r, k);
+          AT(r, k) -= tau * tt;
+          AT(r, k + 1) -= (tau * tt) * e0;
+          AT(r, k + 2) -= (tau * tt) * e1;
+        }
+      }
+      __syncthreads();
+    }
+    if (t == 0) {
+      double x[2] = {AT(iuq - 1, iuq - 2), AT(iuq, iuq - 2)};
+      double e1[1], tau, beta;
+      make_hh(x, 2, e1, &tau, &beta);
+      si[6] = beta != 0;
+      if (beta != 0) AT(iuq - 1, iuq - 2) = beta;
+      sc[3] = tau; ess[0] = e1[0];
+    }
+    __syncthreads();
+    if (si[6]) {
+      const double tau = sc[3], e0 = ess[0];
+      for (int c = iuq - 1 + t; c <= iuq; c += blockDim.x) {
+        double tt = e0 * AT(iuq, c);
+        tt += AT(iuq - 1, c);
+        AT(iuq - 1, c) -= tau * tt;
+        AT(iuq, c) -= (tau * e0) * tt;
+      }
+      __syncthreads();
+      for (int r = t; r <= iuq; r += blockDim.x) {
+        double tt = AT(r, iuq) * e0;
+        tt += AT(r, iuq - 1);
+        AT(r, iuq - 1) -= tau * tt;
+        AT(r, iuq) -= (tau * tt) * e0;
+      }
+    }
+    __syncthreads();
+    for (int i = im + 2 + t; i <= iuq; i += blockDim.x) {
+      AT(i, i - 2) = 0;
+      if (i > im + 2) AT(i, i - 3) = 0;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // eigenvalue 2 in Schur-diagonal order (EigenSolver::compute, EigenSolver.h:364-381)
+  if (t == 0) {
+    double f = 0.0;
+    if (n >= 3) {
+      int i = 0;
+      double re2 = 0, im2 = 0;
+      while (i < n && i <= 2) {
+        if (i == n - 1 || AT(i + 1, i) == 0) {
+          if (i == 2) { re2 = AT(i, i); im2 = 0; }
+          ++i;
+        } else {
+          const double p = 0.5 * (AT(i, i) - AT(i + 1, i + 1));
+          const double z = sqrt(fabs(p * p + AT(i + 1, i) * AT(i, i + 1)));
+          if (i == 2) { re2 = AT(i + 1, i + 1) + p; im2 = z; }
+          if (i + 1 == 2) { re2 = AT(i + 1, i + 1) + p; im2 = -z; }
+          i += 2;
+        }
+      }
+      f = (im2 == 0) ? sqrt(re2) : 0.0;
+    }
+    H.feat[angle * 14 + 13] = f;
+  }
+#undef AT
+}
+
+// mean and range over the 4 angles, reference output order (haralick.cpp:96-143)
+__global__ void k_har_final(const HarDesc *hd, OutRef o) {
+  const HarDesc H = hd[blockIdx.x];
+  if (threadIdx.x != 0) return;
+  double temp[28];
+  for (int a = 0; a < 14; a++) {
+    double s = 0, mn = kInf, mx = -kInf;
+    for (int g = 0; g < 4; g++) {
+      const double v = H.feat[g * 14 + a];
+      s += v;
+      if (v < mn) mn = v;
+      if (v > mx) mx = v;
+    }
+    temp[a] = s / 4;
+    temp[a + 14] = mx - mn;
+  }
+  const int ord[28] = {0, 14, 1, 15, 2, 16, 9, 23, 10, 24, 8, 22, 11, 25,
+                       4, 18, 13, 27, 12, 26, 5, 19, 7, 21, 6, 20, 3, 17};
+  double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+  for (int a = 0; a < 28; a++) out[a] = temp[ord[a]];
+}
+
+void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef o, int nimg, int maxn,
+                     hipStream_t s) {
+  int blocks = (maxn + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  k_har_quant<<<dim3(blocks, nimg), 256, 0, s>>>(imgs, st, hd);
+  k_har_lut<<<nimg, 64, 0, s>>>(hd);
+  static const bool lds_ok = hipFuncSetAttribute((const void *)k_har_cooc, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  256 * 257 / 2 * sizeof(uint32_t)) == hipSuccess;
+  (void)lds_ok;
+  k_har_cooc<<<dim3(4, nimg), 1024, 256 * 257 / 2 * sizeof(uint32_t), s>>>(imgs, hd);
+  k_har_stats<<<dim3(4, nimg), 256, 0, s>>>(hd);
+  k_har_eig<<<dim3(4, nimg), 256, 0, s>>>(hd);
+  k_har_final<<<nimg, 64, 0, s>>>(hd, o);
+}
+
+// ======================================================================= Tamura
+// tamura.cpp:10-299.  scratch per image (5P doubles): [0, P) normalised
+// plane, [P, 2P) summed-area table (laufendeSumme), [2P, 3P) phi,
+// [3P, 4P) per-pixel sum_r terms, [4P, 4.5P) best coarseness scale (int).
+__global__ void k_tam_norm(const Img *imgs, const Stats *st, double *const *scr) {
+  const Img im = imgs[blockIdx.y];
+  const Stats S = st[blockIdx.y];
+  const long n = (long)im.h * im.w;
+  double *nr = scr[blockIdx.y];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    nr[i] = (im.p[i] - S.mn) / S.mx;
+}
+
+// summed-area table with the reference's exact recurrence
+//   S(y,x) = ((I(y,x) + S(y,x-1)) + S(y-1,x)) - S(y-1,x-1)
+// evaluated as an anti-diagonal wavefront: one wave per image, lane l owns
+// row y0+l of a 64-row strip, the value above comes from lane l-1.
+__global__ void __launch_bounds__(64) k_tam_sat(const Img *imgs, double *const *scr) {
+  extern __shared__ double rowbuf[];
+  const Img im = imgs[blockIdx.x];
+  const int H = im.h, W = im.w;
+  const long P = (long)H * W;
+  const double *I = scr[blockIdx.x];
+  double *S = scr[blockIdx.x] + P;
+  const int l = threadIdx.x;
+  for (int y0 = 0; y0 < H; y0 += 64) {
+    const int y = y0 + l;
+    double cur = 0, up_prev = 0;
+    for (int t = 0; t < W + 63; t++) {
+      const int x = t - l;
+      const double from_left_lane = __shfl_up(cur, 1, 64);
+      double up;
+      if (l == 0) up = (y0 == 0 || x < 0 || x >= W) ? 0.0 : rowbuf[x];
+      else up = (y < 1) ? 0.0 : from_left_lane;
+      if (x >= 0 && x < W && y < H) {
+        const double links = x < 1 ? 0.0 : cur;
+        const double oben = y < 1 ? 0.0 : up;
+        const double obenlinks = (y < 1 || x < 1) ? 0.0 : up_prev;
+        const double v = I[(long)y * W + x] + links + oben - obenlinks;
+        S[(long)y * W + x] = v;
+        cur = v;
+        up_prev = up;
+      }
+      __syncthreads();
+      if (l == 63 && x >= 0 && x < W) rowbuf[x] = cur;
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ double local_mean(long x, long y, long k, const double *S, long dimx, long dimy) {
+  const long k2 = k / 2;
+  long starty = y - k2, startx = x - k2, stopy = y + k2 - 1, stopx = x + k2 - 1;
+  if (starty < 0) starty = 0;
+  if (startx < 0) startx = 0;
+  if (stopx > dimx - 1) stopx = dimx - 1;
+  if (stopy > dimy - 1) stopy = dimy - 1;
+  const double links = (startx - 1 < 0) ? 0 : S[stopy * dimx + startx - 1];
+  const double oben = (starty - 1 < 0) ? 0 : S[(stopy - 1) * dimx + startx];
+  const double obenlinks = ((starty - 1 < 0) || (startx - 1 < 0)) ? 0 : S[(stopy - 1) * dimx + startx - 1];
+  const double unten = S[stopy * dimx + startx];
+  const long counter = (stopy - starty + 1) * (stopx - startx + 1);
+  return (unten - links - oben + obenlinks) / counter;
+}
+
+// coarseness per pixel (best scale), directionality phi and the sums
+struct TamAcc {
+  double sum_r, m2, m4, mean;
+  long long ksum;
+  int kmin, kmax;
+};
+
+__global__ void __launch_bounds__(256) k_tam_pix(const Img *imgs, double *const *scr) {
+  const Img im = imgs[blockIdx.y];
+  const int H = im.h, W = im.w;
+  const long P = (long)H * W;
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  double *N = scr[blockIdx.y];
+  const double *S = N + P;
+  double *phi = N + 2 * P;
+  const int y = i / W, x = i - (long)y * W;
+  // coarseness: argmax over scales of E_h / E_v (tamura.cpp:183-238)
+  double maxE = 0;
+  int maxk = 0;
+  int len = 1;
+  for (int k = 1; k <= 7; ++k) {
+    const int k2 = len;
+    len *= 2;
+    double eh = 0, ev = 0;
+    if (x + k2 < W && x - k2 >= 0)
+      eh = fabs(local_mean(x + k2, y, len, S, W, H) - local_mean(x - k2, y, len, S, W, H));
+    if (y + k2 < H && y - k2 >= 0)
+      ev = fabs(local_mean(x, y + k2, len, S, W, H) - local_mean(x, y - k2, len, S, W, H));
+    if (eh > maxE) { maxE = eh; maxk = k; }
+    if (ev > maxE) { maxE = ev; maxk = k; }
+  }
+  ((int *)(N + 4 * P))[i] = maxk;
+  // directionality filters after the reference's assignments (tamura.cpp:50-57):
+  // H = [-1 2 1; -2 0 2; -1 -2 -1], V = [1 0 0; 0 0 0; -1 0 0], correlation order
+  // (column offset outer, row offset inner), zero weights skipped (exact).
+  auto px = [&](int yy, int xx) -> double { return N[(long)yy * W + xx]; };
+  double dh = 0.0, dv = 0.0;
+  const bool xm = x - 1 >= 0, xp = x + 1 < W, ym = y - 1 >= 0, yp = y + 1 < H;
+  if (xm) {
+    if (ym) dh += -1 * px(y - 1, x - 1);
+    dh += -2 * px(y, x - 1);
+    if (yp) dh += -1 * px(y + 1, x - 1);
+  }
+  if (ym) dh += 2 * px(y - 1, x);
+  if (yp) dh += -2 * px(y + 1, x);
+  if (xp) {
+    if (ym) dh += 1 * px(y - 1, x + 1);
+    dh += 2 * px(y, x + 1);
+    if (yp) dh += -1 * px(y + 1, x + 1);
+  }
+  if (xm) {
+    if (ym) dv += 1 * px(y - 1, x - 1);
+    if (yp) dv += -1 * px(y + 1, x - 1);
+  }
+  double ph = 0.0;
+  if (dh >= 0.0001) ph = atan(dv / dh) + (M_PI / 2.0 + 0.001);
+  phi[i] = ph;
+  N[3 * P + i] = (dh >= 0.0001) ? (dh * dh + dv * dv) + ph * ph : 0.0;
+}
+
+__global__ void __launch_bounds__(1024) k_tam_final(const Img *imgs, double *const *scr, OutRef o) {
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  __shared__ int hk[3];
+  __shared__ int hd[125];
+  const Img im = imgs[blockIdx.x];
+  const int H = im.h, W = im.w;
+  const long P = (long)H * W;
+  const double *N = scr[blockIdx.x];
+  const double *phi = N + 2 * P;
+  const double *srt = N + 3 * P;
+  const int *kb = (const int *)(N + 4 * P);
+  const int t = threadIdx.x;
+  // contrast: Moments4 of the normalised plane (tamura.cpp:10-28)
+  double s = 0;
+  for (long i = t; i < P; i += blockDim.x) s += N[i];
+  s = block_reduce(s, SumOp(), shd);
+  const double mean = s / P;
+  double m2 = 0, m4 = 0;
+  for (long i = t; i < P; i += blockDim.x) { const double d = N[i] - mean, d2 = d * d; m2 += d2; m4 += d2 * d2; }
+  m2 = block_reduce(m2, SumOp(), shd);
+  m4 = block_reduce(m4, SumOp(), shd);
+  // coarseness sums and min/max of best scale
+  long long ks = 0;
+  int kmn = 99, kmx = -1;
+  double pmn = 1e308, pmx = -1e308, sr = 0;
+  for (long i = t; i < P; i += blockDim.x) {
+    const int k = kb[i];
+    ks += k;
+    kmn = min(kmn, k);
+    kmx = max(kmx, k);
+    pmn = fmin(pmn, phi[i]);
+    pmx = fmax(pmx, phi[i]);
+    sr += srt[i];
+  }
+  ks = block_reduce(ks, SumOp(), shl);
+  kmn = (int)block_reduce((long long)kmn, MinOp(), shl);
+  kmx = (int)block_reduce((long long)kmx, MaxOp(), shl);
+  pmn = block_reduce(pmn, MinOp(), shd);
+  pmx = block_reduce(pmx, MaxOp(), shd);
+  sr = block_reduce(sr, SumOp(), shd);
+  if (t < 3) hk[t] = 0;
+  if (t < 125) hd[t] = 0;
+  __syncthreads();
+  const double ksc = hist_scale((double)kmn, (double)kmx, 3), psc = hist_scale(pmn, pmx, 125);
+  for (long i = t; i < P; i += blockDim.x) {
+    atomicAdd(&hk[hist_bin((double)kb[i], (double)kmn, ksc, 3)], 1);
+    atomicAdd(&hd[hist_bin(phi[i], pmn, psc, 125)], 1);
+  }
+  __syncthreads();
+  if (t == 0) {
+    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    // coarseness (tamura.cpp:240-262); the divisor is unsigned arithmetic
+    const unsigned xD = W, yD = H;
+    double sum = (double)ks;
+    if (yD == 32 || xD == 32) sum /= ((xD + 1 - 32) * (yD + 1 - 32));
+    else sum /= ((yD - 32) * (xD - 32));
+    double hm = 0;
+    for (int k = 0; k < 3; k++) if (hk[k] > hm) hm = hk[k];
+    // directionality (tamura.cpp:75-101)
+    double mxh = 0.0;
+    long fmx = 0;
+    for (long a = 0; a < 125; a++) if (hd[a] > mxh) { mxh = hd[a]; fmx = a; }
+    double ssum = 0;
+    for (long a = 0; a < 125; a++) ssum += hd[a] * (double)((a + 1 - fmx) * (a + 1 - fmx));
+    const double dir = fabs(log(ssum / sr + 0.0000001));
+    // contrast
+    double con;
+    const double stdv = P > 2 ? sqrt(m2 / (P - 1)) : 0.0;
+    const double kurt = P > 4 ? (P * m4) / (m2 * m2) : 0.0;
+    if (stdv < kDblEps) con = 0;
+    else con = stdv / pow(kurt / pow(stdv, 4), 0.25);
+    out[0] = hk[0] / hm;
+    out[1] = hk[1] / hm;
+    out[2] = hk[2] / hm;
+    out[3] = con;
+    out[4] = dir;
+    out[5] = sum;
+  }
+}
+
+void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, OutRef o, int nimg, int maxh, int maxw,
+                   hipStream_t s) {
+  const long maxn = (long)maxh * maxw;
+  int blocks = (int)((maxn + 255) / 256);
+  k_tam_norm<<<dim3(blocks > 2048 ? 2048 : blocks, nimg), 256, 0, s>>>(imgs, st, scratch);
+  k_tam_sat<<<nimg, 64, maxw * sizeof(double), s>>>(imgs, scratch);
+  k_tam_pix<<<dim3(blocks, nimg), 256, 0, s>>>(imgs, scratch);
+  k_tam_final<<<nimg, 1024, 0, s>>>(imgs, scratch, o);
+}
+
+// ================================================================= Comb moments
+// CombFirst4Moments2D (CombFirst4Moments.cpp:78-217): 4 combs x 20 lines, one
+// thread per line running the reference's sequential Moments4 over the
+// line's pixels in raster order.  iis[img*20 + j] are the 20 rounded line
+// offsets (host-computed with the reference's FP accumulation), NaN = unused.
+__global__ void __launch_bounds__(128) k_comb(const Img *imgs, const double *iis, OutRef o) {
+  __shared__ double mm[4][4][20];
+  const Img im = imgs[blockIdx.x];
+  const long m = im.h, n = im.w;
+  const long n2 = n / 2, m2 = m / 2;
+  const int t = threadIdx.x;
+  if (t < 80) {
+    const int comb = t / 20, li = t % 20;
+    const double iid = iis[blockIdx.x * 20 + li];
+    double z[4] = {0, 0, 0, 0};
+    if (iid == iid) {
+      const long ii = (long)iid;
+      double mean = 0, M2 = 0, M3 = 0, M4 = 0;
+      size_t cnt = 0;
+      auto add = [&](double x) {
+        if (!moments_ok(x)) return;
+        const size_t n1 = cnt;
+        cnt = cnt + 1;
+        const double delta = x - mean;
+        const double delta_n = delta / cnt;
+        const double delta_n2 = delta_n * delta_n;
+        const double term1 = delta * delta_n * n1;
+        mean = mean + delta_n;
+        M4 = M4 + term1 * delta_n2 * (cnt * cnt - 3 * cnt + 3) + 6 * delta_n2 * M2 - 4 * delta_n * M3;
+        M3 = M3 + term1 * delta_n * (cnt - 2) - 3 * delta_n * M2;
+        M2 = M2 + term1;
+      };
+      if (comb == 0) {         // |I + ii - J| < 1  ->  x = y + ii
+        for (long y = 0; y < m; y++) { const long x = y + ii; if (x >= 0 && x < n) add(im.p[y * n + x]); }
+      } else if (comb == 1) {  // J1 = n - x  ->  x = n - 1 - y - ii
+        for (long y = 0; y < m; y++) { const long x = n - 1 - y - ii; if (x >= 0 && x < n) add(im.p[y * n + x]); }
+      } else if (comb == 2) {  // vertical: x = n2 - 1 - ii
+        const long x = n2 - 1 - ii;
+        if (x >= 0 && x < n) for (long y = 0; y < m; y++) add(im.p[y * n + x]);
+      } else {                 // horizontal: y = m2 - 1 - ii
+        const long y = m2 - 1 - ii;
+        if (y >= 0 && y < m) for (long x = 0; x < n; x++) add(im.p[y * n + x]);
+      }
+      z[0] = mean;
+      z[1] = cnt > 2 ? sqrt(M2 / (cnt - 1)) : 0.0;
+      z[2] = cnt > 3 ? (sqrt((double)cnt) * M3) / pow(M2, 1.5) : 0.0;
+      z[3] = cnt > 4 ? (cnt * M4) / (M2 * M2) : 0.0;
+    }
+    for (int a = 0; a < 4; a++) mm[comb][a][li] = z[a];
+  }
+  __syncthreads();
+  if (t == 0) {
+    double vec[48];
+    int vi = 0;
+    for (int comb = 0; comb < 4; comb++)
+      for (int a = 0; a < 4; a++) {
+        double hmin = kInf, hmax = -kInf;
+        for (int b = 0; b < 20; b++) { const double v = mm[comb][a][b]; if (v > hmax) hmax = v; if (v < hmin) hmin = v; }
+        const double hs = hist_scale(hmin, hmax, 3);
+        double bins[3] = {0, 0, 0};
+        for (int b = 0; b < 20; b++) bins[hist_bin(mm[comb][a][b], hmin, hs, 3)] += 1.0;
+        for (int b = 0; b < 3; b++) vec[vi++] = bins[b];
+      }
+    const int ord[48] = {45, 46, 47, 36, 37, 38, 42, 43, 44, 39, 40, 41, 33, 34, 35, 24,
+                         25, 26, 30, 31, 32, 27, 28, 29, 9, 10, 11, 0, 1, 2, 6, 7,
+                         8, 3, 4, 5, 21, 22, 23, 12, 13, 14, 18, 19, 20, 15, 16, 17};
+    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    for (int a = 0; a < 48; a++) out[a] = vec[ord[a]];
+  }
+}
+
+void launch_comb(const Img *imgs, const double *iis, OutRef o, int nimg, hipStream_t s) {
+  k_comb<<<nimg, 128, 0, s>>>(imgs, iis, o);
+}
+
+// ======================================================================== Radon
+// radon.cpp:14-91 restated as a gather: each projection bin r of angle a sums,
+// in the reference's (n, m) scan order, pixelLow of sub-pixels with
+// floor(rIdx) == r and (pixel - pixelLow) of those with floor(rIdx) == r-1.
+// floor(rIdx(n, m)) is non-increasing in m, so each column n contributes a
+// contiguous m-range found by binary search.  trig[4][2] = {cos, sin} (glibc).
+__device__ __forceinline__ double radon_ridx(int n, int m, int M, int N, double cs, double sn, int rFirst) {
+  const int xO = (N - 1) / 2, yO = (M - 1) / 2;
+  const double xT = (-xO - 0.25) + 0.5 * n;
+  const double yT = (-yO - 0.25) + 0.5 * (2 * M - 1 - m);
+  const double xc = xT * cs - rFirst;
+  const double ys = yT * sn;
+  return xc + ys;
+}
+
+__global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const double *trig, double *const *proj) {
+  const Img im = imgs[blockIdx.z];
+  const int M = im.h, N = im.w;
+  const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
+                                   (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
+  const int rFirst = -rLast, rSize = rLast - rFirst + 1;
+  const int a = blockIdx.y;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rSize) return;
+  const double cs = trig[2 * a], sn = trig[2 * a + 1];
+  double acc = 0.0;
+  const int M2 = 2 * M;
+  for (int nn = 0; nn < 2 * N; nn++) {
+    const double *col = im.p + nn / 2;
+    // floor(rIdx) at the ends of the column
+    const int fl0 = (int)radon_ridx(nn, 0, M, N, cs, sn, rFirst);
+    const int fl1 = (int)radon_ridx(nn, M2 - 1, M, N, cs, sn, rFirst);
+    if (fl0 < r - 1 || fl1 > r) continue;
+    // first m with floor <= r
+    int lo = 0, hi = M2;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)radon_ridx(nn, mid, M, N, cs, sn, rFirst) <= r) hi = mid; else lo = mid + 1; }
+    const int m0 = lo;
+    lo = m0; hi = M2;
+    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)radon_ridx(nn, mid, M, N, cs, sn, rFirst) < r - 1) hi = mid; else lo = mid + 1; }
+    const int m1 = lo;
+    for (int mm = m0; mm < m1; mm++) {
+      double pixel = col[(long)(mm / 2) * N];
+      if (pixel) {
+        pixel *= 0.25;
+        const double rIdx = radon_ridx(nn, mm, M, N, cs, sn, rFirst);
+        const int rLow = (int)rIdx;
+        const double pixelLow = pixel * (1 - rIdx + rLow);
+        if (rLow == r) acc += pixelLow;
+        else acc += pixel - pixelLow;
+      }
+    }
+  }
+  proj[blockIdx.z][(long)a * rSize + r] = acc;
+}
+
+// yTable / xTable entries are exact multiples of 0.25, so the closed forms
+// above equal the reference's running sums.
+__global__ void k_radon_hist(const Img *imgs, double *const *proj, OutRef o) {
+  const Img im = imgs[blockIdx.x];
+  const int M = im.h, N = im.w;
+  const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
+                                   (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
+  const int rSize = 2 * rLast + 1;
+  if (threadIdx.x != 0) return;
+  double v[12];
+  int vi = 0;
+  for (int a = 0; a < 4; a++) {
+    const double *p = proj[blockIdx.x] + (long)a * rSize;
+    double hmin = kInf, hmax = -kInf;
+    for (int i = 0; i < rSize; i++) { if (p[i] > hmax) hmax = p[i]; if (p[i] < hmin) hmin = p[i]; }
+    const double hs = hist_scale(hmin, hmax, 3);
+    double bins[3] = {0, 0, 0};
+    for (int i = 0; i < rSize; i++) bins[hist_bin(p[i], hmin, hs, 3)] += 1.0;
+    for (int b = 0; b < 3; b++) v[vi++] = bins[b];
+  }
+  const int ord[12] = {0, 1, 2, 9, 10, 11, 3, 4, 5, 6, 7, 8};
+  double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+  for (int a = 0; a < 12; a++) out[a] = v[ord[a]];
+}
+
+void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
+                  hipStream_t s) {
+  k_radon_proj<<<dim3((maxbins + 255) / 256, 4, nimg), 256, 0, s>>>(imgs, trig, proj);
+  k_radon_hist<<<nimg, 64, 0, s>>>(imgs, proj, o);
+}
+
+}  // namespace wnd

@@ -1,0 +1,108 @@
+// Kernel descriptors and launcher declarations (host + device visible).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "common.hpp"
+
+namespace wnd {
+
+struct LoadDesc {
+  const void *src;
+  int h, w, stride;
+  int dtype;  // 0 uint16, 1 float32, 2 float64, 3 uint8
+};
+
+// C[r][c] = sum_a (A[r*sar + a*sak] * B[a*N + c]) / 2   (M x K) (K x N)
+struct GemmDesc {
+  const double *A;
+  const double *B;
+  double *C;
+  int M, K, N;
+  long sar, sak;
+};
+
+struct FftPlan {
+  const double2 *tw;  // tw[k] = exp(-2 pi i k / n)
+  int n, nst;
+  int radix[20];
+};
+
+// Chebyshev-Fourier coefficient tables for one (possibly downsampled) size
+struct CfDesc {
+  const double *img;    // plane the coefficients are taken from
+  const int *kk;        // in-disk pixel indices, column-major order (x outer)
+  const double *tn;     // [np][NN] Chebyshev T(2r-1), T0 = 0.5
+  const double *fc;     // [np][NN] Ftrm*cos(mf*phi)
+  const double *fs;     // [np][NN] Ftrm*sin(-mf*phi)
+  int np, NN;
+  int w, h;             // plane size (row-major [h][w])
+  double *partial;      // [nchunk][2*NN*NN]
+};
+
+struct DsDesc {  // area-weighted Downsample (cmatrix.cpp:579-720)
+  const double *in;
+  double *tmp;          // [oh][ow] after the x pass
+  double *out;          // [nh][nw]
+  int oh, ow, nh, nw, cx, cy;  // cx/cy: number of x / y steps
+  const double *xs;     // x positions (cx+1 entries, sequential accumulation)
+  const double *ys;
+  double dx, dy;
+};
+
+void launch_load(const LoadDesc *d, Img *dst, int nimg, long maxn, hipStream_t s);
+void launch_stats(const Img *imgs, Stats *st, int nimg, hipStream_t s);
+void launch_edge(const Img *in, const Img *out, int nimg, long maxn, hipStream_t s);
+void launch_wavelet(const Img *in, double *const *g, const int *gstride, const Img *out, int nimg, int maxh,
+                    int maxw, hipStream_t s);
+void launch_cheb_gemm(const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s);
+void launch_fft(const Img *in, double2 *const *f1, const FftPlan *plw, const FftPlan *plh, const Img *out, int nimg,
+                int maxh, int maxw, hipStream_t s);
+void launch_sort(const Img *imgs, uint64_t *const *buf, const long *cap, int nimg, hipStream_t s);
+void launch_median_gini(const Img *imgs, uint64_t *const *buf, const Stats *st, double *median_out, OutRef pix,
+                        OutRef gini, int nimg, hipStream_t s);
+void launch_msh(const Img *imgs, const Stats *st, OutRef o, int nimg, hipStream_t s);
+void launch_fractal(const Img *imgs, OutRef o, int nimg, hipStream_t s);
+void launch_hist_values(const Img *vals, int nbins, OutRef o, int nimg, hipStream_t s);
+
+// textures (k_texture.hip)
+struct HarDesc {
+  const Img *img;
+  uint8_t *gray;        // [n] quantised plane
+  uint32_t *cooc;       // [4][256*257/2] upper-triangle counts per angle
+  double *P;            // [4][ng*ng] normalised matrices (ng <= 256)
+  double *Q;            // [4][ng*ng]
+  double *feat;         // [4][14]
+  int *ng;              // number of tones present
+  int *lut;             // [256] tone -> rank
+  int *count;           // [4] pair counts
+};
+void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef o, int nimg, int maxn,
+                     hipStream_t s);
+void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, OutRef o, int nimg, int maxh, int maxw,
+                   hipStream_t s);
+void launch_comb(const Img *imgs, const double *iis, OutRef o, int nimg, hipStream_t s);
+void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
+                  hipStream_t s);
+
+// objects / edges / gabor / zernike / chebyshev-fourier (k_objects.hip, k_spectral.hip)
+struct ObjDesc {
+  int *label;            // [n]
+  int *area;             // [n+1] per label
+  long long *xm, *ym;    // [n+1]
+  uint64_t *keys;        // [2*(n+1)] sort buffers
+  double *dist;          // [n+1]
+  int *count;            // [1]
+};
+void launch_objects(const Img *imgs, const Stats *st, int invert, const ObjDesc *od, OutRef o, int nimg, int maxn,
+                    hipStream_t s);
+void launch_prewitt(const Img *imgs, const Img *mag, const Img *dir, int nimg, long maxn, hipStream_t s);
+void launch_edge_final(const Img *mag, const Img *dir, const Stats *st2, const double *medians, OutRef o, int nimg,
+                       hipStream_t s);
+void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
+                  int maxw, hipStream_t s);
+void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
+                    hipStream_t s);
+void launch_cf(const CfDesc *cd, int nimg, int maxchunk, OutRef o, hipStream_t s);
+void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
+
+}  // namespace wnd

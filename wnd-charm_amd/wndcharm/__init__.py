@@ -1,0 +1,407 @@
+"""wndcharm -- Python surface of the MI355X WND-CHARM feature engine.
+
+Mirrors the names of the reference's SWIG module (wndcharm/swig/wndcharm.i:32-40,
+Tasks.i:15-35) that callers use on the feature path:
+
+    plan = wndcharm.StdFeatureComputationPlans.getFeatureSetLong()
+    vec = wndcharm.DoubleVector(plan.n_features)
+    wndcharm.FeatureComputationPlanExecutor(plan).run(px_plane, vec, 0)
+
+(the call sequence of FeatureVector.GenerateFeatures, wndcharm/FeatureVector.py:724-758),
+plus batch entry points that take many images per launch.  Everything runs on
+the GPU through the C ABI of lib/libwndgpu.so (include/wndgpu.h); there is no
+CPU implementation behind these names, and a missing library or GPU raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ['ImageMatrix', 'PyImageMatrix', 'DoubleVector', 'FeatureComputationPlan', 'StdFeatureComputationPlans',
+           'FeatureComputationPlanExecutor', 'ImageTransform', 'FeatureAlgorithm', 'compute_features',
+           'normalize_train', 'normalize_test', 'fisher_weights', 'wnd5', 'WndGpuError', 'library_path']
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+library_path = os.environ.get('WNDGPU_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libwndgpu.so'))
+
+
+class WndGpuError(RuntimeError):
+    pass
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_lib = None
+
+
+class _Image(ctypes.Structure):
+    _fields_ = [('data', _P), ('height', _I), ('width', _I), ('stride', _I), ('dtype', _I)]
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(library_path):
+        raise WndGpuError('libwndgpu.so not found at %s: build it with __graft_entry__.build()' % library_path)
+    lib = ctypes.CDLL(library_path)
+    sig = {
+        'wndgpu_create': (_I, [_I, ctypes.POINTER(_P)]),
+        'wndgpu_destroy': (None, [_P]),
+        'wndgpu_last_error': (ctypes.c_char_p, [_P]),
+        'wndgpu_version': (ctypes.c_char_p, []),
+        'wndgpu_plan_standard': (_I, [_I, ctypes.POINTER(_P)]),
+        'wndgpu_plan_from_groups': (_I, [ctypes.POINTER(ctypes.c_char_p), _I, ctypes.POINTER(_P)]),
+        'wndgpu_plan_destroy': (None, [_P]),
+        'wndgpu_plan_n_features': (_I, [_P]),
+        'wndgpu_plan_feature_name': (ctypes.c_char_p, [_P, _I]),
+        'wndgpu_plan_group_offset': (_I, [_P, ctypes.c_char_p]),
+        'wndgpu_compute': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P]),
+        'wndgpu_compute_device': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P]),
+        'wndgpu_transform': (_I, [_P, ctypes.c_char_p, _P, _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        'wndgpu_algorithm': (_I, [_P, ctypes.c_char_p, _P, _I, _I, _P, ctypes.POINTER(_I)]),
+        'wndgpu_algorithm_n_features': (_I, [ctypes.c_char_p]),
+        'wndgpu_normalize_train': (_I, [_P, _P, _I, _I, _P, _P]),
+        'wndgpu_normalize_test': (_I, [_P, _P, _I, _P, _P]),
+        'wndgpu_fisher': (_I, [_P, _P, _P, _I, _I, _I, _D, _P]),
+        'wndgpu_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
+        'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+_ctx = None
+
+
+def _context():
+    """The process-wide context on device WNDGPU_DEVICE (default 0)."""
+    global _ctx
+    if _ctx is None:
+        lib = _load()
+        c = _P()
+        rc = lib.wndgpu_create(int(os.environ.get('WNDGPU_DEVICE', '0')), ctypes.byref(c))
+        if rc != 0:
+            raise WndGpuError('wndgpu_create failed (%d): no usable GPU' % rc)
+        _ctx = c
+    return _ctx
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = _load().wndgpu_last_error(_ctx).decode() if _ctx is not None else ''
+        raise WndGpuError('%s failed (%d): %s' % (what, rc, msg))
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- pixel plane
+class ImageMatrix(object):
+    """FP64 row-major pixel plane (ImageMatrix, src/cmatrix.h:164-339)."""
+
+    def __init__(self, pixels=None):
+        self._px = None
+        if pixels is not None:
+            self.set_pixels(pixels)
+
+    def allocate(self, w, h):
+        self._px = np.zeros((int(h), int(w)), np.float64)
+
+    def set_pixels(self, a):
+        a = np.asarray(a)
+        if a.ndim != 2:
+            raise ValueError('pixel plane must be 2-D')
+        self._px = np.ascontiguousarray(a, dtype=np.float64)
+
+    @property
+    def width(self):
+        return 0 if self._px is None else self._px.shape[1]
+
+    @property
+    def height(self):
+        return 0 if self._px is None else self._px.shape[0]
+
+    def data_ptr(self):
+        return None if self._px is None else self._px.ctypes.data
+
+    def as_ndarray(self):
+        return self._px
+
+    def OpenImage(self, path, downsample=0, bounding_rect=None, mean=0, stdev=0):
+        """Loads an 8/16-bit grayscale TIFF (ImageMatrix::OpenImage, cmatrix.cpp:252-294);
+        returns 1 on success, 0 on failure like the reference."""
+        try:
+            from PIL import Image
+            a = np.array(Image.open(path))
+        except Exception:
+            return 0
+        if a.ndim != 2:
+            return 0
+        if bounding_rect is not None:
+            x, y, w, h = bounding_rect
+            a = a[y:y + h, x:x + w]
+        self.set_pixels(a.astype(np.float64))
+        return 1
+
+
+class PyImageMatrix(ImageMatrix):
+    """numpy view helper of the reference (wndcharm/PyImageMatrix.py:41-63)."""
+
+
+class DoubleVector(object):
+    """std::vector<double> stand-in of the SWIG module (Tasks.i:15-17)."""
+
+    def __init__(self, n=0):
+        self._a = np.zeros(int(n), np.float64)
+
+    def __len__(self):
+        return len(self._a)
+
+    def __getitem__(self, i):
+        return self._a[i]
+
+    def __setitem__(self, i, v):
+        self._a[i] = v
+
+    def __iter__(self):
+        return iter(self._a.tolist())
+
+    def resize(self, n):
+        a = np.zeros(int(n), np.float64)
+        k = min(n, len(self._a))
+        a[:k] = self._a[:k]
+        self._a = a
+
+    def as_ndarray(self):
+        return self._a
+
+
+# ---------------------------------------------------------------- plans
+class FeatureComputationPlan(object):
+    """FeatureComputationPlan (src/Tasks.h:272-300, src/Tasks.cpp:118-168)."""
+
+    def __init__(self, name='', _handle=None, _vec_type=0):
+        self.name = name
+        self._groups = []
+        self._handle = _handle
+        self.feature_vec_type = _vec_type
+        self._final = _handle is not None
+
+    def add(self, group_name):
+        if self._final:
+            raise WndGpuError('Attempt to call FeatureComputationPlan::add() to a finalized plan')
+        if group_name in self._groups:
+            raise WndGpuError('Calling FeatureComputationPlan::add() with duplicate FeatureGroup')
+        self._groups.append(group_name)
+
+    def finalize(self):
+        if self._final:
+            return
+        lib = _load()
+        arr = (ctypes.c_char_p * len(self._groups))(*[g.encode() for g in self._groups])
+        h = _P()
+        rc = lib.wndgpu_plan_from_groups(arr, len(self._groups), ctypes.byref(h))
+        if rc != 0:
+            raise WndGpuError('unknown or duplicate feature group in %r' % (self._groups,))
+        self._handle = h
+        self._final = True
+
+    def isFinalized(self):
+        return self._final
+
+    @property
+    def handle(self):
+        self.finalize()
+        return self._handle
+
+    @property
+    def n_features(self):
+        return _load().wndgpu_plan_n_features(self.handle)
+
+    def getFeatureNameByIndex(self, i):
+        return _load().wndgpu_plan_feature_name(self.handle, int(i)).decode()
+
+    def feature_names(self):
+        return [self.getFeatureNameByIndex(i) for i in range(self.n_features)]
+
+    def getFGoffset(self, group_name):
+        return _load().wndgpu_plan_group_offset(self.handle, group_name.encode())
+
+
+class StdFeatureComputationPlans(object):
+    """StdFeatureComputationPlans (src/Tasks.h:304-377): process singletons."""
+    _cache = {}
+
+    @classmethod
+    def _get(cls, s, vec_type, name):
+        if s not in cls._cache:
+            h = _P()
+            _check(_load().wndgpu_plan_standard(s, ctypes.byref(h)), 'wndgpu_plan_standard')
+            cls._cache[s] = FeatureComputationPlan(name, _handle=h, _vec_type=vec_type)
+        return cls._cache[s]
+
+    @classmethod
+    def getFeatureSet(cls):
+        return cls._get(0, 1, 'Standard Feature Set')
+
+    @classmethod
+    def getFeatureSetLong(cls):
+        return cls._get(1, 2, 'Long Feature Set')
+
+
+def _as_image_struct(a):
+    if a.dtype == np.uint16:
+        dt = 0
+    elif a.dtype == np.float32:
+        dt = 1
+    elif a.dtype == np.float64:
+        dt = 2
+    elif a.dtype == np.uint8:
+        dt = 3
+    else:
+        a = a.astype(np.float64)
+        dt = 2
+    a = np.ascontiguousarray(a)
+    return a, _Image(ctypes.c_void_p(a.ctypes.data), a.shape[0], a.shape[1], a.shape[1], dt)
+
+
+def compute_features(images, plan=None, return_status=False):
+    """Features of a batch of 2-D arrays (uint8/uint16/float32/float64, sizes
+    may differ) in one pass of the DAG on the GPU.  Returns [n][n_features]
+    float64; rows of rejected images are NaN (status != 0)."""
+    if plan is None:
+        plan = StdFeatureComputationPlans.getFeatureSetLong()
+    keep = []
+    structs = (_Image * len(images))()
+    for i, im in enumerate(images):
+        a, s = _as_image_struct(np.asarray(im))
+        keep.append(a)
+        structs[i] = s
+    nf = plan.n_features
+    out = np.full((len(images), nf), np.nan)
+    status = np.zeros(len(images), np.int32)
+    rc = _load().wndgpu_compute(_context(), plan.handle, structs, len(images), _ptr(out), _ptr(status))
+    if rc not in (0, -4):
+        _check(rc, 'wndgpu_compute')
+    if return_status:
+        return out, status
+    if rc == -4:
+        raise WndGpuError('images rejected, status=%s' % status.tolist())
+    return out
+
+
+class FeatureComputationPlanExecutor(object):
+    """FeatureComputationPlanExecutor (src/Tasks.h:335-357, Tasks.cpp:275-295)."""
+
+    def __init__(self, plan):
+        self.plan = plan
+
+    def run(self, source_mat, feature_mat, dest_row):
+        px = source_mat.as_ndarray() if hasattr(source_mat, 'as_ndarray') else np.asarray(source_mat)
+        if px is None:
+            raise WndGpuError('Attempt to execute a FeatureComputationPlan with an empty ImageMatrix')
+        nf = self.plan.n_features
+        row = compute_features([px], self.plan)[0]
+        dst = feature_mat.as_ndarray() if hasattr(feature_mat, 'as_ndarray') else feature_mat
+        if len(dst) < nf * (dest_row + 1):
+            raise WndGpuError('feature matrix too small for row %d' % dest_row)
+        dst[nf * dest_row:nf * (dest_row + 1)] = row
+
+
+class ImageTransform(object):
+    """ImageTransform::execute by registered name (src/ImageTransforms.h:45-51)."""
+
+    def __init__(self, name):
+        self.name = name
+
+    def execute(self, matrix_in, matrix_out=None):
+        a = np.ascontiguousarray(matrix_in.as_ndarray() if hasattr(matrix_in, 'as_ndarray') else matrix_in,
+                                 dtype=np.float64)
+        h, w = a.shape
+        out = np.zeros((2 * ((h + 9) // 2)) * (2 * ((w + 9) // 2)))
+        oh, ow = _I(), _I()
+        _check(_load().wndgpu_transform(_context(), self.name.encode(), _ptr(a), h, w, _ptr(out), ctypes.byref(oh),
+                                        ctypes.byref(ow)), 'ImageTransform %s' % self.name)
+        r = out[:oh.value * ow.value].reshape(oh.value, ow.value)
+        if matrix_out is not None:
+            matrix_out.set_pixels(r)
+            return matrix_out
+        return r
+
+
+class FeatureAlgorithm(object):
+    """FeatureAlgorithm::execute by registered name (src/FeatureAlgorithms.h:280-290)."""
+
+    def __init__(self, name):
+        self.name = name
+        self.n_features = _load().wndgpu_algorithm_n_features(name.encode())
+        if self.n_features < 0:
+            raise WndGpuError('unknown feature algorithm %r' % name)
+
+    def execute(self, matrix_in):
+        a = np.ascontiguousarray(matrix_in.as_ndarray() if hasattr(matrix_in, 'as_ndarray') else matrix_in,
+                                 dtype=np.float64)
+        out = np.zeros(self.n_features)
+        n = _I()
+        _check(_load().wndgpu_algorithm(_context(), self.name.encode(), _ptr(a), a.shape[0], a.shape[1], _ptr(out),
+                                        ctypes.byref(n)), 'FeatureAlgorithm %s' % self.name)
+        return out
+
+
+# ---------------------------------------------------------------- classifier
+def normalize_train(data):
+    """TrainingSet::normalize: returns (normalised copy, mins, maxs)."""
+    d = np.ascontiguousarray(np.array(data, np.float64))
+    n, f = d.shape
+    mins, maxs = np.zeros(f), np.zeros(f)
+    _check(_load().wndgpu_normalize_train(_context(), _ptr(d), n, f, _ptr(mins), _ptr(maxs)), 'normalize_train')
+    return d, mins, maxs
+
+
+def normalize_test(vec, mins, maxs):
+    v = np.ascontiguousarray(np.array(vec, np.float64))
+    mins = np.ascontiguousarray(mins, np.float64)
+    maxs = np.ascontiguousarray(maxs, np.float64)
+    _check(_load().wndgpu_normalize_test(_context(), _ptr(v), len(v), _ptr(mins), _ptr(maxs)), 'normalize_test')
+    return v
+
+
+def fisher_weights(data, classes, n_classes, used_fraction=0.15):
+    d = np.ascontiguousarray(data, np.float64)
+    c = np.ascontiguousarray(classes, np.int32)
+    w = np.zeros(d.shape[1])
+    _check(_load().wndgpu_fisher(_context(), _ptr(d), _ptr(c), d.shape[0], d.shape[1], int(n_classes),
+                                 float(used_fraction), _ptr(w)), 'fisher')
+    return w
+
+
+def wnd5(train, classes, n_classes, weights, test):
+    """classify2 for a batch of normalised test vectors: (probabilities,
+    norm_factor, predicted class 1..n_classes)."""
+    tr = np.ascontiguousarray(train, np.float64)
+    c = np.ascontiguousarray(classes, np.int32)
+    w = np.ascontiguousarray(weights, np.float64)
+    te = np.ascontiguousarray(np.atleast_2d(test), np.float64)
+    nt = te.shape[0]
+    prob = np.zeros((nt, n_classes))
+    norm = np.zeros(nt)
+    pred = np.zeros(nt, np.int64)
+    _check(_load().wndgpu_wnd5(_context(), _ptr(tr), _ptr(c), tr.shape[0], tr.shape[1], int(n_classes), _ptr(w),
+                               _ptr(te), nt, _ptr(prob), _ptr(norm), _ptr(pred)), 'wnd5')
+    return prob, norm, pred
+
+
+def last_node_times():
+    """(name, ms) per launch of the last compute, when WNDGPU_TIMING is set."""
+    lib = _load()
+    ms = np.zeros(1024)
+    names = (ctypes.c_char_p * 1024)()
+    k = lib.wndgpu_last_node_times(_context(), _ptr(ms), names, 1024)
+    return [(names[i].decode(), float(ms[i])) for i in range(k)]
