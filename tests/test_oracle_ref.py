@@ -141,3 +141,20 @@ def test_chebyshev_vs_reference_bitexact(shape):
     lib.ref_cheb1d(ptr(t), n, h, n, ptr(b))
     got = orc_transform(T_CHEBYSHEV, img)
     assert np.array_equal(got, b.reshape(n, n))
+
+
+@pytest.mark.parametrize('shape', [(128, 128), (113, 113), (70, 100), (61, 122), (33, 97)])
+def test_fourier_is_the_dft(shape):
+    """The oracle's Fourier node (ImageMatrix::fft2, cmatrix.cpp:1107-1160) is
+    |DFT2| of the plane with the reference's Hermitian fill of rows >= H/2+1;
+    checked against numpy's FFT (an independent implementation)."""
+    img = synth(*shape, seed=3)
+    got = orc_transform(T_FOURIER, img)
+    mag = np.abs(np.fft.fft2(img))
+    h, w = shape
+    hh = h // 2 + 1
+    np.testing.assert_allclose(got[:hh], mag[:hh], rtol=1e-10, atol=1e-9 * mag.max())
+    # lower rows are exact copies of the mirrored upper rows
+    for y in range(hh, h):
+        assert got[y, 0] == got[h - y, 0]
+        assert np.array_equal(got[y, 1:], got[h - y, :0:-1])
