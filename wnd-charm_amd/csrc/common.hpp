@@ -23,11 +23,11 @@ struct Stats {
   long long n;
 };
 
-// Output row addressing: feature vector of image i starts at out + i*stride.
+// Output addressing: rows[i] is where entry i of a launch writes its
+// feature group (result row of its image + the group's column offset), so
+// one launch can serve a feature group on several DAG nodes at once.
 struct OutRef {
-  double *out;
-  int stride;  // features per image
-  int off;     // column offset of the feature group
+  double *const *rows;
 };
 
 constexpr double kFltMax = 3.40282346638528859812e+38;

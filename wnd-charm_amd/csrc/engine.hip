@@ -463,245 +463,311 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     N.planes.resize(n);
     for (int i = 0; i < n; i++) N.planes[i] = im[i].p;
   }
-  // ---- feature groups, node by node
+  // ---- feature groups: one launch per algorithm over every (node, image)
+  // entry that needs it, so a launch spans all DAG nodes of the batch.
+  struct Entry {
+    const NodeRun *N;
+    int i;
+    int off;
+  };
+  std::vector<std::vector<Entry>> by_alg(A_COUNT);
   for (const auto &key : plan->nodes) {
-    NodeRun &N = nodes[key];
-    const std::string nm = node_text(key);
-    Img *ni = N.imgs;
-    Stats *nst = N.stats;
-    const int mh = maxof(N.h), mw = maxof(N.w);
-    long maxp = 0;
-    for (int i = 0; i < n; i++) maxp = std::max(maxp, (long)N.h[i] * N.w[i]);
-    int off_pix = -1, off_gini = -1;
-    for (const Group &g : plan->groups) {
-      if (g.chain != key) continue;
-      const OutRef o{B.out, B.nf, g.offset};
-      const std::string gn = g.name;
-      switch (g.alg) {
-        case A_PIXSTATS: off_pix = g.offset; break;
-        case A_GINI: off_gini = g.offset; break;
-        case A_MSH: push(gn, [=]() { launch_msh(ni, nst, o, n, s); }); break;
-        case A_FRACTAL: push(gn, [=]() { launch_fractal(ni, o, n, s); }); break;
-        case A_COMB: {
-          std::vector<double> iis;
-          for (int i = 0; i < n; i++) {
-            std::vector<double> v = comb_offsets(N.h[i]);
-            iis.insert(iis.end(), v.begin(), v.end());
-          }
-          double *d = A.stage(iis);
-          push(gn, [=]() { launch_comb(ni, d, o, n, s); });
-        } break;
-        case A_RADON: {
-          std::vector<double *> pr(n);
-          int maxbins = 0;
-          for (int i = 0; i < n; i++) {
-            const int W = N.w[i], H = N.h[i];
-            const int a = W - 1 - (W - 1) / 2, b = H - 1 - (H - 1) / 2;
-            const int rl = (int)ceil(sqrt((double)(a * a) + (double)(b * b))) + 1;
-            pr[i] = A.dalloc_n<double>(4 * (size_t)(2 * rl + 1));
-            maxbins = std::max(maxbins, 2 * rl + 1);
-          }
-          double **prd = A.stage(pr);
-          const double *trig = (const double *)c->rtrig.p;
-          push(gn, [=]() { launch_radon(ni, trig, prd, o, n, maxbins, s); });
-        } break;
-        case A_TAMURA: {
-          std::vector<double *> sc(n);
-          for (int i = 0; i < n; i++) sc[i] = A.dalloc_n<double>(5 * (size_t)N.h[i] * N.w[i] + 8);
-          double **scd = A.stage(sc);
-          push(gn, [=]() { launch_tamura(ni, nst, scd, o, n, mh, mw, s); });
-        } break;
-        case A_HARALICK: {
-          std::vector<HarDesc> hd(n);
-          int *lut_all = A.dalloc_n<int>(256 * (size_t)n);
-          for (int i = 0; i < n; i++) {
-            HarDesc &H = hd[i];
-            H.img = nullptr;
-            H.gray = A.dalloc_n<uint8_t>((size_t)N.h[i] * N.w[i]);
-            H.cooc = nullptr;
-            H.P = A.dalloc_n<double>(4 * 256 * 256);
-            H.Q = A.dalloc_n<double>(4 * 256 * 256);
-            H.feat = A.dalloc_n<double>(4 * 14);
-            H.ng = A.dalloc_n<int>(1);
-            H.lut = lut_all + 256 * i;
-            H.count = A.dalloc_n<int>(4);
-          }
-          HarDesc *hdd = A.stage(hd);
-          const long mp = maxp;
-          push(gn, [=]() {
-            hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)n, s);
-            launch_haralick(ni, nst, hdd, o, n, (int)mp, s);
-          });
-        } break;
-        case A_ZERNIKE: {
-          const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
-          double *part = A.dalloc_n<double>((size_t)n * (4 + nchunk * 144));
-          const double *ht = (const double *)c->zern.p;
-          push(gn, [=]() { launch_zernike(ni, part, nchunk, ht, o, n, s); });
-        } break;
-        case A_CHEBYSHEV: {
-          std::vector<GemmDesc> g1(n), g2(n);
-          std::vector<Img> vals(n);
-          int maxN = 0, maxH = 0;
-          for (int i = 0; i < n; i++) {
-            const int h = N.h[i], w = N.w[i], NN = std::min(20, std::min(h, w));
-            const double *bx = nullptr, *by = nullptr;
-            if (!A.sizing) {
-              int r;
-              if ((r = get_cheb(c, w, NN, &bx))) return r;
-              if ((r = get_cheb(c, h, NN, &by))) return r;
-            }
-            double *c1 = A.dalloc_n<double>((size_t)h * NN);
-            double *c2 = A.dalloc_n<double>((size_t)NN * NN);
-            g1[i] = GemmDesc{N.planes[i], bx, c1, h, w, NN, (long)w, 1};
-            g2[i] = GemmDesc{c1, by, c2, NN, h, NN, 1, (long)NN};
-            vals[i] = Img{c2, NN, NN};
-            maxN = std::max(maxN, NN);
-            maxH = std::max(maxH, h);
-          }
-          GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
-          Img *vd = A.stage(vals);
-          push(gn, [=]() {
-            launch_cheb_gemm(d1, n, maxH, maxN, s);
-            launch_cheb_gemm(d2, n, maxN, maxN, s);
-            launch_hist_values(vd, 32, o, n, s);
-          });
-        } break;
-        case A_CHEB_FOURIER: {
-          // downsample planes above 300x300 (ImageMatrix::ChebyshevFourierTransform2D)
-          std::vector<DsDesc> ds;
-          std::vector<CfDesc> cd(n);
-          int maxchunk = 0, moh = 0, mow = 0, mnh = 0, mnw = 0;
-          for (int i = 0; i < n; i++) {
-            int h = N.h[i], w = N.w[i];
-            const double *plane = N.planes[i];
-            if ((long)w * h > 300 * 300) {
-              double rr = std::min(300.0 / (double)w, 300.0 / (double)h);
-              if (rr > 1) rr = 1;
-              const double dx = 1 / rr;
-              const int nw = (int)(unsigned)(rr * w), nh = (int)(unsigned)(rr * h);
-              DsDesc d;
-              d.in = plane;
-              d.tmp = A.dalloc_n<double>((size_t)h * w);
-              d.out = A.dalloc_n<double>((size_t)nh * nw);
-              d.oh = h; d.ow = w; d.nh = nh; d.nw = nw;
-              d.dx = dx; d.dy = dx;
-              d.xs = d.ys = nullptr;
-              d.cx = d.cy = 0;
-              if (!A.sizing) {
-                int r;
-                if ((r = get_dspos(c, w, dx, &d.xs, &d.cx))) return r;
-                if ((r = get_dspos(c, h, dx, &d.ys, &d.cy))) return r;
-              }
-              ds.push_back(d);
-              moh = std::max(moh, h); mow = std::max(mow, w); mnh = std::max(mnh, nh); mnw = std::max(mnw, nw);
-              plane = d.out;
-              h = nh;
-              w = nw;
-            }
-            CfDev *t = nullptr;
-            CfDesc C;
-            C.img = plane;
-            C.w = w;
-            C.h = h;
-            if (!A.sizing) {
-              int r;
-              if ((r = get_cf(c, w, h, &t))) return r;
-              C.kk = (const int *)t->kk.p;
-              C.tn = (const double *)t->tn.p;
-              C.fc = (const double *)t->fc.p;
-              C.fs = (const double *)t->fs.p;
-              C.np = t->np;
-              C.NN = t->NN;
-            } else {
-              C.kk = nullptr; C.tn = C.fc = C.fs = nullptr;
-              C.np = w * h;  // upper bound for sizing
-              C.NN = 23;
-            }
-            const int nch = (C.np + 127) / 128;
-            C.partial = A.dalloc_n<double>((size_t)nch * 2 * 23 * 23 + 8);
-            maxchunk = std::max(maxchunk, nch);
-            cd[i] = C;
-          }
-          CfDesc *cdd = A.stage(cd);
-          DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
-          const int nds = (int)ds.size();
-          push(gn, [=]() {
-            if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
-            launch_cf(cdd, n, maxchunk, o, s);
-          });
-        } break;
-        case A_EDGE: {
-          std::vector<Img> md(2 * n), mg(n), dr(n);
-          for (int i = 0; i < n; i++) {
-            const size_t P = (size_t)N.h[i] * N.w[i];
-            mg[i] = Img{A.dalloc_n<double>(P), N.h[i], N.w[i]};
-            dr[i] = Img{A.dalloc_n<double>(P), N.h[i], N.w[i]};
-            md[2 * i] = mg[i];
-            md[2 * i + 1] = dr[i];
-          }
-          std::vector<uint64_t *> sb(2 * n);
-          std::vector<long> cap(2 * n);
-          for (int i = 0; i < 2 * n; i++) {
-            cap[i] = (long)md[i].h * md[i].w;
-            sb[i] = A.dalloc_n<uint64_t>(2 * (size_t)cap[i]);
-          }
-          Img *mgd = A.stage(mg), *drd = A.stage(dr), *mdd = A.stage(md);
-          uint64_t **sbd = A.stage(sb);
-          long *capd = A.stage(cap);
-          Stats *st2 = A.dalloc_n<Stats>(2 * n);
-          double *med = A.dalloc_n<double>(2 * n);
-          const long mp = maxp;
-          push(gn, [=]() {
-            launch_prewitt(ni, mgd, drd, n, mp, s);
-            launch_stats(mdd, st2, 2 * n, s);
-            launch_sort(mdd, sbd, capd, 2 * n, s);
-            launch_median_gini(mdd, sbd, st2, med, OutRef{nullptr, 0, 0}, OutRef{nullptr, 0, 0}, 2 * n, s);
-            launch_edge_final(mgd, drd, st2, med, o, n, s);
-          });
-        } break;
-        case A_OTSU:
-        case A_INV_OTSU: {
-          std::vector<ObjDesc> od(n);
-          for (int i = 0; i < n; i++) {
-            const size_t P = (size_t)N.h[i] * N.w[i];
-            od[i].label = A.dalloc_n<int>(P);
-            od[i].area = A.dalloc_n<int>(P + 1);
-            od[i].xm = A.dalloc_n<long long>(P + 1);
-            od[i].ym = A.dalloc_n<long long>(P + 1);
-            od[i].keys = A.dalloc_n<uint64_t>(2 * (P + 1));
-            od[i].dist = A.dalloc_n<double>(P + 1);
-            od[i].count = A.dalloc_n<int>(1);
-          }
-          ObjDesc *odd = A.stage(od);
-          const int inv = g.alg == A_INV_OTSU;
-          const long mp = maxp;
-          push(gn, [=]() { launch_objects(ni, nst, inv, odd, o, n, (int)mp, s); });
-        } break;
-        case A_GABOR: {
-          std::vector<double *> en(n);
-          for (int i = 0; i < n; i++) en[i] = A.dalloc_n<double>(8 * (size_t)N.h[i] * N.w[i]);
-          double **end_ = A.stage(en);
-          const double *gk = (const double *)c->gabor.p;
-          push(gn, [=]() { launch_gabor(ni, gk, end_, o, n, mh, mw, s); });
-        } break;
-        default: break;
-      }
+    const NodeRun &N = nodes[key];
+    for (const Group &g : plan->groups)
+      if (g.chain == key)
+        for (int i = 0; i < n; i++) by_alg[g.alg].push_back(Entry{&N, i, g.offset});
+  }
+  auto row_of = [&](const Entry &e) { return A.sizing ? (double *)nullptr : B.out + (size_t)e.i * B.nf + e.off; };
+  // common per-launch descriptors: planes, gathered stats, output rows
+  struct Launch {
+    Img *imgs;
+    Stats *st;
+    OutRef o;
+    int ne, mh, mw;
+    long maxp;
+    const Stats *const *stsrc;
+  };
+  auto make_launch = [&](const std::vector<Entry> &E) {
+    Launch L;
+    std::vector<Img> im(E.size());
+    std::vector<const Stats *> sp(E.size());
+    std::vector<double *> rows(E.size());
+    L.mh = L.mw = 0;
+    L.maxp = 0;
+    for (size_t k = 0; k < E.size(); k++) {
+      const NodeRun &N = *E[k].N;
+      const int i = E[k].i;
+      im[k] = Img{N.planes[i], N.h[i], N.w[i]};
+      sp[k] = N.stats + i;
+      rows[k] = row_of(E[k]);
+      L.mh = std::max(L.mh, N.h[i]);
+      L.mw = std::max(L.mw, N.w[i]);
+      L.maxp = std::max(L.maxp, (long)N.h[i] * N.w[i]);
     }
-    if (off_pix >= 0 || off_gini >= 0) {
-      std::vector<uint64_t *> sb(n);
-      std::vector<long> cap(n);
-      for (int i = 0; i < n; i++) {
-        cap[i] = (long)N.h[i] * N.w[i];
-        sb[i] = A.dalloc_n<uint64_t>(2 * (size_t)cap[i]);
+    L.ne = (int)E.size();
+    L.imgs = A.stage(im);
+    L.stsrc = A.stage(sp);
+    L.st = A.dalloc_n<Stats>(E.size());
+    L.o = OutRef{A.stage(rows)};
+    return L;
+  };
+  for (int alg = 0; alg < A_COUNT; alg++) {
+    const std::vector<Entry> &E = by_alg[alg];
+    if (E.empty() || alg == A_GINI) continue;
+    const std::string gn = kAlgNames[alg];
+    if (alg == A_PIXSTATS) continue;  // handled with Gini below
+    Launch L = make_launch(E);
+    Img *ni = L.imgs;
+    Stats *nst = L.st;
+    const Stats *const *sts = L.stsrc;
+    const OutRef o = L.o;
+    const int ne = L.ne, mh = L.mh, mw = L.mw;
+    const long maxp = L.maxp;
+    auto gather = [=]() { launch_gather_stats(sts, nst, ne, s); };
+    switch (alg) {
+      case A_MSH: push(gn, [=]() { gather(); launch_msh(ni, nst, o, ne, s); }); break;
+      case A_FRACTAL: push(gn, [=]() { launch_fractal(ni, o, ne, s); }); break;
+      case A_COMB: {
+        std::vector<double> iis;
+        for (const Entry &e : E) {
+          std::vector<double> v = comb_offsets(e.N->h[e.i]);
+          iis.insert(iis.end(), v.begin(), v.end());
+        }
+        double *d = A.stage(iis);
+        push(gn, [=]() { launch_comb(ni, d, o, ne, s); });
+      } break;
+      case A_RADON: {
+        std::vector<double *> pr(ne);
+        int maxbins = 0;
+        for (int k = 0; k < ne; k++) {
+          const int W = E[k].N->w[E[k].i], H = E[k].N->h[E[k].i];
+          const int a = W - 1 - (W - 1) / 2, b = H - 1 - (H - 1) / 2;
+          const int rl = (int)ceil(sqrt((double)(a * a) + (double)(b * b))) + 1;
+          pr[k] = A.dalloc_n<double>(4 * (size_t)(2 * rl + 1));
+          maxbins = std::max(maxbins, 2 * rl + 1);
+        }
+        double **prd = A.stage(pr);
+        const double *trig = (const double *)c->rtrig.p;
+        push(gn, [=]() { launch_radon(ni, trig, prd, o, ne, maxbins, mh, mw, s); });
+      } break;
+      case A_TAMURA: {
+        std::vector<double *> sc(ne);
+        for (int k = 0; k < ne; k++)
+          sc[k] = A.dalloc_n<double>(5 * (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i] + 8);
+        double **scd = A.stage(sc);
+        push(gn, [=]() { gather(); launch_tamura(ni, nst, scd, o, ne, mh, mw, s); });
+      } break;
+      case A_HARALICK: {
+        std::vector<HarDesc> hd(ne);
+        int *lut_all = A.dalloc_n<int>(256 * (size_t)ne);
+        for (int k = 0; k < ne; k++) {
+          HarDesc &H = hd[k];
+          H.img = nullptr;
+          H.gray = A.dalloc_n<uint8_t>((size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i]);
+          H.cooc = nullptr;
+          H.P = A.dalloc_n<double>(4 * 256 * 256);
+          H.Q = A.dalloc_n<double>(4 * 256 * 256);
+          H.feat = A.dalloc_n<double>(4 * 14);
+          H.ng = A.dalloc_n<int>(1);
+          H.lut = lut_all + 256 * k;
+          H.count = A.dalloc_n<int>(4);
+        }
+        HarDesc *hdd = A.stage(hd);
+        push(gn, [=]() {
+          gather();
+          hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)ne, s);
+          launch_haralick(ni, nst, hdd, o, ne, (int)maxp, s);
+        });
+      } break;
+      case A_ZERNIKE: {
+        const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
+        double *part = A.dalloc_n<double>((size_t)ne * (4 + nchunk * 144));
+        const double *ht = (const double *)c->zern.p;
+        push(gn, [=]() { launch_zernike(ni, part, nchunk, ht, o, ne, s); });
+      } break;
+      case A_CHEBYSHEV: {
+        std::vector<GemmDesc> g1(ne), g2(ne);
+        std::vector<Img> vals(ne);
+        int maxN = 0, maxH = 0;
+        for (int k = 0; k < ne; k++) {
+          const int h = E[k].N->h[E[k].i], w = E[k].N->w[E[k].i], NN = std::min(20, std::min(h, w));
+          const double *bx = nullptr, *by = nullptr;
+          if (!A.sizing) {
+            int r;
+            if ((r = get_cheb(c, w, NN, &bx))) return r;
+            if ((r = get_cheb(c, h, NN, &by))) return r;
+          }
+          double *c1 = A.dalloc_n<double>((size_t)h * NN);
+          double *c2 = A.dalloc_n<double>((size_t)NN * NN);
+          g1[k] = GemmDesc{E[k].N->planes[E[k].i], bx, c1, h, w, NN, (long)w, 1};
+          g2[k] = GemmDesc{c1, by, c2, NN, h, NN, 1, (long)NN};
+          vals[k] = Img{c2, NN, NN};
+          maxN = std::max(maxN, NN);
+          maxH = std::max(maxH, h);
+        }
+        GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
+        Img *vd = A.stage(vals);
+        push(gn, [=]() {
+          launch_cheb_gemm(d1, ne, maxH, maxN, s);
+          launch_cheb_gemm(d2, ne, maxN, maxN, s);
+          launch_hist_values(vd, 32, o, ne, s);
+        });
+      } break;
+      case A_CHEB_FOURIER: {
+        // downsample planes above 300x300 (ImageMatrix::ChebyshevFourierTransform2D)
+        std::vector<DsDesc> ds;
+        std::vector<CfDesc> cd(ne);
+        int maxchunk = 0, moh = 0, mow = 0, mnh = 0, mnw = 0;
+        for (int k = 0; k < ne; k++) {
+          int h = E[k].N->h[E[k].i], w = E[k].N->w[E[k].i];
+          const double *plane = E[k].N->planes[E[k].i];
+          if ((long)w * h > 300 * 300) {
+            double rr = std::min(300.0 / (double)w, 300.0 / (double)h);
+            if (rr > 1) rr = 1;
+            const double dx = 1 / rr;
+            const int nw = (int)(unsigned)(rr * w), nh = (int)(unsigned)(rr * h);
+            DsDesc d;
+            d.in = plane;
+            d.tmp = A.dalloc_n<double>((size_t)h * w);
+            d.out = A.dalloc_n<double>((size_t)nh * nw);
+            d.oh = h; d.ow = w; d.nh = nh; d.nw = nw;
+            d.dx = dx; d.dy = dx;
+            d.xs = d.ys = nullptr;
+            d.cx = d.cy = 0;
+            if (!A.sizing) {
+              int r;
+              if ((r = get_dspos(c, w, dx, &d.xs, &d.cx))) return r;
+              if ((r = get_dspos(c, h, dx, &d.ys, &d.cy))) return r;
+            }
+            ds.push_back(d);
+            moh = std::max(moh, h); mow = std::max(mow, w); mnh = std::max(mnh, nh); mnw = std::max(mnw, nw);
+            plane = d.out;
+            h = nh;
+            w = nw;
+          }
+          CfDesc C;
+          C.img = plane;
+          C.w = w;
+          C.h = h;
+          if (!A.sizing) {
+            CfDev *t = nullptr;
+            int r;
+            if ((r = get_cf(c, w, h, &t))) return r;
+            C.kk = (const int *)t->kk.p;
+            C.tn = (const double *)t->tn.p;
+            C.fc = (const double *)t->fc.p;
+            C.fs = (const double *)t->fs.p;
+            C.np = t->np;
+            C.NN = t->NN;
+          } else {
+            C.kk = nullptr; C.tn = C.fc = C.fs = nullptr;
+            C.np = w * h;  // upper bound for sizing
+            C.NN = 23;
+          }
+          const int nch = (C.np + 127) / 128;
+          C.partial = A.dalloc_n<double>((size_t)nch * 2 * 23 * 23 + 8);
+          maxchunk = std::max(maxchunk, nch);
+          cd[k] = C;
+        }
+        CfDesc *cdd = A.stage(cd);
+        DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
+        const int nds = (int)ds.size();
+        push(gn, [=]() {
+          if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
+          launch_cf(cdd, ne, maxchunk, o, s);
+        });
+      } break;
+      case A_EDGE: {
+        std::vector<Img> md(2 * ne), mg(ne), dr(ne);
+        for (int k = 0; k < ne; k++) {
+          const int h = E[k].N->h[E[k].i], w = E[k].N->w[E[k].i];
+          const size_t P = (size_t)h * w;
+          mg[k] = Img{A.dalloc_n<double>(P), h, w};
+          dr[k] = Img{A.dalloc_n<double>(P), h, w};
+          md[2 * k] = mg[k];
+          md[2 * k + 1] = dr[k];
+        }
+        std::vector<uint64_t *> sb(2 * ne);
+        std::vector<long> cap(2 * ne);
+        for (int k = 0; k < 2 * ne; k++) {
+          cap[k] = (long)md[k].h * md[k].w;
+          sb[k] = A.dalloc_n<uint64_t>(2 * (size_t)cap[k]);
+        }
+        Img *mgd = A.stage(mg), *drd = A.stage(dr), *mdd = A.stage(md);
+        uint64_t **sbd = A.stage(sb);
+        long *capd = A.stage(cap);
+        Stats *st2 = A.dalloc_n<Stats>(2 * ne);
+        double *med = A.dalloc_n<double>(2 * ne);
+        push(gn, [=]() {
+          launch_prewitt(ni, mgd, drd, ne, maxp, s);
+          launch_stats(mdd, st2, 2 * ne, s);
+          launch_sort(mdd, sbd, capd, 2 * ne, s);
+          launch_median_gini(mdd, sbd, st2, med, OutRef{nullptr}, OutRef{nullptr}, 2 * ne, s);
+          launch_edge_final(mgd, drd, st2, med, o, ne, s);
+        });
+      } break;
+      case A_OTSU:
+      case A_INV_OTSU: {
+        std::vector<ObjDesc> od(ne);
+        for (int k = 0; k < ne; k++) {
+          const size_t P = (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i];
+          od[k].label = A.dalloc_n<int>(P);
+          od[k].area = A.dalloc_n<int>(P + 1);
+          od[k].xm = A.dalloc_n<long long>(P + 1);
+          od[k].ym = A.dalloc_n<long long>(P + 1);
+          od[k].keys = A.dalloc_n<uint64_t>(2 * (P + 1));
+          od[k].dist = A.dalloc_n<double>(P + 1);
+          od[k].count = A.dalloc_n<int>(1);
+        }
+        ObjDesc *odd = A.stage(od);
+        const int inv = alg == A_INV_OTSU;
+        push(gn, [=]() { gather(); launch_objects(ni, nst, inv, odd, o, ne, (int)maxp, s); });
+      } break;
+      case A_GABOR: {
+        std::vector<double *> en(ne);
+        for (int k = 0; k < ne; k++) en[k] = A.dalloc_n<double>(8 * (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i]);
+        double **end_ = A.stage(en);
+        const double *gk = (const double *)c->gabor.p;
+        push(gn, [=]() { launch_gabor(ni, gk, end_, o, ne, mh, mw, s); });
+      } break;
+      default: break;
+    }
+  }
+  // Pixel Intensity Statistics and Gini share one sort of each plane
+  {
+    std::map<std::pair<const NodeRun *, int>, std::pair<int, int>> ps;  // -> (pix off, gini off)
+    std::vector<std::pair<const NodeRun *, int>> order;
+    for (int alg : {A_PIXSTATS, A_GINI})
+      for (const Entry &e : by_alg[alg]) {
+        auto key = std::make_pair(e.N, e.i);
+        if (!ps.count(key)) { ps[key] = {-1, -1}; order.push_back(key); }
+        (alg == A_PIXSTATS ? ps[key].first : ps[key].second) = e.off;
       }
+    if (!order.empty()) {
+      std::vector<Entry> E;
+      for (auto &k : order) E.push_back(Entry{k.first, k.second, 0});
+      Launch L = make_launch(E);
+      std::vector<double *> pr(E.size()), gr(E.size());
+      std::vector<uint64_t *> sb(E.size());
+      std::vector<long> cap(E.size());
+      for (size_t k = 0; k < E.size(); k++) {
+        const auto &o2 = ps[order[k]];
+        pr[k] = (o2.first >= 0 && !A.sizing) ? B.out + (size_t)E[k].i * B.nf + o2.first : nullptr;
+        gr[k] = (o2.second >= 0 && !A.sizing) ? B.out + (size_t)E[k].i * B.nf + o2.second : nullptr;
+        cap[k] = (long)E[k].N->h[E[k].i] * E[k].N->w[E[k].i];
+        sb[k] = A.dalloc_n<uint64_t>(2 * (size_t)cap[k]);
+      }
+      const OutRef op{A.stage(pr)}, og{A.stage(gr)};
       uint64_t **sbd = A.stage(sb);
       long *capd = A.stage(cap);
-      const OutRef op{off_pix >= 0 ? B.out : nullptr, B.nf, off_pix};
-      const OutRef og{off_gini >= 0 ? B.out : nullptr, B.nf, off_gini};
-      push("Pixel Intensity Statistics / Gini " + nm, [=]() {
-        launch_sort(ni, sbd, capd, n, s);
-        launch_median_gini(ni, sbd, nst, nullptr, op, og, n, s);
+      Img *ni = L.imgs;
+      Stats *nst = L.st;
+      const Stats *const *sts = L.stsrc;
+      const int ne = L.ne;
+      push("Pixel Intensity Statistics / Gini", [=]() {
+        launch_gather_stats(sts, nst, ne, s);
+        launch_sort(ni, sbd, capd, ne, s);
+        launch_median_gini(ni, sbd, nst, nullptr, op, og, ne, s);
       });
     }
   }

@@ -322,17 +322,17 @@ __global__ void __launch_bounds__(1024) k_median_gini(const Img *imgs, uint64_t 
   g = block_reduce(g, SumOp(), shd);
   if (threadIdx.x == 0) {
     if (median_out) median_out[blockIdx.x] = med;
-    if (pix.out) {
-      double *o = pix.out + (long)blockIdx.x * pix.stride + pix.off;
+    if (pix.rows && pix.rows[blockIdx.x]) {
+      double *o = pix.rows[blockIdx.x];
       const Stats S = st[blockIdx.x];
       o[0] = S.mean; o[1] = med; o[2] = S.stdv; o[3] = S.mn; o[4] = S.mx;
     }
-    if (gini.out) {
+    if (gini.rows && gini.rows[blockIdx.x]) {
       const double mean = cnt > 0 ? s / cnt : 0.0;
       double r;
       if (cnt <= 1 || mean <= 0.0) r = 0.0;
       else r = g / (mean * cnt * (cnt - 1));
-      gini.out[(long)blockIdx.x * gini.stride + gini.off] = r;
+      gini.rows[blockIdx.x][0] = r;
     }
   }
 }
@@ -359,7 +359,7 @@ __global__ void __launch_bounds__(1024) k_msh(const Img *imgs, const Stats *st, 
   if (threadIdx.x == 0) {
     double mx = 0;
     for (int a = 0; a < 24; a++) if (cnt[a] > mx) mx = cnt[a];
-    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    double *out = o.rows[blockIdx.x];
     for (int a = 0; a < 24; a++) out[a] = (double)cnt[a] / mx;
   }
 }
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(256) k_fractal(const Img *imgs, OutRef o) {
   if (step < 1) step = 1;
   const int bin = blockIdx.x;
   const int k = 1 + bin * step;
-  double *out = o.out + (long)blockIdx.y * o.stride + o.off;
+  double *out = o.rows[blockIdx.y];
   if (bin >= 20) return;
   if (k >= K) {
     if (threadIdx.x == 0) out[bin] = 0.0;
@@ -414,11 +414,21 @@ __global__ void __launch_bounds__(256) k_hist_values(const Img *vals, int nbins,
   const double sc = hist_scale(mn, mx, nbins);
   for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[hist_bin(v.p[i], mn, sc, nbins)], 1);
   __syncthreads();
-  if (threadIdx.x < nbins) o.out[(long)blockIdx.x * o.stride + o.off + threadIdx.x] = cnt[threadIdx.x];
+  if (threadIdx.x < nbins) o.rows[blockIdx.x][threadIdx.x] = cnt[threadIdx.x];
+}
+
+// gathers per-entry statistics into one array (entries of a combined launch
+// come from different DAG nodes)
+__global__ void k_gather_stats(const Stats *const *src, Stats *dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = *src[i];
 }
 
 // ===========================================================================
 // launchers
+void launch_gather_stats(const Stats *const *src, Stats *dst, int n, hipStream_t s) {
+  k_gather_stats<<<(n + 255) / 256, 256, 0, s>>>(src, dst, n);
+}
 void launch_load(const LoadDesc *d, Img *dst, int nimg, long maxn, hipStream_t s) {
   int blocks = (int)((maxn + 255) / 256);
   if (blocks > 4096) blocks = 4096;

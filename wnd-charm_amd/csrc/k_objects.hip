@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(1024) k_objects(const Img *imgs, const Stats *
   va = block_reduce(va, SumOp(), shd);
   vd = block_reduce(vd, SumOp(), shd);
   if (t == 0) {
-    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    double *out = o.rows[blockIdx.x];
     int k = 0;
     for (int j = 0; j < 10; j++) out[k++] = hist[j];
     out[k++] = amax;
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(256) k_edge_final(const Img *mag, const Img *d
       sum += (DH[b] + DH[b + 4]);
     }
     if (sum > 0) homog = (DH[0] + DH[1]) / sum;
-    double *c = o.out + (long)blockIdx.x * o.stride + o.off;
+    double *c = o.rows[blockIdx.x];
     int h = 0;
     c[h++] = (double)area;
     for (int j = 0; j < 4; j++) c[h++] = diff[j];
@@ -389,7 +389,7 @@ __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *co
   const Img im = imgs[blockIdx.x];
   const long n = (long)im.h * im.w;
   const double *E0 = energy[blockIdx.x];
-  double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+  double *out = o.rows[blockIdx.x];
   long long orig = 0;
   for (int f = 0; f < 8; f++) {
     const double *E = E0 + (long)f * n;

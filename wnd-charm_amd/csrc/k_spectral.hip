@@ -117,7 +117,7 @@ __global__ void k_zern_final(const Img *imgs, const double *partial, int nchunk,
   }
   ar *= ar;
   ai *= ai;
-  o.out[(long)blockIdx.x * o.stride + o.off + e] = fabs(sqrt(ar + ai));
+  o.rows[blockIdx.x][e] = fabs(sqrt(ar + ai));
 }
 
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) 
     for (int k = threadIdx.x; k < NN2; k += blockDim.x) atomicAdd(&bins[hist_bin(coeff[k], mn, hs, 32)], 1);
   }
   __syncthreads();
-  if (threadIdx.x < 32) o.out[(long)blockIdx.x * o.stride + o.off + threadIdx.x] = bins[threadIdx.x];
+  if (threadIdx.x < 32) o.rows[blockIdx.x][threadIdx.x] = bins[threadIdx.x];
 }
 
 void launch_cf(const CfDesc *cd, int nimg, int maxchunk, OutRef o, hipStream_t s) {

@@ -219,8 +219,16 @@ __device__ void make_hh(const double *x, int m, double *ess, double *tau, double
 // in place) followed by the Francis QR on the active windows.  Only the
 // values of index 0..2 of the Schur diagonal are consumed (f14 = sqrt of
 // index 2 when that eigenvalue is real).
-__global__ void __launch_bounds__(256) k_har_eig(const HarDesc *hd) {
+constexpr int QB = 32;       // bulge-chase steps per LDS window
+constexpr int QW = QB + 4;   // window side
+constexpr int HT = 32;       // Hessenberg: rows per left-dot tile (HT x 256)
+constexpr int HC = 32;       // Hessenberg: columns per right-dot tile (256 x HC)
+__global__ void __launch_bounds__(256) k_har_hess(const HarDesc *hd) {
   __shared__ double ess[256];
+  __shared__ double tv[256];
+  __shared__ double tile[HT * 256];
+  __shared__ double Rw[QW * QW];
+  __shared__ double rf[QB][4];
   __shared__ double sc[8];
   __shared__ int si[8];
   const int angle = blockIdx.x;
@@ -229,14 +237,21 @@ __global__ void __launch_bounds__(256) k_har_eig(const HarDesc *hd) {
   double *A = H.Q + (long)angle * n * n;
   const int t = threadIdx.x;
 #define AT(r, c) A[(long)(r) * n + (c)]
-  // ---- Hessenberg (HessenbergDecomposition::_compute)
+  // ---- Hessenberg (HessenbergDecomposition::_compute).  Per column i the
+  // reflector comes from one thread's sequential sums; the two dot-product
+  // passes stage row / column tiles of the trailing block in LDS so every
+  // thread still accumulates its own column (left) or row (right) in the
+  // reference's order while the loads stay coalesced; the rank-1 updates are
+  // element-parallel.
+  double *colbuf = tile;  // reuse the tile buffer for the reflector column
   for (int i = 0; i < n - 1; ++i) {
     const int rem = n - i - 1;
+    for (int k = t; k < rem; k += blockDim.x) colbuf[k] = AT(i + 1 + k, i);
+    __syncthreads();
     if (t == 0) {
-      // column i below the diagonal -> reflector
       double tail = 0;
-      for (int k = 1; k < rem; k++) { const double v = AT(i + 1 + k, i); tail += v * v; }
-      const double c0 = AT(i + 1, i);
+      for (int k = 1; k < rem; k++) tail += colbuf[k] * colbuf[k];
+      const double c0 = colbuf[0];
       double tau, beta;
       if (tail == 0.0) {
         tau = 0; beta = c0;
@@ -244,37 +259,86 @@ __global__ void __launch_bounds__(256) k_har_eig(const HarDesc *hd) {
       } else {
         double b = sqrt(c0 * c0 + tail);
         if (c0 >= 0) b = -b;
-        for (int k = 1; k < rem; k++) ess[k - 1] = AT(i + 1 + k, i) / (c0 - b);
+        for (int k = 1; k < rem; k++) ess[k - 1] = colbuf[k] / (c0 - b);
         tau = (b - c0) / b;
         beta = b;
       }
-      AT(i + 1, i) = beta;
-      for (int k = 1; k < rem; k++) AT(i + 1 + k, i) = 0.0;  // matrixH() keeps only the Hessenberg part
       sc[0] = tau;
+      sc[1] = beta;
     }
     __syncthreads();
     const double tau = sc[0];
-    // A = H A on rows i+1..n-1, columns i+1..n-1
-    for (int c = i + 1 + t; c < n; c += blockDim.x) {
-      if (rem == 1) { AT(i + 1, c) *= (1 - tau); continue; }
-      double tt = 0;
-      for (int r = 1; r < rem; r++) tt += ess[r - 1] * AT(i + 1 + r, c);
-      tt += AT(i + 1, c);
-      AT(i + 1, c) -= tau * tt;
-      for (int r = 1; r < rem; r++) AT(i + 1 + r, c) -= (tau * ess[r - 1]) * tt;
+    if (t == 0) AT(i + 1, i) = sc[1];
+    for (int k = 1 + t; k < rem; k += blockDim.x) AT(i + 1 + k, i) = 0.0;  // matrixH(): Hessenberg part only
+    if (rem == 1) {
+      if (t == 0) AT(i + 1, i + 1) *= (1 - tau);
+      __syncthreads();
+      for (int r = t; r < n; r += blockDim.x) AT(r, i + 1) *= (1 - tau);
+      __syncthreads();
+      continue;
+    }
+    // A = H A on rows i+1..n-1, columns i+1..n-1: thread t owns column i+1+t
+    const int c = i + 1 + t;
+    double acc = 0;
+    for (int rt = 1; rt < rem; rt += HT) {
+      const int nr = (rem - rt < HT) ? rem - rt : HT;
+      for (int e = t; e < nr * rem; e += blockDim.x) {
+        const int rr = e / rem, cc = e - rr * rem;
+        tile[rr * 256 + cc] = AT(i + 1 + rt + rr, i + 1 + cc);
+      }
+      __syncthreads();
+      if (t < rem)
+        for (int rr = 0; rr < nr; rr++) acc += ess[rt + rr - 1] * tile[rr * 256 + t];
+      __syncthreads();
+    }
+    if (t < rem) tv[t] = acc + AT(i + 1, c);
+    __syncthreads();
+    for (int e = t; e < rem * rem; e += blockDim.x) {
+      const int rr = e / rem, cc = e - rr * rem;
+      const double tt = tv[cc];
+      if (rr == 0) AT(i + 1, i + 1 + cc) -= tau * tt;
+      else AT(i + 1 + rr, i + 1 + cc) -= (tau * ess[rr - 1]) * tt;
     }
     __syncthreads();
-    // A = A H on all rows, columns i+1..n-1
-    for (int r = t; r < n; r += blockDim.x) {
-      if (rem == 1) { AT(r, i + 1) *= (1 - tau); continue; }
-      double tt = 0;
-      for (int c = 1; c < rem; c++) tt += AT(r, i + 1 + c) * ess[c - 1];
-      tt += AT(r, i + 1);
-      AT(r, i + 1) -= tau * tt;
-      for (int c = 1; c < rem; c++) AT(r, i + 1 + c) -= (tau * tt) * ess[c - 1];
+    // A = A H on rows 0..n-1, columns i+1..n-1: thread t owns row t
+    double acc2 = 0;
+    for (int ct = 1; ct < rem; ct += HC) {
+      const int ncol = (rem - ct < HC) ? rem - ct : HC;
+      for (int e = t; e < n * ncol; e += blockDim.x) {
+        const int r = e / ncol, cc = e - r * ncol;
+        tile[r * HC + cc] = AT(r, i + 1 + ct + cc);
+      }
+      __syncthreads();
+      if (t < n)
+        for (int cc = 0; cc < ncol; cc++) acc2 += tile[t * HC + cc] * ess[ct + cc - 1];
+      __syncthreads();
+    }
+    if (t < n) tv[t] = acc2 + AT(t, i + 1);
+    __syncthreads();
+    for (int e = t; e < n * rem; e += blockDim.x) {
+      const int r = e / rem, cc = e - r * rem;
+      const double tt = tv[r];
+      if (cc == 0) AT(r, i + 1) -= tau * tt;
+      else AT(r, i + 1 + cc) -= (tau * tt) * ess[cc - 1];
     }
     __syncthreads();
   }
+#undef AT
+}
+
+// Francis QR on the Hessenberg matrix left in H.Q by k_har_hess.
+__global__ void __launch_bounds__(256) k_har_qr(const HarDesc *hd) {
+  __shared__ double ess[256];
+  __shared__ double Rw[QW * QW];
+  __shared__ double rf[QB][4];
+  __shared__ double sc[8];
+  __shared__ int si[8];
+  const int angle = blockIdx.x;
+  const HarDesc H = hd[blockIdx.y];
+  const int n = *H.ng;
+  double *A = H.Q + (long)angle * n * n;
+  const int t = threadIdx.x;
+#define AT(r, c) A[(long)(r) * n + (c)]
   // ---- real Schur form (RealSchur::compute without U); left updates are
   // restricted to columns <= iu (entries right of the active window are
   // never read again), right updates cover rows 0..min(iu,k+3) as in Eigen.
@@ -433,42 +497,110 @@ __global__ void __launch_bounds__(256) k_har_eig(const HarDesc *hd) {
     const int im = si[2];
     const double fv0 = sc[0], fv1 = sc[1], fv2 = sc[2];
     __syncthreads();
-    for (int k = im; k <= iuq - 2; ++k) {
-      if (t == 0) {
-        const bool first = (k == im);
-        double x[3];
-        if (first) { x[0] = fv0; x[1] = fv1; x[2] = fv2; }
-        else { x[0] = AT(k, k - 1); x[1] = AT(k + 1, k - 1); x[2] = AT(k + 2, k - 1); }
-        double e2[2], tau, beta;
-        make_hh(x, 3, e2, &tau, &beta);
-        si[6] = beta != 0;
-        if (beta != 0) {
-          if (first && k > il) AT(k, k - 1) = -AT(k, k - 1);
-          else if (!first) AT(k, k - 1) = beta;
-        }
-        sc[3] = tau; ess[0] = e2[0]; ess[1] = e2[1];
+    // Bulge chase in windows of QB steps.  Window steps [k0, k1) read and
+    // update only entries inside R = [a, b) x [a, b) (a = k0-1, b = k1+3),
+    // which live in LDS while the window runs; afterwards the window's
+    // reflectors are applied, in step order, to the strips outside R
+    // (rows k0..k1+1 right of R: left reflectors only; columns k0..k1+1
+    // above R: right reflectors only).  Every entry therefore receives the
+    // same reflectors in the same order as in the unwindowed chase.
+    for (int k0 = im; k0 <= iuq - 2; k0 += QB) {
+      const int k1 = (k0 + QB < iuq - 1) ? k0 + QB : iuq - 1;
+      const int a = k0 - 1 > 0 ? k0 - 1 : 0;
+      const int b = (k1 + 3 < iuq + 1) ? k1 + 3 : iuq + 1;
+      const int wd = b - a;
+      for (int e = t; e < wd * wd; e += blockDim.x) {
+        const int r = e / wd, c = e - r * wd;
+        Rw[r * QW + c] = AT(a + r, a + c);
       }
       __syncthreads();
-      if (si[6]) {
-        const double tau = sc[3], e0 = ess[0], e1 = ess[1];
-        for (int c = k + t; c <= iuq; c += blockDim.x) {
-          double tt = e0 * AT(k + 1, c);
-          tt += e1 * AT(k + 2, c);
-          tt += AT(k, c);
-          AT(k, c) -= tau * tt;
-          AT(k + 1, c) -= (tau * e0) * tt;
-          AT(k + 2, c) -= (tau * e1) * tt;
+#define RW(r, c) Rw[((r) - a) * QW + ((c) - a)]
+      for (int k = k0; k < k1; ++k) {
+        if (t == 0) {
+          const bool first = (k == im);
+          double x[3];
+          if (first) { x[0] = fv0; x[1] = fv1; x[2] = fv2; }
+          else { x[0] = RW(k, k - 1); x[1] = RW(k + 1, k - 1); x[2] = RW(k + 2, k - 1); }
+          double e2[2], tau, beta;
+          make_hh(x, 3, e2, &tau, &beta);
+          if (beta != 0) {
+            if (first && k > il) RW(k, k - 1) = -RW(k, k - 1);
+            else if (!first) RW(k, k - 1) = beta;
+          }
+          rf[k - k0][0] = beta != 0 ? tau : 0.0;
+          rf[k - k0][1] = e2[0];
+          rf[k - k0][2] = e2[1];
+          rf[k - k0][3] = beta != 0 ? 1.0 : 0.0;
         }
         __syncthreads();
-        const int rlast = (iuq < k + 3 ? iuq : k + 3);
-        for (int r = t; r <= rlast; r += blockDim.x) {
-          double tt = AT(r, k + 1) * e0;
-          tt += AT(r, k + 2) * e1;
-          tt += AT(r, k);
-          AT(r, k) -= tau * tt;
-          AT(r, k + 1) -= (tau * tt) * e0;
-          AT(r, k + 2) -= (tau * tt) * e1;
+        if (rf[k - k0][3] != 0.0) {
+          const double tau = rf[k - k0][0], e0 = rf[k - k0][1], e1 = rf[k - k0][2];
+          for (int c = k + t; c < b; c += blockDim.x) {
+            double tt = e0 * RW(k + 1, c);
+            tt += e1 * RW(k + 2, c);
+            tt += RW(k, c);
+            RW(k, c) -= tau * tt;
+            RW(k + 1, c) -= (tau * e0) * tt;
+            RW(k + 2, c) -= (tau * e1) * tt;
+          }
+          __syncthreads();
+          const int rlast = (iuq < k + 3 ? iuq : k + 3);
+          for (int r = a + t; r <= rlast; r += blockDim.x) {
+            double tt = RW(r, k + 1) * e0;
+            tt += RW(r, k + 2) * e1;
+            tt += RW(r, k);
+            RW(r, k) -= tau * tt;
+            RW(r, k + 1) -= (tau * tt) * e0;
+            RW(r, k + 2) -= (tau * tt) * e1;
+          }
         }
+        __syncthreads();
+      }
+      for (int e = t; e < wd * wd; e += blockDim.x) {
+        const int r = e / wd, c = e - r * wd;
+        AT(a + r, a + c) = Rw[r * QW + c];
+      }
+#undef RW
+      // strip right of R: rows k0 .. k1+1, columns b .. iuq (left reflectors)
+      const int nrow = k1 + 2 - k0;
+      for (int c = b + t; c <= iuq; c += blockDim.x) {
+        double v[QB + 2];
+#pragma unroll
+        for (int q = 0; q < QB + 2; q++) v[q] = (q < nrow) ? AT(k0 + q, c) : 0.0;
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+          if (k0 + q < k1 && rf[q][3] != 0.0) {
+            const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+            double tt = e0 * v[q + 1];
+            tt += e1 * v[q + 2];
+            tt += v[q];
+            v[q] -= tau * tt;
+            v[q + 1] -= (tau * e0) * tt;
+            v[q + 2] -= (tau * e1) * tt;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < QB + 2; q++) if (q < nrow) AT(k0 + q, c) = v[q];
+      }
+      // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors)
+      for (int r = t; r < a; r += blockDim.x) {
+        double v[QB + 2];
+#pragma unroll
+        for (int q = 0; q < QB + 2; q++) v[q] = (q < nrow) ? AT(r, k0 + q) : 0.0;
+#pragma unroll
+        for (int q = 0; q < QB; q++) {
+          if (k0 + q < k1 && rf[q][3] != 0.0) {
+            const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+            double tt = v[q + 1] * e0;
+            tt += v[q + 2] * e1;
+            tt += v[q];
+            v[q] -= tau * tt;
+            v[q + 1] -= (tau * tt) * e0;
+            v[q + 2] -= (tau * tt) * e1;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < QB + 2; q++) if (q < nrow) AT(r, k0 + q) = v[q];
       }
       __syncthreads();
     }
@@ -548,7 +680,7 @@ __global__ void k_har_final(const HarDesc *hd, OutRef o) {
   }
   const int ord[28] = {0, 14, 1, 15, 2, 16, 9, 23, 10, 24, 8, 22, 11, 25,
                        4, 18, 13, 27, 12, 26, 5, 19, 7, 21, 6, 20, 3, 17};
-  double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+  double *out = o.rows[blockIdx.x];
   for (int a = 0; a < 28; a++) out[a] = temp[ord[a]];
 }
 
@@ -563,7 +695,8 @@ void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef
   (void)lds_ok;
   k_har_cooc<<<dim3(4, nimg), 1024, 256 * 257 / 2 * sizeof(uint32_t), s>>>(imgs, hd);
   k_har_stats<<<dim3(4, nimg), 256, 0, s>>>(hd);
-  k_har_eig<<<dim3(4, nimg), 256, 0, s>>>(hd);
+  k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
+  k_har_qr<<<dim3(4, nimg), 256, 0, s>>>(hd);
   k_har_final<<<nimg, 64, 0, s>>>(hd, o);
 }
 
@@ -744,7 +877,7 @@ __global__ void __launch_bounds__(1024) k_tam_final(const Img *imgs, double *con
   }
   __syncthreads();
   if (t == 0) {
-    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    double *out = o.rows[blockIdx.x];
     // coarseness (tamura.cpp:240-262); the divisor is unsigned arithmetic
     const unsigned xD = W, yD = H;
     double sum = (double)ks;
@@ -850,7 +983,7 @@ __global__ void __launch_bounds__(128) k_comb(const Img *imgs, const double *iis
     const int ord[48] = {45, 46, 47, 36, 37, 38, 42, 43, 44, 39, 40, 41, 33, 34, 35, 24,
                          25, 26, 30, 31, 32, 27, 28, 29, 9, 10, 11, 0, 1, 2, 6, 7,
                          8, 3, 4, 5, 21, 22, 23, 12, 13, 14, 18, 19, 20, 15, 16, 17};
-    double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+    double *out = o.rows[blockIdx.x];
     for (int a = 0; a < 48; a++) out[a] = vec[ord[a]];
   }
 }
@@ -874,44 +1007,65 @@ __device__ __forceinline__ double radon_ridx(int n, int m, int M, int N, double 
   return xc + ys;
 }
 
+// Block = (angle, image).  The x*cos and y*sin tables of the reference
+// (radon.cpp:58-63) are built in LDS with the reference's operations, so
+// rIdx = xC[n] + yS[m] is bit-identical; each thread then gathers its bins.
 __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const double *trig, double *const *proj) {
-  const Img im = imgs[blockIdx.z];
+  extern __shared__ double tabs[];
+  const Img im = imgs[blockIdx.y];
   const int M = im.h, N = im.w;
   const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
                                    (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
   const int rFirst = -rLast, rSize = rLast - rFirst + 1;
-  const int a = blockIdx.y;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= rSize) return;
+  const int a = blockIdx.x;
   const double cs = trig[2 * a], sn = trig[2 * a + 1];
-  double acc = 0.0;
-  const int M2 = 2 * M;
-  for (int nn = 0; nn < 2 * N; nn++) {
-    const double *col = im.p + nn / 2;
-    // floor(rIdx) at the ends of the column
-    const int fl0 = (int)radon_ridx(nn, 0, M, N, cs, sn, rFirst);
-    const int fl1 = (int)radon_ridx(nn, M2 - 1, M, N, cs, sn, rFirst);
-    if (fl0 < r - 1 || fl1 > r) continue;
-    // first m with floor <= r
-    int lo = 0, hi = M2;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)radon_ridx(nn, mid, M, N, cs, sn, rFirst) <= r) hi = mid; else lo = mid + 1; }
-    const int m0 = lo;
-    lo = m0; hi = M2;
-    while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)radon_ridx(nn, mid, M, N, cs, sn, rFirst) < r - 1) hi = mid; else lo = mid + 1; }
-    const int m1 = lo;
-    for (int mm = m0; mm < m1; mm++) {
-      double pixel = col[(long)(mm / 2) * N];
-      if (pixel) {
-        pixel *= 0.25;
-        const double rIdx = radon_ridx(nn, mm, M, N, cs, sn, rFirst);
-        const int rLow = (int)rIdx;
-        const double pixelLow = pixel * (1 - rIdx + rLow);
-        if (rLow == r) acc += pixelLow;
-        else acc += pixel - pixelLow;
+  const int xO = (N - 1) / 2, yO = (M - 1) / 2;
+  const int N2 = 2 * N, M2 = 2 * M;
+  double *xC = tabs, *yS = tabs + N2;
+  for (int p = threadIdx.x; p < N2; p += blockDim.x) xC[p] = ((-xO - 0.25) + 0.5 * p) * cs - rFirst;
+  for (int p = threadIdx.x; p < M2; p += blockDim.x) yS[p] = ((-yO - 0.25) + 0.5 * (M2 - 1 - p)) * sn;
+  __syncthreads();
+  double *out = proj[blockIdx.y] + (long)a * rSize;
+  for (int r = threadIdx.x; r < rSize; r += blockDim.x) {
+    double acc = 0.0;
+    // [m0, m1): sub-rows of column nn whose floor(rIdx) is r or r-1; floor is
+    // non-increasing in m, and the range moves slowly with nn, so it is
+    // tracked by stepping (binary search after a gap).
+    int m0 = 0, m1 = 0;
+    bool valid = false;
+    for (int nn = 0; nn < N2; nn++) {
+      const double xc = xC[nn];
+      if ((int)(xc + yS[0]) < r - 1 || (int)(xc + yS[M2 - 1]) > r) { valid = false; continue; }
+      if (!valid) {
+        int lo = 0, hi = M2;
+        while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)(xc + yS[mid]) <= r) hi = mid; else lo = mid + 1; }
+        m0 = lo;
+        lo = m0; hi = M2;
+        while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)(xc + yS[mid]) < r - 1) hi = mid; else lo = mid + 1; }
+        m1 = lo;
+        valid = true;
+      } else {
+        while (m0 > 0 && (int)(xc + yS[m0 - 1]) <= r) m0--;
+        while (m0 < M2 && (int)(xc + yS[m0]) > r) m0++;
+        if (m1 < m0) m1 = m0;
+        while (m1 > m0 && (int)(xc + yS[m1 - 1]) < r - 1) m1--;
+        while (m1 < M2 && (int)(xc + yS[m1]) >= r - 1) m1++;
+      }
+      const double *col = im.p + nn / 2;
+      for (int mm = m0; mm < m1; mm++) {
+        double pixel = col[(long)(mm / 2) * N];
+        if (pixel) {
+          pixel *= 0.25;
+          const double rIdx = (xc + yS[mm]);
+          const int rLow = (int)rIdx;
+          const double pixelLow = pixel * (1 - rIdx + rLow);
+          if (rLow == r) acc += pixelLow;
+          else acc += pixel - pixelLow;
+        }
       }
     }
+    out[r] = acc;
   }
-  proj[blockIdx.z][(long)a * rSize + r] = acc;
 }
 
 // yTable / xTable entries are exact multiples of 0.25, so the closed forms
@@ -935,13 +1089,17 @@ __global__ void k_radon_hist(const Img *imgs, double *const *proj, OutRef o) {
     for (int b = 0; b < 3; b++) v[vi++] = bins[b];
   }
   const int ord[12] = {0, 1, 2, 9, 10, 11, 3, 4, 5, 6, 7, 8};
-  double *out = o.out + (long)blockIdx.x * o.stride + o.off;
+  double *out = o.rows[blockIdx.x];
   for (int a = 0; a < 12; a++) out[a] = v[ord[a]];
 }
 
 void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
-                  hipStream_t s) {
-  k_radon_proj<<<dim3((maxbins + 255) / 256, 4, nimg), 256, 0, s>>>(imgs, trig, proj);
+                  int maxh, int maxw, hipStream_t s) {
+  const size_t radon_lds = (2 * (size_t)maxw + 2 * (size_t)maxh) * sizeof(double);
+  if (radon_lds > 65536)
+    (void)hipFuncSetAttribute((const void *)k_radon_proj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)radon_lds);
+  (void)maxbins;
+  k_radon_proj<<<dim3(4, nimg), 256, radon_lds, s>>>(imgs, trig, proj);
   k_radon_hist<<<nimg, 64, 0, s>>>(imgs, proj, o);
 }
 

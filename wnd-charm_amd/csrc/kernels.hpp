@@ -51,6 +51,7 @@ struct DsDesc {  // area-weighted Downsample (cmatrix.cpp:579-720)
 
 void launch_load(const LoadDesc *d, Img *dst, int nimg, long maxn, hipStream_t s);
 void launch_stats(const Img *imgs, Stats *st, int nimg, hipStream_t s);
+void launch_gather_stats(const Stats *const *src, Stats *dst, int n, hipStream_t s);
 void launch_edge(const Img *in, const Img *out, int nimg, long maxn, hipStream_t s);
 void launch_wavelet(const Img *in, double *const *g, const int *gstride, const Img *out, int nimg, int maxh,
                     int maxw, hipStream_t s);
@@ -82,7 +83,7 @@ void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, Out
                    hipStream_t s);
 void launch_comb(const Img *imgs, const double *iis, OutRef o, int nimg, hipStream_t s);
 void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
-                  hipStream_t s);
+                  int maxh, int maxw, hipStream_t s);
 
 // objects / edges / gabor / zernike / chebyshev-fourier (k_objects.hip, k_spectral.hip)
 struct ObjDesc {
