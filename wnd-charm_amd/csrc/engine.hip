@@ -568,6 +568,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           H.P = A.dalloc_n<double>(4 * 256 * 256);
           H.Q = A.dalloc_n<double>(4 * 256 * 256);
           H.feat = A.dalloc_n<double>(4 * 14);
+          H.marg = A.dalloc_n<double>(4 * 2 * 256);
           H.ng = A.dalloc_n<int>(1);
           H.lut = lut_all + 256 * k;
           H.count = A.dalloc_n<int>(4);

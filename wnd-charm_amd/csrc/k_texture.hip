@@ -13,6 +13,15 @@
 
 namespace wnd {
 
+// Memory fence for lanes of one wavefront exchanging values through LDS:
+// a wave's LDS operations execute in issue order, so only compiler
+// reordering has to be prevented.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ===================================================================== Haralick
 // quantisation (haralick.cpp:29-37) and tone presence
 __global__ void k_har_quant(const Img *imgs, const Stats *st, const HarDesc *hd) {
@@ -130,8 +139,9 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     mean4 += i * p;
     idm += p / (1 + (i - j) * (i - j));
     ent += p * log10(p + EPS) / LOG2;
-    hxy1 -= p * log10(px[i] * py[j] + EPS) / LOG2;
-    hxy2 -= px[i] * py[j] * log10(px[i] * py[j] + EPS) / LOG2;
+    const double lg = log10(px[i] * py[j] + EPS);
+    hxy1 -= p * lg / LOG2;
+    hxy2 -= px[i] * py[j] * lg / LOG2;
   }
   f1 = block_reduce(f1, SumOp(), shd);
   tmp3 = block_reduce(tmp3, SumOp(), shd);
@@ -186,14 +196,63 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     F[11] = (mxh == 0) ? 1.0 : (hxy - hxy1) / mxh;
     F[12] = sqrt(fabs(1 - exp(-2.0 * (hxy2 - hxy))));
   }
-  // Q(i,j) = sum_k (P[i][k]*P[j][k]) / (px[i]*py[k] + 1e-13), sequential in k
+  if (t < ng) {
+    H.marg[angle * 512 + t] = px[t];
+    H.marg[angle * 512 + 256 + t] = py[t];
+  }
+}
+
+// Q (_Q_calc_eigen, CVIPtexture.cpp:961-987):
+//   Q(i,j) = sum_k P(i,k) P(j,k) / (px(i) py(k) + 1e-13), accumulated in k order.
+// A term whose product is zero adds +0.0 and leaves the sum bit-identical,
+// so each row visits only the k with P(i,k) != 0 (compacted per wavefront)
+// and skips P(j,k) == 0.  P is symmetric (pairs are counted both ways), so
+// P(j,k) is read as P(k,j): along row k, coalesced over j.
+constexpr int QRB = 4;  // row blocks per angle
+__global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd) {
+  __shared__ int lk[4][256];
+  __shared__ double la[4][256], ld[4][256];
+  const int angle = blockIdx.x / QRB, rb = blockIdx.x - angle * QRB;
+  const HarDesc H = hd[blockIdx.y];
+  const int ng = *H.ng;
+  const double *P = H.P + (long)angle * ng * ng;
+  const double *px = H.marg + angle * 512, *py = px + 256;
   double *Q = H.Q + (long)angle * ng * ng;
-  for (int e = t; e < ng * ng; e += blockDim.x) {
-    const int i = e / ng, j = e - i * ng;
-    double s = 0;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = rb * 4 + w; i < ng; i += QRB * 4) {
     const double pxi = px[i];
-    for (int k = 0; k < ng; k++) s += (P[i * ng + k] * P[j * ng + k]) / (pxi * py[k] + 0.0000000000001);
-    Q[e] = s;
+    int cnt = 0;
+    for (int k0 = 0; k0 < ng; k0 += 64) {
+      const int k = k0 + lane;
+      const double a = k < ng ? P[(long)i * ng + k] : 0.0;
+      const unsigned long long m = __ballot(a != 0.0);
+      if (a != 0.0) {
+        const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
+        lk[w][pos] = k;
+        la[w][pos] = a;
+        ld[w][pos] = pxi * py[k] + 0.0000000000001;
+      }
+      cnt += __popcll(m);
+    }
+    wave_sync();
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const int j0 = lane, j1 = lane + 64, j2 = lane + 128, j3 = lane + 192;
+    for (int kk = 0; kk < cnt; kk++) {
+      const double *row = P + (long)lk[w][kk] * ng;
+      const double a = la[w][kk], d = ld[w][kk];
+      const double b0 = j0 < ng ? row[j0] : 0.0, b1 = j1 < ng ? row[j1] : 0.0;
+      const double b2 = j2 < ng ? row[j2] : 0.0, b3 = j3 < ng ? row[j3] : 0.0;
+      if (b0 != 0.0) s0 += (a * b0) / d;
+      if (b1 != 0.0) s1 += (a * b1) / d;
+      if (b2 != 0.0) s2 += (a * b2) / d;
+      if (b3 != 0.0) s3 += (a * b3) / d;
+    }
+    double *qr = Q + (long)i * ng;
+    if (j0 < ng) qr[j0] = s0;
+    if (j1 < ng) qr[j1] = s1;
+    if (j2 < ng) qr[j2] = s2;
+    if (j3 < ng) qr[j3] = s3;
+    wave_sync();  // the lists are rebuilt for the next row
   }
 }
 
@@ -216,60 +275,60 @@ __device__ void make_hh(const double *x, int m, double *ess, double *tau, double
 }
 
 // One block per (angle, image): Hessenberg reduction of Q (n x n, row-major,
-// in place) followed by the Francis QR on the active windows.  Only the
+// in place), then (k_har_qr) the Francis QR on the active windows.  Only the
 // values of index 0..2 of the Schur diagonal are consumed (f14 = sqrt of
 // index 2 when that eigenvalue is real).
-constexpr int QB = 32;       // bulge-chase steps per LDS window
-constexpr int QW = QB + 4;   // window side
-constexpr int HT = 32;       // Hessenberg: rows per left-dot tile (HT x 256)
-constexpr int HC = 32;       // Hessenberg: columns per right-dot tile (256 x HC)
-__global__ void __launch_bounds__(256) k_har_hess(const HarDesc *hd) {
+constexpr int QB = 60;       // bulge-chase steps per LDS window
+constexpr int QW = QB + 4;   // window side (one wavefront lane per row/column)
+constexpr int QWP = QW + 1;  // padded LDS row of the window
+constexpr int HC = 16;       // Hessenberg: columns per right-dot tile (n x HC)
+constexpr int HCP = HC + 1;  // padded tile row (bank-conflict free row reads)
+constexpr int HB = 8;        // independent loads in flight per thread
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_har_hess(const HarDesc *hd) {
   __shared__ double ess[256];
   __shared__ double tv[256];
-  __shared__ double tile[HT * 256];
-  __shared__ double Rw[QW * QW];
-  __shared__ double rf[QB][4];
-  __shared__ double sc[8];
-  __shared__ int si[8];
+  __shared__ double tile[256 * HCP];
+  __shared__ double sc[4];
   const int angle = blockIdx.x;
   const HarDesc H = hd[blockIdx.y];
   const int n = *H.ng;
   double *A = H.Q + (long)angle * n * n;
   const int t = threadIdx.x;
 #define AT(r, c) A[(long)(r) * n + (c)]
-  // ---- Hessenberg (HessenbergDecomposition::_compute).  Per column i the
-  // reflector comes from one thread's sequential sums; the two dot-product
-  // passes stage row / column tiles of the trailing block in LDS so every
-  // thread still accumulates its own column (left) or row (right) in the
-  // reference's order while the loads stay coalesced; the rank-1 updates are
-  // element-parallel.
-  double *colbuf = tile;  // reuse the tile buffer for the reflector column
+  // HessenbergDecomposition::_compute, column by column.  Per column i:
+  // the reflector norm is one thread's sequential sum, the essential part
+  // is element-parallel; A = H A has thread c own column c (dot over rows in
+  // order, then the rank-1 update of the same column); A = A H stages
+  // n x HC column tiles in LDS so thread r accumulates row r in column order.
   for (int i = 0; i < n - 1; ++i) {
     const int rem = n - i - 1;
-    for (int k = t; k < rem; k += blockDim.x) colbuf[k] = AT(i + 1 + k, i);
+    for (int k = t; k < rem; k += blockDim.x) tile[k] = AT(i + 1 + k, i);
     __syncthreads();
     if (t == 0) {
       double tail = 0;
-      for (int k = 1; k < rem; k++) tail += colbuf[k] * colbuf[k];
-      const double c0 = colbuf[0];
-      double tau, beta;
+      for (int k = 1; k < rem; k++) tail += tile[k] * tile[k];
+      const double c0 = tile[0];
       if (tail == 0.0) {
-        tau = 0; beta = c0;
-        for (int k = 1; k < rem; k++) ess[k - 1] = 0;
+        sc[0] = 0; sc[1] = c0; sc[2] = 0; sc[3] = 0;
       } else {
         double b = sqrt(c0 * c0 + tail);
         if (c0 >= 0) b = -b;
-        for (int k = 1; k < rem; k++) ess[k - 1] = colbuf[k] / (c0 - b);
-        tau = (b - c0) / b;
-        beta = b;
+        sc[0] = (b - c0) / b;
+        sc[1] = b;
+        sc[2] = c0 - b;
+        sc[3] = 1;
       }
-      sc[0] = tau;
-      sc[1] = beta;
     }
     __syncthreads();
     const double tau = sc[0];
+    {
+      const bool nz = sc[3] != 0;
+      const double den = sc[2];
+      for (int k = 1 + t; k < rem; k += blockDim.x) ess[k - 1] = nz ? tile[k] / den : 0.0;
+    }
     if (t == 0) AT(i + 1, i) = sc[1];
     for (int k = 1 + t; k < rem; k += blockDim.x) AT(i + 1 + k, i) = 0.0;  // matrixH(): Hessenberg part only
+    __syncthreads();
     if (rem == 1) {
       if (t == 0) AT(i + 1, i + 1) *= (1 - tau);
       __syncthreads();
@@ -277,49 +336,62 @@ __global__ void __launch_bounds__(256) k_har_hess(const HarDesc *hd) {
       __syncthreads();
       continue;
     }
-    // A = H A on rows i+1..n-1, columns i+1..n-1: thread t owns column i+1+t
-    const int c = i + 1 + t;
-    double acc = 0;
-    for (int rt = 1; rt < rem; rt += HT) {
-      const int nr = (rem - rt < HT) ? rem - rt : HT;
-      for (int e = t; e < nr * rem; e += blockDim.x) {
-        const int rr = e / rem, cc = e - rr * rem;
-        tile[rr * 256 + cc] = AT(i + 1 + rt + rr, i + 1 + cc);
+    // A = H A on rows i+1..n-1, columns i+1..n-1 (HB loads in flight per thread)
+    for (int c = i + 1 + t; c < n; c += blockDim.x) {
+      const double *p = &AT(i + 2, c);
+      const int m = rem - 1;
+      double acc = 0;
+      int r = 0;
+      for (; r + HB <= m; r += HB) {
+        double v[HB];
+#pragma unroll
+        for (int j = 0; j < HB; j++) v[j] = p[(long)(r + j) * n];
+#pragma unroll
+        for (int j = 0; j < HB; j++) acc += ess[r + j] * v[j];
       }
-      __syncthreads();
-      if (t < rem)
-        for (int rr = 0; rr < nr; rr++) acc += ess[rt + rr - 1] * tile[rr * 256 + t];
-      __syncthreads();
+      for (; r < m; r++) acc += ess[r] * p[(long)r * n];
+      const double tt = acc + AT(i + 1, c);
+      AT(i + 1, c) -= tau * tt;
+      double *q = &AT(i + 2, c);
+      for (r = 0; r + HB <= m; r += HB) {
+        double v[HB];
+#pragma unroll
+        for (int j = 0; j < HB; j++) v[j] = q[(long)(r + j) * n];
+#pragma unroll
+        for (int j = 0; j < HB; j++) q[(long)(r + j) * n] = v[j] - (tau * ess[r + j]) * tt;
+      }
+      for (; r < m; r++) q[(long)r * n] -= (tau * ess[r]) * tt;
     }
-    if (t < rem) tv[t] = acc + AT(i + 1, c);
     __syncthreads();
-    for (int e = t; e < rem * rem; e += blockDim.x) {
-      const int rr = e / rem, cc = e - rr * rem;
-      const double tt = tv[cc];
-      if (rr == 0) AT(i + 1, i + 1 + cc) -= tau * tt;
-      else AT(i + 1 + rr, i + 1 + cc) -= (tau * ess[rr - 1]) * tt;
-    }
-    __syncthreads();
-    // A = A H on rows 0..n-1, columns i+1..n-1: thread t owns row t
+    // A = A H on rows 0..n-1, columns i+1..n-1
     double acc2 = 0;
+    const int lc = t & (HC - 1), lr = t / HC;
     for (int ct = 1; ct < rem; ct += HC) {
       const int ncol = (rem - ct < HC) ? rem - ct : HC;
-      for (int e = t; e < n * ncol; e += blockDim.x) {
-        const int r = e / ncol, cc = e - r * ncol;
-        tile[r * HC + cc] = AT(r, i + 1 + ct + cc);
+      if (lc < ncol) {
+#pragma unroll 16
+        for (int r = lr; r < n; r += 256 / HC) tile[r * HCP + lc] = AT(r, i + 1 + ct + lc);
       }
       __syncthreads();
       if (t < n)
-        for (int cc = 0; cc < ncol; cc++) acc2 += tile[t * HC + cc] * ess[ct + cc - 1];
+        for (int cc = 0; cc < ncol; cc++) acc2 += tile[t * HCP + cc] * ess[ct + cc - 1];
       __syncthreads();
     }
     if (t < n) tv[t] = acc2 + AT(t, i + 1);
     __syncthreads();
-    for (int e = t; e < n * rem; e += blockDim.x) {
-      const int r = e / rem, cc = e - r * rem;
-      const double tt = tv[r];
-      if (cc == 0) AT(r, i + 1) -= tau * tt;
-      else AT(r, i + 1 + cc) -= (tau * tt) * ess[cc - 1];
+    // rank-1 update, thread per column (column i+1 has factor 1: x*1.0 is exact)
+    for (int cc = t; cc < rem; cc += blockDim.x) {
+      double *p = &AT(0, i + 1 + cc);
+      const double f = cc == 0 ? 1.0 : ess[cc - 1];
+      int r = 0;
+      for (; r + HB <= n; r += HB) {
+        double v[HB];
+#pragma unroll
+        for (int j = 0; j < HB; j++) v[j] = p[(long)(r + j) * n];
+#pragma unroll
+        for (int j = 0; j < HB; j++) p[(long)(r + j) * n] = v[j] - (tau * tv[r + j]) * f;
+      }
+      for (; r < n; r++) p[(long)r * n] -= (tau * tv[r]) * f;
     }
     __syncthreads();
   }
@@ -327,12 +399,14 @@ __global__ void __launch_bounds__(256) k_har_hess(const HarDesc *hd) {
 }
 
 // Francis QR on the Hessenberg matrix left in H.Q by k_har_hess.
-__global__ void __launch_bounds__(256) k_har_qr(const HarDesc *hd) {
-  __shared__ double ess[256];
-  __shared__ double Rw[QW * QW];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_har_qr(const HarDesc *hd) {
+  __shared__ double Rw[QW * QWP];
   __shared__ double rf[QB][4];
   __shared__ double sc[8];
   __shared__ int si[8];
+  __shared__ int ired[8];
+  __shared__ double dred[8];
+  __shared__ int s_iu;
   const int angle = blockIdx.x;
   const HarDesc H = hd[blockIdx.y];
   const int n = *H.ng;
@@ -342,111 +416,110 @@ __global__ void __launch_bounds__(256) k_har_qr(const HarDesc *hd) {
   // ---- real Schur form (RealSchur::compute without U); left updates are
   // restricted to columns <= iu (entries right of the active window are
   // never read again), right updates cover rows 0..min(iu,k+3) as in Eigen.
-  __shared__ double normv;
+  double norm;
   {
     double s = 0;
     for (int e = t; e < n * n; e += blockDim.x) {
       const int r = e / n, c = e - r * n;
       if (c >= (r - 1 > 0 ? r - 1 : 0)) s += fabs(A[e]);
     }
-    s = block_reduce(s, SumOp(), sc);
-    if (t == 0) normv = s;
+    norm = block_reduce(s, SumOp(), dred);
   }
-  __syncthreads();
-  const double norm = normv;
   const double eps = kDblEps;
-  // decision state lives in thread 0; the block follows through si / sc
-  __shared__ int s_iu;
+  // decision state (iu, iteration counts, exceptional shift) lives in
+  // thread 0; the block follows through s_iu / si / sc
   int iu = n - 1, iter = 0, totalIter = 0;
   double exshift = 0;
   if (t == 0) s_iu = n - 1;
   __syncthreads();
   while (norm != 0) {
     __syncthreads();  // every thread has consumed the previous decision
+    const int iub = s_iu;
+    if (iub < 0) break;
+    // findSmallSubdiagEntry: the largest res in (0, iu] whose subdiagonal
+    // entry is negligible, else 0 -- every candidate tested at once
+    int cand = 0;
+    for (int res = iub - t; res > 0; res -= blockDim.x) {
+      double s = fabs(AT(res - 1, res - 1)) + fabs(AT(res, res));
+      if (s == 0.0) s = norm;
+      if (fabs(AT(res, res - 1)) < eps * s) { cand = res; break; }
+    }
+    const int il = block_reduce(cand, MaxOp(), ired);
     if (t == 0) {
       int action = 3;
-      if (iu >= 0) {
-        int il = iu;
-        while (il > 0) {
-          double s = fabs(AT(il - 1, il - 1)) + fabs(AT(il, il));
-          if (s == 0.0) s = norm;
-          if (fabs(AT(il, il - 1)) < eps * s) break;
-          il--;
+      if (il == iu) {  // one root
+        AT(iu, iu) = AT(iu, iu) + exshift;
+        if (iu > 0) AT(iu, iu - 1) = 0;
+        iu--; iter = 0;
+        action = 0;
+      } else if (il == iu - 1) {  // two roots (RealSchur::splitOffTwoRows)
+        action = 1;
+        const double p = 0.5 * (AT(iu - 1, iu - 1) - AT(iu, iu));
+        const double q = p * p + AT(iu, iu - 1) * AT(iu - 1, iu);
+        AT(iu, iu) += exshift;
+        AT(iu - 1, iu - 1) += exshift;
+        si[3] = q >= 0;
+        if (q >= 0) {
+          const double z = sqrt(fabs(q));
+          const double pp = (p >= 0) ? p + z : p - z, qq = AT(iu, iu - 1);
+          double c, s;  // JacobiRotation::makeGivens, real case
+          if (qq == 0) { c = pp < 0 ? -1 : 1; s = 0; }
+          else if (pp == 0) { c = 0; s = qq < 0 ? 1 : -1; }
+          else if (fabs(pp) > fabs(qq)) {
+            const double tt = qq / pp;
+            double u = sqrt(1 + tt * tt);
+            if (pp < 0) u = -u;
+            c = 1 / u; s = -tt * c;
+          } else {
+            const double tt = pp / qq;
+            double u = sqrt(1 + tt * tt);
+            if (qq < 0) u = -u;
+            s = -1 / u; c = -tt * s;
+          }
+          sc[1] = c; sc[2] = s;
+          for (int j = iu - 1; j <= iu; j++) {  // rot.adjoint() on the left, columns <= iu
+            const double xv = AT(iu - 1, j), yv = AT(iu, j);
+            AT(iu - 1, j) = c * xv - s * yv;
+            AT(iu, j) = s * xv + c * yv;
+          }
         }
-        if (il == iu) {  // one root
-          AT(iu, iu) = AT(iu, iu) + exshift;
-          if (iu > 0) AT(iu, iu - 1) = 0;
-          iu--; iter = 0;
-          action = 0;
-        } else if (il == iu - 1) {  // two roots (RealSchur::splitOffTwoRows)
-          action = 1;
-          const double p = 0.5 * (AT(iu - 1, iu - 1) - AT(iu, iu));
-          const double q = p * p + AT(iu, iu - 1) * AT(iu - 1, iu);
-          AT(iu, iu) += exshift;
-          AT(iu - 1, iu - 1) += exshift;
-          si[3] = q >= 0;
-          if (q >= 0) {
-            const double z = sqrt(fabs(q));
-            const double pp = (p >= 0) ? p + z : p - z, qq = AT(iu, iu - 1);
-            double c, s;  // JacobiRotation::makeGivens, real case
-            if (qq == 0) { c = pp < 0 ? -1 : 1; s = 0; }
-            else if (pp == 0) { c = 0; s = qq < 0 ? 1 : -1; }
-            else if (fabs(pp) > fabs(qq)) {
-              const double tt = qq / pp;
-              double u = sqrt(1 + tt * tt);
-              if (pp < 0) u = -u;
-              c = 1 / u; s = -tt * c;
-            } else {
-              const double tt = pp / qq;
-              double u = sqrt(1 + tt * tt);
-              if (qq < 0) u = -u;
-              s = -1 / u; c = -tt * s;
-            }
-            sc[1] = c; sc[2] = s;
-            for (int j = iu - 1; j <= iu; j++) {  // rot.adjoint() on the left, columns <= iu
-              const double xv = AT(iu - 1, j), yv = AT(iu, j);
-              AT(iu - 1, j) = c * xv - s * yv;
-              AT(iu, j) = s * xv + c * yv;
-            }
+        si[1] = iu;  // rows 0..iu get the right rotation
+        iu -= 2; iter = 0;
+      } else {  // Francis QR step (RealSchur::computeShift)
+        double sh0 = AT(iu, iu), sh1 = AT(iu - 1, iu - 1), sh2 = AT(iu, iu - 1) * AT(iu - 1, iu);
+        double shift_amt = 0;
+        int shift_kind = 0;
+        if (iter == 10) {
+          exshift += sh0;
+          shift_kind = 1; shift_amt = sh0;
+          const double s = fabs(AT(iu, iu - 1)) + fabs(AT(iu - 1, iu - 2));
+          sh0 = 0.75 * s; sh1 = 0.75 * s; sh2 = -0.4375 * s * s;
+        }
+        if (iter == 30) {
+          double s = (sh1 - sh0) / 2.0;
+          s = s * s + sh2;
+          if (s > 0) {
+            s = sqrt(s);
+            if (sh1 < sh0) s = -s;
+            s = s + (sh1 - sh0) / 2.0;
+            s = sh0 - sh2 / s;
+            exshift += s;
+            shift_kind = 2; shift_amt = s;
+            sh0 = sh1 = sh2 = 0.964;
           }
-          si[1] = iu;  // rows 0..iu get the right rotation
-          iu -= 2; iter = 0;
-        } else {  // Francis QR step (RealSchur::computeShift / initFrancisQRStep)
-          double sh0 = AT(iu, iu), sh1 = AT(iu - 1, iu - 1), sh2 = AT(iu, iu - 1) * AT(iu - 1, iu);
-          double shift_amt = 0;
-          int shift_kind = 0;
-          if (iter == 10) {
-            exshift += sh0;
-            shift_kind = 1; shift_amt = sh0;
-            const double s = fabs(AT(iu, iu - 1)) + fabs(AT(iu - 1, iu - 2));
-            sh0 = 0.75 * s; sh1 = 0.75 * s; sh2 = -0.4375 * s * s;
-          }
-          if (iter == 30) {
-            double s = (sh1 - sh0) / 2.0;
-            s = s * s + sh2;
-            if (s > 0) {
-              s = sqrt(s);
-              if (sh1 < sh0) s = -s;
-              s = s + (sh1 - sh0) / 2.0;
-              s = sh0 - sh2 / s;
-              exshift += s;
-              shift_kind = 2; shift_amt = s;
-              sh0 = sh1 = sh2 = 0.964;
-            }
-          }
-          iter++; totalIter++;
-          if (totalIter > 40 * n) action = 3;
-          else {
-            action = 2;
-            si[4] = shift_kind;
-            sc[3] = shift_amt;
-            sc[4] = sh0; sc[5] = sh1; sc[6] = sh2;
-            si[1] = il;
-            si[5] = iu;
-          }
+        }
+        iter++; totalIter++;
+        if (totalIter > 40 * n) action = 3;
+        else {
+          action = 2;
+          si[4] = shift_kind;
+          sc[3] = shift_amt;
+          sc[4] = sh0; sc[5] = sh1; sc[6] = sh2;
+          si[5] = iu;
         }
       }
       si[0] = action;
+      s_iu = action == 3 ? -1 : iu;
     }
     __syncthreads();
     const int action = si[0];
@@ -465,142 +538,179 @@ __global__ void __launch_bounds__(256) k_har_qr(const HarDesc *hd) {
         if (t == 0) AT(iu2, iu2 - 1) = 0;
       }
       if (t == 0 && iu2 > 1) AT(iu2 - 1, iu2 - 2) = 0;
-      __syncthreads();
-      continue;
+      continue;  // the loop head synchronises
     }
     // ---- Francis QR step on the window [il, iuq]
-    const int il = si[1], iuq = si[5];
+    const int iuq = si[5];
     if (si[4]) {
       const double amt = sc[3];
       for (int i = t; i <= iuq; i += blockDim.x) AT(i, i) -= amt;
       __syncthreads();
     }
-    if (t == 0) {
-      const double sh0 = sc[4], sh1 = sc[5], sh2 = sc[6];
-      int im;
-      double v0 = 0, v1 = 0, v2 = 0;
-      for (im = iuq - 2; im >= il; --im) {
-        const double Tmm = AT(im, im);
-        const double r = sh0 - Tmm, s = sh1 - Tmm;
-        v0 = (r * s - sh2) / AT(im + 1, im) + AT(im, im + 1);
-        v1 = AT(im + 1, im + 1) - Tmm - r - s;
-        v2 = AT(im + 2, im + 1);
-        if (im == il) break;
-        const double lhs = AT(im, im - 1) * (fabs(v1) + fabs(v2));
-        const double rhs = v0 * (fabs(AT(im - 1, im - 1)) + fabs(Tmm) + fabs(AT(im + 1, im + 1)));
-        if (fabs(lhs) < eps * rhs) break;
-      }
-      si[2] = im;
-      sc[0] = v0; sc[1] = v1; sc[2] = v2;
+    // initFrancisQRStep: the largest im in (il, iuq-2] passing the
+    // small-subdiagonal test, else il -- every candidate tested at once
+    const double sh0 = sc[4], sh1 = sc[5], sh2 = sc[6];
+    cand = il;
+    for (int m = iuq - 2 - t; m > il; m -= blockDim.x) {
+      const double Tmm = AT(m, m);
+      const double r = sh0 - Tmm, s = sh1 - Tmm;
+      const double v0 = (r * s - sh2) / AT(m + 1, m) + AT(m, m + 1);
+      const double v1 = AT(m + 1, m + 1) - Tmm - r - s;
+      const double v2 = AT(m + 2, m + 1);
+      const double lhs = AT(m, m - 1) * (fabs(v1) + fabs(v2));
+      const double rhs = v0 * (fabs(AT(m - 1, m - 1)) + fabs(Tmm) + fabs(AT(m + 1, m + 1)));
+      if (fabs(lhs) < eps * rhs) { cand = m; break; }
     }
-    __syncthreads();
-    const int im = si[2];
-    const double fv0 = sc[0], fv1 = sc[1], fv2 = sc[2];
-    __syncthreads();
+    const int im = block_reduce(cand, MaxOp(), ired);
+    double fv0, fv1, fv2;
+    {
+      const double Tmm = AT(im, im);
+      const double r = sh0 - Tmm, s = sh1 - Tmm;
+      fv0 = (r * s - sh2) / AT(im + 1, im) + AT(im, im + 1);
+      fv1 = AT(im + 1, im + 1) - Tmm - r - s;
+      fv2 = AT(im + 2, im + 1);
+    }
     // Bulge chase in windows of QB steps.  Window steps [k0, k1) read and
     // update only entries inside R = [a, b) x [a, b) (a = k0-1, b = k1+3),
-    // which live in LDS while the window runs; afterwards the window's
-    // reflectors are applied, in step order, to the strips outside R
-    // (rows k0..k1+1 right of R: left reflectors only; columns k0..k1+1
-    // above R: right reflectors only).  Every entry therefore receives the
-    // same reflectors in the same order as in the unwindowed chase.
+    // which live in LDS while wavefront 0 chases the bulge through them;
+    // afterwards the window's reflectors are applied, in step order, to the
+    // strips outside R (rows k0..k1+1 right of R: left reflectors only;
+    // columns k0..k1+1 above R: right reflectors only).  Every entry
+    // therefore receives the same reflectors in the same order as in the
+    // unwindowed chase.
     for (int k0 = im; k0 <= iuq - 2; k0 += QB) {
       const int k1 = (k0 + QB < iuq - 1) ? k0 + QB : iuq - 1;
       const int a = k0 - 1 > 0 ? k0 - 1 : 0;
       const int b = (k1 + 3 < iuq + 1) ? k1 + 3 : iuq + 1;
       const int wd = b - a;
-      for (int e = t; e < wd * wd; e += blockDim.x) {
-        const int r = e / wd, c = e - r * wd;
-        Rw[r * QW + c] = AT(a + r, a + c);
+      for (int r = t >> 6; r < wd; r += 4)
+        if ((t & 63) < wd) Rw[r * QWP + (t & 63)] = AT(a + r, a + (t & 63));
+      __syncthreads();
+#define RW(r, c) Rw[((r) - a) * QWP + ((c) - a)]
+      if (t < 64) {
+        for (int k = k0; k < k1; ++k) {
+          const bool first = (k == im);
+          double x0, x1, x2;
+          if (first) { x0 = fv0; x1 = fv1; x2 = fv2; }
+          else { x0 = RW(k, k - 1); x1 = RW(k + 1, k - 1); x2 = RW(k + 2, k - 1); }
+          // makeHouseholder of (x0, x1, x2), every lane
+          double tail = 0;
+          tail += x1 * x1;
+          tail += x2 * x2;
+          double tau = 0, e0 = 0, e1 = 0, beta = x0;
+          if (tail != 0.0) {
+            double bb = sqrt(x0 * x0 + tail);
+            if (x0 >= 0) bb = -bb;
+            e0 = x1 / (x0 - bb);
+            e1 = x2 / (x0 - bb);
+            tau = (bb - x0) / bb;
+            beta = bb;
+          }
+          wave_sync();
+          if (t == 0) {
+            rf[k - k0][0] = beta != 0 ? tau : 0.0;
+            rf[k - k0][1] = e0;
+            rf[k - k0][2] = e1;
+            rf[k - k0][3] = beta != 0 ? 1.0 : 0.0;
+          }
+          if (beta != 0) {
+            if (t == 0) {
+              if (first && k > il) RW(k, k - 1) = -RW(k, k - 1);
+              else if (!first) RW(k, k - 1) = beta;
+            }
+            const int c = k + t;
+            if (c < b) {
+              double tt = e0 * RW(k + 1, c);
+              tt += e1 * RW(k + 2, c);
+              tt += RW(k, c);
+              RW(k, c) -= tau * tt;
+              RW(k + 1, c) -= (tau * e0) * tt;
+              RW(k + 2, c) -= (tau * e1) * tt;
+            }
+            wave_sync();
+            const int rlast = (iuq < k + 3 ? iuq : k + 3);
+            const int r = a + t;
+            if (r <= rlast) {
+              double tt = RW(r, k + 1) * e0;
+              tt += RW(r, k + 2) * e1;
+              tt += RW(r, k);
+              RW(r, k) -= tau * tt;
+              RW(r, k + 1) -= (tau * tt) * e0;
+              RW(r, k + 2) -= (tau * tt) * e1;
+            }
+            wave_sync();
+          }
+        }
       }
       __syncthreads();
-#define RW(r, c) Rw[((r) - a) * QW + ((c) - a)]
-      for (int k = k0; k < k1; ++k) {
-        if (t == 0) {
-          const bool first = (k == im);
-          double x[3];
-          if (first) { x[0] = fv0; x[1] = fv1; x[2] = fv2; }
-          else { x[0] = RW(k, k - 1); x[1] = RW(k + 1, k - 1); x[2] = RW(k + 2, k - 1); }
-          double e2[2], tau, beta;
-          make_hh(x, 3, e2, &tau, &beta);
-          if (beta != 0) {
-            if (first && k > il) RW(k, k - 1) = -RW(k, k - 1);
-            else if (!first) RW(k, k - 1) = beta;
-          }
-          rf[k - k0][0] = beta != 0 ? tau : 0.0;
-          rf[k - k0][1] = e2[0];
-          rf[k - k0][2] = e2[1];
-          rf[k - k0][3] = beta != 0 ? 1.0 : 0.0;
-        }
-        __syncthreads();
-        if (rf[k - k0][3] != 0.0) {
-          const double tau = rf[k - k0][0], e0 = rf[k - k0][1], e1 = rf[k - k0][2];
-          for (int c = k + t; c < b; c += blockDim.x) {
-            double tt = e0 * RW(k + 1, c);
-            tt += e1 * RW(k + 2, c);
-            tt += RW(k, c);
-            RW(k, c) -= tau * tt;
-            RW(k + 1, c) -= (tau * e0) * tt;
-            RW(k + 2, c) -= (tau * e1) * tt;
-          }
-          __syncthreads();
-          const int rlast = (iuq < k + 3 ? iuq : k + 3);
-          for (int r = a + t; r <= rlast; r += blockDim.x) {
-            double tt = RW(r, k + 1) * e0;
-            tt += RW(r, k + 2) * e1;
-            tt += RW(r, k);
-            RW(r, k) -= tau * tt;
-            RW(r, k + 1) -= (tau * tt) * e0;
-            RW(r, k + 2) -= (tau * tt) * e1;
-          }
-        }
-        __syncthreads();
-      }
-      for (int e = t; e < wd * wd; e += blockDim.x) {
-        const int r = e / wd, c = e - r * wd;
-        AT(a + r, a + c) = Rw[r * QW + c];
-      }
+      for (int r = t >> 6; r < wd; r += 4)
+        if ((t & 63) < wd) AT(a + r, a + (t & 63)) = Rw[r * QWP + (t & 63)];
 #undef RW
-      // strip right of R: rows k0 .. k1+1, columns b .. iuq (left reflectors)
-      const int nrow = k1 + 2 - k0;
+      // strip right of R: rows k0 .. k1+1, columns b .. iuq (left reflectors).
+      // Reflector q touches rows q..q+2 of the strip, so after it row q is
+      // final: each thread streams down its column with a 3-value window.
+      const int nstep = k1 - k0;
       for (int c = b + t; c <= iuq; c += blockDim.x) {
-        double v[QB + 2];
+        double *p = &AT(k0, c);
+        double u0 = p[0], u1 = p[n];
+        for (int q0 = 0; q0 < nstep; q0 += HB) {
+          double w[HB];
 #pragma unroll
-        for (int q = 0; q < QB + 2; q++) v[q] = (q < nrow) ? AT(k0 + q, c) : 0.0;
+          for (int j = 0; j < HB; j++) w[j] = (q0 + j < nstep) ? p[(long)(q0 + j + 2) * n] : 0.0;
 #pragma unroll
-        for (int q = 0; q < QB; q++) {
-          if (k0 + q < k1 && rf[q][3] != 0.0) {
-            const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
-            double tt = e0 * v[q + 1];
-            tt += e1 * v[q + 2];
-            tt += v[q];
-            v[q] -= tau * tt;
-            v[q + 1] -= (tau * e0) * tt;
-            v[q + 2] -= (tau * e1) * tt;
+          for (int j = 0; j < HB; j++) {
+            const int q = q0 + j;
+            if (q < nstep) {
+              double u2 = w[j];
+              if (rf[q][3] != 0.0) {
+                const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+                double tt = e0 * u1;
+                tt += e1 * u2;
+                tt += u0;
+                u0 -= tau * tt;
+                u1 -= (tau * e0) * tt;
+                u2 -= (tau * e1) * tt;
+              }
+              p[(long)q * n] = u0;
+              u0 = u1;
+              u1 = u2;
+            }
           }
         }
-#pragma unroll
-        for (int q = 0; q < QB + 2; q++) if (q < nrow) AT(k0 + q, c) = v[q];
+        p[(long)nstep * n] = u0;
+        p[(long)(nstep + 1) * n] = u1;
       }
-      // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors)
+      // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors),
+      // streamed along each row the same way
       for (int r = t; r < a; r += blockDim.x) {
-        double v[QB + 2];
+        double *p = &AT(r, k0);
+        double u0 = p[0], u1 = p[1];
+        for (int q0 = 0; q0 < nstep; q0 += HB) {
+          double w[HB];
 #pragma unroll
-        for (int q = 0; q < QB + 2; q++) v[q] = (q < nrow) ? AT(r, k0 + q) : 0.0;
+          for (int j = 0; j < HB; j++) w[j] = (q0 + j < nstep) ? p[q0 + j + 2] : 0.0;
 #pragma unroll
-        for (int q = 0; q < QB; q++) {
-          if (k0 + q < k1 && rf[q][3] != 0.0) {
-            const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
-            double tt = v[q + 1] * e0;
-            tt += v[q + 2] * e1;
-            tt += v[q];
-            v[q] -= tau * tt;
-            v[q + 1] -= (tau * tt) * e0;
-            v[q + 2] -= (tau * tt) * e1;
+          for (int j = 0; j < HB; j++) {
+            const int q = q0 + j;
+            if (q < nstep) {
+              double u2 = w[j];
+              if (rf[q][3] != 0.0) {
+                const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+                double tt = u1 * e0;
+                tt += u2 * e1;
+                tt += u0;
+                u0 -= tau * tt;
+                u1 -= (tau * tt) * e0;
+                u2 -= (tau * tt) * e1;
+              }
+              p[q] = u0;
+              u0 = u1;
+              u1 = u2;
+            }
           }
         }
-#pragma unroll
-        for (int q = 0; q < QB + 2; q++) if (q < nrow) AT(r, k0 + q) = v[q];
+        p[nstep] = u0;
+        p[nstep + 1] = u1;
       }
       __syncthreads();
     }
@@ -610,17 +720,18 @@ __global__ void __launch_bounds__(256) k_har_qr(const HarDesc *hd) {
       make_hh(x, 2, e1, &tau, &beta);
       si[6] = beta != 0;
       if (beta != 0) AT(iuq - 1, iuq - 2) = beta;
-      sc[3] = tau; ess[0] = e1[0];
+      sc[3] = tau; sc[7] = e1[0];
     }
     __syncthreads();
     if (si[6]) {
-      const double tau = sc[3], e0 = ess[0];
-      for (int c = iuq - 1 + t; c <= iuq; c += blockDim.x) {
-        double tt = e0 * AT(iuq, c);
-        tt += AT(iuq - 1, c);
-        AT(iuq - 1, c) -= tau * tt;
-        AT(iuq, c) -= (tau * e0) * tt;
-      }
+      const double tau = sc[3], e0 = sc[7];
+      if (t == 0)
+        for (int c = iuq - 1; c <= iuq; c++) {
+          double tt = e0 * AT(iuq, c);
+          tt += AT(iuq - 1, c);
+          AT(iuq - 1, c) -= tau * tt;
+          AT(iuq, c) -= (tau * e0) * tt;
+        }
       __syncthreads();
       for (int r = t; r <= iuq; r += blockDim.x) {
         double tt = AT(r, iuq) * e0;
@@ -629,12 +740,10 @@ __global__ void __launch_bounds__(256) k_har_qr(const HarDesc *hd) {
         AT(r, iuq) -= (tau * tt) * e0;
       }
     }
-    __syncthreads();
     for (int i = im + 2 + t; i <= iuq; i += blockDim.x) {
       AT(i, i - 2) = 0;
       if (i > im + 2) AT(i, i - 3) = 0;
     }
-    __syncthreads();
   }
   __syncthreads();
   // eigenvalue 2 in Schur-diagonal order (EigenSolver::compute, EigenSolver.h:364-381)
@@ -695,6 +804,7 @@ void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef
   (void)lds_ok;
   k_har_cooc<<<dim3(4, nimg), 1024, 256 * 257 / 2 * sizeof(uint32_t), s>>>(imgs, hd);
   k_har_stats<<<dim3(4, nimg), 256, 0, s>>>(hd);
+  k_har_q<<<dim3(4 * QRB, nimg), 256, 0, s>>>(hd);
   k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
   k_har_qr<<<dim3(4, nimg), 256, 0, s>>>(hd);
   k_har_final<<<nimg, 64, 0, s>>>(hd, o);

@@ -73,6 +73,7 @@ struct HarDesc {
   double *P;            // [4][ng*ng] normalised matrices (ng <= 256)
   double *Q;            // [4][ng*ng]
   double *feat;         // [4][14]
+  double *marg;         // [4][2][256] row / column marginals px, py
   int *ng;              // number of tones present
   int *lut;             // [256] tone -> rank
   int *count;           // [4] pair counts
