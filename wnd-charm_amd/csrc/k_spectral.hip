@@ -14,7 +14,7 @@ namespace wnd {
 // pixels are a fixed-order tree.
 constexpr int ZL = 15;
 constexpr int ZPAIRS = 72;  // (n, m), 0 <= m <= n <= 15, n - m even
-__device__ __forceinline__ int zoff(int n) {  // pairs before order n
+__host__ __device__ constexpr int zoff(int n) {  // pairs before order n
   int s = 0;
   for (int k = 0; k < n; k++) s += k / 2 + 1;
   return s;
@@ -44,65 +44,96 @@ __global__ void __launch_bounds__(1024) k_zern_moments(const Img *imgs, double *
   }
 }
 
-// 256 threads = 16 pixel streams x 16 orders n; block = one pixel chunk
-__global__ void __launch_bounds__(256) k_zern_accum(const Img *imgs, double *partial, int nchunk, const double *htab) {
-  __shared__ double red[16][2 * ZPAIRS];
-  const Img im = imgs[blockIdx.y];
-  const long P = (long)im.h * im.w;
-  const long csz = (P + nchunk - 1) / nchunk;
-  const long p0 = blockIdx.x * csz, p1 = (p0 + csz < P) ? p0 + csz : P;
-  double *hdr = partial + (long)blockIdx.y * (4 + (long)nchunk * 2 * ZPAIRS);
-  const double cxm = hdr[0], cym = hdr[1], sum = hdr[2];
-  const int t = threadIdx.x, n = t & 15, stream = t >> 4;
+// One wavefront per group of orders [N0, N1] (groups balanced by work),
+// lanes over pixels; the group's accumulators stay in registers (every loop
+// is unrolled at compile time) and are reduced across lanes at the end.
+template <int N0, int N1>
+__device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int lane, double cxm, double cym, double sum,
+                                           const double *H1, const double *H2, const double *H3, double *dst) {
+  constexpr int NP = [] {
+    int c = 0;
+    for (int n = N0; n <= N1; n++) c += n / 2 + 1;
+    return c;
+  }();
   const double rad = (double)(im.w < im.h ? im.w : im.h);
   const double PI = 3.14159265358979323846264338328;
-  const double *H1 = htab, *H2 = htab + 32 * 32, *H3 = htab + 2 * 32 * 32;
-  double AR[8], AI[8];
+  double AR[NP], AI[NP];
 #pragma unroll
-  for (int k = 0; k < 8; k++) AR[k] = AI[k] = 0.0;
-  for (long p = p0 + stream; p < p1; p += 16) {
+  for (int k = 0; k < NP; k++) AR[k] = AI[k] = 0.0;
+  for (long p = p0 + lane; p < p1; p += 64) {
     const int j = p / im.w, i = p - (long)j * im.w;  // i: column, j: row
     const double x = (i + 1 - cxm) / rad;
     const double y = (j + 1 - cym) / rad;
     const double r2 = x * x + y * y;
     const double r = sqrt(r2);
     if (r < kDblEps || r > 1.0) continue;
-    double R[ZL + 1];
+    double R[N1 + 1];
     R[0] = 1;
-    for (int k = 1; k <= n; k++) R[k] = r * R[k - 1];
-    double COST[ZL + 1], SINT[ZL + 1];
+#pragma unroll
+    for (int k = 1; k <= N1; k++) R[k] = r * R[k - 1];
+    double COST[N1 + 1], SINT[N1 + 1];
     const double a = COST[0] = x / r;
     const double b = SINT[0] = y / r;
-    for (int m = 1; m <= n; m++) {
+#pragma unroll
+    for (int m = 1; m <= N1; m++) {
       COST[m] = a * COST[m - 1] - b * SINT[m - 1];
       SINT[m] = a * SINT[m - 1] + b * COST[m - 1];
     }
     const double f = im.p[p] / sum;
-    const double const_t = (n + 1) * f / PI;
-    const double Rn = R[n];
-    const double Rnm2 = n >= 2 ? R[n - 2] : 0.0;
-    double Rnmp2 = 0, Rnmp4 = 0, Rnm;
-    int slot = n / 2;  // m = n sits at the last slot of this order
-    for (int m = n; m >= 0; m -= 2, slot--) {
-      if (m == n) { Rnm = Rn; Rnmp4 = Rn; }
-      else if (m == n - 2) { const double Rnnm2 = n * Rn - (n - 1) * Rnm2; Rnm = Rnnm2; Rnmp2 = Rnnm2; }
-      else {
-        Rnm = H1[n * 32 + m] * Rnmp4 + (H2[n * 32 + m] + (H3[n * 32 + m] / r2)) * Rnmp2;
-        Rnmp4 = Rnmp2;
-        Rnmp2 = Rnm;
+    int base = 0;
+#pragma unroll
+    for (int n = N0; n <= N1; n++) {
+      const double const_t = (n + 1) * f / PI;
+      const double Rn = R[n];
+      const double Rnm2 = n >= 2 ? R[n >= 2 ? n - 2 : 0] : 0.0;
+      double Rnmp2 = 0, Rnmp4 = 0, Rnm = 0;
+#pragma unroll
+      for (int m = n; m >= 0; m -= 2) {
+        if (m == n) { Rnm = Rn; Rnmp4 = Rn; }
+        else if (m == n - 2) { const double Rnnm2 = n * Rn - (n - 1) * Rnm2; Rnm = Rnnm2; Rnmp2 = Rnnm2; }
+        else {
+          Rnm = H1[n * 32 + m] * Rnmp4 + (H2[n * 32 + m] + (H3[n * 32 + m] / r2)) * Rnmp2;
+          Rnmp4 = Rnmp2;
+          Rnmp2 = Rnm;
+        }
+        AR[base + (m >> 1)] += const_t * Rnm * COST[m];
+        AI[base + (m >> 1)] -= const_t * Rnm * SINT[m];
       }
-      AR[slot] += const_t * Rnm * COST[m];
-      AI[slot] -= const_t * Rnm * SINT[m];
+      base += n / 2 + 1;
     }
   }
-  const int base = zoff(n), cnt = n / 2 + 1;
-  for (int k = 0; k < cnt; k++) { red[stream][2 * (base + k)] = AR[k]; red[stream][2 * (base + k) + 1] = AI[k]; }
-  __syncthreads();
+  const int off = zoff(N0);
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    double vr = AR[k], vi = AI[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      vr += __shfl_xor(vr, o, 64);
+      vi += __shfl_xor(vi, o, 64);
+    }
+    if (lane == 0) {
+      dst[2 * (off + k)] = vr;
+      dst[2 * (off + k) + 1] = vi;
+    }
+  }
+}
+
+// block = one pixel chunk of one image; wave w handles order group w
+__global__ void __launch_bounds__(256) k_zern_accum(const Img *imgs, double *partial, int nchunk, const double *htab) {
+  const Img im = imgs[blockIdx.y];
+  const long P = (long)im.h * im.w;
+  const long csz = (P + nchunk - 1) / nchunk;
+  const long p0 = blockIdx.x * csz, p1 = (p0 + csz < P) ? p0 + csz : P;
+  double *hdr = partial + (long)blockIdx.y * (4 + (long)nchunk * 2 * ZPAIRS);
+  const double cxm = hdr[0], cym = hdr[1], sum = hdr[2];
+  const double *H1 = htab, *H2 = htab + 32 * 32, *H3 = htab + 2 * 32 * 32;
   double *dst = hdr + 4 + (long)blockIdx.x * 2 * ZPAIRS;
-  for (int e = t; e < 2 * ZPAIRS; e += 256) {
-    double s = 0;
-    for (int q = 0; q < 16; q++) s += red[q][e];
-    dst[e] = s;
+  const int lane = threadIdx.x & 63;
+  switch (threadIdx.x >> 6) {
+    case 0: zern_group<0, 7>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    case 1: zern_group<8, 10>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    case 2: zern_group<11, 13>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    default: zern_group<14, 15>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
   }
 }
 

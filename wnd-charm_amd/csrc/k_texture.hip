@@ -1117,65 +1117,97 @@ __device__ __forceinline__ double radon_ridx(int n, int m, int M, int N, double 
   return xc + ys;
 }
 
-// Block = (angle, image).  The x*cos and y*sin tables of the reference
-// (radon.cpp:58-63) are built in LDS with the reference's operations, so
-// rIdx = xC[n] + yS[m] is bit-identical; each thread then gathers its bins.
-__global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const double *trig, double *const *proj) {
-  extern __shared__ double tabs[];
+// Block = (angle, group of 256 bins, image); thread = bin r.  The y*sin
+// table of the reference (radon.cpp:58-63) is built in LDS and x*cos
+// recomputed with the reference's operations, so rIdx is bit-identical.
+// floor(rIdx) is monotone in the sub-column n (x*cos is) and non-increasing
+// in the sub-row m, so the sub-columns touching bin r form an interval
+// (binary search) and, per sub-column, the sub-rows with floor(rIdx) in
+// {r-1, r} form a range that moves slowly with n (tracked by stepping).
+// Each thread walks its own (n, m) sequence as one flat loop, so the lanes
+// of a wave stay busy whatever the angle.
+__device__ __forceinline__ double radon_xc(int nn, int xO, double cs, int rFirst) {
+  return ((-xO - 0.25) + 0.5 * nn) * cs - rFirst;
+}
+
+__global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const double *trig, double *const *proj,
+                                                    int nbg) {
+  extern __shared__ double yS[];
   const Img im = imgs[blockIdx.y];
   const int M = im.h, N = im.w;
   const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
                                    (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
   const int rFirst = -rLast, rSize = rLast - rFirst + 1;
-  const int a = blockIdx.x;
+  const int a = blockIdx.x / nbg, r0 = (blockIdx.x - a * nbg) * 256;
+  if (r0 >= rSize) return;
   const double cs = trig[2 * a], sn = trig[2 * a + 1];
   const int xO = (N - 1) / 2, yO = (M - 1) / 2;
-  const int N2 = 2 * N, M2 = 2 * M;
-  double *xC = tabs, *yS = tabs + N2;
-  for (int p = threadIdx.x; p < N2; p += blockDim.x) xC[p] = ((-xO - 0.25) + 0.5 * p) * cs - rFirst;
+  const int M2 = 2 * M, N2 = 2 * N;
   for (int p = threadIdx.x; p < M2; p += blockDim.x) yS[p] = ((-yO - 0.25) + 0.5 * (M2 - 1 - p)) * sn;
   __syncthreads();
-  double *out = proj[blockIdx.y] + (long)a * rSize;
-  for (int r = threadIdx.x; r < rSize; r += blockDim.x) {
-    double acc = 0.0;
-    // [m0, m1): sub-rows of column nn whose floor(rIdx) is r or r-1; floor is
-    // non-increasing in m, and the range moves slowly with nn, so it is
-    // tracked by stepping (binary search after a gap).
-    int m0 = 0, m1 = 0;
-    bool valid = false;
-    for (int nn = 0; nn < N2; nn++) {
-      const double xc = xC[nn];
-      if ((int)(xc + yS[0]) < r - 1 || (int)(xc + yS[M2 - 1]) > r) { valid = false; continue; }
-      if (!valid) {
-        int lo = 0, hi = M2;
-        while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)(xc + yS[mid]) <= r) hi = mid; else lo = mid + 1; }
-        m0 = lo;
-        lo = m0; hi = M2;
-        while (lo < hi) { const int mid = (lo + hi) >> 1; if ((int)(xc + yS[mid]) < r - 1) hi = mid; else lo = mid + 1; }
-        m1 = lo;
-        valid = true;
-      } else {
-        while (m0 > 0 && (int)(xc + yS[m0 - 1]) <= r) m0--;
-        while (m0 < M2 && (int)(xc + yS[m0]) > r) m0++;
-        if (m1 < m0) m1 = m0;
-        while (m1 > m0 && (int)(xc + yS[m1 - 1]) < r - 1) m1--;
-        while (m1 < M2 && (int)(xc + yS[m1]) >= r - 1) m1++;
-      }
-      const double *col = im.p + nn / 2;
-      for (int mm = m0; mm < m1; mm++) {
-        double pixel = col[(long)(mm / 2) * N];
-        if (pixel) {
-          pixel *= 0.25;
-          const double rIdx = (xc + yS[mm]);
-          const int rLow = (int)rIdx;
-          const double pixelLow = pixel * (1 - rIdx + rLow);
-          if (rLow == r) acc += pixelLow;
-          else acc += pixel - pixelLow;
-        }
-      }
-    }
-    out[r] = acc;
+  const int r = r0 + threadIdx.x;
+  if (r >= rSize) return;
+  const double yS0 = yS[0], ySL = yS[M2 - 1];
+  // column nn is skipped when floor(xc + yS0) < r-1 (A) or floor(xc + ySL) > r (B)
+  auto predA = [&](int nn) { return (int)(radon_xc(nn, xO, cs, rFirst) + yS0) < r - 1; };
+  auto predB = [&](int nn) { return (int)(radon_xc(nn, xO, cs, rFirst) + ySL) > r; };
+  int lo = 0, hi = N2;
+  if (cs > 0) {  // A: true then false; B: false then true
+    int l = 0, h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (!predA(mid)) h = mid; else l = mid + 1; }
+    lo = l;
+    l = lo; h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (predB(mid)) h = mid; else l = mid + 1; }
+    hi = l;
+  } else if (cs < 0) {  // B: true then false; A: false then true
+    int l = 0, h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (!predB(mid)) h = mid; else l = mid + 1; }
+    lo = l;
+    l = lo; h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (predA(mid)) h = mid; else l = mid + 1; }
+    hi = l;
+  } else if (predA(0) || predB(0)) {
+    hi = 0;
   }
+  double acc = 0.0;
+  int nn = lo, mm = 0, m0 = 0, m1 = 0;
+  double xc = 0;
+  bool live = lo < hi;
+  if (live) {
+    xc = radon_xc(nn, xO, cs, rFirst);
+    int l = 0, h = M2;
+    while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) <= r) h = mid; else l = mid + 1; }
+    m0 = l;
+    l = m0; h = M2;
+    while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) < r - 1) h = mid; else l = mid + 1; }
+    m1 = l;
+    mm = m0;
+  }
+  while (live) {
+    if (mm < m1) {
+      double pixel = im.p[(long)(mm >> 1) * N + (nn >> 1)];
+      if (pixel) {
+        pixel *= 0.25;
+        const double rIdx = (xc + yS[mm]);
+        const int rLow = (int)rIdx;
+        const double pixelLow = pixel * (1 - rIdx + rLow);
+        if (rLow == r) acc += pixelLow;
+        else acc += pixel - pixelLow;
+      }
+      mm++;
+    } else if (++nn >= hi) {
+      live = false;
+    } else {
+      xc = radon_xc(nn, xO, cs, rFirst);
+      while (m0 > 0 && (int)(xc + yS[m0 - 1]) <= r) m0--;
+      while (m0 < M2 && (int)(xc + yS[m0]) > r) m0++;
+      if (m1 < m0) m1 = m0;
+      while (m1 > m0 && (int)(xc + yS[m1 - 1]) < r - 1) m1--;
+      while (m1 < M2 && (int)(xc + yS[m1]) >= r - 1) m1++;
+      mm = m0;
+    }
+  }
+  proj[blockIdx.y][(long)a * rSize + r] = acc;
 }
 
 // yTable / xTable entries are exact multiples of 0.25, so the closed forms
@@ -1205,11 +1237,12 @@ __global__ void k_radon_hist(const Img *imgs, double *const *proj, OutRef o) {
 
 void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
                   int maxh, int maxw, hipStream_t s) {
-  const size_t radon_lds = (2 * (size_t)maxw + 2 * (size_t)maxh) * sizeof(double);
+  (void)maxw;
+  const size_t radon_lds = 2 * (size_t)maxh * sizeof(double);  // y*sin table
   if (radon_lds > 65536)
     (void)hipFuncSetAttribute((const void *)k_radon_proj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)radon_lds);
-  (void)maxbins;
-  k_radon_proj<<<dim3(4, nimg), 256, radon_lds, s>>>(imgs, trig, proj);
+  const int nbg = (maxbins + 255) / 256;
+  k_radon_proj<<<dim3(4 * nbg, nimg), 256, radon_lds, s>>>(imgs, trig, proj, nbg);
   k_radon_hist<<<nimg, 64, 0, s>>>(imgs, proj, o);
 }
 
