@@ -6,15 +6,22 @@ feature DAG (11 transform nodes, 124 feature groups) computed in FP64.  One
 step = one pass of the DAG over one batch of images already resident in
 HBM (wndgpu_compute_device); value = images/s over the timed steps.
 
-Also reported: the roofline position of the dominant kernel (per-launch HIP
-events on the engine's stream during the timed region) and the CPU baseline
-(the FP64 C++ oracle, one image per thread on the host cores, on a bounded
-sample of the same images).
+Also reported:
+  roofline      the dominant single-kernel launch (largest mean HIP-event
+                time over the timed steps, events recorded on the engine's
+                stream around every launch) against MI355X's FP64 peak:
+                k_har_hess does (exactly) hess_flops(Ng) per Q matrix,
+                k_har_qr reports the flops its Francis sweeps issued;
+                traffic (HBM bytes per launch) is read from the rocprofv3
+                --pmc summary committed under profiles/ when present.
+  cpu_baseline  the FP64 C++ oracle, one image per thread on the host
+                cores, on a bounded sample of the same images.
 """
 import argparse
 import ctypes
 import json
 import os
+import re
 import sys
 import threading
 import time
@@ -23,6 +30,25 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, 'wnd-charm_amd'))
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+
+
+def hess_flops(n):
+    """Floating-point operations of the Householder Hessenberg reduction of an
+    n x n matrix as k_har_hess (and the oracle) perform it: per column i
+    (rem = n-i-1 >= 2) the reflector (2(rem-1) for the norm, 4 for beta/tau,
+    rem-1 divisions), A = H A over rem columns and A = A H over n rows, each
+    line costing 2(rem-1)+1 for the dot product and 2+3(rem-1) for the update."""
+    f = 0
+    for i in range(n - 1):
+        rem = n - i - 1
+        if rem == 1:
+            f += 1 + n
+            continue
+        line = 2 * (rem - 1) + 1 + 2 + 3 * (rem - 1)
+        f += 3 * (rem - 1) + 4 + rem * line + n * line
+    return f
 
 
 def cpu_baseline(imgs, seconds=12.0):
@@ -52,21 +78,42 @@ def cpu_baseline(imgs, seconds=12.0):
     for t in th:
         t.join()
     el = time.time() - t0
-    return {'value': done[0] / el, 'unit': 'feature vectors/s', 'cores': cores, 'kind': 'port',
+    return {'value': round(done[0] / el, 3), 'unit': 'feature vectors/s', 'cores': cores, 'kind': 'port',
             'sample': '%d long-set vectors of 256x256 images in %.1f s, one image per thread' % (done[0], el)}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from profiles/<kernel>_traffic.txt
+    (a line 'traffic_bytes_per_launch: X'), or None."""
+    path = os.path.join(ROOT, 'profiles', '%s_traffic.txt' % kernel)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r'traffic_bytes_per_launch:\s*([0-9.eE+]+)', open(path).read())
+    return float(m.group(1)) if m else None
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--batch', type=int, default=400)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--copy-probe', action='store_true',
+                    help='only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'copy_probe_bytes': src.numel() * 8, 'launches': 3}))
+        return
     if args.gpus != 1:
-        print('bench.py measures one GPU: every run here uses a single MI355X; '
-              'multi-GPU sharding is described in DESIGN.md but not run', file=sys.stderr)
+        print('bench.py measures one GPU: this build runs and reports a single MI355X '
+              '(images shard across GPUs with no exchange; see DESIGN.md)', file=sys.stderr)
         sys.exit(2)
     import torch
     import wndcharm
@@ -90,26 +137,38 @@ def main():
         if rc != 0:
             raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
 
+    # per-launch HIP events on the engine stream, recorded inside the timed steps
+    os.environ['WNDGPU_TIMING'] = '1'
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    acc = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
+        for name, ms in wndcharm.last_node_times():
+            acc[name] = acc.get(name, 0.0) + ms
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_step = el * 1000 / args.steps
     value = len(imgs) * args.steps / el
-    # per-launch times of one more pass (events on the engine stream)
-    os.environ['WNDGPU_TIMING'] = '1'
-    step()
-    times = wndcharm.last_node_times()
-    del os.environ['WNDGPU_TIMING']
-    agg = {}
-    for name, ms in times:
-        grp = name.split(' (')[0]
-        agg[grp] = agg.get(grp, 0.0) + ms
-    top = sorted(agg.items(), key=lambda kv: -kv[1])
+    mean = {k: v / args.steps for k, v in acc.items()}
+
+    # dominant single-kernel launch and its FP64 roofline
+    kern = {k: v for k, v in mean.items() if '[k_' in k}
+    dom = max(kern, key=kern.get)
+    kname = re.search(r'\[(k_\w+)\]', dom).group(1)
+    tones, schur = wndcharm.last_haralick_work()
+    flops = float(4 * sum(hess_flops(int(n)) for n in tones)) if kname == 'k_har_hess' else float(schur.sum())
+    achieved = flops / (mean[dom] * 1e-3) / 1e12
+    roof = {'kernel': kname, 'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': FP64_PEAK_TFLOPS,
+            'unit': 'TFLOP/s', 'frac': round(achieved / FP64_PEAK_TFLOPS, 5), 'traffic': profiled_traffic(kname),
+            'launch_ms': round(mean[dom], 3), 'flops_per_launch': flops, 'matrices_per_launch': 4 * len(tones)}
+    stage = {}
+    for k, v in mean.items():
+        key = k if '[' in k else k.split(' (')[0]
+        stage[key] = stage.get(key, 0.0) + v
+    top = sorted(stage.items(), key=lambda kv: -kv[1])
     line = {
         'metric': 'feature vectors/sec (2919-feat)', 'value': round(value, 3), 'unit': 'feature vectors/s',
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_step, 3),
@@ -117,7 +176,8 @@ def main():
         'data': 'synthetic (seeded grating/blob/Poisson generator, 256x256 uint16)',
         'config': {'workload': 'long-set 2919 features, %d images 256x256 per step, plain FP64' % len(imgs),
                    'images_per_step': len(imgs), 'set': 'long', 'precision': 'fp64'},
-        'stage_ms': {k: round(v, 3) for k, v in top[:12]},
+        'roofline': roof,
+        'stage_ms': {k: round(v, 3) for k, v in top[:14]},
     }
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(imgs[:64])

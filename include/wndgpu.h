@@ -109,10 +109,19 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
                 int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
                 double *norm_factor, long *predicted);
 
-/* Per-node timings of the last wndgpu_compute (ms, one entry per DAG node
- * in execution order), the counterpart of the reference's TIME lines
+/* Per-launch timings of the last wndgpu_compute / wndgpu_compute_device made
+ * with WNDGPU_TIMING set in the environment (ms from a HIP event pair around
+ * each launch, in execution order; launches are named by feature group, and
+ * the Haralick group by step), the counterpart of the reference's TIME lines
  * (src/Tasks.cpp:249-253).  Returns the number of entries written. */
 int wndgpu_last_node_times(wndgpu_ctx *ctx, double *ms, const char **names, int cap);
+
+/* Work record of the Haralick groups of the last compute, for roofline
+ * accounting: tones[k] = Ng of Haralick entry k (one (image, node) pair),
+ * schur_flops[4k+a] = floating-point operations the Francis QR issued for
+ * angle a.  Returns the number of entries (<= cap are written).  No
+ * counterpart in the reference (Ng is local to CVIPtexture.cpp:400-414). */
+int wndgpu_last_haralick_work(wndgpu_ctx *ctx, int *tones, double *schur_flops, int cap);
 
 #ifdef __cplusplus
 }

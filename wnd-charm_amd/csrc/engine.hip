@@ -204,6 +204,14 @@ struct wndgpu_ctx {
   std::map<int, DevTable> comb;
   DevTable gabor, zern, rtrig;
   std::vector<std::pair<std::string, double>> times;
+  // per-launch timing (WNDGPU_TIMING): an event pair around every launch,
+  // read after the compute call has synchronised
+  std::vector<hipEvent_t> evpool;
+  std::vector<std::string> tnames;
+  // Haralick entries of the last compute: tone counts and Schur-iteration flops
+  int *har_ng = nullptr;
+  double *har_work = nullptr;
+  int har_n = 0;
   bool sized = false;
 };
 
@@ -354,6 +362,7 @@ static void out_size(int kind, int h, int w, int *oh, int *ow) {
 // also appended to `launches` in execution order.
 static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, Arena &A,
                        std::vector<std::pair<std::string, std::function<void()>>> &launches) {
+  if (!A.sizing) c->har_n = 0;
   hipStream_t s = c->stream;
   const int n = B.n;
   std::map<std::vector<int>, NodeRun> nodes;
@@ -560,6 +569,8 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
       case A_HARALICK: {
         std::vector<HarDesc> hd(ne);
         int *lut_all = A.dalloc_n<int>(256 * (size_t)ne);
+        int *ng_all = A.dalloc_n<int>(ne);
+        double *work_all = A.dalloc_n<double>(4 * (size_t)ne);
         for (int k = 0; k < ne; k++) {
           HarDesc &H = hd[k];
           H.img = nullptr;
@@ -569,16 +580,25 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           H.Q = A.dalloc_n<double>(4 * 256 * 256);
           H.feat = A.dalloc_n<double>(4 * 14);
           H.marg = A.dalloc_n<double>(4 * 2 * 256);
-          H.ng = A.dalloc_n<int>(1);
+          H.work = work_all + 4 * k;
+          H.ng = ng_all + k;
           H.lut = lut_all + 256 * k;
           H.count = A.dalloc_n<int>(4);
         }
+        if (!A.sizing) {
+          c->har_ng = ng_all;
+          c->har_work = work_all;
+          c->har_n = ne;
+        }
         HarDesc *hdd = A.stage(hd);
-        push(gn, [=]() {
+        push(gn + " [co-occurrence, f1-f13, Q]", [=]() {
           gather();
           hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)ne, s);
-          launch_haralick(ni, nst, hdd, o, ne, (int)maxp, s);
+          launch_haralick_prep(ni, nst, hdd, ne, (int)maxp, s);
         });
+        push(gn + " [k_har_hess]", [=]() { launch_haralick_hess(hdd, ne, s); });
+        push(gn + " [k_har_qr]", [=]() { launch_haralick_qr(hdd, ne, s); });
+        push(gn + " [mean, range]", [=]() { launch_haralick_final(hdd, o, ne, s); });
       } break;
       case A_ZERNIKE: {
         const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
@@ -812,25 +832,20 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   if ((r = build_batch(c, plan, B, A2, launches))) return r;
   HIPCHK(c, hipMemcpyAsync(c->hdesc, c->host, A2.h, hipMemcpyHostToDevice, c->stream));
   c->times.clear();
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (timing) {
-    HIPCHK(c, hipEventCreate(&e0));
-    HIPCHK(c, hipEventCreate(&e1));
-  }
-  for (auto &L : launches) {
-    if (timing) hipEventRecord(e0, c->stream);
-    L.second();
-    if (timing) {
-      hipEventRecord(e1, c->stream);
-      hipEventSynchronize(e1);
-      float ms = 0;
-      hipEventElapsedTime(&ms, e0, e1);
-      c->times.emplace_back(L.first, ms);
+  c->tnames.clear();
+  if (timing)
+    while (c->evpool.size() < 2 * launches.size()) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      c->evpool.push_back(e);
     }
-  }
-  if (timing) {
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+  for (size_t k = 0; k < launches.size(); k++) {
+    if (timing) hipEventRecord(c->evpool[2 * k], c->stream);
+    launches[k].second();
+    if (timing) {
+      hipEventRecord(c->evpool[2 * k + 1], c->stream);
+      c->tnames.push_back(launches[k].first);
+    }
   }
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -868,6 +883,7 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   for (auto &kv : c->dspos) fr(kv.second);
   for (auto &kv : c->comb) fr(kv.second);
   fr(c->gabor); fr(c->zern); fr(c->rtrig);
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->dev) hipFree(c->dev);
   if (c->hdesc) hipFree(c->hdesc);
   if (c->host) hipHostFree(c->host);
@@ -992,6 +1008,12 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   }
   hipFreeAsync(dout, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < c->tnames.size(); k++) {
+    float ms = 0;
+    hipEventElapsedTime(&ms, c->evpool[2 * k], c->evpool[2 * k + 1]);
+    c->times.emplace_back(c->tnames[k], ms);
+  }
+  c->tnames.clear();
   return rejected ? fail(c, WNDGPU_E_IMAGE, "some images were rejected (see status)") : WNDGPU_OK;
 }
 
@@ -1184,6 +1206,15 @@ int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f
   HIPCHK(c, hipMemcpy(norm, nm.p, (size_t)ntest * 8, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(pred, pd.p, (size_t)ntest * sizeof(long), hipMemcpyDeviceToHost));
   return WNDGPU_OK;
+}
+
+int wndgpu_last_haralick_work(wndgpu_ctx *c, int *tones, double *schur_flops, int cap) {
+  if (!c || c->har_n == 0) return 0;
+  const int k = cap < c->har_n ? cap : c->har_n;
+  if (k > 0 && tones) HIPCHK(c, hipMemcpy(tones, c->har_ng, (size_t)k * sizeof(int), hipMemcpyDeviceToHost));
+  if (k > 0 && schur_flops)
+    HIPCHK(c, hipMemcpy(schur_flops, c->har_work, 4 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost));
+  return c->har_n;
 }
 
 int wndgpu_last_node_times(wndgpu_ctx *c, double *ms, const char **names, int cap) {

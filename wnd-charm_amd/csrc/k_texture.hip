@@ -430,6 +430,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   // thread 0; the block follows through s_iu / si / sc
   int iu = n - 1, iter = 0, totalIter = 0;
   double exshift = 0;
+  double work = 0;  // thread 0: flops of the updates issued (12 per 3-vector, 6 per 2-vector / rotation)
   if (t == 0) s_iu = n - 1;
   __syncthreads();
   while (norm != 0) {
@@ -484,6 +485,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
           }
         }
         si[1] = iu;  // rows 0..iu get the right rotation
+        if (q >= 0) work += 6.0 * (2 + iu + 1);
         iu -= 2; iter = 0;
       } else {  // Francis QR step (RealSchur::computeShift)
         double sh0 = AT(iu, iu), sh1 = AT(iu - 1, iu - 1), sh2 = AT(iu, iu - 1) * AT(iu - 1, iu);
@@ -512,6 +514,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         if (totalIter > 40 * n) action = 3;
         else {
           action = 2;
+          if (shift_kind) work += iu + 1;
           si[4] = shift_kind;
           sc[3] = shift_amt;
           sc[4] = sh0; sc[5] = sh1; sc[6] = sh2;
@@ -562,6 +565,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       if (fabs(lhs) < eps * rhs) { cand = m; break; }
     }
     const int im = block_reduce(cand, MaxOp(), ired);
+    if (t == 0)
+      for (int k = im; k <= iuq - 2; k++) work += 12.0 * ((iuq - k + 1) + ((iuq < k + 3 ? iuq : k + 3) + 1));
     double fv0, fv1, fv2;
     {
       const double Tmm = AT(im, im);
@@ -719,7 +724,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       double e1[1], tau, beta;
       make_hh(x, 2, e1, &tau, &beta);
       si[6] = beta != 0;
-      if (beta != 0) AT(iuq - 1, iuq - 2) = beta;
+      if (beta != 0) { AT(iuq - 1, iuq - 2) = beta; work += 6.0 * (2 + iuq + 1); }
       sc[3] = tau; sc[7] = e1[0];
     }
     __syncthreads();
@@ -767,6 +772,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       f = (im2 == 0) ? sqrt(re2) : 0.0;
     }
     H.feat[angle * 14 + 13] = f;
+    H.work[angle] = work;
   }
 #undef AT
 }
@@ -793,8 +799,7 @@ __global__ void k_har_final(const HarDesc *hd, OutRef o) {
   for (int a = 0; a < 28; a++) out[a] = temp[ord[a]];
 }
 
-void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef o, int nimg, int maxn,
-                     hipStream_t s) {
+void launch_haralick_prep(const Img *imgs, const Stats *st, const HarDesc *hd, int nimg, int maxn, hipStream_t s) {
   int blocks = (maxn + 255) / 256;
   if (blocks > 1024) blocks = 1024;
   k_har_quant<<<dim3(blocks, nimg), 256, 0, s>>>(imgs, st, hd);
@@ -805,8 +810,14 @@ void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef
   k_har_cooc<<<dim3(4, nimg), 1024, 256 * 257 / 2 * sizeof(uint32_t), s>>>(imgs, hd);
   k_har_stats<<<dim3(4, nimg), 256, 0, s>>>(hd);
   k_har_q<<<dim3(4 * QRB, nimg), 256, 0, s>>>(hd);
+}
+void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s) {
   k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
+}
+void launch_haralick_qr(const HarDesc *hd, int nimg, hipStream_t s) {
   k_har_qr<<<dim3(4, nimg), 256, 0, s>>>(hd);
+}
+void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s) {
   k_har_final<<<nimg, 64, 0, s>>>(hd, o);
 }
 

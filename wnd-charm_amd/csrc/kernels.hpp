@@ -74,12 +74,17 @@ struct HarDesc {
   double *Q;            // [4][ng*ng]
   double *feat;         // [4][14]
   double *marg;         // [4][2][256] row / column marginals px, py
+  double *work;         // [4] floating-point operations of the Schur iteration (reported, not used)
   int *ng;              // number of tones present
   int *lut;             // [256] tone -> rank
   int *count;           // [4] pair counts
 };
-void launch_haralick(const Img *imgs, const Stats *st, const HarDesc *hd, OutRef o, int nimg, int maxn,
-                     hipStream_t s);
+// Haralick in four stream-ordered steps: quantise / co-occurrence / f1..f13
+// / Q, Hessenberg reduction of Q, Francis QR (f14), mean and range output
+void launch_haralick_prep(const Img *imgs, const Stats *st, const HarDesc *hd, int nimg, int maxn, hipStream_t s);
+void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s);
+void launch_haralick_qr(const HarDesc *hd, int nimg, hipStream_t s);
+void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s);
 void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, OutRef o, int nimg, int maxh, int maxw,
                    hipStream_t s);
 void launch_comb(const Img *imgs, const double *iis, OutRef o, int nimg, hipStream_t s);

@@ -67,6 +67,7 @@ def _load():
         'wndgpu_fisher': (_I, [_P, _P, _P, _I, _I, _I, _D, _P]),
         'wndgpu_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
         'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
+        'wndgpu_last_haralick_work': (_I, [_P, _P, _P, _I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -405,3 +406,13 @@ def last_node_times():
     names = (ctypes.c_char_p * 1024)()
     k = lib.wndgpu_last_node_times(_context(), _ptr(ms), names, 1024)
     return [(names[i].decode(), float(ms[i])) for i in range(k)]
+
+
+def last_haralick_work():
+    """(tones [k], schur_flops [k, 4]) of the Haralick entries of the last compute."""
+    lib = _load()
+    n = lib.wndgpu_last_haralick_work(_context(), None, None, 0)
+    tones = np.zeros(max(n, 1), np.int32)
+    flops = np.zeros((max(n, 1), 4))
+    lib.wndgpu_last_haralick_work(_context(), _ptr(tones), _ptr(flops), n)
+    return tones[:n], flops[:n]
