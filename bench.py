@@ -99,6 +99,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--batch', type=int, default=400)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--host-io', action='store_true',
+                    help='only time wndgpu_compute with host image and feature buffers (PCIe-inclusive rate)')
     ap.add_argument('--copy-probe', action='store_true',
                     help='only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
@@ -122,6 +124,17 @@ def main():
     imgs, cls = synthetic.config_images('long_256', limit=args.batch)
     plan = wndcharm.StdFeatureComputationPlans.getFeatureSetLong()
     nf = plan.n_features
+    if args.host_io:
+        wndcharm.compute_features(imgs, plan)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            wndcharm.compute_features(imgs, plan)
+        el = time.perf_counter() - t0
+        print(json.dumps({'host_io_feature_vectors_per_s': round(len(imgs) * args.steps / el, 3),
+                          'images_per_step': len(imgs), 'steps': args.steps,
+                          'note': 'uint16 images and FP64 features in host memory, transfers included'}))
+        return
     dev_imgs = [torch.from_numpy(im.view(np.int16)).cuda() for im in imgs]
     out = torch.zeros((len(imgs), nf), dtype=torch.float64, device='cuda')
     structs = (wndcharm._Image * len(imgs))()

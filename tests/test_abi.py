@@ -55,3 +55,20 @@ def test_custom_plan_parsing():
     bad.add('No Such Algorithm ()')
     with pytest.raises(wndcharm.WndGpuError):
         bad.finalize()
+
+
+def test_cpp_facade_fails_loudly_without_gpu(tmp_path):
+    """The compiled C++ caller has no host fallback: with no GPU it reports
+    the engine error and exits non-zero."""
+    exe = os.path.join(ROOT, 'wnd-charm_amd', 'lib', 'facade_check')
+    if not os.path.exists(exe):
+        pytest.skip('facade_check not built')
+    if os.path.exists('/dev/kfd'):
+        pytest.skip('a GPU is visible here; covered by tests/test_gpu_facade.py')
+    import subprocess
+    import numpy as np
+    src = str(tmp_path / 'img.f64')
+    np.zeros((32, 32)).tofile(src)
+    r = subprocess.run([exe, src, '32', '32', str(tmp_path / 'out.f64')], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert 'no usable MI355X GPU' in r.stderr
