@@ -4,7 +4,8 @@
  * out-of-line members).  TEST INFRASTRUCTURE: used only to pin the oracle.
  *
  *   ref_texture  -> Extract_Texture_Features (src/textures/haralick/CVIPtexture.cpp:384-452)
- *   ref_eigen    -> Eigen 3.1.2 EigenSolver<MatrixXd>(Q,false).eigenvalues() (src/Eigen)
+ *   ref_eigen    -> Eigen 3.1.2 eigenvalues() of a row-major matrix, the call
+ *                   f14_maxcorr makes on Q.matrix() (CVIPtexture.cpp:1015)
  *   ref_radon    -> radon + vd_RadonTextures input path (src/transforms/radon.cpp:14-91)
  *   ref_wavelet  -> Symlet5(0,1).transform2D on a DataGrid2D
  *                   (src/transforms/wavelet/Symlet5.cpp:10-59, Wavelet.cpp:136-434)
@@ -39,7 +40,7 @@ int ref_texture(const unsigned char *grays, int rows, int cols, int angle, doubl
 }
 
 int ref_eigen(const double *Q, int n, double *re, double *im) {
-  Eigen::MatrixXd m(n, n);
+  Eigen::Matrix<double, Eigen::Dynamic, Eigen::Dynamic, Eigen::RowMajor> m(n, n);
   for (int i = 0; i < n; i++) for (int j = 0; j < n; j++) m(i, j) = Q[(size_t)i * n + j];
   Eigen::VectorXcd ev = m.eigenvalues();
   for (int i = 0; i < n; i++) { re[i] = ev(i).real(); im[i] = ev(i).imag(); }

@@ -22,6 +22,38 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- Eigen 3.1.2 evaluation orders (SSE2, two doubles per packet) used by
+// the reference's f14 on row-major arrays (CVIPtexture.cpp:961-1023):
+// eigen_redux is DenseBase::sum() over a contiguous range with linear
+// vectorized traversal (Core/Redux.h:193-248); s is the index of the first
+// 16-byte-aligned term.  x(k) supplies term k.
+template <class F>
+__device__ __forceinline__ double eigen_redux(int m, int s, F x) {
+  if (s > m) s = m;
+  const int as = ((m - s) / 2) * 2, as2 = ((m - s) / 4) * 4;
+  const int end = s + as, end2 = s + as2;
+  double res;
+  if (as) {
+    double a0 = x(s), a1 = x(s + 1);
+    if (as > 2) {
+      double b0 = x(s + 2), b1 = x(s + 3);
+      for (int k = s + 4; k < end2; k += 4) {
+        a0 = a0 + x(k); a1 = a1 + x(k + 1);
+        b0 = b0 + x(k + 2); b1 = b1 + x(k + 3);
+      }
+      a0 = a0 + b0; a1 = a1 + b1;
+      if (end > end2) { a0 = a0 + x(end2); a1 = a1 + x(end2 + 1); }
+    }
+    res = a0 + a1;
+    for (int k = 0; k < s; k++) res = res + x(k);
+    for (int k = end; k < m; k++) res = res + x(k);
+  } else {
+    res = x(0);
+    for (int k = 1; k < m; k++) res = res + x(k);
+  }
+  return res;
+}
+
 // ===================================================================== Haralick
 // quantisation (haralick.cpp:29-37) and tone presence
 __global__ void k_har_quant(const Img *imgs, const Stats *st, const HarDesc *hd) {
@@ -196,19 +228,51 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     F[11] = (mxh == 0) ? 1.0 : (hxy - hxy1) / mxh;
     F[12] = sqrt(fabs(1 - exp(-2.0 * (hxy2 - hxy))));
   }
+  // marginals of _Q_calc_eigen: px = P.rowwise().sum() (row t of the
+  // row-major array starts at linear index t*ng), py = P.colwise().sum()
+  // (strided: a running sum, equal to py above)
   if (t < ng) {
-    H.marg[angle * 512 + t] = px[t];
+    const double *row = P + (long)t * ng;
+    H.marg[angle * 512 + t] = eigen_redux(ng, (int)(((long)t * ng) & 1), [&](int k) { return row[k]; });
     H.marg[angle * 512 + 256 + t] = py[t];
   }
 }
 
 // Q (_Q_calc_eigen, CVIPtexture.cpp:961-987):
-//   Q(i,j) = sum_k P(i,k) P(j,k) / (px(i) py(k) + 1e-13), accumulated in k order.
-// A term whose product is zero adds +0.0 and leaves the sum bit-identical,
-// so each row visits only the k with P(i,k) != 0 (compacted per wavefront)
-// and skips P(j,k) == 0.  P is symmetric (pairs are counted both ways), so
-// P(j,k) is read as P(k,j): along row k, coalesced over j.
+//   Q(i,j) = (P.row(i)*P.row(j) / (px(i) py + 1e-13)).sum(),
+// an Eigen redux over an aligned expression: terms k < 4*floor(Ng/4) go to
+// four accumulators by k mod 4, which are paired (0+2, 1+3); the next two
+// terms (if Ng mod 4 >= 2) join the pairs, the pairs are added, and an odd
+// last term comes last.  Every term is >= +0, so an accumulator that
+// starts at +0 and skips zero terms holds the same bits: each row visits
+// only the k with P(i,k) != 0 (compacted per wavefront), and skips
+// P(j,k) == 0.  P is symmetric (pairs are counted both ways), so P(j,k) is
+// read as P(k,j): along row k, coalesced over j.
 constexpr int QRB = 4;  // row blocks per angle
+struct Redux4 {
+  double a[4], e0, e1, tl;
+};
+__device__ __forceinline__ void r4_add(Redux4 &r, int k, int ng, double term) {
+  const int as2 = (ng / 4) * 4, as = (ng / 2) * 2;
+  if (ng >= 4 && k < as2) r.a[k & 3] += term;
+  else if (ng >= 4 && k < as) { if (k == as2) r.e0 += term; else r.e1 += term; }
+  else if (ng < 4 && k < as) r.a[k] += term;  // ng 2..3: the single packet (x0, x1)
+  else r.tl += term;                           // odd last term (or x0 when ng == 1)
+}
+__device__ __forceinline__ double r4_finish(const Redux4 &r, int ng) {
+  if (ng == 1) return r.tl;
+  double res;
+  if (ng >= 4) {
+    double l0 = r.a[0] + r.a[2], l1 = r.a[1] + r.a[3];
+    if ((ng / 2) * 2 > (ng / 4) * 4) { l0 = l0 + r.e0; l1 = l1 + r.e1; }
+    res = l0 + l1;
+  } else {
+    res = r.a[0] + r.a[1];
+  }
+  if (ng & 1) res = res + r.tl;
+  return res;
+}
+
 __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd) {
   __shared__ int lk[4][256];
   __shared__ double la[4][256], ld[4][256];
@@ -230,28 +294,23 @@ __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd) {
         const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
         lk[w][pos] = k;
         la[w][pos] = a;
-        ld[w][pos] = pxi * py[k] + 0.0000000000001;
+        ld[w][pos] = py[k] * pxi + 0.0000000000001;
       }
       cnt += __popcll(m);
     }
     wave_sync();
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    const int j0 = lane, j1 = lane + 64, j2 = lane + 128, j3 = lane + 192;
-    for (int kk = 0; kk < cnt; kk++) {
-      const double *row = P + (long)lk[w][kk] * ng;
-      const double a = la[w][kk], d = ld[w][kk];
-      const double b0 = j0 < ng ? row[j0] : 0.0, b1 = j1 < ng ? row[j1] : 0.0;
-      const double b2 = j2 < ng ? row[j2] : 0.0, b3 = j3 < ng ? row[j3] : 0.0;
-      if (b0 != 0.0) s0 += (a * b0) / d;
-      if (b1 != 0.0) s1 += (a * b1) / d;
-      if (b2 != 0.0) s2 += (a * b2) / d;
-      if (b3 != 0.0) s3 += (a * b3) / d;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int j = lane + 64 * q;
+      if (j >= ng) break;
+      Redux4 r = {{0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
+      for (int kk = 0; kk < cnt; kk++) {
+        const int k = lk[w][kk];
+        const double b = P[(long)k * ng + j];
+        if (b != 0.0) r4_add(r, k, ng, (la[w][kk] * b) / ld[w][kk]);
+      }
+      Q[(long)i * ng + j] = r4_finish(r, ng);
     }
-    double *qr = Q + (long)i * ng;
-    if (j0 < ng) qr[j0] = s0;
-    if (j1 < ng) qr[j1] = s1;
-    if (j2 < ng) qr[j2] = s2;
-    if (j3 < ng) qr[j3] = s3;
     wave_sync();  // the lists are rebuilt for the next row
   }
 }
@@ -336,24 +395,49 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       __syncthreads();
       continue;
     }
-    // A = H A on rows i+1..n-1, columns i+1..n-1 (HB loads in flight per thread)
+    // A = H A on rows i+1..n-1, columns i+1..n-1.  The column dot product
+    // follows Eigen's ColMajor GEMV on bottom^T (general_matrix_vector_product
+    // <ColMajor>, GeneralMatrixVector.h:31-285): inner terms in groups of four
+    // (g, g+o1, g+2, g+o3), added pairwise for outputs in the packet range and
+    // left to right for an odd last output; leftover terms one by one; a
+    // skipped leading term last.
+    const bool odd_n = n & 1;
+    const int o1 = odd_n ? 3 : 1, o3 = odd_n ? 1 : 3;
     for (int c = i + 1 + t; c < n; c += blockDim.x) {
       const double *p = &AT(i + 2, c);
       const int m = rem - 1;
-      double acc = 0;
-      int r = 0;
-      for (; r + HB <= m; r += HB) {
-        double v[HB];
-#pragma unroll
-        for (int j = 0; j < HB; j++) v[j] = p[(long)(r + j) * n];
-#pragma unroll
-        for (int j = 0; j < HB; j++) acc += ess[r + j] * v[j];
+      const long L = (long)(i + 2) * n + (i + 1);
+      const int skip = ((L & 1) && odd_n && m > 0) ? 1 : 0;
+      const bool packet = (c - (i + 1)) < (rem & ~1);
+      const int bound = ((m - skip) / 4) * 4 + skip;
+      double acc = 0.0;
+      int g = skip;
+      for (; g + 4 < bound; g += 8) {  // two groups per pass: eight loads in flight
+        const double a0 = p[(long)g * n] * ess[g], b0 = p[(long)(g + o1) * n] * ess[g + o1];
+        const double c0 = p[(long)(g + 2) * n] * ess[g + 2], d0 = p[(long)(g + o3) * n] * ess[g + o3];
+        const double a1 = p[(long)(g + 4) * n] * ess[g + 4], b1 = p[(long)(g + 4 + o1) * n] * ess[g + 4 + o1];
+        const double c1 = p[(long)(g + 6) * n] * ess[g + 6], d1 = p[(long)(g + 4 + o3) * n] * ess[g + 4 + o3];
+        if (packet) {
+          acc = acc + ((a0 + b0) + (c0 + d0));
+          acc = acc + ((a1 + b1) + (c1 + d1));
+        } else {
+          acc = (((acc + a0) + b0) + c0) + d0;
+          acc = (((acc + a1) + b1) + c1) + d1;
+        }
       }
-      for (; r < m; r++) acc += ess[r] * p[(long)r * n];
+      for (; g < bound; g += 4) {
+        const double a0 = p[(long)g * n] * ess[g], b0 = p[(long)(g + o1) * n] * ess[g + o1];
+        const double c0 = p[(long)(g + 2) * n] * ess[g + 2], d0 = p[(long)(g + o3) * n] * ess[g + o3];
+        if (packet) acc = acc + ((a0 + b0) + (c0 + d0));
+        else acc = (((acc + a0) + b0) + c0) + d0;
+      }
+      for (int j = bound; j < m; j++) acc = acc + p[(long)j * n] * ess[j];
+      if (skip) acc = acc + p[0] * ess[0];
       const double tt = acc + AT(i + 1, c);
       AT(i + 1, c) -= tau * tt;
       double *q = &AT(i + 2, c);
-      for (r = 0; r + HB <= m; r += HB) {
+      int r = 0;
+      for (; r + HB <= m; r += HB) {
         double v[HB];
 #pragma unroll
         for (int j = 0; j < HB; j++) v[j] = q[(long)(r + j) * n];
@@ -363,8 +447,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       for (; r < m; r++) q[(long)r * n] -= (tau * ess[r]) * tt;
     }
     __syncthreads();
-    // A = A H on rows 0..n-1, columns i+1..n-1
-    double acc2 = 0;
+    // A = A H on rows 0..n-1, columns i+1..n-1.  The row dot product follows
+    // Eigen's RowMajor GEMV with an aligned right-hand side
+    // (GeneralMatrixVector.h:289-552): even and odd inner terms in two lanes,
+    // the lanes added, then an odd last term.
+    const int mr = rem - 1, asz = mr & ~1;
+    double l0 = 0.0, l1 = 0.0, tl = 0.0;
     const int lc = t & (HC - 1), lr = t / HC;
     for (int ct = 1; ct < rem; ct += HC) {
       const int ncol = (rem - ct < HC) ? rem - ct : HC;
@@ -374,10 +462,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       }
       __syncthreads();
       if (t < n)
-        for (int cc = 0; cc < ncol; cc++) acc2 += tile[t * HCP + cc] * ess[ct + cc - 1];
+        for (int cc = 0; cc < ncol; cc++) {
+          const int j = ct + cc - 1;
+          const double x = tile[t * HCP + cc] * ess[j];
+          if (j < asz) {
+            if (j & 1) l1 = l1 + x;
+            else l0 = l0 + x;
+          } else {
+            tl = x;
+          }
+        }
       __syncthreads();
     }
-    if (t < n) tv[t] = acc2 + AT(t, i + 1);
+    if (t < n) {
+      double tmp = 0.0;
+      if (asz > 0) tmp = tmp + (l0 + l1);
+      if (mr & 1) tmp = tmp + tl;
+      tv[t] = (0.0 + tmp) + AT(t, i + 1);
+    }
     __syncthreads();
     // rank-1 update, thread per column (column i+1 has factor 1: x*1.0 is exact)
     for (int cc = t; cc < rem; cc += blockDim.x) {
@@ -416,15 +518,23 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   // ---- real Schur form (RealSchur::compute without U); left updates are
   // restricted to columns <= iu (entries right of the active window are
   // never read again), right updates cover rows 0..min(iu,k+3) as in Eigen.
-  double norm;
-  {
-    double s = 0;
-    for (int e = t; e < n * n; e += blockDim.x) {
-      const int r = e / n, c = e - r * n;
-      if (c >= (r - 1 > 0 ? r - 1 : 0)) s += fabs(A[e]);
-    }
-    norm = block_reduce(s, SumOp(), dred);
+  // RealSchur::computeNormOfT (RealSchur.h:274-285) on row-major storage:
+  // each row segment T(j, max(j-1,0)..n-1) is a contiguous Eigen redux, and
+  // the segment sums are added row by row
+  __shared__ double segs[256];
+  for (int j = t; j < n; j += blockDim.x) {
+    const int k0 = j - 1 > 0 ? j - 1 : 0;
+    const double *row = A + (long)j * n + k0;
+    segs[j] = eigen_redux(n - k0, (int)(((long)j * n + k0) & 1), [&](int k) { return fabs(row[k]); });
   }
+  __syncthreads();
+  if (t == 0) {
+    double sn = 0;
+    for (int j = 0; j < n; j++) sn += segs[j];
+    dred[0] = sn;
+  }
+  __syncthreads();
+  const double norm = dred[0];
   const double eps = kDblEps;
   // decision state (iu, iteration counts, exceptional shift) lives in
   // thread 0; the block follows through s_iu / si / sc
