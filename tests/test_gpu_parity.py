@@ -67,6 +67,23 @@ def test_algorithm(alg, src, node):
     compare(got, want, mask, '%s on %s %s' % (ALG_NAMES[alg], src, node))
 
 
+@pytest.mark.parametrize('node', ['root', 'F', 'W', 'C', 'E'])
+@pytest.mark.parametrize('src', ['t1_113', 'synth_128'])
+def test_haralick_f14_bit_exact(src, node):
+    """Haralick f14 (mean and range over the angles of sqrt(eigenvalue 2 of
+    Q)) equals the oracle bit for bit: Q, the Hessenberg reduction and the
+    Francis QR run in the reference's Eigen orders.  (f1..f13 use parallel
+    sums and are held to 1e-6 by test_algorithm.)"""
+    print('haralick exact', src, node, flush=True)
+    img = root_images()[src]
+    if node != 'root':
+        img = orc_transform({'F': 0, 'C': 1, 'W': 2, 'E': 3}[node], img)
+    got = np.asarray(wndcharm.FeatureAlgorithm('Haralick Textures').execute(img))
+    want = orc_algorithm(ALG_NAMES.index('Haralick Textures'), img)
+    f14 = [16, 17]
+    assert np.array_equal(got[f14], want[f14]), (got[f14], want[f14])
+
+
 def test_long_set_batch_uneven_sizes():
     imgs = [load_npz('img_t1_s01_c05_ij.npz')['pixels'], synthetic.image(2002, 6, (128, 128)),
             synthetic.image(2003, 24, (96, 140))]

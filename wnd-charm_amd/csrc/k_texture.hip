@@ -250,24 +250,33 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
 // read as P(k,j): along row k, coalesced over j.
 constexpr int QRB = 4;  // row blocks per angle
 struct Redux4 {
-  double a[4], e0, e1, tl;
+  double a0, a1, a2, a3, e0, e1, tl;
 };
-__device__ __forceinline__ void r4_add(Redux4 &r, int k, int ng, double term) {
-  const int as2 = (ng / 4) * 4, as = (ng / 2) * 2;
-  if (ng >= 4 && k < as2) r.a[k & 3] += term;
-  else if (ng >= 4 && k < as) { if (k == as2) r.e0 += term; else r.e1 += term; }
-  else if (ng < 4 && k < as) r.a[k] += term;  // ng 2..3: the single packet (x0, x1)
-  else r.tl += term;                           // odd last term (or x0 when ng == 1)
+// k is the same for every lane of a wavefront, so the branches are uniform
+__device__ __forceinline__ void r4_add(Redux4 &r, int k, int as2, int as, double term) {
+  if (k < as2 || (as2 == 0 && k < as)) {  // four accumulators; ng 2..3: the single packet (x0, x1)
+    switch (k & 3) {
+      case 0: r.a0 += term; break;
+      case 1: r.a1 += term; break;
+      case 2: r.a2 += term; break;
+      default: r.a3 += term; break;
+    }
+  } else if (k < as) {
+    if (k == as2) r.e0 += term;
+    else r.e1 += term;
+  } else {
+    r.tl += term;  // odd last term (or x0 when ng == 1)
+  }
 }
 __device__ __forceinline__ double r4_finish(const Redux4 &r, int ng) {
   if (ng == 1) return r.tl;
   double res;
   if (ng >= 4) {
-    double l0 = r.a[0] + r.a[2], l1 = r.a[1] + r.a[3];
+    double l0 = r.a0 + r.a2, l1 = r.a1 + r.a3;
     if ((ng / 2) * 2 > (ng / 4) * 4) { l0 = l0 + r.e0; l1 = l1 + r.e1; }
     res = l0 + l1;
   } else {
-    res = r.a[0] + r.a[1];
+    res = r.a0 + r.a1;
   }
   if (ng & 1) res = res + r.tl;
   return res;
@@ -299,18 +308,25 @@ __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd) {
       cnt += __popcll(m);
     }
     wave_sync();
+    // four output columns per lane, one pass over the row's list
+    const int as2 = (ng / 4) * 4, as = (ng / 2) * 2;
+    Redux4 r[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int j = lane + 64 * q;
-      if (j >= ng) break;
-      Redux4 r = {{0.0, 0.0, 0.0, 0.0}, 0.0, 0.0, 0.0};
-      for (int kk = 0; kk < cnt; kk++) {
-        const int k = lk[w][kk];
-        const double b = P[(long)k * ng + j];
-        if (b != 0.0) r4_add(r, k, ng, (la[w][kk] * b) / ld[w][kk]);
-      }
-      Q[(long)i * ng + j] = r4_finish(r, ng);
+    for (int q = 0; q < 4; q++) r[q] = Redux4{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int kk = 0; kk < cnt; kk++) {
+      const int k = lk[w][kk];
+      const double a = la[w][kk], d = ld[w][kk];
+      const double *row = P + (long)k * ng;
+      double b[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) b[q] = (lane + 64 * q < ng) ? row[lane + 64 * q] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (b[q] != 0.0) r4_add(r[q], k, as2, as, (a * b[q]) / d);
     }
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (lane + 64 * q < ng) Q[(long)i * ng + lane + 64 * q] = r4_finish(r[q], ng);
     wave_sync();  // the lists are rebuilt for the next row
   }
 }
@@ -343,7 +359,7 @@ constexpr int QWP = QW + 1;  // padded LDS row of the window
 constexpr int HC = 16;       // Hessenberg: columns per right-dot tile (n x HC)
 constexpr int HCP = HC + 1;  // padded tile row (bank-conflict free row reads)
 constexpr int HB = 8;        // independent loads in flight per thread
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_har_hess(const HarDesc *hd) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_har_hess(const HarDesc *hd) {
   __shared__ double ess[256];
   __shared__ double tv[256];
   __shared__ double tile[256 * HCP];
@@ -461,17 +477,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         for (int r = lr; r < n; r += 256 / HC) tile[r * HCP + lc] = AT(r, i + 1 + ct + lc);
       }
       __syncthreads();
-      if (t < n)
-        for (int cc = 0; cc < ncol; cc++) {
-          const int j = ct + cc - 1;
-          const double x = tile[t * HCP + cc] * ess[j];
-          if (j < asz) {
-            if (j & 1) l1 = l1 + x;
-            else l0 = l0 + x;
-          } else {
-            tl = x;
-          }
+      if (t < n) {
+        const int jb = ct - 1;  // even: HC is even and the first tile starts at j = 0
+        const int nl = (jb + ncol <= asz) ? ncol : asz - jb;  // lane terms in this tile
+        const double *tr = tile + t * HCP;
+        int cc = 0;
+        for (; cc + 1 < nl; cc += 2) {
+          l0 = l0 + tr[cc] * ess[jb + cc];
+          l1 = l1 + tr[cc + 1] * ess[jb + cc + 1];
         }
+        if (cc < nl) l0 = l0 + tr[cc] * ess[jb + cc];
+        if ((mr & 1) && jb + ncol == mr) tl = tr[ncol - 1] * ess[mr - 1];
+      }
       __syncthreads();
     }
     if (t < n) {
