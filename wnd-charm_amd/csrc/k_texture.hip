@@ -353,8 +353,8 @@ __device__ void make_hh(const double *x, int m, double *ess, double *tau, double
 // in place), then (k_har_qr) the Francis QR on the active windows.  Only the
 // values of index 0..2 of the Schur diagonal are consumed (f14 = sqrt of
 // index 2 when that eigenvalue is real).
-constexpr int QB = 60;       // bulge-chase steps per LDS window
-constexpr int QW = QB + 4;   // window side (one wavefront lane per row/column)
+constexpr int QB = 28;       // bulge-chase steps per LDS window
+constexpr int QW = QB + 4;   // window side (<= 64: one wavefront lane per row/column)
 constexpr int QWP = QW + 1;  // padded LDS row of the window
 constexpr int HC = 16;       // Hessenberg: columns per right-dot tile (n x HC)
 constexpr int HCP = HC + 1;  // padded tile row (bank-conflict free row reads)
@@ -518,7 +518,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 }
 
 // Francis QR on the Hessenberg matrix left in H.Q by k_har_hess.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_har_qr(const HarDesc *hd) {
+// One wavefront per (angle, image): the Schur iteration is a chain of
+// dependent steps, so many matrices per CU (small LDS window, one wave each)
+// beat many waves per matrix.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_har_qr(const HarDesc *hd) {
   __shared__ double Rw[QW * QWP];
   __shared__ double rf[QB][4];
   __shared__ double sc[8];
@@ -715,8 +718,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const int a = k0 - 1 > 0 ? k0 - 1 : 0;
       const int b = (k1 + 3 < iuq + 1) ? k1 + 3 : iuq + 1;
       const int wd = b - a;
-      for (int r = t >> 6; r < wd; r += 4)
-        if ((t & 63) < wd) Rw[r * QWP + (t & 63)] = AT(a + r, a + (t & 63));
+      for (int r = t >> 5; r < wd; r += 2)
+        if ((t & 31) < wd) Rw[r * QWP + (t & 31)] = AT(a + r, a + (t & 31));
       __syncthreads();
 #define RW(r, c) Rw[((r) - a) * QWP + ((c) - a)]
       if (t < 64) {
@@ -775,8 +778,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         }
       }
       __syncthreads();
-      for (int r = t >> 6; r < wd; r += 4)
-        if ((t & 63) < wd) AT(a + r, a + (t & 63)) = Rw[r * QWP + (t & 63)];
+      for (int r = t >> 5; r < wd; r += 2)
+        if ((t & 31) < wd) AT(a + r, a + (t & 31)) = Rw[r * QWP + (t & 31)];
 #undef RW
       // strip right of R: rows k0 .. k1+1, columns b .. iuq (left reflectors).
       // Reflector q touches rows q..q+2 of the strip, so after it row q is
@@ -784,67 +787,77 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
       const int nstep = k1 - k0;
       for (int c = b + t; c <= iuq; c += blockDim.x) {
         double *p = &AT(k0, c);
-        double u0 = p[0], u1 = p[n];
-        for (int q0 = 0; q0 < nstep; q0 += HB) {
-          double w[HB];
+        double w[QB + 2];
+        {
+          const double *pl = p;  // running pointer: one live address
 #pragma unroll
-          for (int j = 0; j < HB; j++) w[j] = (q0 + j < nstep) ? p[(long)(q0 + j + 2) * n] : 0.0;
-#pragma unroll
-          for (int j = 0; j < HB; j++) {
-            const int q = q0 + j;
-            if (q < nstep) {
-              double u2 = w[j];
-              if (rf[q][3] != 0.0) {
-                const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
-                double tt = e0 * u1;
-                tt += e1 * u2;
-                tt += u0;
-                u0 -= tau * tt;
-                u1 -= (tau * e0) * tt;
-                u2 -= (tau * e1) * tt;
-              }
-              p[(long)q * n] = u0;
-              u0 = u1;
-              u1 = u2;
-            }
+          for (int q = 0; q < QB + 2; q++) {
+            w[q] = (q < nstep + 2) ? *pl : 0.0;
+            pl += n;
           }
         }
-        p[(long)nstep * n] = u0;
-        p[(long)(nstep + 1) * n] = u1;
-      }
-      // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors),
-      // streamed along each row the same way
-      for (int r = t; r < a; r += blockDim.x) {
-        double *p = &AT(r, k0);
-        double u0 = p[0], u1 = p[1];
-        for (int q0 = 0; q0 < nstep; q0 += HB) {
-          double w[HB];
+        double *ps = p;
+        double u0 = w[0], u1 = w[1];
 #pragma unroll
-          for (int j = 0; j < HB; j++) w[j] = (q0 + j < nstep) ? p[q0 + j + 2] : 0.0;
-#pragma unroll
-          for (int j = 0; j < HB; j++) {
-            const int q = q0 + j;
-            if (q < nstep) {
-              double u2 = w[j];
-              if (rf[q][3] != 0.0) {
-                const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
-                double tt = u1 * e0;
-                tt += u2 * e1;
-                tt += u0;
-                u0 -= tau * tt;
-                u1 -= (tau * tt) * e0;
-                u2 -= (tau * tt) * e1;
-              }
-              p[q] = u0;
-              u0 = u1;
-              u1 = u2;
+        for (int q = 0; q < QB; q++) {
+          if (q < nstep) {
+            double u2 = w[q + 2];
+            if (rf[q][3] != 0.0) {
+              const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+              double tt = e0 * u1;
+              tt += e1 * u2;
+              tt += u0;
+              u0 -= tau * tt;
+              u1 -= (tau * e0) * tt;
+              u2 -= (tau * e1) * tt;
             }
+            *ps = u0;
+            ps += n;
+            u0 = u1;
+            u1 = u2;
           }
+          __asm__ volatile("" ::: "memory");  // keep the reflector loads step by step
         }
-        p[nstep] = u0;
-        p[nstep + 1] = u1;
+        ps[0] = u0;
+        ps[n] = u1;
       }
+      // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors).
+      // Staged through the (now free) window buffer in chunks of 32 rows with
+      // coalesced loads and stores; one lane streams each row through the
+      // reflectors in LDS.
       __syncthreads();
+      for (int r0 = 0; r0 < a; r0 += 32) {
+        const int nr = (a - r0 < 32) ? a - r0 : 32;
+        const int nc = nstep + 2;
+        for (int rr = t >> 5; rr < nr; rr += 2)
+          if ((t & 31) < nc) Rw[rr * QWP + (t & 31)] = AT(r0 + rr, k0 + (t & 31));
+        __syncthreads();
+        if (t < nr) {
+          double *row = Rw + t * QWP;
+          double u0 = row[0], u1 = row[1];
+          for (int q = 0; q < nstep; q++) {
+            double u2 = row[q + 2];
+            if (rf[q][3] != 0.0) {
+              const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+              double tt = u1 * e0;
+              tt += u2 * e1;
+              tt += u0;
+              u0 -= tau * tt;
+              u1 -= (tau * tt) * e0;
+              u2 -= (tau * tt) * e1;
+            }
+            row[q] = u0;
+            u0 = u1;
+            u1 = u2;
+          }
+          row[nstep] = u0;
+          row[nstep + 1] = u1;
+        }
+        __syncthreads();
+        for (int rr = t >> 5; rr < nr; rr += 2)
+          if ((t & 31) < nc) AT(r0 + rr, k0 + (t & 31)) = Rw[rr * QWP + (t & 31)];
+        __syncthreads();
+      }
     }
     if (t == 0) {
       double x[2] = {AT(iuq - 1, iuq - 2), AT(iuq, iuq - 2)};
@@ -942,7 +955,7 @@ void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s) {
   k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
 }
 void launch_haralick_qr(const HarDesc *hd, int nimg, hipStream_t s) {
-  k_har_qr<<<dim3(4, nimg), 256, 0, s>>>(hd);
+  k_har_qr<<<dim3(4, nimg), 64, 0, s>>>(hd);
 }
 void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s) {
   k_har_final<<<nimg, 64, 0, s>>>(hd, o);
