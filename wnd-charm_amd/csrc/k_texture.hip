@@ -718,8 +718,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       const int a = k0 - 1 > 0 ? k0 - 1 : 0;
       const int b = (k1 + 3 < iuq + 1) ? k1 + 3 : iuq + 1;
       const int wd = b - a;
-      for (int r = t >> 5; r < wd; r += 2)
-        if ((t & 31) < wd) Rw[r * QWP + (t & 31)] = AT(a + r, a + (t & 31));
+      {  // 32 x 32 tile, two rows per wave instruction, all loads in flight
+        const int cl = t & 31;
+        double v[QW / 2];
+#pragma unroll
+        for (int it = 0; it < QW / 2; it++) {
+          const int r = (t >> 5) + 2 * it;
+          v[it] = (r < wd && cl < wd) ? AT(a + r, a + cl) : 0.0;
+        }
+#pragma unroll
+        for (int it = 0; it < QW / 2; it++) {
+          const int r = (t >> 5) + 2 * it;
+          if (r < wd && cl < wd) Rw[r * QWP + cl] = v[it];
+        }
+      }
       __syncthreads();
 #define RW(r, c) Rw[((r) - a) * QWP + ((c) - a)]
       if (t < 64) {
@@ -829,8 +841,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       for (int r0 = 0; r0 < a; r0 += 32) {
         const int nr = (a - r0 < 32) ? a - r0 : 32;
         const int nc = nstep + 2;
-        for (int rr = t >> 5; rr < nr; rr += 2)
-          if ((t & 31) < nc) Rw[rr * QWP + (t & 31)] = AT(r0 + rr, k0 + (t & 31));
+        {
+          const int cl = t & 31;
+          double v[QW / 2];
+#pragma unroll
+          for (int it = 0; it < QW / 2; it++) {
+            const int rr = (t >> 5) + 2 * it;
+            v[it] = (rr < nr && cl < nc) ? AT(r0 + rr, k0 + cl) : 0.0;
+          }
+#pragma unroll
+          for (int it = 0; it < QW / 2; it++) {
+            const int rr = (t >> 5) + 2 * it;
+            if (rr < nr && cl < nc) Rw[rr * QWP + cl] = v[it];
+          }
+        }
         __syncthreads();
         if (t < nr) {
           double *row = Rw + t * QWP;
