@@ -638,7 +638,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         // downsample planes above 300x300 (ImageMatrix::ChebyshevFourierTransform2D)
         std::vector<DsDesc> ds;
         std::vector<CfDesc> cd(ne);
-        int maxchunk = 0, moh = 0, mow = 0, mnh = 0, mnw = 0;
+        int moh = 0, mow = 0, mnh = 0, mnw = 0;
         for (int k = 0; k < ne; k++) {
           int h = E[k].N->h[E[k].i], w = E[k].N->w[E[k].i];
           const double *plane = E[k].N->planes[E[k].i];
@@ -685,17 +685,37 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
             C.np = w * h;  // upper bound for sizing
             C.NN = 23;
           }
-          const int nch = (C.np + 127) / 128;
-          C.partial = A.dalloc_n<double>((size_t)nch * 2 * 23 * 23 + 8);
-          maxchunk = std::max(maxchunk, nch);
+          C.sum = A.dalloc_n<double>(2 * 23 * 23);
           cd[k] = C;
         }
+        // groups of up to CF_GROUP entries sharing one table set (same plane size)
+        std::vector<CfGroup> grp(ne);  // sized for the worst case (one entry per group)
+        int ngrp = 0;
+        {
+          std::map<const int *, std::vector<int>> by_table;
+          for (int k = 0; k < ne; k++) by_table[cd[k].kk].push_back(k);
+          for (auto &kv : by_table)
+            for (size_t q = 0; q < kv.second.size(); q += CF_GROUP) {
+              CfGroup &G = grp[ngrp++];
+              const CfDesc &C0 = cd[kv.second[q]];
+              G.kk = C0.kk; G.tn = C0.tn; G.fc = C0.fc; G.fs = C0.fs;
+              G.np = C0.np; G.NN = C0.NN;
+              G.ne = (int)std::min<size_t>(CF_GROUP, kv.second.size() - q);
+              for (int g = 0; g < CF_GROUP; g++) {
+                const CfDesc &Cg = cd[kv.second[q + std::min(g, G.ne - 1)]];
+                G.img[g] = Cg.img;
+                G.sum[g] = Cg.sum;
+              }
+            }
+          if (A.sizing) ngrp = ne;
+        }
+        CfGroup *grpd = A.stage(grp);
         CfDesc *cdd = A.stage(cd);
         DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
         const int nds = (int)ds.size();
         push(gn, [=]() {
           if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
-          launch_cf(cdd, ne, maxchunk, o, s);
+          launch_cf(grpd, ngrp, cdd, ne, o, s);
         });
       } break;
       case A_EDGE: {

@@ -162,38 +162,70 @@ void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *
 // ChebyshevFourier2D (src/transforms/ChebyshevFourier.cpp:59-160), N = 11,
 // NN = 23: sum_r[im][a] += (Ftrm*cos(mf*phi) * T_a(2r-1)) * I  over in-disk
 // pixels; the geometry tables are host-computed with the reference's libm
-// calls.  Block = 128-pixel chunk staged in LDS; every thread owns <= 5 of
-// the 2*NN*NN outputs and walks the chunk in the reference's pixel order.
-constexpr int CF_CHUNK = 128;
+// calls.
+constexpr int CF_CHUNK = 64;
 constexpr int CF_NN = 23;
 
-__global__ void __launch_bounds__(256) k_cf_partial(const CfDesc *cds) {
-  __shared__ double tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN], iv[CF_CHUNK];
-  const CfDesc C = cds[blockIdx.y];
-  const int nchunk = (C.np + CF_CHUNK - 1) / CF_CHUNK;
-  if ((int)blockIdx.x >= nchunk) return;
-  const int NN = C.NN, NN2 = NN * NN;
-  const int p0 = blockIdx.x * CF_CHUNK;
-  const int np = (C.np - p0 < CF_CHUNK) ? C.np - p0 : CF_CHUNK;
-  for (int e = threadIdx.x; e < np * NN; e += 256) {
-    const int p = e / NN, a = e - p * NN;
-    tn[p][a] = C.tn[(long)(p0 + p) * NN + a];
-    fc[p][a] = C.fc[(long)(p0 + p) * NN + a];
-    fs[p][a] = C.fs[(long)(p0 + p) * NN + a];
+// Block = one group of entries sharing the basis tables.  Each thread owns
+// up to five of the 2*NN*NN coefficients and keeps one running sum per
+// coefficient and entry across all pixel chunks, so every sum is taken in
+// the reference's pixel order (ChebyshevFourier.cpp:112-135); the tables
+// of a chunk are staged in LDS once for all entries of the group.
+__global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
+  __shared__ double tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN];
+  __shared__ double iv[CF_GROUP][CF_CHUNK];
+  const CfGroup G = groups[blockIdx.x];
+  const int NN = G.NN, NN2 = NN * NN, ne = G.ne;
+  const int t = threadIdx.x;
+  // this thread's coefficients: o = t + 256 q, q < 5 (2*23*23 = 1058 <= 1280)
+  int oim[5], oa[5];
+  bool ore[5], on[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int o = t + 256 * q;
+    on[q] = o < 2 * NN2;
+    const int part = o >= NN2, k = part ? o - NN2 : o;
+    ore[q] = !part;
+    oim[q] = on[q] ? k / NN : 0;
+    oa[q] = on[q] ? k - (k / NN) * NN : 0;
   }
-  for (int p = threadIdx.x; p < np; p += 256) iv[p] = C.img[C.kk[p0 + p]];
-  __syncthreads();
-  double *dst = C.partial + (long)blockIdx.x * 2 * NN2;
-  for (int o = threadIdx.x; o < 2 * NN2; o += 256) {
-    const int part = o / NN2, k = o - part * NN2;
-    const int im = k / NN, a = k - im * NN;
-    double acc = 0;
-    if (part == 0)
-      for (int p = 0; p < np; p++) acc += (fc[p][im] * tn[p][a]) * iv[p];
-    else
-      for (int p = 0; p < np; p++) acc += (fs[p][im] * tn[p][a]) * iv[p];
-    dst[o] = acc;
+  double acc[CF_GROUP][5];
+#pragma unroll
+  for (int g = 0; g < CF_GROUP; g++)
+#pragma unroll
+    for (int q = 0; q < 5; q++) acc[g][q] = 0.0;
+  for (int p0 = 0; p0 < G.np; p0 += CF_CHUNK) {
+    const int np = (G.np - p0 < CF_CHUNK) ? G.np - p0 : CF_CHUNK;
+    __syncthreads();  // the previous chunk is consumed
+    for (int e = t; e < np * NN; e += 256) {
+      const int p = e / NN, a = e - p * NN;
+      tn[p][a] = G.tn[(long)(p0 + p) * NN + a];
+      fc[p][a] = G.fc[(long)(p0 + p) * NN + a];
+      fs[p][a] = G.fs[(long)(p0 + p) * NN + a];
+    }
+    if (t < np) {
+      const int kidx = G.kk[p0 + t];
+#pragma unroll
+      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = g < ne ? G.img[g][kidx] : 0.0;
+    }
+    __syncthreads();
+    for (int p = 0; p < np; p++) {
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        if (!on[q]) continue;
+        const double f = ore[q] ? fc[p][oim[q]] : fs[p][oim[q]];
+        const double ft = f * tn[p][oa[q]];
+#pragma unroll
+        for (int g = 0; g < CF_GROUP; g++) acc[g][q] += ft * iv[g][p];
+      }
+    }
   }
+#pragma unroll
+  for (int g = 0; g < CF_GROUP; g++)
+    if (g < ne)
+#pragma unroll
+      for (int q = 0; q < 5; q++)
+        if (on[q]) G.sum[g][t + 256 * q] = acc[g][q];
 }
 
 __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) {
@@ -202,14 +234,9 @@ __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) 
   __shared__ int bins[32];
   const CfDesc C = cds[blockIdx.x];
   const int NN = C.NN, NN2 = NN * NN;
-  const int nchunk = (C.np + CF_CHUNK - 1) / CF_CHUNK;
   double mn = kInf, mx = -kInf;
   for (int k = threadIdx.x; k < NN2; k += blockDim.x) {
-    double sr = 0, si = 0;
-    for (int c = 0; c < nchunk; c++) {
-      sr += C.partial[(long)c * 2 * NN2 + k];
-      si += C.partial[(long)c * 2 * NN2 + NN2 + k];
-    }
+    const double sr = C.sum[k], si = C.sum[NN2 + k];
     const double v = sqrt(sr * sr + si * si);
     coeff[k] = v;
     mn = fmin(mn, v);
@@ -227,8 +254,8 @@ __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) 
   if (threadIdx.x < 32) o.rows[blockIdx.x][threadIdx.x] = bins[threadIdx.x];
 }
 
-void launch_cf(const CfDesc *cd, int nimg, int maxchunk, OutRef o, hipStream_t s) {
-  k_cf_partial<<<dim3(maxchunk, nimg), 256, 0, s>>>(cd);
+void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s) {
+  k_cf_accum<<<ngroups, 256, 0, s>>>(groups);
   k_cf_final<<<nimg, 1024, 0, s>>>(cd, o);
 }
 

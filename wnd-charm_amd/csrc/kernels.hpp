@@ -36,7 +36,17 @@ struct CfDesc {
   const double *fs;     // [np][NN] Ftrm*sin(-mf*phi)
   int np, NN;
   int w, h;             // plane size (row-major [h][w])
-  double *partial;      // [nchunk][2*NN*NN]
+  double *sum;          // [2][NN*NN] real and imaginary coefficient sums
+};
+
+// up to CF_GROUP entries whose planes share one set of basis tables
+constexpr int CF_GROUP = 4;
+struct CfGroup {
+  const int *kk;
+  const double *tn, *fc, *fs;
+  int np, NN, ne;
+  const double *img[CF_GROUP];
+  double *sum[CF_GROUP];
 };
 
 struct DsDesc {  // area-weighted Downsample (cmatrix.cpp:579-720)
@@ -109,7 +119,7 @@ void launch_gabor(const Img *imgs, const double *kernels, double *const *energy,
                   int maxw, hipStream_t s);
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
                     hipStream_t s);
-void launch_cf(const CfDesc *cd, int nimg, int maxchunk, OutRef o, hipStream_t s);
+void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s);
 void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
 
 }  // namespace wnd
