@@ -30,6 +30,19 @@ struct OutRef {
   double *const *rows;
 };
 
+// XCD-aware block mapping.  MI355X deals consecutive workgroups round-robin
+// to its 8 XCDs, each with its own L2.  For a 1-D grid of 8 * per_item *
+// ceil(items / 8) blocks, this gives every block of one item the same XCD
+// (item = its blocks' shared input, e.g. one image plane), so the item is
+// fetched into one L2 only.  Returns false for padding blocks.
+__device__ __forceinline__ bool xcd_item(int per_item, int items, int &item, int &sub) {
+  const int b = blockIdx.x, xcd = b & 7, k = b >> 3;
+  sub = k % per_item;
+  item = (k / per_item) * 8 + xcd;
+  return item < items;
+}
+inline int xcd_grid(int per_item, int items) { return 8 * per_item * ((items + 7) / 8); }
+
 constexpr double kFltMax = 3.40282346638528859812e+38;
 constexpr double kDblEps = 2.2204460492503131e-16;
 constexpr double kInf = 10E200;  // the reference's INF (src/cmatrix.h:48)

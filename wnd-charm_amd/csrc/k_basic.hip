@@ -366,28 +366,34 @@ __global__ void __launch_bounds__(1024) k_msh(const Img *imgs, const Stats *st, 
 
 // ---------------------------------------------------------------------------
 // Fractal features (FeatureAlgorithms.cpp:232-264): one block per (k, image)
-__global__ void __launch_bounds__(256) k_fractal(const Img *imgs, OutRef o) {
+__global__ void __launch_bounds__(256) k_fractal(const Img *imgs, OutRef o, int nimg) {
   __shared__ double shd[4];
-  const Img im = imgs[blockIdx.y];
+  int ent, bin;
+  if (!xcd_item(20, nimg, ent, bin)) return;  // the 20 offsets of one plane share an XCD
+  const Img im = imgs[ent];
   const int W = im.w, H = im.h;
   const int K = (W > H ? H : W) / 5;
   int step = K / 20;
   if (step < 1) step = 1;
-  const int bin = blockIdx.x;
   const int k = 1 + bin * step;
-  double *out = o.rows[blockIdx.y];
-  if (bin >= 20) return;
+  double *out = o.rows[ent];
   if (k >= K) {
     if (threadIdx.x == 0) out[bin] = 0.0;
     return;
   }
+  // sum |I(y,x) - I(y+k,x)| + |I(y,x) - I(y,x+k)| in one pass: rows by
+  // 4-row groups of 64-lane row segments
   double s = 0;
-  const long nv = (long)(H - k) * W;
-  for (long t = threadIdx.x; t < nv; t += blockDim.x) s += fabs(im.p[t] - im.p[t + (long)k * W]);
-  const long nh = (long)H * (W - k);
-  for (long t = threadIdx.x; t < nh; t += blockDim.x) {
-    const int y = t / (W - k), x = t - (long)y * (W - k);
-    s += fabs(im.p[(long)y * W + x] - im.p[(long)y * W + x + k]);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int y = wv; y < H; y += 4) {
+    const double *row = im.p + (long)y * W;
+    const double *down = row + (long)k * W;
+    const bool hasdown = y + k < H;
+    for (int x = lane; x < W; x += 64) {
+      const double v = row[x];
+      if (hasdown) s += fabs(v - down[x]);
+      if (x + k < W) s += fabs(v - row[x + k]);
+    }
   }
   s = block_reduce(s, SumOp(), shd);
   if (threadIdx.x == 0) out[bin] = s / (W * (W - k) + H * (H - k));
@@ -470,7 +476,7 @@ void launch_msh(const Img *imgs, const Stats *st, OutRef o, int nimg, hipStream_
   k_msh<<<nimg, 1024, 0, s>>>(imgs, st, o);
 }
 void launch_fractal(const Img *imgs, OutRef o, int nimg, hipStream_t s) {
-  k_fractal<<<dim3(20, nimg), 256, 0, s>>>(imgs, o);
+  k_fractal<<<xcd_grid(20, nimg), 256, 0, s>>>(imgs, o, nimg);
 }
 void launch_hist_values(const Img *vals, int nbins, OutRef o, int nimg, hipStream_t s) {
   k_hist_values<<<nimg, 256, 0, s>>>(vals, nbins, o);

@@ -1306,14 +1306,16 @@ __device__ __forceinline__ double radon_xc(int nn, int xO, double cs, int rFirst
 }
 
 __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const double *trig, double *const *proj,
-                                                    int nbg) {
+                                                    int nbg, int nimg) {
   extern __shared__ double yS[];
-  const Img im = imgs[blockIdx.y];
+  int ent, sub;
+  if (!xcd_item(4 * nbg, nimg, ent, sub)) return;
+  const Img im = imgs[ent];
   const int M = im.h, N = im.w;
   const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
                                    (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
   const int rFirst = -rLast, rSize = rLast - rFirst + 1;
-  const int a = blockIdx.x / nbg, r0 = (blockIdx.x - a * nbg) * 256;
+  const int a = sub / nbg, r0 = (sub - a * nbg) * 256;
   if (r0 >= rSize) return;
   const double cs = trig[2 * a], sn = trig[2 * a + 1];
   const int xO = (N - 1) / 2, yO = (M - 1) / 2;
@@ -1382,7 +1384,7 @@ __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const doubl
       mm = m0;
     }
   }
-  proj[blockIdx.y][(long)a * rSize + r] = acc;
+  proj[ent][(long)a * rSize + r] = acc;
 }
 
 // yTable / xTable entries are exact multiples of 0.25, so the closed forms
@@ -1417,7 +1419,7 @@ void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutR
   if (radon_lds > 65536)
     (void)hipFuncSetAttribute((const void *)k_radon_proj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)radon_lds);
   const int nbg = (maxbins + 255) / 256;
-  k_radon_proj<<<dim3(4 * nbg, nimg), 256, radon_lds, s>>>(imgs, trig, proj, nbg);
+  k_radon_proj<<<xcd_grid(4 * nbg, nimg), 256, radon_lds, s>>>(imgs, trig, proj, nbg, nimg);
   k_radon_hist<<<nimg, 64, 0, s>>>(imgs, proj, o);
 }
 
