@@ -340,10 +340,16 @@ void launch_edge_final(const Img *mag, const Img *dir, const Stats *st2, const d
 // with separate multiply and add; taps that fall outside the image add
 // +-0, which leaves the sum unchanged.
 #define GB_N 38
-#define GB_TR 16
-#define GB_TC 64
+#define GB_TR 32          // output rows per block
+#define GB_TC 64          // output columns per block
+#define GB_LD (GB_TC + GB_N - 1 + 1)
+// Block = 32 x 64 outputs of one filter on one image; thread = 8 adjacent
+// outputs of one row.  Taps are taken in groups of eight: the 15 input values
+// a group needs for the thread's 8 outputs are read from LDS once into
+// registers, and every output still adds its taps in the reference's (j, i)
+// order (conv2comp, gabor.cpp:19-52).
 __global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const double *kernels, double *const *energy) {
-  __shared__ double tile[GB_TR + GB_N - 1][GB_TC + GB_N - 1 + 1];
+  __shared__ double tile[GB_TR + GB_N - 1][GB_LD];
   __shared__ double2 wk[GB_N * GB_N];
   const Img im = imgs[blockIdx.z];
   const int f = blockIdx.y % 8;
@@ -354,31 +360,49 @@ __global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const doubl
   const double *K = kernels + (long)f * GB_N * GB_N * 2;
   for (int e = t; e < GB_N * GB_N; e += 256) wk[e] = make_double2(K[2 * e], K[2 * e + 1]);
   // input rows b0-18 .. b0+GB_TR+18, columns a0-18 .. a0+GB_TC+18
-  for (int e = t; e < (GB_TR + GB_N - 1) * (GB_TC + GB_N - 1); e += 256) {
-    const int r = e / (GB_TC + GB_N - 1), c = e - r * (GB_TC + GB_N - 1);
-    const int y = b0 - 18 + r, x = a0 - 18 + c;
-    tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+  for (int r = t / 64; r < GB_TR + GB_N - 1; r += 4) {
+    const int y = b0 - 18 + r;
+    for (int c = t % 64; c < GB_TC + GB_N - 1; c += 64) {
+      const int x = a0 - 18 + c;
+      tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+    }
   }
   __syncthreads();
-  const int ty = t / 16, tx = t % 16;
-  const int b = b0 + ty, a = a0 + tx * 4;
-  double re[4] = {0, 0, 0, 0}, imv[4] = {0, 0, 0, 0};
-  const int rb = ty + 37, cb = tx * 4 + 37;
+  const int ty = t / 8, tx = t % 8;
+  const int b = b0 + ty, a = a0 + tx * 8;
+  double re[8], imv[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) re[q] = imv[q] = 0.0;
+  const int rb = ty + 37, cb = tx * 8 + 37;
   for (int j = 0; j < GB_N; j++) {
     const double *row = &tile[rb - j][0];
-    for (int i = 0; i < GB_N; i++) {
-      const double2 w = wk[j * GB_N + i];
+    const double2 *wr = &wk[j * GB_N];
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const double v = row[cb + q - i];
-        re[q] += v * w.x;
-        imv[q] += v * w.y;
+    for (int i0 = 0; i0 < GB_N; i0 += 8) {
+      // u[k] = row[cb - i0 - 7 + k]: value of output q, tap i is u[q - (i - i0) + 7]
+      // taps in this group: D = min(8, GB_N - i0); slots k < 8 - D are unused
+      const int D = (GB_N - i0 < 8) ? GB_N - i0 : 8;
+      double u[15];
+#pragma unroll
+      for (int k = 0; k < 15; k++) u[k] = (k >= 8 - D) ? row[cb - i0 - 7 + k] : 0.0;
+#pragma unroll
+      for (int di = 0; di < 8; di++) {
+        if (i0 + di < GB_N) {
+          const double2 w = wr[i0 + di];
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const double v = u[q - di + 7];
+            re[q] += v * w.x;
+            imv[q] += v * w.y;
+          }
+        }
       }
     }
   }
   double *E = energy[blockIdx.z] + (long)f * im.h * im.w;
   if (b < im.h)
-    for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int q = 0; q < 8; q++)
       if (a + q < im.w) E[(long)b * im.w + a + q] = sqrt(re[q] * re[q] + imv[q] * imv[q]);
 }
 

@@ -282,11 +282,13 @@ __device__ __forceinline__ double r4_finish(const Redux4 &r, int ng) {
   return res;
 }
 
-__global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd) {
+__global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd, int nimg) {
   __shared__ int lk[4][256];
   __shared__ double la[4][256], ld[4][256];
-  const int angle = blockIdx.x / QRB, rb = blockIdx.x - angle * QRB;
-  const HarDesc H = hd[blockIdx.y];
+  int item, rb;
+  if (!xcd_item(QRB, 4 * nimg, item, rb)) return;  // the row blocks of one P share an XCD
+  const int angle = item & 3;
+  const HarDesc H = hd[item >> 2];
   const int ng = *H.ng;
   const double *P = H.P + (long)angle * ng * ng;
   const double *px = H.marg + angle * 512, *py = px + 256;
@@ -973,7 +975,7 @@ void launch_haralick_prep(const Img *imgs, const Stats *st, const HarDesc *hd, i
   (void)lds_ok;
   k_har_cooc<<<dim3(4, nimg), 1024, 256 * 257 / 2 * sizeof(uint32_t), s>>>(imgs, hd);
   k_har_stats<<<dim3(4, nimg), 256, 0, s>>>(hd);
-  k_har_q<<<dim3(4 * QRB, nimg), 256, 0, s>>>(hd);
+  k_har_q<<<xcd_grid(QRB, 4 * nimg), 256, 0, s>>>(hd, nimg);
 }
 void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s) {
   k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
