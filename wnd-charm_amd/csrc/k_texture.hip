@@ -1012,27 +1012,48 @@ __global__ void __launch_bounds__(64) k_tam_sat(const Img *imgs, double *const *
   const double *I = scr[blockIdx.x];
   double *S = scr[blockIdx.x] + P;
   const int l = threadIdx.x;
+  constexpr int PF = 16;  // inputs loaded ahead per lane
   for (int y0 = 0; y0 < H; y0 += 64) {
     const int y = y0 + l;
+    const double *Irow = I + (long)y * W;
     double cur = 0, up_prev = 0;
-    for (int t = 0; t < W + 63; t++) {
-      const int x = t - l;
-      const double from_left_lane = __shfl_up(cur, 1, 64);
-      double up;
-      if (l == 0) up = (y0 == 0 || x < 0 || x >= W) ? 0.0 : rowbuf[x];
-      else up = (y < 1) ? 0.0 : from_left_lane;
-      if (x >= 0 && x < W && y < H) {
-        const double links = x < 1 ? 0.0 : cur;
-        const double oben = y < 1 ? 0.0 : up;
-        const double obenlinks = (y < 1 || x < 1) ? 0.0 : up_prev;
-        const double v = I[(long)y * W + x] + links + oben - obenlinks;
-        S[(long)y * W + x] = v;
-        cur = v;
-        up_prev = up;
+    double nx[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+      const int x = k - l;
+      nx[k] = (y < H && x >= 0 && x < W) ? Irow[x] : 0.0;
+    }
+    for (int t0 = 0; t0 < W + 63; t0 += PF) {
+      double in[PF];
+#pragma unroll
+      for (int k = 0; k < PF; k++) in[k] = nx[k];
+#pragma unroll
+      for (int k = 0; k < PF; k++) {  // the next group's inputs, in flight during this group
+        const int x = t0 + PF + k - l;
+        nx[k] = (y < H && x >= 0 && x < W) ? Irow[x] : 0.0;
       }
-      __syncthreads();
-      if (l == 63 && x >= 0 && x < W) rowbuf[x] = cur;
-      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < PF; k++) {
+        const int t = t0 + k;
+        if (t >= W + 63) break;
+        const int x = t - l;
+        const double from_left_lane = __shfl_up(cur, 1, 64);
+        double up;
+        if (l == 0) up = (y0 == 0 || x < 0 || x >= W) ? 0.0 : rowbuf[x];
+        else up = (y < 1) ? 0.0 : from_left_lane;
+        if (x >= 0 && x < W && y < H) {
+          const double links = x < 1 ? 0.0 : cur;
+          const double oben = y < 1 ? 0.0 : up;
+          const double obenlinks = (y < 1 || x < 1) ? 0.0 : up_prev;
+          const double v = in[k] + links + oben - obenlinks;
+          S[(long)y * W + x] = v;
+          cur = v;
+          up_prev = up;
+        }
+        __syncthreads();
+        if (l == 63 && x >= 0 && x < W) rowbuf[x] = cur;
+        __syncthreads();
+      }
     }
   }
 }
@@ -1059,13 +1080,15 @@ struct TamAcc {
   int kmin, kmax;
 };
 
-__global__ void __launch_bounds__(256) k_tam_pix(const Img *imgs, double *const *scr) {
-  const Img im = imgs[blockIdx.y];
+__global__ void __launch_bounds__(256) k_tam_pix(const Img *imgs, double *const *scr, int nimg, int per_img) {
+  int ent, blk;
+  if (!xcd_item(per_img, nimg, ent, blk)) return;  // one plane's SAT stays in one L2
+  const Img im = imgs[ent];
   const int H = im.h, W = im.w;
   const long P = (long)H * W;
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long i = blk * (long)blockDim.x + threadIdx.x;
   if (i >= P) return;
-  double *N = scr[blockIdx.y];
+  double *N = scr[ent];
   const double *S = N + P;
   double *phi = N + 2 * P;
   const int y = i / W, x = i - (long)y * W;
@@ -1200,7 +1223,7 @@ void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, Out
   int blocks = (int)((maxn + 255) / 256);
   k_tam_norm<<<dim3(blocks > 2048 ? 2048 : blocks, nimg), 256, 0, s>>>(imgs, st, scratch);
   k_tam_sat<<<nimg, 64, maxw * sizeof(double), s>>>(imgs, scratch);
-  k_tam_pix<<<dim3(blocks, nimg), 256, 0, s>>>(imgs, scratch);
+  k_tam_pix<<<xcd_grid(blocks, nimg), 256, 0, s>>>(imgs, scratch, nimg, blocks);
   k_tam_final<<<nimg, 1024, 0, s>>>(imgs, scratch, o);
 }
 
