@@ -10,6 +10,18 @@
 
 namespace wnd {
 
+// Global-memory view of a device pointer.  Pointers read from descriptor
+// structs are generic to the compiler, so their accesses become FLAT
+// instructions, which count against the LDS wait counter as well: a kernel
+// that mixes LDS and global traffic would wait for its global loads at
+// every LDS access.  Kernels address global memory through gptr instead.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(T *p) {
+  return (gptr<T>)p;
+}
+
 // One FP64 row-major plane on the device (an ImageMatrix pixel plane).
 struct Img {
   double *p;

@@ -358,165 +358,259 @@ __device__ void make_hh(const double *x, int m, double *ess, double *tau, double
 constexpr int QB = 28;       // bulge-chase steps per LDS window
 constexpr int QW = QB + 4;   // window side (<= 64: one wavefront lane per row/column)
 constexpr int QWP = QW + 1;  // padded LDS row of the window
-constexpr int HC = 16;       // Hessenberg: columns per right-dot tile (n x HC)
-constexpr int HCP = HC + 1;  // padded tile row (bank-conflict free row reads)
-constexpr int HB = 8;        // independent loads in flight per thread
+constexpr int HT = 16;       // Hessenberg: columns per row-dot tile (n x HT in LDS)
+constexpr int HTP = HT + 1;  // padded tile row (bank-conflict free row reads)
+constexpr int HR = 16;       // rows in flight per thread in the column pass
+
+// Reflector of column s below the diagonal (Eigen makeHouseholderInPlace),
+// x[0..rm-1] in LDS: one thread's sequential squared norm of the tail (the
+// column is strided in row-major storage, so Eigen sums it in order), then
+// the essential part element-parallel.  Writes beta and the zeros of the
+// Hessenberg form into column s.  Returns tau.
+__device__ __forceinline__ double hess_reflector(gptr<double> A, int n, int s, int rm, const double *x, double *ess,
+                                                 double *sc) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    double tail = 0;
+    for (int k = 1; k < rm; k++) tail += x[k] * x[k];
+    const double c0 = x[0];
+    if (tail == 0.0) {
+      sc[0] = 0; sc[1] = c0; sc[2] = 0; sc[3] = 0;
+    } else {
+      double b = sqrt(c0 * c0 + tail);
+      if (c0 >= 0) b = -b;
+      sc[0] = (b - c0) / b;
+      sc[1] = b;
+      sc[2] = c0 - b;
+      sc[3] = 1;
+    }
+  }
+  __syncthreads();
+  const double tau = sc[0];
+  const bool nz = sc[3] != 0;
+  const double den = sc[2];
+  for (int k = 1 + t; k < rm; k += blockDim.x) ess[k - 1] = nz ? x[k] / den : 0.0;
+  if (t == 0) A[(long)(s + 1) * n + s] = sc[1];
+  for (int k = 1 + t; k < rm; k += blockDim.x) A[(long)(s + 1 + k) * n + s] = 0.0;  // matrixH(): Hessenberg part only
+  __syncthreads();
+  return tau;
+}
+
+// Column pass after step i (columns c = i+2..n-1, one thread per column;
+// threads left over split the rows above the dot region):
+//   upd  A = A H of step i on every row: A(r,c) -= tu[r] * ess[c-i-2], with
+//        tu[r] = tau * (row dot), Eigen's (tau * tmp) * essential^T
+//   dot  the left dot of step i+1 on the updated column, in Eigen's ColMajor
+//        GEMV order on bottom^T (general_matrix_vector_product<ColMajor>,
+//        GeneralMatrixVector.h:31-285): inner terms (rows i+3..n-1 times nx)
+//        in groups of four (g, g+o1, g+2, g+o3), added pairwise for outputs
+//        in the packet range and left to right for an odd last output;
+//        leftover terms one by one; a skipped leading term last; then
+//        + A(i+2, c) (applyHouseholderOnTheLeft's tmp += row(0)).  -> tw[c]
+__device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool upd, const double *ess,
+                                             const double *tu, bool dot, const double *nx, double *tw) {
+  const int t = threadIdx.x;
+  const int mB = n - i - 2;
+  if (mB <= 0) return;
+  const int k = min(8, (int)blockDim.x / mB);
+  const int cc = t % mB, sub = t / mB;
+  if (sub >= k) return;
+  const int c = i + 2 + cc;
+  const double f = upd ? ess[cc] : 0.0;
+  const int d0 = dot ? i + 2 : n;  // first row of the dot region
+  if (upd && !(dot && k > 1 && sub == 0)) {
+    const int parts = (dot && k > 1) ? k - 1 : k, ps = (dot && k > 1) ? sub - 1 : sub;
+    const int ra = d0 * ps / parts, rb = d0 * (ps + 1) / parts;
+    unsigned o = (unsigned)(ra * n + c);
+    int r = ra;
+    for (; r + HR <= rb; r += HR, o += HR * n) {
+      double v[HR];
+#pragma unroll
+      for (int q = 0; q < HR; q++) v[q] = A[o + q * n];
+#pragma unroll
+      for (int q = 0; q < HR; q++) A[o + q * n] = v[q] - tu[r + q] * f;
+    }
+    for (; r < rb; r++, o += n) A[o] = A[o] - tu[r] * f;
+  }
+  if (!dot || sub != 0) return;
+  const int s = i + 1;       // the step whose left dot this is
+  const int ms = n - s - 2;  // its inner terms: rows s+2..n-1
+  const bool odd_n = n & 1;
+  const long L = (long)(s + 2) * n + (s + 1);  // bottom^T(0,0) in row-major storage
+  const int skip = ((L & 1) && odd_n && ms > 0) ? 1 : 0;
+  const bool packet = cc < ((n - s - 1) & ~1);
+  const int bound = ((ms - skip) / 4) * 4 + skip;
+  const unsigned ot = (unsigned)((s + 1) * n + c);
+  double top = A[ot];
+  if (upd) { top = top - tu[s + 1] * f; A[ot] = top; }
+  gptr<double> p = A + (ot + n);  // row s+2: term 0
+  double x0 = 0.0;
+  if (skip) {
+    double v = p[0];
+    if (upd) { v = v - tu[s + 2] * f; p[0] = v; }
+    x0 = v * nx[0];
+  }
+  double acc = 0.0;
+  int g = skip;
+  for (; g + 16 <= bound; g += 16) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = p[(unsigned)((g + q) * n)];
+    if (upd) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) {
+        v[q] = v[q] - tu[s + 2 + g + q] * f;
+        p[(unsigned)((g + q) * n)] = v[q];
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 16; h += 4) {
+      const double a = v[h] * nx[g + h], p1 = v[h + 1] * nx[g + h + 1];
+      const double c2 = v[h + 2] * nx[g + h + 2], p3 = v[h + 3] * nx[g + h + 3];
+      const double b = odd_n ? p3 : p1, d = odd_n ? p1 : p3;
+      if (packet) acc = acc + ((a + b) + (c2 + d));
+      else acc = (((acc + a) + b) + c2) + d;
+    }
+  }
+  for (; g < bound; g += 4) {
+    double v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = p[(unsigned)((g + q) * n)];
+    if (upd) {
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        v[q] = v[q] - tu[s + 2 + g + q] * f;
+        p[(unsigned)((g + q) * n)] = v[q];
+      }
+    }
+    const double a = v[0] * nx[g], p1 = v[1] * nx[g + 1], c2 = v[2] * nx[g + 2], p3 = v[3] * nx[g + 3];
+    const double b = odd_n ? p3 : p1, d = odd_n ? p1 : p3;
+    if (packet) acc = acc + ((a + b) + (c2 + d));
+    else acc = (((acc + a) + b) + c2) + d;
+  }
+  for (int j = bound; j < ms; j++) {
+    double v = p[(unsigned)(j * n)];
+    if (upd) { v = v - tu[s + 2 + j] * f; p[(unsigned)(j * n)] = v; }
+    acc = acc + v * nx[j];
+  }
+  if (skip) acc = acc + x0;
+  tw[c] = acc + top;
+}
+
+// One block per (angle, image): HessenbergDecomposition::_compute
+// (Eigenvalues/HessenbergDecomposition.h:292-318) of Q (n x n, row-major, in
+// place), two passes over the trailing columns per step i:
+//   tile pass    columns i+1..n-1 in n x HT LDS tiles.  The loading thread
+//                applies A = H A of step i to rows i+1..n-1 (row i+1 -=
+//                tau*tw, bottom -= (tau*ess)*tw; tw = the left dot from the
+//                previous column pass) and writes them back; thread r then
+//                accumulates row r's dot in Eigen's RowMajor GEMV order
+//                (general_matrix_vector_product<RowMajor>,
+//                GeneralMatrixVector.h:289-552): even and odd terms in two
+//                lanes, the lanes added, an odd last term, + A(r, i+1).
+//   column i+1   A = A H on column i+1 (factor 1), then the reflector of
+//                step i+1 from it.
+//   column pass  A = A H on columns i+2..n-1 fused with the left dot of
+//                step i+1 (hess_colpass).
+// Each element of the trailing block is read and written twice per step.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_har_hess(const HarDesc *hd) {
-  __shared__ double ess[256];
-  __shared__ double tv[256];
-  __shared__ double tile[256 * HCP];
+  __shared__ double tile[256 * HTP];
+  __shared__ double essb[2][256];  // reflector of step i and of step i+1
+  __shared__ double tw[256];       // left dot of the current step, by column
+  __shared__ double tu[256];       // tau * right dot of the current step, by row
+  __shared__ double xcol[256];
   __shared__ double sc[4];
   const int angle = blockIdx.x;
   const HarDesc H = hd[blockIdx.y];
   const int n = *H.ng;
-  double *A = H.Q + (long)angle * n * n;
+  if (n < 2) return;
+  gptr<double> A = to_global(H.Q + (long)angle * n * n);
   const int t = threadIdx.x;
-#define AT(r, c) A[(long)(r) * n + (c)]
-  // HessenbergDecomposition::_compute, column by column.  Per column i:
-  // the reflector norm is one thread's sequential sum, the essential part
-  // is element-parallel; A = H A has thread c own column c (dot over rows in
-  // order, then the rank-1 update of the same column); A = A H stages
-  // n x HC column tiles in LDS so thread r accumulates row r in column order.
-  for (int i = 0; i < n - 1; ++i) {
-    const int rem = n - i - 1;
-    for (int k = t; k < rem; k += blockDim.x) tile[k] = AT(i + 1 + k, i);
-    __syncthreads();
-    if (t == 0) {
-      double tail = 0;
-      for (int k = 1; k < rem; k++) tail += tile[k] * tile[k];
-      const double c0 = tile[0];
-      if (tail == 0.0) {
-        sc[0] = 0; sc[1] = c0; sc[2] = 0; sc[3] = 0;
-      } else {
-        double b = sqrt(c0 * c0 + tail);
-        if (c0 >= 0) b = -b;
-        sc[0] = (b - c0) / b;
-        sc[1] = b;
-        sc[2] = c0 - b;
-        sc[3] = 1;
-      }
-    }
-    __syncthreads();
-    const double tau = sc[0];
-    {
-      const bool nz = sc[3] != 0;
-      const double den = sc[2];
-      for (int k = 1 + t; k < rem; k += blockDim.x) ess[k - 1] = nz ? tile[k] / den : 0.0;
-    }
-    if (t == 0) AT(i + 1, i) = sc[1];
-    for (int k = 1 + t; k < rem; k += blockDim.x) AT(i + 1 + k, i) = 0.0;  // matrixH(): Hessenberg part only
-    __syncthreads();
-    if (rem == 1) {
-      if (t == 0) AT(i + 1, i + 1) *= (1 - tau);
-      __syncthreads();
-      for (int r = t; r < n; r += blockDim.x) AT(r, i + 1) *= (1 - tau);
-      __syncthreads();
-      continue;
-    }
-    // A = H A on rows i+1..n-1, columns i+1..n-1.  The column dot product
-    // follows Eigen's ColMajor GEMV on bottom^T (general_matrix_vector_product
-    // <ColMajor>, GeneralMatrixVector.h:31-285): inner terms in groups of four
-    // (g, g+o1, g+2, g+o3), added pairwise for outputs in the packet range and
-    // left to right for an odd last output; leftover terms one by one; a
-    // skipped leading term last.
-    const bool odd_n = n & 1;
-    const int o1 = odd_n ? 3 : 1, o3 = odd_n ? 1 : 3;
-    for (int c = i + 1 + t; c < n; c += blockDim.x) {
-      const double *p = &AT(i + 2, c);
-      const int m = rem - 1;
-      const long L = (long)(i + 2) * n + (i + 1);
-      const int skip = ((L & 1) && odd_n && m > 0) ? 1 : 0;
-      const bool packet = (c - (i + 1)) < (rem & ~1);
-      const int bound = ((m - skip) / 4) * 4 + skip;
-      double acc = 0.0;
-      int g = skip;
-      for (; g + 4 < bound; g += 8) {  // two groups per pass: eight loads in flight
-        const double a0 = p[(long)g * n] * ess[g], b0 = p[(long)(g + o1) * n] * ess[g + o1];
-        const double c0 = p[(long)(g + 2) * n] * ess[g + 2], d0 = p[(long)(g + o3) * n] * ess[g + o3];
-        const double a1 = p[(long)(g + 4) * n] * ess[g + 4], b1 = p[(long)(g + 4 + o1) * n] * ess[g + 4 + o1];
-        const double c1 = p[(long)(g + 6) * n] * ess[g + 6], d1 = p[(long)(g + 4 + o3) * n] * ess[g + 4 + o3];
-        if (packet) {
-          acc = acc + ((a0 + b0) + (c0 + d0));
-          acc = acc + ((a1 + b1) + (c1 + d1));
-        } else {
-          acc = (((acc + a0) + b0) + c0) + d0;
-          acc = (((acc + a1) + b1) + c1) + d1;
-        }
-      }
-      for (; g < bound; g += 4) {
-        const double a0 = p[(long)g * n] * ess[g], b0 = p[(long)(g + o1) * n] * ess[g + o1];
-        const double c0 = p[(long)(g + 2) * n] * ess[g + 2], d0 = p[(long)(g + o3) * n] * ess[g + o3];
-        if (packet) acc = acc + ((a0 + b0) + (c0 + d0));
-        else acc = (((acc + a0) + b0) + c0) + d0;
-      }
-      for (int j = bound; j < m; j++) acc = acc + p[(long)j * n] * ess[j];
-      if (skip) acc = acc + p[0] * ess[0];
-      const double tt = acc + AT(i + 1, c);
-      AT(i + 1, c) -= tau * tt;
-      double *q = &AT(i + 2, c);
-      int r = 0;
-      for (; r + HB <= m; r += HB) {
-        double v[HB];
+  for (int k = t; k < n - 1; k += blockDim.x) xcol[k] = A[(long)(1 + k) * n];
+  __syncthreads();
+  double tau = hess_reflector(A, n, 0, n - 1, xcol, essb[0], sc);
+  if (n - 1 >= 2) hess_colpass(A, n, -1, false, nullptr, nullptr, true, essb[0], tw);
+  __syncthreads();
+  int pp = 0;
+  const int lc = t % HT, lr = t / HT;
+  for (int i = 0; i + 2 < n; ++i) {
+    const int rem = n - i - 1, m = rem - 1, asz = m & ~1;
+    const double *ess = essb[pp];
+    double rv[16];
+    auto load = [&](int ct) {
+      if (ct + lc < rem) {
+        int r0 = lr;
+        asm volatile("" : "+v"(r0));  // per-row values are formed at use, not hoisted out of the column loop
+        const int c = i + 1 + ct + lc;
+        // rows past the end re-read row n-1: branch-free, so all 16 loads issue back to back
 #pragma unroll
-        for (int j = 0; j < HB; j++) v[j] = q[(long)(r + j) * n];
-#pragma unroll
-        for (int j = 0; j < HB; j++) q[(long)(r + j) * n] = v[j] - (tau * ess[r + j]) * tt;
+        for (int q = 0; q < 16; q++) rv[q] = A[(unsigned)(min(r0 + 16 * q, n - 1) * n + c)];
       }
-      for (; r < m; r++) q[(long)r * n] -= (tau * ess[r]) * tt;
-    }
-    __syncthreads();
-    // A = A H on rows 0..n-1, columns i+1..n-1.  The row dot product follows
-    // Eigen's RowMajor GEMV with an aligned right-hand side
-    // (GeneralMatrixVector.h:289-552): even and odd inner terms in two lanes,
-    // the lanes added, then an odd last term.
-    const int mr = rem - 1, asz = mr & ~1;
-    double l0 = 0.0, l1 = 0.0, tl = 0.0;
-    const int lc = t & (HC - 1), lr = t / HC;
-    for (int ct = 1; ct < rem; ct += HC) {
-      const int ncol = (rem - ct < HC) ? rem - ct : HC;
+    };
+    double l0 = 0.0, l1 = 0.0, tl = 0.0, col0 = 0.0;
+    load(0);
+    for (int ct = 0; ct < rem; ct += HT) {
+      const int ncol = (rem - ct < HT) ? rem - ct : HT;
       if (lc < ncol) {
-#pragma unroll 16
-        for (int r = lr; r < n; r += 256 / HC) tile[r * HCP + lc] = AT(r, i + 1 + ct + lc);
+        const int c = i + 1 + ct + lc;
+        const double w = tw[c];
+        int r0 = lr;
+        asm volatile("" : "+v"(r0));
+        const unsigned o = (unsigned)(r0 * n + c);
+        double *tl0 = tile + (r0 * HTP + lc);
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+          const int r = r0 + 16 * q;
+          double v = rv[q];
+          if (r > i && r < n) {
+            const double fac = (r == i + 1) ? tau : tau * ess[r - i - 2];
+            v = v - fac * w;
+            A[o + 16 * q * n] = v;
+          }
+          tl0[16 * q * HTP] = v;  // rows >= n are never read
+        }
       }
       __syncthreads();
+      if (ct + HT < rem) load(ct + HT);  // next tile's loads in flight during the row sums
       if (t < n) {
-        const int jb = ct - 1;  // even: HC is even and the first tile starts at j = 0
-        const int nl = (jb + ncol <= asz) ? ncol : asz - jb;  // lane terms in this tile
-        const double *tr = tile + t * HCP;
-        int cc = 0;
-        for (; cc + 1 < nl; cc += 2) {
-          l0 = l0 + tr[cc] * ess[jb + cc];
-          l1 = l1 + tr[cc + 1] * ess[jb + cc + 1];
+        const double *tr = tile + t * HTP;
+#pragma unroll
+        for (int cc = 0; cc < HT; cc++) {
+          if (cc < ncol) {
+            const int j = ct + cc - 1;  // ct is even: j is even for odd cc
+            const double v = tr[cc];
+            if (j < 0) col0 = v;
+            else if (j < asz) {
+              if (cc & 1) l0 = l0 + v * ess[j];
+              else l1 = l1 + v * ess[j];
+            } else tl = v * ess[j];
+          }
         }
-        if (cc < nl) l0 = l0 + tr[cc] * ess[jb + cc];
-        if ((mr & 1) && jb + ncol == mr) tl = tr[ncol - 1] * ess[mr - 1];
       }
       __syncthreads();
     }
     if (t < n) {
       double tmp = 0.0;
       if (asz > 0) tmp = tmp + (l0 + l1);
-      if (mr & 1) tmp = tmp + tl;
-      tv[t] = (0.0 + tmp) + AT(t, i + 1);
+      if (m & 1) tmp = tmp + tl;
+      const double tt = (0.0 + tmp) + col0;
+      const double tuv = tau * tt;
+      tu[t] = tuv;
+      const double v = col0 - tuv;  // column i+1 of A = A H: (tau*tmp) * 1
+      A[(long)t * n + i + 1] = v;
+      if (t >= i + 2) xcol[t - i - 2] = v;
     }
     __syncthreads();
-    // rank-1 update, thread per column (column i+1 has factor 1: x*1.0 is exact)
-    for (int cc = t; cc < rem; cc += blockDim.x) {
-      double *p = &AT(0, i + 1 + cc);
-      const double f = cc == 0 ? 1.0 : ess[cc - 1];
-      int r = 0;
-      for (; r + HB <= n; r += HB) {
-        double v[HB];
-#pragma unroll
-        for (int j = 0; j < HB; j++) v[j] = p[(long)(r + j) * n];
-#pragma unroll
-        for (int j = 0; j < HB; j++) p[(long)(r + j) * n] = v[j] - (tau * tv[r + j]) * f;
-      }
-      for (; r < n; r++) p[(long)r * n] -= (tau * tv[r]) * f;
-    }
+    const double tau_n = hess_reflector(A, n, i + 1, m, xcol, essb[pp ^ 1], sc);
+    hess_colpass(A, n, i, true, ess, tu, m >= 2, essb[pp ^ 1], tw);
     __syncthreads();
+    pp ^= 1;
+    tau = tau_n;
   }
-#undef AT
+  // last step (one row below the diagonal): Eigen scales by (1 - tau)
+  if (t == 0) A[(long)(n - 1) * n + (n - 1)] *= (1 - tau);
+  __syncthreads();
+  for (int r = t; r < n; r += blockDim.x) A[(long)r * n + (n - 1)] *= (1 - tau);
 }
 
 // Francis QR on the Hessenberg matrix left in H.Q by k_har_hess.
@@ -534,7 +628,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   const int angle = blockIdx.x;
   const HarDesc H = hd[blockIdx.y];
   const int n = *H.ng;
-  double *A = H.Q + (long)angle * n * n;
+  gptr<double> A = to_global(H.Q + (long)angle * n * n);
   const int t = threadIdx.x;
 #define AT(r, c) A[(long)(r) * n + (c)]
   // ---- real Schur form (RealSchur::compute without U); left updates are
@@ -543,10 +637,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
   // RealSchur::computeNormOfT (RealSchur.h:274-285) on row-major storage:
   // each row segment T(j, max(j-1,0)..n-1) is a contiguous Eigen redux, and
   // the segment sums are added row by row
-  __shared__ double segs[256];
+  double *segs = Rw;  // the window buffer is free until the first window
   for (int j = t; j < n; j += blockDim.x) {
     const int k0 = j - 1 > 0 ? j - 1 : 0;
-    const double *row = A + (long)j * n + k0;
+    gptr<const double> row = A + ((long)j * n + k0);
     segs[j] = eigen_redux(n - k0, (int)(((long)j * n + k0) & 1), [&](int k) { return fabs(row[k]); });
   }
   __syncthreads();
@@ -721,16 +815,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       const int b = (k1 + 3 < iuq + 1) ? k1 + 3 : iuq + 1;
       const int wd = b - a;
       {  // 32 x 32 tile, two rows per wave instruction, all loads in flight
+        int rb = t >> 5;
+        asm volatile("" : "+v"(rb));  // per-row values are formed here, not hoisted out of the sweep loop
         const int cl = t & 31;
+        const int cc = min(cl, wd - 1);  // out-of-window lanes re-read in-window entries (unused)
         double v[QW / 2];
 #pragma unroll
-        for (int it = 0; it < QW / 2; it++) {
-          const int r = (t >> 5) + 2 * it;
-          v[it] = (r < wd && cl < wd) ? AT(a + r, a + cl) : 0.0;
-        }
+        for (int it = 0; it < QW / 2; it++) v[it] = A[(unsigned)((a + min(rb + 2 * it, wd - 1)) * n + a + cc)];
 #pragma unroll
         for (int it = 0; it < QW / 2; it++) {
-          const int r = (t >> 5) + 2 * it;
+          const int r = rb + 2 * it;
           if (r < wd && cl < wd) Rw[r * QWP + cl] = v[it];
         }
       }
@@ -800,17 +894,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       // final: each thread streams down its column with a 3-value window.
       const int nstep = k1 - k0;
       for (int c = b + t; c <= iuq; c += blockDim.x) {
-        double *p = &AT(k0, c);
+        gptr<double> p = A + ((long)k0 * n + c);
         double w[QB + 2];
         {
-          const double *pl = p;  // running pointer: one live address
+          gptr<const double> pl = p;  // running pointer: one live address
 #pragma unroll
           for (int q = 0; q < QB + 2; q++) {
             w[q] = (q < nstep + 2) ? *pl : 0.0;
             pl += n;
           }
         }
-        double *ps = p;
+        gptr<double> ps = p;
         double u0 = w[0], u1 = w[1];
 #pragma unroll
         for (int q = 0; q < QB; q++) {
@@ -844,16 +938,16 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         const int nr = (a - r0 < 32) ? a - r0 : 32;
         const int nc = nstep + 2;
         {
+          int rb = t >> 5;
+          asm volatile("" : "+v"(rb));
           const int cl = t & 31;
+          const int cc = min(cl, nc - 1);
           double v[QW / 2];
 #pragma unroll
-          for (int it = 0; it < QW / 2; it++) {
-            const int rr = (t >> 5) + 2 * it;
-            v[it] = (rr < nr && cl < nc) ? AT(r0 + rr, k0 + cl) : 0.0;
-          }
+          for (int it = 0; it < QW / 2; it++) v[it] = A[(unsigned)((r0 + min(rb + 2 * it, nr - 1)) * n + k0 + cc)];
 #pragma unroll
           for (int it = 0; it < QW / 2; it++) {
-            const int rr = (t >> 5) + 2 * it;
+            const int rr = rb + 2 * it;
             if (rr < nr && cl < nc) Rw[rr * QWP + cl] = v[it];
           }
         }
