@@ -194,21 +194,45 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
   for (int g = 0; g < CF_GROUP; g++)
 #pragma unroll
     for (int q = 0; q < 5; q++) acc[g][q] = 0.0;
+  // the next chunk's table entries and pixels are fetched into registers
+  // while the current chunk is consumed from LDS
+  constexpr int CF_PER = (CF_CHUNK * CF_NN + 255) / 256;
+  const gptr<const double> Gtn = to_global(G.tn), Gfc = to_global(G.fc), Gfs = to_global(G.fs);
+  double rt[CF_PER], rc[CF_PER], rs[CF_PER], rv[CF_GROUP];
+  auto fetch = [&](int p0) {
+    const int np = (G.np - p0 < CF_CHUNK) ? G.np - p0 : CF_CHUNK;
+    const long base = (long)p0 * NN;
+#pragma unroll
+    for (int k = 0; k < CF_PER; k++) {
+      const int e = min(t + 256 * k, np * NN - 1);  // past the chunk: re-read its last entry (unused)
+      rt[k] = Gtn[base + e];
+      rc[k] = Gfc[base + e];
+      rs[k] = Gfs[base + e];
+    }
+    const int kidx = to_global(G.kk)[p0 + min(t, np - 1)];
+#pragma unroll
+    for (int g = 0; g < CF_GROUP; g++) rv[g] = g < ne ? to_global((const double *)G.img[g])[kidx] : 0.0;
+  };
+  fetch(0);
   for (int p0 = 0; p0 < G.np; p0 += CF_CHUNK) {
     const int np = (G.np - p0 < CF_CHUNK) ? G.np - p0 : CF_CHUNK;
     __syncthreads();  // the previous chunk is consumed
-    for (int e = t; e < np * NN; e += 256) {
-      const int p = e / NN, a = e - p * NN;
-      tn[p][a] = G.tn[(long)(p0 + p) * NN + a];
-      fc[p][a] = G.fc[(long)(p0 + p) * NN + a];
-      fs[p][a] = G.fs[(long)(p0 + p) * NN + a];
+#pragma unroll
+    for (int k = 0; k < CF_PER; k++) {
+      const int e = t + 256 * k;
+      if (e < np * NN) {
+        const int p = e / NN, a = e - p * NN;
+        tn[p][a] = rt[k];
+        fc[p][a] = rc[k];
+        fs[p][a] = rs[k];
+      }
     }
     if (t < np) {
-      const int kidx = G.kk[p0 + t];
 #pragma unroll
-      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = g < ne ? G.img[g][kidx] : 0.0;
+      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = rv[g];
     }
     __syncthreads();
+    if (p0 + CF_CHUNK < G.np) fetch(p0 + CF_CHUNK);
     for (int p = 0; p < np; p++) {
 #pragma unroll
       for (int q = 0; q < 5; q++) {

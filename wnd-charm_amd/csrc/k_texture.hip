@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd, int nimg) {
     int cnt = 0;
     for (int k0 = 0; k0 < ng; k0 += 64) {
       const int k = k0 + lane;
-      const double a = k < ng ? P[(long)i * ng + k] : 0.0;
+      const double a = k < ng ? to_global(P)[(long)i * ng + k] : 0.0;
       const unsigned long long m = __ballot(a != 0.0);
       if (a != 0.0) {
         const int pos = cnt + __popcll(m & ((1ull << lane) - 1));
@@ -315,13 +315,42 @@ __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd, int nimg) {
     Redux4 r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) r[q] = Redux4{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int kk = 0; kk < cnt; kk++) {
+    // list entries in order, four at a time so that 16 row loads are in flight
+    const gptr<const double> Pg = to_global(P);
+    int kk = 0;
+    for (; kk + 4 <= cnt; kk += 4) {
+      int k[4];
+      double a[4], d[4], b[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        k[u] = lk[w][kk + u];
+        a[u] = la[w][kk + u];
+        d[u] = ld[w][kk + u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int col = lane + 64 * q;
+          const double v = Pg[(unsigned)(k[u] * ng + min(col, ng - 1))];
+          b[u][q] = col < ng ? v : 0.0;
+        }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (b[u][q] != 0.0) r4_add(r[q], k[u], as2, as, (a[u] * b[u][q]) / d[u]);
+    }
+    for (; kk < cnt; kk++) {
       const int k = lk[w][kk];
       const double a = la[w][kk], d = ld[w][kk];
-      const double *row = P + (long)k * ng;
       double b[4];
 #pragma unroll
-      for (int q = 0; q < 4; q++) b[q] = (lane + 64 * q < ng) ? row[lane + 64 * q] : 0.0;
+      for (int q = 0; q < 4; q++) {
+        const int col = lane + 64 * q;
+        const double v = Pg[(unsigned)(k * ng + min(col, ng - 1))];
+        b[q] = col < ng ? v : 0.0;
+      }
 #pragma unroll
       for (int q = 0; q < 4; q++)
         if (b[q] != 0.0) r4_add(r[q], k, as2, as, (a * b[q]) / d);
@@ -1465,23 +1494,32 @@ __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const doubl
   } else if (predA(0) || predB(0)) {
     hi = 0;
   }
+  // columns nn in [lo, hi) in order, and within a column the sub-rows
+  // [m0, m1) whose bin is r or r-1 in order: the reference's per-bin
+  // accumulation order.  [m0, m1) moves monotonically with nn.
+  const gptr<const double> P = to_global((const double *)im.p);
   double acc = 0.0;
-  int nn = lo, mm = 0, m0 = 0, m1 = 0;
-  double xc = 0;
-  bool live = lo < hi;
-  if (live) {
-    xc = radon_xc(nn, xO, cs, rFirst);
-    int l = 0, h = M2;
-    while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) <= r) h = mid; else l = mid + 1; }
-    m0 = l;
-    l = m0; h = M2;
-    while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) < r - 1) h = mid; else l = mid + 1; }
-    m1 = l;
-    mm = m0;
-  }
-  while (live) {
-    if (mm < m1) {
-      double pixel = im.p[(long)(mm >> 1) * N + (nn >> 1)];
+  int m0 = 0, m1 = 0;
+  for (int nn = lo; nn < hi; nn++) {
+    const double xc = radon_xc(nn, xO, cs, rFirst);
+    if (nn == lo) {
+      int l = 0, h = M2;
+      while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) <= r) h = mid; else l = mid + 1; }
+      m0 = l;
+      l = m0; h = M2;
+      while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) < r - 1) h = mid; else l = mid + 1; }
+      m1 = l;
+    } else {
+      while (m0 > 0 && (int)(xc + yS[m0 - 1]) <= r) m0--;
+      while (m0 < M2 && (int)(xc + yS[m0]) > r) m0++;
+      if (m1 < m0) m1 = m0;
+      while (m1 > m0 && (int)(xc + yS[m1 - 1]) < r - 1) m1--;
+      while (m1 < M2 && (int)(xc + yS[m1]) >= r - 1) m1++;
+    }
+    const gptr<const double> col = P + (nn >> 1);
+#pragma unroll 4
+    for (int mm = m0; mm < m1; mm++) {
+      double pixel = col[(unsigned)((mm >> 1) * N)];
       if (pixel) {
         pixel *= 0.25;
         const double rIdx = (xc + yS[mm]);
@@ -1490,17 +1528,6 @@ __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const doubl
         if (rLow == r) acc += pixelLow;
         else acc += pixel - pixelLow;
       }
-      mm++;
-    } else if (++nn >= hi) {
-      live = false;
-    } else {
-      xc = radon_xc(nn, xO, cs, rFirst);
-      while (m0 > 0 && (int)(xc + yS[m0 - 1]) <= r) m0--;
-      while (m0 < M2 && (int)(xc + yS[m0]) > r) m0++;
-      if (m1 < m0) m1 = m0;
-      while (m1 > m0 && (int)(xc + yS[m1 - 1]) < r - 1) m1--;
-      while (m1 < M2 && (int)(xc + yS[m1]) >= r - 1) m1++;
-      mm = m0;
     }
   }
   proj[ent][(long)a * rSize + r] = acc;
