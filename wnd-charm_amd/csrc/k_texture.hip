@@ -387,9 +387,10 @@ __device__ void make_hh(const double *x, int m, double *ess, double *tau, double
 constexpr int QB = 28;       // bulge-chase steps per LDS window
 constexpr int QW = QB + 4;   // window side (<= 64: one wavefront lane per row/column)
 constexpr int QWP = QW + 1;  // padded LDS row of the window
-constexpr int HT = 16;       // Hessenberg: columns per row-dot tile (n x HT in LDS)
+constexpr int HT = 32;       // Hessenberg: columns per row-dot tile (n x HT in LDS)
 constexpr int HTP = HT + 1;  // padded tile row (bank-conflict free row reads)
-constexpr int HR = 16;       // rows in flight per thread in the column pass
+constexpr int HS = 256 / HT; // row stride of the tile-loading threads; each loads HT rows
+constexpr int HR = 16;       // rows in flight per thread in the column pass (a multiple of 4)
 
 // Reflector of column s below the diagonal (Eigen makeHouseholderInPlace),
 // x[0..rm-1] in LDS: one thread's sequential squared norm of the tail (the
@@ -481,19 +482,19 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   }
   double acc = 0.0;
   int g = skip;
-  for (; g + 16 <= bound; g += 16) {
-    double v[16];
+  for (; g + HR <= bound; g += HR) {
+    double v[HR];
 #pragma unroll
-    for (int q = 0; q < 16; q++) v[q] = p[(unsigned)((g + q) * n)];
+    for (int q = 0; q < HR; q++) v[q] = p[(unsigned)((g + q) * n)];
     if (upd) {
 #pragma unroll
-      for (int q = 0; q < 16; q++) {
+      for (int q = 0; q < HR; q++) {
         v[q] = v[q] - tu[s + 2 + g + q] * f;
         p[(unsigned)((g + q) * n)] = v[q];
       }
     }
 #pragma unroll
-    for (int h = 0; h < 16; h += 4) {
+    for (int h = 0; h < HR; h += 4) {
       const double a = v[h] * nx[g + h], p1 = v[h + 1] * nx[g + h + 1];
       const double c2 = v[h + 2] * nx[g + h + 2], p3 = v[h + 3] * nx[g + h + 3];
       const double b = odd_n ? p3 : p1, d = odd_n ? p1 : p3;
@@ -542,7 +543,9 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
 //   column pass  A = A H on columns i+2..n-1 fused with the left dot of
 //                step i+1 (hess_colpass).
 // Each element of the trailing block is read and written twice per step.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_har_hess(const HarDesc *hd) {
+// Two blocks per CU (waves_per_eu 2): the ~500 matrices in flight then fit
+// in the MALL, which serves the re-reads of later steps.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_har_hess(const HarDesc *hd) {
   __shared__ double tile[256 * HTP];
   __shared__ double essb[2][256];  // reflector of step i and of step i+1
   __shared__ double tw[256];       // left dot of the current step, by column
@@ -565,15 +568,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
   for (int i = 0; i + 2 < n; ++i) {
     const int rem = n - i - 1, m = rem - 1, asz = m & ~1;
     const double *ess = essb[pp];
-    double rv[16];
+    double rv[HT];
     auto load = [&](int ct) {
       if (ct + lc < rem) {
         int r0 = lr;
         asm volatile("" : "+v"(r0));  // per-row values are formed at use, not hoisted out of the column loop
         const int c = i + 1 + ct + lc;
-        // rows past the end re-read row n-1: branch-free, so all 16 loads issue back to back
+        // rows past the end re-read row n-1: branch-free, so all HT loads issue back to back
 #pragma unroll
-        for (int q = 0; q < 16; q++) rv[q] = A[(unsigned)(min(r0 + 16 * q, n - 1) * n + c)];
+        for (int q = 0; q < HT; q++) rv[q] = A[(unsigned)(min(r0 + HS * q, n - 1) * n + c)];
       }
     };
     double l0 = 0.0, l1 = 0.0, tl = 0.0, col0 = 0.0;
@@ -588,15 +591,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         const unsigned o = (unsigned)(r0 * n + c);
         double *tl0 = tile + (r0 * HTP + lc);
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-          const int r = r0 + 16 * q;
+        for (int q = 0; q < HT; q++) {
+          const int r = r0 + HS * q;
           double v = rv[q];
           if (r > i && r < n) {
             const double fac = (r == i + 1) ? tau : tau * ess[r - i - 2];
             v = v - fac * w;
-            A[o + 16 * q * n] = v;
+            A[o + HS * q * n] = v;
           }
-          tl0[16 * q * HTP] = v;  // rows >= n are never read
+          tl0[HS * q * HTP] = v;  // rows >= n are never read
         }
       }
       __syncthreads();
