@@ -134,15 +134,17 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
   const int angle = blockIdx.x;
   const HarDesc H = hd[blockIdx.y];
   const int ng = *H.ng;
-  const double *P = H.P + (long)angle * ng * ng;
+  const gptr<const double> P = to_global((const double *)H.P + (long)angle * ng * ng);
   const int t = threadIdx.x;
   const double LOG2 = log10(2.0), EPS = 0.000000001;
-  // exact per-row / per-column marginals (sequential, as the reference loops)
+  // exact per-row / per-column marginals (sequential, as the reference
+  // loops).  P is symmetric (k_har_cooc stores the counts as a triangle),
+  // so row t equals column t term by term and both sums walk the column,
+  // which the lanes read coalesced.
   if (t < ng) {
-    double a = 0, b = 0;
-    for (int j = 0; j < ng; j++) a += P[t * ng + j];
+    double b = 0;
     for (int i = 0; i < ng; i++) b += P[i * ng + t];
-    px[t] = a;
+    px[t] = b;
     py[t] = b;
   }
   // Pxpy[i+j] (anti-diagonals, raster order) and Pxpy[|i-j|] (f2, f10, f11)
@@ -229,11 +231,11 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     F[12] = sqrt(fabs(1 - exp(-2.0 * (hxy2 - hxy))));
   }
   // marginals of _Q_calc_eigen: px = P.rowwise().sum() (row t of the
-  // row-major array starts at linear index t*ng), py = P.colwise().sum()
-  // (strided: a running sum, equal to py above)
+  // row-major array starts at linear index t*ng; its terms are read down
+  // column t, equal by symmetry), py = P.colwise().sum() (strided: a
+  // running sum, equal to py above)
   if (t < ng) {
-    const double *row = P + (long)t * ng;
-    H.marg[angle * 512 + t] = eigen_redux(ng, (int)(((long)t * ng) & 1), [&](int k) { return row[k]; });
+    H.marg[angle * 512 + t] = eigen_redux(ng, (int)(((long)t * ng) & 1), [&](int k) { return P[(long)k * ng + t]; });
     H.marg[angle * 512 + 256 + t] = py[t];
   }
 }
