@@ -389,6 +389,9 @@ __device__ void make_hh(const double *x, int m, double *ess, double *tau, double
 constexpr int QB = 28;       // bulge-chase steps per LDS window
 constexpr int QW = QB + 4;   // window side (<= 64: one wavefront lane per row/column)
 constexpr int QWP = QW + 1;  // padded LDS row of the window
+// window and strip tiles are loaded as two rows of 32 lanes per wave instruction
+static_assert(QW <= 32 && QW % 2 == 0 && QB + 2 <= QW, "QR window tiles must fit 32-lane rows");
+static_assert(QW * QWP >= 256, "the window buffer also holds the n <= 256 row sums of the Schur norm");
 constexpr int HT = 32;       // Hessenberg: columns per row-dot tile (n x HT in LDS)
 constexpr int HTP = HT + 1;  // padded tile row (bank-conflict free row reads)
 constexpr int HS = 256 / HT; // row stride of the tile-loading threads; each loads HT rows
@@ -964,12 +967,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         ps[n] = u1;
       }
       // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors).
-      // Staged through the (now free) window buffer in chunks of 32 rows with
+      // Staged through the (now free) window buffer in chunks of QW rows with
       // coalesced loads and stores; one lane streams each row through the
       // reflectors in LDS.
       __syncthreads();
-      for (int r0 = 0; r0 < a; r0 += 32) {
-        const int nr = (a - r0 < 32) ? a - r0 : 32;
+      for (int r0 = 0; r0 < a; r0 += QW) {  // chunks of QW rows: a chunk fills at most the window buffer
+        const int nr = (a - r0 < QW) ? a - r0 : QW;
         const int nc = nstep + 2;
         {
           int rb = t >> 5;
