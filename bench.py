@@ -7,13 +7,15 @@ step = one pass of the DAG over one batch of images already resident in
 HBM (wndgpu_compute_device); value = images/s over the timed steps.
 
 Also reported:
-  roofline      the dominant single-kernel launch (largest mean HIP-event
-                time over the timed steps, events recorded on the engine's
-                stream around every launch) against MI355X's FP64 peak:
-                k_har_hess does (exactly) hess_flops(Ng) per Q matrix,
-                k_har_qr reports the flops its Francis sweeps issued;
-                traffic (HBM bytes per launch) is read from the rocprofv3
-                --pmc summary committed under profiles/ when present.
+  roofline      the dominant kernel (largest HIP-event time per step,
+                events recorded around every launch on the stream that
+                launch runs on) against MI355X's FP64 peak, per launch
+                (k_har_hess runs as two launches per step, one per half of
+                the Haralick matrices): k_har_hess does (exactly)
+                hess_flops(Ng) per Q matrix, k_har_qr reports the flops its
+                Francis sweeps issued; traffic (HBM bytes per launch) is read
+                from the rocprofv3 --pmc summary committed under profiles/
+                when present.
   cpu_baseline  the FP64 C++ oracle, one image per thread on the host
                 cores, on a bounded sample of the same images.
 """
@@ -155,28 +157,32 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    acc = {}
+    acc, cnt = {}, {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
         for name, ms in wndcharm.last_node_times():
             acc[name] = acc.get(name, 0.0) + ms
+            cnt[name] = cnt.get(name, 0) + 1
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     ms_step = el * 1000 / args.steps
     value = len(imgs) * args.steps / el
     mean = {k: v / args.steps for k, v in acc.items()}
 
-    # dominant single-kernel launch and its FP64 roofline
+    # dominant kernel (largest time per step) and its FP64 roofline, per launch
     kern = {k: v for k, v in mean.items() if '[k_' in k}
     dom = max(kern, key=kern.get)
     kname = re.search(r'\[(k_\w+)\]', dom).group(1)
+    per_step = cnt[dom] // args.steps  # launches of that kernel per step
     tones, schur = wndcharm.last_haralick_work()
     flops = float(4 * sum(hess_flops(int(n)) for n in tones)) if kname == 'k_har_hess' else float(schur.sum())
-    achieved = flops / (mean[dom] * 1e-3) / 1e12
+    launch_ms = acc[dom] / cnt[dom]
+    achieved = flops / per_step / (launch_ms * 1e-3) / 1e12
     roof = {'kernel': kname, 'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': FP64_PEAK_TFLOPS,
             'unit': 'TFLOP/s', 'frac': round(achieved / FP64_PEAK_TFLOPS, 5), 'traffic': profiled_traffic(kname),
-            'launch_ms': round(mean[dom], 3), 'flops_per_launch': flops, 'matrices_per_launch': 4 * len(tones)}
+            'launch_ms': round(launch_ms, 3), 'launches_per_step': per_step, 'flops_per_launch': flops / per_step,
+            'matrices_per_launch': 4 * len(tones) / per_step}
     stage = {}
     for k, v in mean.items():
         key = k if '[' in k else k.split(' (')[0]

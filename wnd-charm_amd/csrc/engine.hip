@@ -189,6 +189,13 @@ struct NodeRun {
 struct wndgpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // The Haralick chain (co-occurrence .. Schur) runs on `side`, forked from
+  // `stream` once every transform node is issued and joined back before the
+  // batch completes: its memory- and latency-bound eigen kernels overlap
+  // the arithmetic-bound feature kernels on `stream`.  Both streams have the
+  // default priority.
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   std::string err;
   // arena
   char *dev = nullptr;
@@ -358,17 +365,30 @@ static void out_size(int kind, int h, int w, int *oh, int *ow) {
   else { *oh = h; *ow = w; }
 }
 
+// One launch closure of a batch: its name (timing), the kernels, and whether
+// it runs on the side stream.
+struct LaunchItem {
+  std::string name;
+  std::function<void()> f;
+  bool side;
+};
+
 // Build (or size) every launch of a batch; when !A.sizing the launches are
-// also appended to `launches` in execution order.
+// also appended to `launches` in execution order, and *first_feature is the
+// index of the first feature launch (the ones before it compute the DAG's
+// transform nodes).
 static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, Arena &A,
-                       std::vector<std::pair<std::string, std::function<void()>>> &launches) {
+                       std::vector<LaunchItem> &launches, size_t *first_feature) {
   if (!A.sizing) c->har_n = 0;
   hipStream_t s = c->stream;
   const int n = B.n;
   std::map<std::vector<int>, NodeRun> nodes;
   auto maxof = [](const std::vector<int> &v) { return *std::max_element(v.begin(), v.end()); };
   auto push = [&](const std::string &name, std::function<void()> f) {
-    if (!A.sizing) launches.emplace_back(name, std::move(f));
+    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), false});
+  };
+  auto push_side = [&](const std::string &name, std::function<void()> f) {
+    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), true});
   };
   // ---- root: load + stats
   {
@@ -479,6 +499,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     int i;
     int off;
   };
+  *first_feature = launches.size();
   std::vector<std::vector<Entry>> by_alg(A_COUNT);
   for (const auto &key : plan->nodes) {
     const NodeRun &N = nodes[key];
@@ -591,14 +612,15 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           c->har_n = ne;
         }
         HarDesc *hdd = A.stage(hd);
-        push(gn + " [co-occurrence, f1-f13, Q]", [=]() {
-          gather();
-          hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)ne, s);
-          launch_haralick_prep(ni, nst, hdd, ne, (int)maxp, s);
+        hipStream_t s2 = c->side;
+        push_side(gn + " [co-occurrence, f1-f13, Q]", [=]() {
+          launch_gather_stats(sts, nst, ne, s2);
+          hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)ne, s2);
+          launch_haralick_prep(ni, nst, hdd, ne, (int)maxp, s2);
         });
-        push(gn + " [k_har_hess]", [=]() { launch_haralick_hess(hdd, ne, s); });
-        push(gn + " [k_har_qr]", [=]() { launch_haralick_qr(hdd, ne, s); });
-        push(gn + " [mean, range]", [=]() { launch_haralick_final(hdd, o, ne, s); });
+        push_side(gn + " [k_har_hess]", [=]() { launch_haralick_hess(hdd, ne, s2); });
+        push_side(gn + " [k_har_qr]", [=]() { launch_haralick_qr(hdd, ne, s2); });
+        push_side(gn + " [mean, range]", [=]() { launch_haralick_final(hdd, o, ne, s2); });
       } break;
       case A_ZERNIKE: {
         const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
@@ -828,10 +850,11 @@ static int ensure_tables(wndgpu_ctx *c) {
 static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timing) {
   int r;
   if ((r = ensure_tables(c))) return r;
-  std::vector<std::pair<std::string, std::function<void()>>> launches;
+  std::vector<LaunchItem> launches;
+  size_t first_feature = 0;
   // pass 1: sizes
   Arena A1{c, true};
-  if ((r = build_batch(c, plan, B, A1, launches))) return r;
+  if ((r = build_batch(c, plan, B, A1, launches, &first_feature))) return r;
   if (A1.d + 4096 > c->dev_cap) {
     if (c->dev) hipFree(c->dev);
     c->dev = nullptr;
@@ -849,7 +872,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   }
   // pass 2: real addresses
   Arena A2{c, false};
-  if ((r = build_batch(c, plan, B, A2, launches))) return r;
+  if ((r = build_batch(c, plan, B, A2, launches, &first_feature))) return r;
   HIPCHK(c, hipMemcpyAsync(c->hdesc, c->host, A2.h, hipMemcpyHostToDevice, c->stream));
   c->times.clear();
   c->tnames.clear();
@@ -859,13 +882,36 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
       HIPCHK(c, hipEventCreate(&e));
       c->evpool.push_back(e);
     }
-  for (size_t k = 0; k < launches.size(); k++) {
-    if (timing) hipEventRecord(c->evpool[2 * k], c->stream);
-    launches[k].second();
+  // transform nodes on `stream`; then the side-stream launches, forked
+  // after the nodes; then the other feature launches; the join makes
+  // `stream` wait for the side stream
+  size_t slot = 0;
+  auto issue = [&](const LaunchItem &L) {
+    hipStream_t st = L.side ? c->side : c->stream;
+    if (timing) hipEventRecord(c->evpool[2 * slot], st);
+    L.f();
     if (timing) {
-      hipEventRecord(c->evpool[2 * k + 1], c->stream);
-      c->tnames.push_back(launches[k].first);
+      hipEventRecord(c->evpool[2 * slot + 1], st);
+      c->tnames.push_back(L.name);
     }
+    slot++;
+  };
+  for (size_t k = 0; k < first_feature; k++) issue(launches[k]);
+  bool forked = false;
+  for (size_t k = first_feature; k < launches.size(); k++)
+    if (launches[k].side) {
+      if (!forked) {
+        HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
+        forked = true;
+      }
+      issue(launches[k]);
+    }
+  for (size_t k = first_feature; k < launches.size(); k++)
+    if (!launches[k].side) issue(launches[k]);
+  if (forked) {
+    HIPCHK(c, hipEventRecord(c->join_ev, c->side));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
   }
   HIPCHK(c, hipGetLastError());
   return 0;
@@ -887,6 +933,16 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     delete c;
     return WNDGPU_E_DEVICE;
   }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+    if (c->join_ev) hipEventDestroy(c->join_ev);
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
+    if (c->side) hipStreamDestroy(c->side);
+    hipStreamDestroy(c->stream);
+    delete c;
+    return WNDGPU_E_DEVICE;
+  }
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS
   *out = c;
   return WNDGPU_OK;
@@ -895,6 +951,7 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
 void wndgpu_destroy(wndgpu_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
+  hipStreamSynchronize(c->side);
   hipStreamSynchronize(c->stream);
   auto fr = [](DevTable &t) { if (t.p) hipFree(t.p); t.p = nullptr; };
   for (auto &kv : c->cheb) fr(kv.second);
@@ -907,6 +964,9 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   if (c->dev) hipFree(c->dev);
   if (c->hdesc) hipFree(c->hdesc);
   if (c->host) hipHostFree(c->host);
+  hipEventDestroy(c->fork_ev);
+  hipEventDestroy(c->join_ev);
+  hipStreamDestroy(c->side);
   hipStreamDestroy(c->stream);
   delete c;
 }
