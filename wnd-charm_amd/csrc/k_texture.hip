@@ -401,9 +401,11 @@ constexpr int HR = 16;       // rows in flight per thread in the column pass (a 
 // x[0..rm-1] in LDS: one thread's sequential squared norm of the tail (the
 // column is strided in row-major storage, so Eigen sums it in order), then
 // the essential part element-parallel.  Writes beta and the zeros of the
-// Hessenberg form into column s.  Returns tau.
+// Hessenberg form into column s, and the reflector into row s of `store`
+// (essential part, tau in column n-1) for the rows above the active block.
+// Returns tau.
 __device__ __forceinline__ double hess_reflector(gptr<double> A, int n, int s, int rm, const double *x, double *ess,
-                                                 double *sc) {
+                                                 double *sc, gptr<double> store) {
   const int t = threadIdx.x;
   if (t == 0) {
     double tail = 0;
@@ -424,8 +426,15 @@ __device__ __forceinline__ double hess_reflector(gptr<double> A, int n, int s, i
   const double tau = sc[0];
   const bool nz = sc[3] != 0;
   const double den = sc[2];
-  for (int k = 1 + t; k < rm; k += blockDim.x) ess[k - 1] = nz ? x[k] / den : 0.0;
-  if (t == 0) A[(long)(s + 1) * n + s] = sc[1];
+  for (int k = 1 + t; k < rm; k += blockDim.x) {
+    const double e = nz ? x[k] / den : 0.0;
+    ess[k - 1] = e;
+    store[(long)s * n + k - 1] = e;
+  }
+  if (t == 0) {
+    A[(long)(s + 1) * n + s] = sc[1];
+    store[(long)s * n + n - 1] = tau;
+  }
   for (int k = 1 + t; k < rm; k += blockDim.x) A[(long)(s + 1 + k) * n + s] = 0.0;  // matrixH(): Hessenberg part only
   __syncthreads();
   return tau;
@@ -433,7 +442,8 @@ __device__ __forceinline__ double hess_reflector(gptr<double> A, int n, int s, i
 
 // Column pass after step i (columns c = i+2..n-1, one thread per column;
 // threads left over split the rows above the dot region):
-//   upd  A = A H of step i on every row: A(r,c) -= tu[r] * ess[c-i-2], with
+//   upd  A = A H of step i on rows i+1..n-1 (the rows above are deferred):
+//        A(r,c) -= tu[r] * ess[c-i-2], with
 //        tu[r] = tau * (row dot), Eigen's (tau * tmp) * essential^T
 //   dot  the left dot of step i+1 on the updated column, in Eigen's ColMajor
 //        GEMV order on bottom^T (general_matrix_vector_product<ColMajor>,
@@ -455,7 +465,8 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   const int d0 = dot ? i + 2 : n;  // first row of the dot region
   if (upd && !(dot && k > 1 && sub == 0)) {
     const int parts = (dot && k > 1) ? k - 1 : k, ps = (dot && k > 1) ? sub - 1 : sub;
-    const int ra = d0 * ps / parts, rb = d0 * (ps + 1) / parts;
+    const int base = i + 1;  // upd implies i >= 0
+    const int ra = base + (d0 - base) * ps / parts, rb = base + (d0 - base) * (ps + 1) / parts;
     unsigned o = (unsigned)(ra * n + c);
     int r = ra;
     for (; r + HR <= rb; r += HR, o += HR * n) {
@@ -534,20 +545,26 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
 
 // One block per (angle, image): HessenbergDecomposition::_compute
 // (Eigenvalues/HessenbergDecomposition.h:292-318) of Q (n x n, row-major, in
-// place), two passes over the trailing columns per step i:
-//   tile pass    columns i+1..n-1 in n x HT LDS tiles.  The loading thread
-//                applies A = H A of step i to rows i+1..n-1 (row i+1 -=
+// place).  Rows 0..i only ever receive the right reflectors of steps >= i,
+// and nothing else reads them, so step i works on the active rows
+// i+1..n-1 and the rows above are brought up to date at the end (deferred
+// pass); every element still receives the same operations in the same
+// order.  Two passes over the active block per step:
+//   tile pass    columns i+1..n-1 in HT-column LDS tiles of the active rows.
+//                The loading thread applies A = H A of step i (row i+1 -=
 //                tau*tw, bottom -= (tau*ess)*tw; tw = the left dot from the
-//                previous column pass) and writes them back; thread r then
-//                accumulates row r's dot in Eigen's RowMajor GEMV order
+//                previous column pass) and writes it back; the row's thread
+//                then accumulates the row dot in Eigen's RowMajor GEMV order
 //                (general_matrix_vector_product<RowMajor>,
 //                GeneralMatrixVector.h:289-552): even and odd terms in two
 //                lanes, the lanes added, an odd last term, + A(r, i+1).
 //   column i+1   A = A H on column i+1 (factor 1), then the reflector of
-//                step i+1 from it.
+//                step i+1 from it (kept in the co-occurrence buffer, free
+//                once Q is built, for the deferred pass).
 //   column pass  A = A H on columns i+2..n-1 fused with the left dot of
 //                step i+1 (hess_colpass).
-// Each element of the trailing block is read and written twice per step.
+// Eigen's last step (one row below the diagonal) scales by 1 - tau with
+// tau = 0 (a one-element reflector): an exact no-op, not performed.
 // Two blocks per CU (waves_per_eu 2): the ~500 matrices in flight then fit
 // in the MALL, which serves the re-reads of later steps.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_har_hess(const HarDesc *hd) {
@@ -562,16 +579,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int n = *H.ng;
   if (n < 2) return;
   gptr<double> A = to_global(H.Q + (long)angle * n * n);
+  gptr<double> Rf = to_global(H.P + (long)angle * n * n);  // reflector store
   const int t = threadIdx.x;
   for (int k = t; k < n - 1; k += blockDim.x) xcol[k] = A[(long)(1 + k) * n];
   __syncthreads();
-  double tau = hess_reflector(A, n, 0, n - 1, xcol, essb[0], sc);
+  double tau = hess_reflector(A, n, 0, n - 1, xcol, essb[0], sc, Rf);
   if (n - 1 >= 2) hess_colpass(A, n, -1, false, nullptr, nullptr, true, essb[0], tw);
   __syncthreads();
   int pp = 0;
   const int lc = t % HT, lr = t / HT;
   for (int i = 0; i + 2 < n; ++i) {
     const int rem = n - i - 1, m = rem - 1, asz = m & ~1;
+    const int rb = i + 1;  // first active row; tile row k holds row rb + k
     const double *ess = essb[pp];
     double rv[HT];
     auto load = [&](int ct) {
@@ -581,7 +600,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const int c = i + 1 + ct + lc;
         // rows past the end re-read row n-1: branch-free, so all HT loads issue back to back
 #pragma unroll
-        for (int q = 0; q < HT; q++) rv[q] = A[(unsigned)(min(r0 + HS * q, n - 1) * n + c)];
+        for (int q = 0; q < HT; q++) rv[q] = A[(unsigned)(min(rb + r0 + HS * q, n - 1) * n + c)];
       }
     };
     double l0 = 0.0, l1 = 0.0, tl = 0.0, col0 = 0.0;
@@ -593,13 +612,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const double w = tw[c];
         int r0 = lr;
         asm volatile("" : "+v"(r0));
-        const unsigned o = (unsigned)(r0 * n + c);
+        const unsigned o = (unsigned)((rb + r0) * n + c);
         double *tl0 = tile + (r0 * HTP + lc);
 #pragma unroll
         for (int q = 0; q < HT; q++) {
-          const int r = r0 + HS * q;
+          const int r = rb + r0 + HS * q;
           double v = rv[q];
-          if (r > i && r < n) {
+          if (r < n) {
             const double fac = (r == i + 1) ? tau : tau * ess[r - i - 2];
             v = v - fac * w;
             A[o + HS * q * n] = v;
@@ -609,7 +628,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       }
       __syncthreads();
       if (ct + HT < rem) load(ct + HT);  // next tile's loads in flight during the row sums
-      if (t < n) {
+      if (t < rem) {
         const double *tr = tile + t * HTP;
 #pragma unroll
         for (int cc = 0; cc < HT; cc++) {
@@ -626,28 +645,97 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       }
       __syncthreads();
     }
-    if (t < n) {
+    if (t < rem) {
+      const int r = rb + t;
       double tmp = 0.0;
       if (asz > 0) tmp = tmp + (l0 + l1);
       if (m & 1) tmp = tmp + tl;
       const double tt = (0.0 + tmp) + col0;
       const double tuv = tau * tt;
-      tu[t] = tuv;
+      tu[r] = tuv;
       const double v = col0 - tuv;  // column i+1 of A = A H: (tau*tmp) * 1
-      A[(long)t * n + i + 1] = v;
-      if (t >= i + 2) xcol[t - i - 2] = v;
+      A[(long)r * n + i + 1] = v;
+      if (r >= i + 2) xcol[r - i - 2] = v;
     }
     __syncthreads();
-    const double tau_n = hess_reflector(A, n, i + 1, m, xcol, essb[pp ^ 1], sc);
+    const double tau_n = hess_reflector(A, n, i + 1, m, xcol, essb[pp ^ 1], sc, Rf);
     hess_colpass(A, n, i, true, ess, tu, m >= 2, essb[pp ^ 1], tw);
     __syncthreads();
     pp ^= 1;
     tau = tau_n;
   }
-  // last step (one row below the diagonal): Eigen scales by (1 - tau)
-  if (t == 0) A[(long)(n - 1) * n + (n - 1)] *= (1 - tau);
-  __syncthreads();
-  for (int r = t; r < n; r += blockDim.x) A[(long)r * n + (n - 1)] *= (1 - tau);
+  // Deferred pass: row r (r <= n-3) receives the right reflectors of steps
+  // r..n-3 in order, exactly as the column passes would have applied them:
+  // row dot in the RowMajor GEMV order (lanes l0 / l1 of one row are the
+  // two threads of a pair), tu = tau * (dot + A(r, i+1)), A(r, i+1) -= tu,
+  // A(r, c) -= tu * ess[c-i-2].  Chunks of rows r0.. are staged in LDS with
+  // their live columns r0+1..n-1 only, as many rows as the tile buffer
+  // holds (at most 128); reflectors stream through a two-deep register
+  // pipeline (one element per thread) into two LDS buffers.
+  double *rows = tile;
+  const int lane = t & 63, wv = t >> 6;
+  for (int r0 = 0; r0 < n - 2;) {
+    const int cb = r0 + 1, wd = n - cb, wp = wd | 1;  // staged columns cb..n-1, odd row stride
+    int nr = (256 * HTP) / wp;
+    if (nr > 128) nr = 128;
+    if (nr > n - 2 - r0) nr = n - 2 - r0;
+    for (int rr = wv; rr < nr; rr += 4)
+      for (int c = lane; c < wd; c += 64) rows[rr * wp + c] = A[(long)(r0 + rr) * n + cb + c];
+    // reflectors r0 and r0+1 in flight; r0 goes to LDS buffer 0
+    auto fetch = [&](int i, double &e, double &tv) {
+      e = 0.0;
+      tv = 0.0;
+      if (i <= n - 3) {
+        if (t < n - i - 2) e = Rf[(long)i * n + t];
+        tv = Rf[(long)i * n + n - 1];
+      }
+    };
+    double e1, t1, e2, t2;
+    fetch(r0, e1, t1);
+    fetch(r0 + 1, e2, t2);
+    essb[0][t] = e1;
+    double ti = t1;
+    e1 = e2;
+    t1 = t2;
+    int pb = 0;
+    __syncthreads();
+    for (int i = r0; i <= n - 3; i++) {
+      fetch(i + 2, e2, t2);  // two steps ahead
+      const double *eb = essb[pb];
+      const int m = n - i - 2, asz = m & ~1;
+      const int na = (i + 1 - r0 < nr) ? i + 1 - r0 : nr;  // rows r0..i of the chunk take part
+      if (t < 2 * na) {
+        const int rr = t >> 1, par = t & 1;
+        const double *row = rows + rr * wp + (i + 2 - cb);
+        double l = 0.0;
+#pragma unroll 4
+        for (int j = par; j < asz; j += 2) l = l + row[j] * eb[j];
+        const double l1 = __shfl_xor(l, 1);
+        if (par == 0) {
+          double tmp = 0.0;
+          if (asz > 0) tmp = tmp + (l + l1);
+          if (m & 1) tmp = tmp + row[m - 1] * eb[m - 1];
+          tu[rr] = ti * ((0.0 + tmp) + rows[rr * wp + (i + 1 - cb)]);
+        }
+      }
+      __syncthreads();
+      for (int rr = wv; rr < na; rr += 4) {
+        const double tv = tu[rr];
+        double *row = rows + rr * wp + (i + 1 - cb);
+        for (int c = lane; c <= m; c += 64) row[c] = c == 0 ? row[c] - tv : row[c] - tv * eb[c - 1];
+      }
+      essb[pb ^ 1][t] = e1;  // reflector i+1
+      ti = t1;
+      e1 = e2;
+      t1 = t2;
+      pb ^= 1;
+      __syncthreads();
+    }
+    for (int rr = wv; rr < nr; rr += 4)
+      for (int c = lane; c < wd; c += 64) A[(long)(r0 + rr) * n + cb + c] = rows[rr * wp + c];
+    __syncthreads();
+    r0 += nr;
+  }
 }
 
 // Francis QR on the Hessenberg matrix left in H.Q by k_har_hess.
