@@ -681,7 +681,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (nr > n - 2 - r0) nr = n - 2 - r0;
     for (int rr = wv; rr < nr; rr += 4)
       for (int c = lane; c < wd; c += 64) rows[rr * wp + c] = A[(long)(r0 + rr) * n + cb + c];
-    // reflectors r0 and r0+1 in flight; r0 goes to LDS buffer 0
+    // reflector r0 in LDS buffer 0; reflectors r0+1..r0+4 in flight in a
+    // ring of four register slots, slot k holding reflector r0+s with
+    // s = k (mod 4): the step loop is unrolled by four so that every slot
+    // is named statically and consumed four steps after its load
     auto fetch = [&](int i, double &e, double &tv) {
       e = 0.0;
       tv = 0.0;
@@ -690,25 +693,27 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         tv = Rf[(long)i * n + n - 1];
       }
     };
-    double e1, t1, e2, t2;
-    fetch(r0, e1, t1);
-    fetch(r0 + 1, e2, t2);
-    essb[0][t] = e1;
-    double ti = t1;
-    e1 = e2;
-    t1 = t2;
-    int pb = 0;
+    double e0, t0, e1, t1, e2, t2, e3, t3;
+    fetch(r0, e0, t0);
+    essb[0][t] = e0;
+    double ti = t0;
+    fetch(r0 + 1, e1, t1);
+    fetch(r0 + 2, e2, t2);
+    fetch(r0 + 3, e3, t3);
+    fetch(r0 + 4, e0, t0);
     __syncthreads();
-    for (int i = r0; i <= n - 3; i++) {
-      fetch(i + 2, e2, t2);  // two steps ahead
-      const double *eb = essb[pb];
+    // step i = r0 + s; afterwards reflector i+1 (slot (s+1) mod 4) goes to
+    // LDS and that slot is refilled with reflector i+5
+    auto step = [&](int s, double &en, double &tn) {
+      const int i = r0 + s;
+      const double *eb = essb[s & 1];
       const int m = n - i - 2, asz = m & ~1;
       const int na = (i + 1 - r0 < nr) ? i + 1 - r0 : nr;  // rows r0..i of the chunk take part
       if (t < 2 * na) {
         const int rr = t >> 1, par = t & 1;
         const double *row = rows + rr * wp + (i + 2 - cb);
         double l = 0.0;
-#pragma unroll 4
+#pragma unroll 8
         for (int j = par; j < asz; j += 2) l = l + row[j] * eb[j];
         const double l1 = __shfl_xor(l, 1);
         if (par == 0) {
@@ -719,17 +724,32 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         }
       }
       __syncthreads();
-      for (int rr = wv; rr < na; rr += 4) {
-        const double tv = tu[rr];
-        double *row = rows + rr * wp + (i + 1 - cb);
-        for (int c = lane; c <= m; c += 64) row[c] = c == 0 ? row[c] - tv : row[c] - tv * eb[c - 1];
+      // column c (c = 0: column i+1, factor 1) of rows wv, wv+4, ..: the
+      // rows' entries are independent, so their LDS accesses overlap
+      for (int c = lane; c <= m; c += 64) {
+        double *col = rows + (i + 1 - cb) + c;
+        if (c == 0) {
+#pragma unroll 4
+          for (int rr = wv; rr < na; rr += 4) col[rr * wp] = col[rr * wp] - tu[rr];
+        } else {
+          const double e = eb[c - 1];
+#pragma unroll 4
+          for (int rr = wv; rr < na; rr += 4) col[rr * wp] = col[rr * wp] - tu[rr] * e;
+        }
       }
-      essb[pb ^ 1][t] = e1;  // reflector i+1
-      ti = t1;
-      e1 = e2;
-      t1 = t2;
-      pb ^= 1;
+      essb[(s + 1) & 1][t] = en;
+      ti = tn;
+      fetch(i + 5, en, tn);
       __syncthreads();
+    };
+    for (int s = 0; r0 + s <= n - 3; s += 4) {
+      step(s, e1, t1);
+      if (r0 + s + 1 > n - 3) break;
+      step(s + 1, e2, t2);
+      if (r0 + s + 2 > n - 3) break;
+      step(s + 2, e3, t3);
+      if (r0 + s + 3 > n - 3) break;
+      step(s + 3, e0, t0);
     }
     for (int rr = wv; rr < nr; rr += 4)
       for (int c = lane; c < wd; c += 64) A[(long)(r0 + rr) * n + cb + c] = rows[rr * wp + c];
