@@ -196,6 +196,7 @@ struct wndgpu_ctx {
   // default priority.
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool serial = false;  // WNDGPU_SERIAL: every launch on `stream` (per-kernel timing without overlap)
   std::string err;
   // arena
   char *dev = nullptr;
@@ -612,7 +613,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           c->har_n = ne;
         }
         HarDesc *hdd = A.stage(hd);
-        hipStream_t s2 = c->side;
+        hipStream_t s2 = c->serial ? c->stream : c->side;
         push_side(gn + " [co-occurrence, f1-f13, Q]", [=]() {
           launch_gather_stats(sts, nst, ne, s2);
           hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)ne, s2);
@@ -887,7 +888,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   // `stream` wait for the side stream
   size_t slot = 0;
   auto issue = [&](const LaunchItem &L) {
-    hipStream_t st = L.side ? c->side : c->stream;
+    hipStream_t st = (L.side && !c->serial) ? c->side : c->stream;
     if (timing) hipEventRecord(c->evpool[2 * slot], st);
     L.f();
     if (timing) {
@@ -900,7 +901,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   bool forked = false;
   for (size_t k = first_feature; k < launches.size(); k++)
     if (launches[k].side) {
-      if (!forked) {
+      if (!forked && !c->serial) {
         HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
         HIPCHK(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
         forked = true;
@@ -943,6 +944,7 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     delete c;
     return WNDGPU_E_DEVICE;
   }
+  c->serial = getenv("WNDGPU_SERIAL") != nullptr;
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS
   *out = c;
   return WNDGPU_OK;

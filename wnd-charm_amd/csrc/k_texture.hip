@@ -810,11 +810,28 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     if (iub < 0) break;
     // findSmallSubdiagEntry: the largest res in (0, iu] whose subdiagonal
     // entry is negligible, else 0 -- every candidate tested at once
+    // (a lane's candidates are res = iub - t - 64k, k < 4 since n <= 256: all
+    // their entries are loaded at once, then tested in descending order)
     int cand = 0;
-    for (int res = iub - t; res > 0; res -= blockDim.x) {
-      double s = fabs(AT(res - 1, res - 1)) + fabs(AT(res, res));
-      if (s == 0.0) s = norm;
-      if (fabs(AT(res, res - 1)) < eps * s) { cand = res; break; }
+    {
+      double dd0[4] = {0, 0, 0, 0}, dd1[4] = {0, 0, 0, 0}, sd[4] = {0, 0, 0, 0};
+      if (n >= 2)  // otherwise there is no candidate
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const int res = min(max(iub - t - 64 * k, 1), n - 1);  // in range; unused when res <= 0
+          dd0[k] = AT(res - 1, res - 1);
+          dd1[k] = AT(res, res);
+          sd[k] = AT(res, res - 1);
+        }
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int res = iub - t - 64 * k;
+        if (res > 0 && cand == 0) {
+          double s = fabs(dd0[k]) + fabs(dd1[k]);
+          if (s == 0.0) s = norm;
+          if (fabs(sd[k]) < eps * s) cand = res;
+        }
+      }
     }
     const int il = block_reduce(cand, MaxOp(), ired);
     if (t == 0) {
@@ -924,16 +941,39 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     // initFrancisQRStep: the largest im in (il, iuq-2] passing the
     // small-subdiagonal test, else il -- every candidate tested at once
     const double sh0 = sc[4], sh1 = sc[5], sh2 = sc[6];
+    // (candidates m = iuq - 2 - t - 64k, k < 4: entries loaded at once, then
+    // tested in descending order)
     cand = il;
-    for (int m = iuq - 2 - t; m > il; m -= blockDim.x) {
-      const double Tmm = AT(m, m);
-      const double r = sh0 - Tmm, s = sh1 - Tmm;
-      const double v0 = (r * s - sh2) / AT(m + 1, m) + AT(m, m + 1);
-      const double v1 = AT(m + 1, m + 1) - Tmm - r - s;
-      const double v2 = AT(m + 2, m + 1);
-      const double lhs = AT(m, m - 1) * (fabs(v1) + fabs(v2));
-      const double rhs = v0 * (fabs(AT(m - 1, m - 1)) + fabs(Tmm) + fabs(AT(m + 1, m + 1)));
-      if (fabs(lhs) < eps * rhs) { cand = m; break; }
+    {
+      double tmm[4] = {0, 0, 0, 0}, t10[4] = {0, 0, 0, 0}, t01[4] = {0, 0, 0, 0}, t11[4] = {0, 0, 0, 0};
+      double t21[4] = {0, 0, 0, 0}, tm1[4] = {0, 0, 0, 0}, tpp[4] = {0, 0, 0, 0};
+      if (n >= 4)  // otherwise there is no candidate m > il >= 0 with m + 2 <= iuq
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int m = min(max(iuq - 2 - t - 64 * k, 1), n - 3);  // in range; unused when m <= il
+        tmm[k] = AT(m, m);
+        t10[k] = AT(m + 1, m);
+        t01[k] = AT(m, m + 1);
+        t11[k] = AT(m + 1, m + 1);
+        t21[k] = AT(m + 2, m + 1);
+        tm1[k] = AT(m, m - 1);
+        tpp[k] = AT(m - 1, m - 1);
+      }
+      bool found = false;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const int m = iuq - 2 - t - 64 * k;
+        if (m > il && !found) {
+          const double Tmm = tmm[k];
+          const double r = sh0 - Tmm, s = sh1 - Tmm;
+          const double v0 = (r * s - sh2) / t10[k] + t01[k];
+          const double v1 = t11[k] - Tmm - r - s;
+          const double v2 = t21[k];
+          const double lhs = tm1[k] * (fabs(v1) + fabs(v2));
+          const double rhs = v0 * (fabs(tpp[k]) + fabs(Tmm) + fabs(t11[k]));
+          if (fabs(lhs) < eps * rhs) { cand = m; found = true; }
+        }
+      }
     }
     const int im = block_reduce(cand, MaxOp(), ired);
     if (t == 0)
