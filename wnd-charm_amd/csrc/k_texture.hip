@@ -317,20 +317,20 @@ __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd, int nimg) {
     Redux4 r[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) r[q] = Redux4{0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    // list entries in order, four at a time so that 16 row loads are in flight
+    // list entries in order, two at a time so that 8 row loads are in flight
     const gptr<const double> Pg = to_global(P);
     int kk = 0;
-    for (; kk + 4 <= cnt; kk += 4) {
-      int k[4];
-      double a[4], d[4], b[4][4];
+    for (; kk + 2 <= cnt; kk += 2) {
+      int k[2];
+      double a[2], d[2], b[2][4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
+      for (int u = 0; u < 2; u++) {
         k[u] = lk[w][kk + u];
         a[u] = la[w][kk + u];
         d[u] = ld[w][kk + u];
       }
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < 2; u++)
 #pragma unroll
         for (int q = 0; q < 4; q++) {
           const int col = lane + 64 * q;
@@ -338,7 +338,7 @@ __global__ void __launch_bounds__(256) k_har_q(const HarDesc *hd, int nimg) {
           b[u][q] = col < ng ? v : 0.0;
         }
 #pragma unroll
-      for (int u = 0; u < 4; u++)
+      for (int u = 0; u < 2; u++)
 #pragma unroll
         for (int q = 0; q < 4; q++)
           if (b[u][q] != 0.0) r4_add(r[q], k[u], as2, as, (a[u] * b[u][q]) / d[u]);
