@@ -489,11 +489,13 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   const unsigned ot = (unsigned)((s + 1) * n + c);
   double top = A[ot];
   if (upd) { top = top - tu[s + 1] * f; A[ot] = top; }
-  gptr<double> p = A + (ot + n);  // row s+2: term 0
+  // rows s+2+j are addressed as A[o0 + j*n]: a uniform base and a 32-bit
+  // offset per load (one register per address)
+  const unsigned o0 = ot + n;  // row s+2: term 0
   double x0 = 0.0;
   if (skip) {
-    double v = p[0];
-    if (upd) { v = v - tu[s + 2] * f; p[0] = v; }
+    double v = A[o0];
+    if (upd) { v = v - tu[s + 2] * f; A[o0] = v; }
     x0 = v * nx[0];
   }
   double acc = 0.0;
@@ -501,12 +503,12 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   for (; g + HR <= bound; g += HR) {
     double v[HR];
 #pragma unroll
-    for (int q = 0; q < HR; q++) v[q] = p[(unsigned)((g + q) * n)];
+    for (int q = 0; q < HR; q++) v[q] = A[o0 + (g + q) * n];
     if (upd) {
 #pragma unroll
       for (int q = 0; q < HR; q++) {
         v[q] = v[q] - tu[s + 2 + g + q] * f;
-        p[(unsigned)((g + q) * n)] = v[q];
+        A[o0 + (g + q) * n] = v[q];
       }
     }
 #pragma unroll
@@ -521,12 +523,12 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   for (; g < bound; g += 4) {
     double v[4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] = p[(unsigned)((g + q) * n)];
+    for (int q = 0; q < 4; q++) v[q] = A[o0 + (g + q) * n];
     if (upd) {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         v[q] = v[q] - tu[s + 2 + g + q] * f;
-        p[(unsigned)((g + q) * n)] = v[q];
+        A[o0 + (g + q) * n] = v[q];
       }
     }
     const double a = v[0] * nx[g], p1 = v[1] * nx[g + 1], c2 = v[2] * nx[g + 2], p3 = v[3] * nx[g + 3];
@@ -535,8 +537,8 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
     else acc = (((acc + a) + b) + c2) + d;
   }
   for (int j = bound; j < ms; j++) {
-    double v = p[(unsigned)(j * n)];
-    if (upd) { v = v - tu[s + 2 + j] * f; p[(unsigned)(j * n)] = v; }
+    double v = A[o0 + j * n];
+    if (upd) { v = v - tu[s + 2 + j] * f; A[o0 + j * n] = v; }
     acc = acc + v * nx[j];
   }
   if (skip) acc = acc + x0;
