@@ -7,7 +7,8 @@
  * GPU stream, its device buffers and the host-computed basis tables.
  *
  * Reference interfaces replaced (paths relative to the reference tree):
- *   wndgpu_plan_standard        StdFeatureComputationPlans::getFeatureSet / getFeatureSetLong
+ *   wndgpu_plan_standard        StdFeatureComputationPlans::getFeatureSet / getFeatureSetLong /
+ *                               getFeatureSetColor / getFeatureSetLongColor
  *                               (src/Tasks.cpp:313-342, plans 354-508)
  *   wndgpu_plan_from_groups     FeatureComputationPlan(name) + add(group name) + finalize()
  *                               (src/Tasks.cpp:118-158, src/FeatureNames.cpp:309-396)
@@ -16,7 +17,8 @@
  *                               (src/Tasks.h:166-170, src/Tasks.cpp:160-168)
  *   wndgpu_compute              FeatureComputationPlanExecutor::run(src, feature_mat, row)
  *                               (src/Tasks.cpp:275-295), for a whole batch of images
- *   wndgpu_transform            ImageTransform::execute(in, out) (src/ImageTransforms.h:45-51,
+ *   wndgpu_transform / wndgpu_transform_image
+ *                               ImageTransform::execute(in, out) (src/ImageTransforms.h:45-51,
  *                               src/ImageTransforms.cpp:63-109) by transform name
  *   wndgpu_algorithm            FeatureAlgorithm::execute(in) (src/FeatureAlgorithms.h:280-290,
  *                               src/FeatureAlgorithms.cpp:54-489) by algorithm name
@@ -41,10 +43,21 @@ enum {
   WNDGPU_E_IMAGE = -4     /* at least one image of the batch was rejected (see status[]) */
 };
 
-enum { WNDGPU_U16 = 0, WNDGPU_F32 = 1, WNDGPU_F64 = 2, WNDGPU_U8 = 3 };
+/* sample types.  RGB8 / RGB16: three interleaved samples per pixel (R, G, B),
+ * `stride` counted in pixels; the image becomes the gray plane RGB2GRAY plus
+ * the colour plane RGB2HSV (ImageMatrix::LoadTIFF with 3 samples per pixel,
+ * src/cmatrix.cpp:134-193).  16-bit samples are scaled to bytes by the global
+ * min / max over the three channels. */
+enum { WNDGPU_U16 = 0, WNDGPU_F32 = 1, WNDGPU_F64 = 2, WNDGPU_U8 = 3, WNDGPU_RGB8 = 4, WNDGPU_RGB16 = 5 };
 
 /* per-image status inside a batch */
-enum { WNDGPU_IMG_OK = 0, WNDGPU_IMG_TOO_SMALL = 1, WNDGPU_IMG_TOO_LARGE = 2, WNDGPU_IMG_BAD_DTYPE = 3 };
+enum {
+  WNDGPU_IMG_OK = 0,
+  WNDGPU_IMG_TOO_SMALL = 1,
+  WNDGPU_IMG_TOO_LARGE = 2,
+  WNDGPU_IMG_BAD_DTYPE = 3,
+  WNDGPU_IMG_NOT_COLOR = 4 /* the plan reads the colour plane and the image is gray */
+};
 
 /* smallest / largest side accepted (the Symlet-5 and Tamura paths need >= 16) */
 #define WNDGPU_MIN_SIDE 16
@@ -57,7 +70,7 @@ typedef struct wndgpu_plan wndgpu_plan;
 typedef struct {
   const void *data;
   int height, width, stride;
-  int dtype; /* WNDGPU_U16 / F32 / F64 / U8 */
+  int dtype; /* WNDGPU_U16 / F32 / F64 / U8 / RGB8 / RGB16 */
 } wndgpu_image;
 
 int wndgpu_create(int device, wndgpu_ctx **ctx);
@@ -65,7 +78,8 @@ void wndgpu_destroy(wndgpu_ctx *ctx);
 const char *wndgpu_last_error(const wndgpu_ctx *ctx);
 const char *wndgpu_version(void);
 
-/* plans: set 0 = short (1059 features), 1 = long (2919 features) */
+/* plans: set 0 = short (1059 features), 1 = long (2919), 2 = short colour
+ * (2199), 3 = long colour (4059); the colour sets need RGB images */
 int wndgpu_plan_standard(int set, wndgpu_plan **plan);
 /* custom plan from feature-group names such as "Haralick Textures (Fourier ())" */
 int wndgpu_plan_from_groups(const char *const *group_names, int n_groups, wndgpu_plan **plan);
@@ -85,11 +99,18 @@ int wndgpu_compute(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image 
 int wndgpu_compute_device(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image *images, int n_images,
                           double *out_device, int *status);
 
-/* One transform ("Fourier", "Chebyshev", "Wavelet", "Edge") on one FP64
- * host plane.  out must hold the output plane: Fourier/Edge h*w, Chebyshev
+/* One transform ("Fourier", "Chebyshev", "Wavelet", "Edge", "Color
+ * Transform") on one FP64 host plane.  out must hold the output plane: Fourier/Edge h*w, Chebyshev
  * n*n with n = min(h,w), Wavelet 2*floor((h+9)/2) * 2*floor((w+9)/2). */
 int wndgpu_transform(wndgpu_ctx *ctx, const char *name, const double *in, int h, int w, double *out, int *out_h,
                      int *out_w);
+/* One transform of one host image of any sample type, by name: the gray
+ * transforms above, "Color Transform" (the identity copy of the gray plane),
+ * and "Color" / "Hue", which read the colour plane of an RGB image
+ * (ColorTransform / HueTransform::execute, src/ImageTransforms.cpp:119-136,
+ * src/cmatrix.cpp:1023-1066).  out as for wndgpu_transform. */
+int wndgpu_transform_image(wndgpu_ctx *ctx, const char *name, const wndgpu_image *image, double *out, int *out_h,
+                           int *out_w);
 /* One feature algorithm by reference name ("Zernike Coefficients", ...) on
  * one FP64 host plane; out gets n_features doubles (returned in *n_out). */
 int wndgpu_algorithm(wndgpu_ctx *ctx, const char *name, const double *in, int h, int w, double *out, int *n_out);

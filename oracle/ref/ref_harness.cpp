@@ -17,6 +17,9 @@
 #include <Eigen/Eigenvalues>
 #include "textures/haralick/CVIPtexture.h"
 #include "transforms/radon.h"
+#include "colors/FuzzyCalc.h"
+extern char rulesfile[];  // FuzzyCalc.cpp:46 (the rules text)
+extern int rules_loaded;  // FuzzyCalc.cpp:44
 #include "transforms/wavelet/Symlet5.h"
 #include "transforms/wavelet/DataGrid2D.h"
 
@@ -24,6 +27,32 @@ void TNx(double *x, double *out, int N, int height);
 void getChCoeff(double *Im, double *out, double *Tj, int N, int width, int height);
 
 extern "C" {
+
+/* The reference's fuzzy colour classifier (colors/FuzzyCalc.cpp:802-831).
+ * Its own first-call loader leaves color_functions[] unterminated:
+ * LoadColorsFunctions (FuzzyCalc.cpp:582-622) returns on reaching "rules:"
+ * before writing the terminating entry, so color_value (625-651) reads past
+ * the array for any hue its class does not cover (undefined behaviour: a
+ * standalone build segfaults).  The rules are loaded here by the
+ * reference's own functions with the rules section hidden while the colour
+ * functions load, so that LoadColorsFunctions runs to the end of the text
+ * and writes its terminator; then FindColor runs on the loaded tables. */
+long ref_find_color(int h, int s, int v) {
+  if (!rules_loaded) {
+    SetColors();
+    char *rules = strstr(rulesfile, "rules:");
+    const char saved = *rules;
+    *rules = '\0';
+    LoadColorsFunctions(rulesfile);
+    *rules = saved;
+    LoadRules(rules);
+    rules_loaded = 1;
+  }
+  return FindColor((short)h, (short)s, (short)v, nullptr);
+}
+void ref_find_color_many(const unsigned char *hsv, long n, long *out) {
+  for (long i = 0; i < n; i++) out[i] = ref_find_color(hsv[3 * i], hsv[3 * i + 1], hsv[3 * i + 2]);
+}
 
 int ref_texture(const unsigned char *grays, int rows, int cols, int angle, double *out14) {
   std::vector<u_int8_t *> rowp(rows);

@@ -18,8 +18,11 @@
 extern "C" {
 #endif
 
-/* transform ids (reference names: "Fourier", "Chebyshev", "Wavelet", "Edge") */
-enum { ORC_T_FOURIER = 0, ORC_T_CHEBYSHEV = 1, ORC_T_WAVELET = 2, ORC_T_EDGE = 3 };
+/* transform ids (reference names: "Fourier", "Chebyshev", "Wavelet", "Edge",
+ * "Color", "Hue"; "Color Transform" is the unregistered name of the standard
+ * colour plans, an identity copy) */
+enum { ORC_T_FOURIER = 0, ORC_T_CHEBYSHEV = 1, ORC_T_WAVELET = 2, ORC_T_EDGE = 3,
+       ORC_T_COLOR = 4, ORC_T_HUE = 5, ORC_T_COLOR_TRANSFORM = 6 };
 
 /* feature algorithm ids, with n_features in brackets */
 enum {
@@ -38,7 +41,8 @@ enum {
   ORC_A_INV_OTSU = 12,     /* Inverse-Otsu Object Features [34] */
   ORC_A_GABOR = 13,        /* Gabor Textures [7] */
   ORC_A_GINI = 14,         /* Gini Coefficient [1] */
-  ORC_A_COUNT = 15
+  ORC_A_COLOR_HIST = 15,   /* Color Histogram [20] (colour images) */
+  ORC_A_COUNT = 16
 };
 
 int orc_alg_n_features(int alg);
@@ -51,9 +55,22 @@ int orc_transform(int kind, const double *in, int h, int w, double *out, int *ou
 /* Feature algorithm on one image; out gets orc_alg_n_features(alg) doubles. */
 int orc_algorithm(int alg, const double *in, int h, int w, double *out);
 
-/* Full plan.  set: 0 = short (1059), 1 = long (2919).  out gets n features. */
+/* Full plan.  set: 0 = short (1059), 1 = long (2919), 2 = short colour
+ * (2199), 3 = long colour (4059).  out gets n features.  orc_features takes
+ * a gray FP64 plane (sets 0, 1); orc_features_rgb an interleaved RGB image
+ * with 8 or 16 bits per sample (any set). */
 int orc_plan_n_features(int set);
 int orc_features(int set, const double *img, int h, int w, double *out);
+int orc_features_rgb(int set, const void *rgb, int bits, int h, int w, double *out);
+
+/* Colour plane of an RGB image: gray = RGB2GRAY, hsv = RGB2HSV bytes. */
+int orc_rgb_planes(const void *rgb, int bits, int h, int w, double *gray, unsigned char *hsv);
+int orc_rgb_to_hsv(int r, int g, int b, unsigned char *hsv3);
+long orc_find_color(int h, int s, int v); /* fuzzy colour class 1..19 */
+void orc_find_color_many(const unsigned char *hsv, long n, long *out); /* n (h, s, v) byte triples */
+/* kind ORC_T_COLOR or ORC_T_HUE of an RGB image; out gets H*W */
+int orc_color_transform(int kind, const void *rgb, int bits, int h, int w, double *out);
+int orc_color_histogram(const void *rgb, int bits, int h, int w, double *out20);
 /* column -> feature-name string of a plan ("Zernike Coefficients (Fourier ()) [3]") */
 const char *orc_feature_name(int set, int column);
 

@@ -46,6 +46,15 @@ def oracle():
         lib.orc_fisher.argtypes = [P, P, I, I, I, D, P]
         lib.orc_wnd5.argtypes = [P, P, I, I, I, P, P, P, P]
         lib.orc_wnd5.restype = ctypes.c_long
+        lib.orc_plan_n_features.argtypes = [I]
+        lib.orc_features_rgb.argtypes = [I, P, I, I, I, P]
+        lib.orc_rgb_planes.argtypes = [P, I, I, I, P, P]
+        lib.orc_rgb_to_hsv.argtypes = [I, I, I, P]
+        lib.orc_find_color.restype = ctypes.c_long
+        lib.orc_find_color.argtypes = [I, I, I]
+        lib.orc_find_color_many.argtypes = [P, ctypes.c_long, P]
+        lib.orc_color_transform.argtypes = [I, P, I, I, I, P]
+        lib.orc_color_histogram.argtypes = [P, I, I, I, P]
         _oracle = lib
     return _oracle
 
@@ -59,27 +68,77 @@ def ref_harness():
     lib.ref_radon.argtypes = [P, I, I, I, I, P]
     lib.ref_wavelet.argtypes = [P, I, I, P, P, P]
     lib.ref_cheb1d.argtypes = [P, I, I, I, P]
+    lib.ref_find_color_many.argtypes = [P, ctypes.c_long, P]
     return lib
 
 
-ALG_NFEAT = [32, 32, 72, 28, 24, 6, 48, 12, 20, 5, 28, 34, 34, 7, 1]
-T_FOURIER, T_CHEBYSHEV, T_WAVELET, T_EDGE = 0, 1, 2, 3
+ALG_NFEAT = [32, 32, 72, 28, 24, 6, 48, 12, 20, 5, 28, 34, 34, 7, 1, 20]
+T_FOURIER, T_CHEBYSHEV, T_WAVELET, T_EDGE, T_COLOR, T_HUE, T_COLOR_TRANSFORM = 0, 1, 2, 3, 4, 5, 6
+# feature counts of the standard sets: short, long, short colour, long colour
+# (src/wndchrm_src/signatures.h:55-57 for the colour sets)
+SET_NFEAT = [1059, 2919, 2199, 4059]
+
+
+def _set_of(long_set):
+    return int(long_set) if isinstance(long_set, bool) else long_set
 
 
 def orc_features(img, long_set=True):
+    """oracle features of a gray plane; long_set: True/False or a set 0..1"""
     img = np.ascontiguousarray(img, dtype=np.float64)
     lib = oracle()
-    n = 2919 if long_set else 1059
-    out = np.zeros(n)
-    assert lib.orc_features(1 if long_set else 0, ptr(img), img.shape[0], img.shape[1], ptr(out)) == 0
+    s = _set_of(long_set)
+    out = np.zeros(SET_NFEAT[s])
+    assert lib.orc_features(s, ptr(img), img.shape[0], img.shape[1], ptr(out)) == 0
+    return out
+
+
+def orc_features_rgb(rgb, set_):
+    """oracle features of an (h, w, 3) uint8 / uint16 RGB image, sets 0..3"""
+    rgb = np.ascontiguousarray(rgb)
+    bits = 8 if rgb.dtype == np.uint8 else 16
+    out = np.zeros(SET_NFEAT[set_])
+    assert oracle().orc_features_rgb(set_, ptr(rgb), bits, rgb.shape[0], rgb.shape[1], ptr(out)) == 0
+    return out
+
+
+def orc_rgb_planes(rgb):
+    """(gray plane, (h, w, 3) HSV bytes) of an RGB image"""
+    rgb = np.ascontiguousarray(rgb)
+    h, w = rgb.shape[:2]
+    gray = np.zeros((h, w))
+    hsv = np.zeros((h, w, 3), np.uint8)
+    assert oracle().orc_rgb_planes(ptr(rgb), 8 if rgb.dtype == np.uint8 else 16, h, w, ptr(gray), ptr(hsv)) == 0
+    return gray, hsv
+
+
+def orc_color_transform(kind, rgb):
+    rgb = np.ascontiguousarray(rgb)
+    h, w = rgb.shape[:2]
+    out = np.zeros((h, w))
+    assert oracle().orc_color_transform(kind, ptr(rgb), 8 if rgb.dtype == np.uint8 else 16, h, w, ptr(out)) == 0
+    return out
+
+
+def orc_color_histogram(rgb):
+    rgb = np.ascontiguousarray(rgb)
+    out = np.zeros(20)
+    assert oracle().orc_color_histogram(ptr(rgb), 8 if rgb.dtype == np.uint8 else 16, rgb.shape[0], rgb.shape[1],
+                                        ptr(out)) == 0
+    return out
+
+
+def orc_find_color_many(hsv):
+    hsv = np.ascontiguousarray(hsv, dtype=np.uint8).reshape(-1, 3)
+    out = np.zeros(len(hsv), np.int64)
+    oracle().orc_find_color_many(ptr(hsv), len(hsv), ptr(out))
     return out
 
 
 def orc_names(long_set=True):
     lib = oracle()
-    s = 1 if long_set else 0
-    n = 2919 if long_set else 1059
-    return [lib.orc_feature_name(s, i).decode() for i in range(n)]
+    s = _set_of(long_set)
+    return [lib.orc_feature_name(s, i).decode() for i in range(SET_NFEAT[s])]
 
 
 def orc_transform(kind, img):
@@ -123,14 +182,14 @@ def load_npz(name):
 ALG_NAMES = ['Chebyshev-Fourier Coefficients', 'Chebyshev Coefficients', 'Zernike Coefficients', 'Haralick Textures',
              'Multiscale Histograms', 'Tamura Textures', 'Comb Moments', 'Radon Coefficients', 'Fractal Features',
              'Pixel Intensity Statistics', 'Edge Features', 'Otsu Object Features', 'Inverse-Otsu Object Features',
-             'Gabor Textures', 'Gini Coefficient']
+             'Gabor Textures', 'Gini Coefficient', 'Color Histogram']
 EXACT = {
     0: list(range(32)), 1: list(range(32)), 2: [], 3: [], 4: list(range(24)), 5: [0, 1, 2, 5],
     6: list(range(48)), 7: list(range(12)), 8: [], 9: [1, 3, 4],
     10: [0] + list(range(1, 14)) + list(range(17, 25)) + [26],
     11: list(range(0, 14)) + [15, 16, 17] + list(range(18, 29)) + [30, 31, 33],
     12: list(range(0, 14)) + [15, 16, 17] + list(range(18, 29)) + [30, 31, 33],
-    13: list(range(7)), 14: [],
+    13: list(range(7)), 14: [], 15: list(range(20)),
 }
 
 

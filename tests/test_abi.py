@@ -36,6 +36,11 @@ def test_standard_plans_match_reference_layout():
     assert longp.n_features == 2919 and shortp.n_features == 1059
     assert longp.feature_names() == orc_names(True)
     assert shortp.feature_names() == orc_names(False)
+    shortc = wndcharm.StdFeatureComputationPlans.getFeatureSetColor()
+    longc = wndcharm.StdFeatureComputationPlans.getFeatureSetLongColor()
+    assert shortc.n_features == 2199 and longc.n_features == 4059
+    assert shortc.feature_names() == orc_names(2)
+    assert longc.feature_names() == orc_names(3)
     # group offsets of the long set (Tasks.cpp:354-400): root group A/B/C, then node blocks
     for g, off in [('Edge Features ()', 0), ('Otsu Object Features ()', 28), ('Gabor Textures ()', 96),
                    ('Chebyshev-Fourier Coefficients (Fourier ())', 383), ('Haralick Textures (Wavelet (Edge ()))', 2639 + 136 + 48)]:

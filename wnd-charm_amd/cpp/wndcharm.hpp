@@ -8,7 +8,8 @@
 //   FeatureAlgorithm::execute       src/FeatureAlgorithms.h:41-51 (n_features, execute)
 //   FeatureComputationPlan          src/Tasks.h:272-313 (add, finalize, isFinalized, n_features,
 //                                   feature_vec_type, getFGoffset, getFeatureNameByIndex)
-//   StdFeatureComputationPlans      src/Tasks.h:356-372 (getFeatureSet, getFeatureSetLong)
+//   StdFeatureComputationPlans      src/Tasks.h:356-372 (getFeatureSet, getFeatureSetLong,
+//                                   getFeatureSetColor, getFeatureSetLongColor)
 //   FeatureComputationPlanExecutor  src/Tasks.h:314-333, run() src/Tasks.cpp:275-295
 //   ComputationTaskInstances lookup src/FeatureNames.cpp:251-296 (tasks by registered name)
 //
@@ -199,12 +200,16 @@ class FeatureComputationPlan {
 struct StdFeatureComputationPlans {
   enum feature_vec_types { fv_unknown = 0, fv_short = 1, fv_long = 2, fv_short_color = 3, fv_long_color = 4 };
   static const FeatureComputationPlan *getFeatureSet() { return get(0, fv_short, "Standard Feature Set"); }
-  static const FeatureComputationPlan *getFeatureSetLong() { return get(1, fv_long, "Standard Feature Set Long"); }
+  static const FeatureComputationPlan *getFeatureSetLong() { return get(1, fv_long, "Long Feature Set"); }
+  static const FeatureComputationPlan *getFeatureSetColor() { return get(2, fv_short_color, "Color Feature Set"); }
+  static const FeatureComputationPlan *getFeatureSetLongColor() {
+    return get(3, fv_long_color, "Long Color Feature Set");
+  }
 
  private:
   static const FeatureComputationPlan *get(int set, int type, const char *nm) {
     static std::mutex mu;
-    static std::unique_ptr<FeatureComputationPlan> plans[2];
+    static std::unique_ptr<FeatureComputationPlan> plans[4];
     std::lock_guard<std::mutex> lk(mu);
     if (!plans[set]) {
       wndgpu_plan *p = nullptr;

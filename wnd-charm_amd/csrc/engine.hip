@@ -37,19 +37,23 @@ void launch_wnd5(const double *train, const int *cls, const double *w, const dou
 using namespace wnd;
 
 // ============================================================ names / plans
-enum { T_FOURIER = 0, T_CHEBYSHEV = 1, T_WAVELET = 2, T_EDGE = 3 };
+// "Color Transform" is the name the reference's standard colour plans use
+// (Tasks.cpp:448-449); no transform registers it, so the reference runs an
+// EmptyTransform, a copy of the image (FeatureNames.cpp:276-285,
+// ImageTransforms.cpp:51-55).  T_COLOR_COPY is that identity.
+enum { T_FOURIER = 0, T_CHEBYSHEV = 1, T_WAVELET = 2, T_EDGE = 3, T_COLOR = 4, T_HUE = 5, T_COLOR_COPY = 6, T_COUNT = 7 };
 enum {
   A_CHEB_FOURIER = 0, A_CHEBYSHEV, A_ZERNIKE, A_HARALICK, A_MSH, A_TAMURA, A_COMB, A_RADON,
-  A_FRACTAL, A_PIXSTATS, A_EDGE, A_OTSU, A_INV_OTSU, A_GABOR, A_GINI, A_COUNT
+  A_FRACTAL, A_PIXSTATS, A_EDGE, A_OTSU, A_INV_OTSU, A_GABOR, A_GINI, A_COLOR_HIST, A_COUNT
 };
-static const int kNFeat[A_COUNT] = {32, 32, 72, 28, 24, 6, 48, 12, 20, 5, 28, 34, 34, 7, 1};
+static const int kNFeat[A_COUNT] = {32, 32, 72, 28, 24, 6, 48, 12, 20, 5, 28, 34, 34, 7, 1, 20};
 // registered names (src/FeatureAlgorithms.cpp:54-489, src/ImageTransforms.cpp:57-103)
 static const char *kAlgNames[A_COUNT] = {
     "Chebyshev-Fourier Coefficients", "Chebyshev Coefficients", "Zernike Coefficients", "Haralick Textures",
     "Multiscale Histograms", "Tamura Textures", "Comb Moments", "Radon Coefficients", "Fractal Features",
     "Pixel Intensity Statistics", "Edge Features", "Otsu Object Features", "Inverse-Otsu Object Features",
-    "Gabor Textures", "Gini Coefficient"};
-static const char *kTrNames[4] = {"Fourier", "Chebyshev", "Wavelet", "Edge"};
+    "Gabor Textures", "Gini Coefficient", "Color Histogram"};
+static const char *kTrNames[T_COUNT] = {"Fourier", "Chebyshev", "Wavelet", "Edge", "Color", "Hue", "Color Transform"};
 
 static std::string trim(const std::string &s) {
   size_t a = s.find_first_not_of(" \t\f\v\n\r"), b = s.find_last_not_of(" \t\f\v\n\r");
@@ -60,7 +64,7 @@ static int alg_index(const std::string &n) {
   return -1;
 }
 static int tr_index(const std::string &n) {
-  for (int i = 0; i < 4; i++) if (n == kTrNames[i]) return i;
+  for (int i = 0; i < T_COUNT; i++) if (n == kTrNames[i]) return i;
   return -1;
 }
 static std::string node_text(const std::vector<int> &chain) {
@@ -79,6 +83,7 @@ struct Group {
 struct wndgpu_plan {
   std::vector<Group> groups;
   int n_features = 0;
+  bool color = false;  // some group reads the colour plane (RGB images only)
   std::vector<std::string> names;
   std::vector<std::vector<int>> nodes;  // DAG nodes, parents before children
 };
@@ -103,6 +108,11 @@ static bool parse_group(const std::string &name, int *alg, std::vector<int> *cha
     rest = trim(inner.substr(p));
   }
   chain->assign(outer_first.rbegin(), outer_first.rend());
+  // the colour plane belongs to the loaded image: Color and Hue apply to it
+  // directly, and the Color Histogram reads it
+  for (size_t k = 0; k < chain->size(); k++)
+    if (((*chain)[k] == T_COLOR || (*chain)[k] == T_HUE) && k != 0) return false;
+  if (*alg == A_COLOR_HIST && !chain->empty()) return false;
   return true;
 }
 
@@ -111,6 +121,7 @@ static void plan_add(wndgpu_plan *p, int alg, const std::vector<int> &chain) {
   for (int i = 0; i < kNFeat[alg]; i++) p->names.push_back(g.name + " [" + std::to_string(i) + "]");
   p->n_features += kNFeat[alg];
   p->groups.push_back(g);
+  if (alg == A_COLOR_HIST || (!chain.empty() && (chain[0] == T_COLOR || chain[0] == T_HUE))) p->color = true;
   std::vector<int> key;
   auto has = [&](const std::vector<int> &k) { return std::find(p->nodes.begin(), p->nodes.end(), k) != p->nodes.end(); };
   if (!has(key)) p->nodes.push_back(key);
@@ -125,17 +136,26 @@ static void plan_finalize(wndgpu_plan *p) {
                    [](const std::vector<int> &a, const std::vector<int> &b) { return a.size() < b.size(); });
 }
 
-// StdFeatureComputationPlans (src/Tasks.cpp:354-508)
+// StdFeatureComputationPlans (src/Tasks.cpp:313-508): set 0 = short, 1 =
+// long, 2 = short colour, 3 = long colour
 static wndgpu_plan *make_standard(int set) {
   wndgpu_plan *p = new wndgpu_plan;
-  const int F = T_FOURIER, C = T_CHEBYSHEV, W = T_WAVELET, E = T_EDGE;
-  if (set == 1) {
-    auto A = [&](std::vector<int> ch) { for (int a : {A_EDGE, A_OTSU, A_INV_OTSU, A_GABOR}) plan_add(p, a, ch); };
-    auto B = [&](std::vector<int> ch) { for (int a : {A_CHEB_FOURIER, A_CHEBYSHEV, A_ZERNIKE}) plan_add(p, a, ch); };
-    auto Cg = [&](std::vector<int> ch) {
-      for (int a : {A_COMB, A_HARALICK, A_MSH, A_TAMURA, A_RADON, A_FRACTAL, A_PIXSTATS, A_GINI}) plan_add(p, a, ch);
-    };
+  const int F = T_FOURIER, C = T_CHEBYSHEV, W = T_WAVELET, E = T_EDGE, H = T_HUE, CT = T_COLOR_COPY;
+  auto A = [&](std::vector<int> ch) { for (int a : {A_EDGE, A_OTSU, A_INV_OTSU, A_GABOR}) plan_add(p, a, ch); };
+  auto B = [&](std::vector<int> ch) { for (int a : {A_CHEB_FOURIER, A_CHEBYSHEV, A_ZERNIKE}) plan_add(p, a, ch); };
+  auto Cg = [&](std::vector<int> ch) {
+    for (int a : {A_COMB, A_HARALICK, A_MSH, A_TAMURA, A_RADON, A_FRACTAL, A_PIXSTATS, A_GINI}) plan_add(p, a, ch);
+  };
+  auto colour = [&]() {  // addColorFeatures, Tasks.cpp:445-459
+    plan_add(p, A_COLOR_HIST, {});
+    B({CT}); Cg({CT});
+    B({H}); Cg({H});
+    B({H, F}); Cg({H, F});
+    B({H, C}); Cg({H, C});
+  };
+  if (set == 1 || set == 3) {
     A({}); B({}); Cg({});
+    if (set == 3) colour();
     B({F}); Cg({F});
     B({W}); Cg({W});
     B({C}); Cg({C});
@@ -161,6 +181,7 @@ static wndgpu_plan *make_standard(int set) {
     plan_add(p, A_TAMURA, {}); plan_add(p, A_TAMURA, {C}); plan_add(p, A_TAMURA, {F, C});
     plan_add(p, A_TAMURA, {F}); plan_add(p, A_TAMURA, {W}); plan_add(p, A_TAMURA, {F, W});
     plan_add(p, A_ZERNIKE, {}); plan_add(p, A_ZERNIKE, {F});
+    if (set == 2) colour();
   }
   plan_finalize(p);
   return p;
@@ -183,6 +204,8 @@ struct NodeRun {
   Img *imgs = nullptr;    // device descriptor array
   Stats *stats = nullptr; // device stats array
   std::vector<double *> planes;
+  std::vector<unsigned char *> hsv;            // colour planes (root of RGB images)
+  unsigned char *const *hsvd = nullptr;        // device copy of hsv
   double ms = 0;
 };
 
@@ -211,6 +234,7 @@ struct wndgpu_ctx {
   std::map<std::pair<int, double>, DevTable> dspos;
   std::map<int, DevTable> comb;
   DevTable gabor, zern, rtrig;
+  DevTable color_lut;  // fuzzy colour class of every (h, s, v) byte triple, built on first use
   std::vector<std::pair<std::string, double>> times;
   // per-launch timing (WNDGPU_TIMING): an event pair around every launch,
   // read after the compute call has synchronised
@@ -406,10 +430,25 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     R.stats = A.dalloc_n<Stats>(n);
     R.planes.resize(n);
     for (int i = 0; i < n; i++) R.planes[i] = im[i].p;
-    LoadDesc *ld = A.stage(B.load);
+    // colour planes (3 bytes per pixel) when the plan reads them; wide RGB
+    // samples need the per-image channel range first
+    std::vector<LoadDesc> ldv = B.load;
+    double *range = nullptr;
+    bool wide = false;
+    for (int i = 0; i < n; i++) wide |= ldv[i].dtype == 5;
+    if (wide) range = A.dalloc_n<double>(2 * (size_t)n);
+    R.hsv.assign(n, nullptr);
+    for (int i = 0; i < n; i++) {
+      if (plan->color) R.hsv[i] = A.dalloc_n<unsigned char>(3 * (size_t)R.h[i] * R.w[i]);
+      ldv[i].hsv = R.hsv[i];
+      ldv[i].range = range ? range + 2 * i : nullptr;
+    }
+    if (plan->color) R.hsvd = A.stage(R.hsv);
+    LoadDesc *ld = A.stage(ldv);
     Img *ri = R.imgs;
     Stats *rs = R.stats;
     push("root", [=]() {
+      if (wide) launch_rgb_range(ld, range, n, s);
       launch_load(ld, ri, n, maxp, s);
       launch_stats(ri, rs, n, s);
     });
@@ -422,6 +461,14 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     NodeRun &N = nodes[key];
     const int kind = key.back();
     N.chain = key;
+    if (kind == T_COLOR_COPY) {  // identity: the node shares its parent's planes and stats
+      N.h = P.h;
+      N.w = P.w;
+      N.imgs = P.imgs;
+      N.stats = P.stats;
+      N.planes = P.planes;
+      continue;
+    }
     N.h.resize(n);
     N.w.resize(n);
     std::vector<Img> im(n);
@@ -437,7 +484,12 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     Img *pin = P.imgs, *pout = N.imgs;
     Stats *nst = N.stats;
     const std::string nm = node_text(key);
-    if (kind == T_EDGE) {
+    if (kind == T_COLOR || kind == T_HUE) {  // innermost only: P is the root
+      const unsigned char *const *hv = P.hsvd;
+      const unsigned char *lut = (const unsigned char *)c->color_lut.p;
+      const int ck = kind == T_COLOR ? 0 : 1;
+      push(nm, [=]() { launch_color_node(hv, lut, pout, ck, n, maxp, s); launch_stats(pout, nst, n, s); });
+    } else if (kind == T_EDGE) {
       push(nm, [=]() { launch_edge(pin, pout, n, maxp, s); launch_stats(pout, nst, n, s); });
     } else if (kind == T_WAVELET) {
       std::vector<double *> g(n);
@@ -558,6 +610,13 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     switch (alg) {
       case A_MSH: push(gn, [=]() { gather(); launch_msh(ni, nst, o, ne, s); }); break;
       case A_FRACTAL: push(gn, [=]() { launch_fractal(ni, o, ne, s); }); break;
+      case A_COLOR_HIST: {  // root entries only
+        std::vector<unsigned char *> hv(ne);
+        for (int k = 0; k < ne; k++) hv[k] = E[k].N->hsv[E[k].i];
+        unsigned char **hvd = A.stage(hv);
+        const unsigned char *lut = (const unsigned char *)c->color_lut.p;
+        push(gn, [=]() { launch_color_hist(hvd, ni, lut, o, ne, s); });
+      } break;
       case A_COMB: {
         std::vector<double> iis;
         for (const Entry &e : E) {
@@ -848,9 +907,21 @@ static int ensure_tables(wndgpu_ctx *c) {
   return 0;
 }
 
+// the colour class of every (h, s, v) byte triple (16 MiB), computed on the
+// device once per context
+static int ensure_color_lut(wndgpu_ctx *c) {
+  if (c->color_lut.p) return 0;
+  HIPCHK(c, hipMalloc(&c->color_lut.p, (size_t)1 << 24));
+  c->color_lut.bytes = (size_t)1 << 24;
+  launch_color_lut((unsigned char *)c->color_lut.p, c->stream);
+  HIPCHK(c, hipGetLastError());
+  return 0;
+}
+
 static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timing) {
   int r;
   if ((r = ensure_tables(c))) return r;
+  if (plan->color && (r = ensure_color_lut(c))) return r;
   std::vector<LaunchItem> launches;
   size_t first_feature = 0;
   // pass 1: sizes
@@ -961,7 +1032,7 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   for (auto &kv : c->cf) { fr(kv.second.kk); fr(kv.second.tn); fr(kv.second.fc); fr(kv.second.fs); }
   for (auto &kv : c->dspos) fr(kv.second);
   for (auto &kv : c->comb) fr(kv.second);
-  fr(c->gabor); fr(c->zern); fr(c->rtrig);
+  fr(c->gabor); fr(c->zern); fr(c->rtrig); fr(c->color_lut);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->dev) hipFree(c->dev);
   if (c->hdesc) hipFree(c->hdesc);
@@ -976,7 +1047,7 @@ void wndgpu_destroy(wndgpu_ctx *c) {
 const char *wndgpu_last_error(const wndgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 int wndgpu_plan_standard(int set, wndgpu_plan **plan) {
-  if (!plan || (set != 0 && set != 1)) return WNDGPU_E_ARG;
+  if (!plan || set < 0 || set > 3) return WNDGPU_E_ARG;
   *plan = make_standard(set);
   return WNDGPU_OK;
 }
@@ -1028,7 +1099,8 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   for (int i = 0; i < n; i++) {
     const wndgpu_image &im = images[i];
     int st = WNDGPU_IMG_OK;
-    if (im.dtype < 0 || im.dtype > 3 || !im.data || im.stride < im.width) st = WNDGPU_IMG_BAD_DTYPE;
+    if (im.dtype < 0 || im.dtype > 5 || !im.data || im.stride < im.width) st = WNDGPU_IMG_BAD_DTYPE;
+    else if (plan->color && im.dtype < 4) st = WNDGPU_IMG_NOT_COLOR;
     else if (im.height < WNDGPU_MIN_SIDE || im.width < WNDGPU_MIN_SIDE) st = WNDGPU_IMG_TOO_SMALL;
     else if (im.height > WNDGPU_MAX_SIDE || im.width > WNDGPU_MAX_SIDE) st = WNDGPU_IMG_TOO_LARGE;
     if (status) status[i] = st;
@@ -1038,7 +1110,7 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   if (idx.empty()) return fail(c, WNDGPU_E_IMAGE, "no valid image in the batch");
   B.n = (int)idx.size();
   // host inputs are staged into one device buffer
-  static const size_t esz[4] = {2, 4, 8, 1};
+  static const size_t esz[6] = {2, 4, 8, 1, 3, 6};  // bytes per pixel
   std::vector<void *> tmp;
   size_t in_bytes = 0;
   std::vector<size_t> in_off(B.n);
@@ -1130,89 +1202,162 @@ int wndgpu_algorithm(wndgpu_ctx *c, const char *name, const double *in, int h, i
   return r;
 }
 
-int wndgpu_transform(wndgpu_ctx *c, const char *name, const double *in, int h, int w, double *out, int *oh,
-                     int *ow) {
-  if (!c || !name || !in || !out) return fail(c, WNDGPU_E_ARG, "null argument");
-  const int t = tr_index(trim(name));
-  if (t < 0) return fail(c, WNDGPU_E_ARG, "unknown transform '%s'", name);
-  if (h < WNDGPU_MIN_SIDE || w < WNDGPU_MIN_SIDE || h > WNDGPU_MAX_SIDE || w > WNDGPU_MAX_SIDE)
-    return fail(c, WNDGPU_E_ARG, "image size %dx%d out of range", w, h);
-  HIPCHK(c, hipSetDevice(c->device));
+struct DBuf {
+  void *p = nullptr;
+  ~DBuf() { if (p) hipFree(p); }
+};
+struct FreeList {
+  std::vector<void *> v;
+  ~FreeList() { for (void *p : v) hipFree(p); }
+};
+
+// One gray transform of a device plane (h x w) into dout (out_size of t):
+// the single-image form of the batched node launches.
+static int transform_dev(wndgpu_ctx *c, int t, double *din, int h, int w, double *dout) {
   int r;
   if ((r = ensure_tables(c))) return r;
   int th, tw;
   out_size(t, h, w, &th, &tw);
-  double *din = nullptr, *dout = nullptr;
-  HIPCHK(c, hipMalloc(&din, (size_t)h * w * 8));
-  HIPCHK(c, hipMalloc(&dout, (size_t)th * tw * 8));
-  HIPCHK(c, hipMemcpy(din, in, (size_t)h * w * 8, hipMemcpyHostToDevice));
-  // single-node plan: the transform is run through the same batched path
   Img ii{din, h, w}, oo{dout, th, tw};
+  FreeList frees;  // every buffer below, freed on every return
+  auto dm = [&](void *p, size_t bytes) {
+    hipError_t e = hipMalloc((void **)p, bytes);
+    if (e == hipSuccess) frees.v.push_back(*(void **)p);
+    return e;
+  };
   Img *dii = nullptr, *doo = nullptr;
-  HIPCHK(c, hipMalloc(&dii, sizeof(Img)));
-  HIPCHK(c, hipMalloc(&doo, sizeof(Img)));
+  HIPCHK(c, dm(&dii, sizeof(Img)));
+  HIPCHK(c, dm(&doo, sizeof(Img)));
   HIPCHK(c, hipMemcpy(dii, &ii, sizeof(Img), hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(doo, &oo, sizeof(Img), hipMemcpyHostToDevice));
   hipStream_t s = c->stream;
-  std::vector<void *> frees;
-  if (t == T_EDGE) {
+  if (t == T_COLOR_COPY) {
+    HIPCHK(c, hipMemcpyAsync(dout, din, (size_t)h * w * 8, hipMemcpyDeviceToDevice, s));
+  } else if (t == T_EDGE) {
     launch_edge(dii, doo, 1, (long)h * w, s);
   } else if (t == T_WAVELET) {
     double *g = nullptr;
     const int gs = ((h + 9) / 2) * w;
-    HIPCHK(c, hipMalloc(&g, 2 * (size_t)gs * 8));
+    HIPCHK(c, dm(&g, 2 * (size_t)gs * 8));
     double **gd = nullptr;
     int *gsd = nullptr;
-    HIPCHK(c, hipMalloc(&gd, sizeof(double *)));
-    HIPCHK(c, hipMalloc(&gsd, sizeof(int)));
+    HIPCHK(c, dm(&gd, sizeof(double *)));
+    HIPCHK(c, dm(&gsd, sizeof(int)));
     HIPCHK(c, hipMemcpy(gd, &g, sizeof(double *), hipMemcpyHostToDevice));
     HIPCHK(c, hipMemcpy(gsd, &gs, sizeof(int), hipMemcpyHostToDevice));
     launch_wavelet(dii, gd, gsd, doo, 1, h, w, s);
-    frees = {g, gd, gsd};
   } else if (t == T_CHEBYSHEV) {
     const int N = std::min(h, w);
     const double *bx, *by;
     if ((r = get_cheb(c, w, N, &bx))) return r;
     if ((r = get_cheb(c, h, N, &by))) return r;
     double *c1 = nullptr;
-    HIPCHK(c, hipMalloc(&c1, (size_t)h * N * 8));
+    HIPCHK(c, dm(&c1, (size_t)h * N * 8));
     GemmDesc g[2] = {GemmDesc{din, bx, c1, h, w, N, (long)w, 1}, GemmDesc{c1, by, dout, N, h, N, 1, (long)N}};
     GemmDesc *gd = nullptr;
-    HIPCHK(c, hipMalloc(&gd, sizeof g));
+    HIPCHK(c, dm(&gd, sizeof g));
     HIPCHK(c, hipMemcpy(gd, g, sizeof g, hipMemcpyHostToDevice));
     launch_cheb_gemm(gd, 1, h, N, s);
     launch_cheb_gemm(gd + 1, 1, N, N, s);
-    frees = {c1, gd};
   } else {
     FftPlan pw, ph;
     if ((r = get_fft(c, w, &pw))) return r;
     if ((r = get_fft(c, h, &ph))) return r;
     double2 *f1 = nullptr;
-    HIPCHK(c, hipMalloc(&f1, (size_t)h * w * sizeof(double2)));
+    HIPCHK(c, dm(&f1, (size_t)h * w * sizeof(double2)));
     void *pd = nullptr;
-    HIPCHK(c, hipMalloc(&pd, 2 * sizeof(FftPlan) + sizeof(double2 *)));
+    HIPCHK(c, dm(&pd, 2 * sizeof(FftPlan) + sizeof(double2 *)));
     FftPlan pls[2] = {pw, ph};
     HIPCHK(c, hipMemcpy(pd, pls, sizeof pls, hipMemcpyHostToDevice));
     double2 **f1d = (double2 **)((char *)pd + 2 * sizeof(FftPlan));
     HIPCHK(c, hipMemcpy(f1d, &f1, sizeof(double2 *), hipMemcpyHostToDevice));
     launch_fft(dii, f1d, (FftPlan *)pd, (FftPlan *)pd + 1, doo, 1, h, w, s);
-    frees = {f1, pd};
   }
   HIPCHK(c, hipStreamSynchronize(s));
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpy(out, dout, (size_t)th * tw * 8, hipMemcpyDeviceToHost));
-  for (void *p : frees) hipFree(p);
-  hipFree(din); hipFree(dout); hipFree(dii); hipFree(doo);
+  return WNDGPU_OK;
+}
+
+int wndgpu_transform(wndgpu_ctx *c, const char *name, const double *in, int h, int w, double *out, int *oh,
+                     int *ow) {
+  if (!c || !name || !in || !out) return fail(c, WNDGPU_E_ARG, "null argument");
+  const int t = tr_index(trim(name));
+  if (t < 0) return fail(c, WNDGPU_E_ARG, "unknown transform '%s'", name);
+  if (t == T_COLOR || t == T_HUE)
+    return fail(c, WNDGPU_E_ARG, "transform '%s' reads the colour plane of an RGB image (wndgpu_transform_image)", name);
+  if (h < WNDGPU_MIN_SIDE || w < WNDGPU_MIN_SIDE || h > WNDGPU_MAX_SIDE || w > WNDGPU_MAX_SIDE)
+    return fail(c, WNDGPU_E_ARG, "image size %dx%d out of range", w, h);
+  HIPCHK(c, hipSetDevice(c->device));
+  int th, tw;
+  out_size(t, h, w, &th, &tw);
+  DBuf din, dout;
+  HIPCHK(c, hipMalloc(&din.p, (size_t)h * w * 8));
+  HIPCHK(c, hipMalloc(&dout.p, (size_t)th * tw * 8));
+  HIPCHK(c, hipMemcpy(din.p, in, (size_t)h * w * 8, hipMemcpyHostToDevice));
+  int r = transform_dev(c, t, (double *)din.p, h, w, (double *)dout.p);
+  if (r) return r;
+  HIPCHK(c, hipMemcpy(out, dout.p, (size_t)th * tw * 8, hipMemcpyDeviceToHost));
+  if (oh) *oh = th;
+  if (ow) *ow = tw;
+  return WNDGPU_OK;
+}
+
+int wndgpu_transform_image(wndgpu_ctx *c, const char *name, const wndgpu_image *im, double *out, int *oh, int *ow) {
+  if (!c || !name || !im || !im->data || !out) return fail(c, WNDGPU_E_ARG, "null argument");
+  const int t = tr_index(trim(name));
+  if (t < 0) return fail(c, WNDGPU_E_ARG, "unknown transform '%s'", name);
+  const int h = im->height, w = im->width;
+  if (im->dtype < 0 || im->dtype > 5 || im->stride < w) return fail(c, WNDGPU_E_ARG, "bad sample type or stride");
+  if (h < WNDGPU_MIN_SIDE || w < WNDGPU_MIN_SIDE || h > WNDGPU_MAX_SIDE || w > WNDGPU_MAX_SIDE)
+    return fail(c, WNDGPU_E_ARG, "image size %dx%d out of range", w, h);
+  const bool colour = t == T_COLOR || t == T_HUE;
+  if (colour && im->dtype < 4)
+    return fail(c, WNDGPU_E_ARG, "transform '%s' reads the colour plane: the image must be RGB", name);
+  HIPCHK(c, hipSetDevice(c->device));
+  int r;
+  if (t == T_COLOR && (r = ensure_color_lut(c))) return r;
+  static const size_t esz[6] = {2, 4, 8, 1, 3, 6};
+  const size_t in_bytes = (size_t)h * im->stride * esz[im->dtype];
+  int th, tw;
+  out_size(t, h, w, &th, &tw);
+  DBuf src, plane, hsv, range, desc, dout;
+  HIPCHK(c, hipMalloc(&src.p, in_bytes));
+  HIPCHK(c, hipMalloc(&plane.p, (size_t)h * w * 8));
+  HIPCHK(c, hipMalloc(&dout.p, (size_t)th * tw * 8));
+  HIPCHK(c, hipMalloc(&range.p, 2 * sizeof(double)));
+  if (colour) HIPCHK(c, hipMalloc(&hsv.p, 3 * (size_t)h * w));
+  HIPCHK(c, hipMemcpy(src.p, im->data, in_bytes, hipMemcpyHostToDevice));
+  // descriptors: LoadDesc, the plane's Img, the colour plane pointer, the output Img
+  struct Desc {
+    LoadDesc L;
+    Img in, out;
+    unsigned char *hsv;
+  } hd;
+  hd.L = LoadDesc{src.p, h, w, im->stride, im->dtype, (unsigned char *)hsv.p, (const double *)range.p};
+  hd.in = Img{(double *)plane.p, h, w};
+  hd.out = Img{(double *)dout.p, th, tw};
+  hd.hsv = (unsigned char *)hsv.p;
+  HIPCHK(c, hipMalloc(&desc.p, sizeof hd));
+  HIPCHK(c, hipMemcpy(desc.p, &hd, sizeof hd, hipMemcpyHostToDevice));
+  Desc *dd = (Desc *)desc.p;
+  hipStream_t s = c->stream;
+  if (im->dtype == 5) launch_rgb_range(&dd->L, (double *)range.p, 1, s);
+  launch_load(&dd->L, &dd->in, 1, (long)h * w, s);
+  if (colour) {
+    launch_color_node(&dd->hsv, (const unsigned char *)c->color_lut.p, &dd->out, t == T_COLOR ? 0 : 1, 1,
+                      (long)h * w, s);
+    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipGetLastError());
+  } else if ((r = transform_dev(c, t, (double *)plane.p, h, w, (double *)dout.p))) {
+    return r;
+  }
+  HIPCHK(c, hipMemcpy(out, dout.p, (size_t)th * tw * 8, hipMemcpyDeviceToHost));
   if (oh) *oh = th;
   if (ow) *ow = tw;
   return WNDGPU_OK;
 }
 
 // ---- classifier
-struct DBuf {
-  void *p = nullptr;
-  ~DBuf() { if (p) hipFree(p); }
-};
 #define DALLOC(ctx, buf, bytes) HIPCHK(ctx, hipMalloc(&(buf).p, (bytes) ? (bytes) : 8))
 
 int wndgpu_normalize_train(wndgpu_ctx *c, double *data, int n, int f, double *mins, double *maxs) {

@@ -4,24 +4,53 @@
 // features.  All FP64; operations whose order fixes integer-valued outputs
 // (wavelet taps, Chebyshev contractions) follow the reference order exactly.
 #include "common.hpp"
+#include "color.hpp"
 #include "kernels.hpp"
 #include "blocksort.hpp"
 
 namespace wnd {
 
 // ---------------------------------------------------------------------------
-// input conversion: uint16 / f32 / f64 source with row stride -> FP64 plane
+// input conversion: uint16 / f32 / f64 / uint8 / RGB source with row stride ->
+// FP64 plane (and the colour plane of an RGB image when the plan reads it)
 __global__ void k_load(const LoadDesc *d, Img *dst) {
   const LoadDesc L = d[blockIdx.y];
   const Img o = dst[blockIdx.y];
   const long n = (long)L.h * L.w;
+  double lo = 0, scale = 0;
+  if (L.dtype == 5) {  // a constant image maps to 0 (the reference divides by zero there)
+    lo = L.range[0];
+    scale = L.range[1] > lo ? (255.0 / (L.range[1] - lo)) : 0.0;
+  }
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
     const int y = i / L.w, x = i - (long)y * L.w;
+    const long e = (long)y * L.stride + x;
     double v;
-    if (L.dtype == 0) v = (double)((const uint16_t *)L.src)[(long)y * L.stride + x];
-    else if (L.dtype == 1) v = (double)((const float *)L.src)[(long)y * L.stride + x];
-    else if (L.dtype == 2) v = ((const double *)L.src)[(long)y * L.stride + x];
-    else v = (double)((const uint8_t *)L.src)[(long)y * L.stride + x];
+    if (L.dtype == 0) v = (double)((const uint16_t *)L.src)[e];
+    else if (L.dtype == 1) v = (double)((const float *)L.src)[e];
+    else if (L.dtype == 2) v = ((const double *)L.src)[e];
+    else if (L.dtype == 3) v = (double)((const uint8_t *)L.src)[e];
+    else {  // RGB (LoadTIFF with 3 samples per pixel, cmatrix.cpp:134-193): gray plane + colour plane
+      unsigned char r, g, b;
+      if (L.dtype == 4) {
+        const uint8_t *p = (const uint8_t *)L.src + 3 * e;
+        r = p[0];
+        g = p[1];
+        b = p[2];
+      } else {
+        const uint16_t *p = (const uint16_t *)L.src + 3 * e;
+        r = (unsigned char)(((double)p[0] - lo) * scale);
+        g = (unsigned char)(((double)p[1] - lo) * scale);
+        b = (unsigned char)(((double)p[2] - lo) * scale);
+      }
+      v = rgb_gray(r, g, b);
+      if (L.hsv) {
+        const HSVb c = rgb_to_hsv(r, g, b);
+        L.hsv[3 * i] = c.h;
+        L.hsv[3 * i + 1] = c.s;
+        L.hsv[3 * i + 2] = c.v;
+      }
+    }
     o.p[i] = v;
   }
 }

@@ -9,7 +9,9 @@ namespace wnd {
 struct LoadDesc {
   const void *src;
   int h, w, stride;
-  int dtype;  // 0 uint16, 1 float32, 2 float64, 3 uint8
+  int dtype;             // 0 uint16, 1 float32, 2 float64, 3 uint8, 4 RGB 8-bit, 5 RGB 16-bit (interleaved)
+  unsigned char *hsv;    // colour plane (H, S, V bytes per pixel) of an RGB image, else null
+  const double *range;   // RGB 16-bit: min, max over the three channels
 };
 
 // C[r][c] = sum_a (A[r*sar + a*sak] * B[a*N + c]) / 2   (M x K) (K x N)
@@ -61,6 +63,12 @@ struct DsDesc {  // area-weighted Downsample (cmatrix.cpp:579-720)
 
 void launch_load(const LoadDesc *d, Img *dst, int nimg, long maxn, hipStream_t s);
 void launch_stats(const Img *imgs, Stats *st, int nimg, hipStream_t s);
+void launch_color_lut(unsigned char *lut, hipStream_t s);
+void launch_color_node(const unsigned char *const *hsv, const unsigned char *lut, const Img *out, int kind, int nimg,
+                       long maxp, hipStream_t s);
+void launch_color_hist(const unsigned char *const *hsv, const Img *imgs, const unsigned char *lut, OutRef o, int nimg,
+                       hipStream_t s);
+void launch_rgb_range(const LoadDesc *d, double *range, int nimg, hipStream_t s);
 void launch_gather_stats(const Stats *const *src, Stats *dst, int n, hipStream_t s);
 void launch_edge(const Img *in, const Img *out, int nimg, long maxn, hipStream_t s);
 void launch_wavelet(const Img *in, double *const *g, const int *gstride, const Img *out, int nimg, int maxh,

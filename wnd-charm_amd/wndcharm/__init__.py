@@ -60,6 +60,8 @@ def _load():
         'wndgpu_compute': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P]),
         'wndgpu_compute_device': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P]),
         'wndgpu_transform': (_I, [_P, ctypes.c_char_p, _P, _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+        'wndgpu_transform_image': (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_Image), _P, ctypes.POINTER(_I),
+                                        ctypes.POINTER(_I)]),
         'wndgpu_algorithm': (_I, [_P, ctypes.c_char_p, _P, _I, _I, _P, ctypes.POINTER(_I)]),
         'wndgpu_algorithm_n_features': (_I, [ctypes.c_char_p]),
         'wndgpu_normalize_train': (_I, [_P, _P, _I, _I, _P, _P]),
@@ -256,8 +258,24 @@ class StdFeatureComputationPlans(object):
     def getFeatureSetLong(cls):
         return cls._get(1, 2, 'Long Feature Set')
 
+    @classmethod
+    def getFeatureSetColor(cls):
+        return cls._get(2, 3, 'Color Feature Set')
+
+    @classmethod
+    def getFeatureSetLongColor(cls):
+        return cls._get(3, 4, 'Long Color Feature Set')
+
 
 def _as_image_struct(a):
+    if a.ndim == 3:  # (h, w, 3) interleaved RGB, 8 or 16 bits per sample
+        if a.shape[2] != 3 or a.dtype not in (np.uint8, np.uint16):
+            raise ValueError('colour images are (h, w, 3) uint8 or uint16 arrays')
+        a = np.ascontiguousarray(a)
+        dt = 4 if a.dtype == np.uint8 else 5
+        return a, _Image(ctypes.c_void_p(a.ctypes.data), a.shape[0], a.shape[1], a.shape[1], dt)
+    if a.ndim != 2:
+        raise ValueError('images are 2-D arrays or (h, w, 3) RGB arrays')
     if a.dtype == np.uint16:
         dt = 0
     elif a.dtype == np.float32:
@@ -274,8 +292,9 @@ def _as_image_struct(a):
 
 
 def compute_features(images, plan=None, return_status=False):
-    """Features of a batch of 2-D arrays (uint8/uint16/float32/float64, sizes
-    may differ) in one pass of the DAG on the GPU.  Returns [n][n_features]
+    """Features of a batch of 2-D arrays (uint8/uint16/float32/float64) or
+    (h, w, 3) RGB arrays (uint8/uint16; the colour plans need these), sizes
+    may differ, in one pass of the DAG on the GPU.  Returns [n][n_features]
     float64; rows of rejected images are NaN (status != 0)."""
     if plan is None:
         plan = StdFeatureComputationPlans.getFeatureSetLong()
@@ -323,13 +342,22 @@ class ImageTransform(object):
         self.name = name
 
     def execute(self, matrix_in, matrix_out=None):
-        a = np.ascontiguousarray(matrix_in.as_ndarray() if hasattr(matrix_in, 'as_ndarray') else matrix_in,
-                                 dtype=np.float64)
-        h, w = a.shape
-        out = np.zeros((2 * ((h + 9) // 2)) * (2 * ((w + 9) // 2)))
+        """matrix_in: a gray plane (ImageMatrix or 2-D array), or an (h, w, 3)
+        uint8 / uint16 RGB array, which "Color" and "Hue" need."""
+        a = matrix_in.as_ndarray() if hasattr(matrix_in, 'as_ndarray') else np.asarray(matrix_in)
         oh, ow = _I(), _I()
-        _check(_load().wndgpu_transform(_context(), self.name.encode(), _ptr(a), h, w, _ptr(out), ctypes.byref(oh),
-                                        ctypes.byref(ow)), 'ImageTransform %s' % self.name)
+        if a.ndim == 3:
+            a, im = _as_image_struct(a)
+            h, w = a.shape[:2]
+            out = np.zeros((2 * ((h + 9) // 2)) * (2 * ((w + 9) // 2)))
+            _check(_load().wndgpu_transform_image(_context(), self.name.encode(), ctypes.byref(im), _ptr(out),
+                                                  ctypes.byref(oh), ctypes.byref(ow)), 'ImageTransform %s' % self.name)
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            h, w = a.shape
+            out = np.zeros((2 * ((h + 9) // 2)) * (2 * ((w + 9) // 2)))
+            _check(_load().wndgpu_transform(_context(), self.name.encode(), _ptr(a), h, w, _ptr(out),
+                                            ctypes.byref(oh), ctypes.byref(ow)), 'ImageTransform %s' % self.name)
         r = out[:oh.value * ow.value].reshape(oh.value, ow.value)
         if matrix_out is not None:
             matrix_out.set_pixels(r)
