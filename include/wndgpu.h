@@ -26,6 +26,13 @@
  *   wndgpu_normalize_test       signatures::normalize (src/wndchrm_src/signatures.cpp:196-215)
  *   wndgpu_fisher               TrainingSet::SetFisherScores (TrainingSet.cpp:2183-2324)
  *   wndgpu_wnd5                 TrainingSet::classify2 (TrainingSet.cpp:2429-2586)
+ *   wndgpu_tiff_read            ImageMatrix::LoadTIFF (src/cmatrix.cpp:85-203), the file part
+ *   wndgpu_sig_write / _read    signatures::SaveToFile / LoadFromFilep
+ *                               (src/wndchrm_src/signatures.cpp:227-304)
+ *   wndgpu_fit_write / _read    TrainingSet::SaveToFile / ReadFromFile (TrainingSet.cpp:339-480)
+ *   wndgpu_weights_write / _read
+ *                               TrainingSet::SaveWeightVector / LoadWeightVector
+ *                               (TrainingSet.cpp:594-642)
  */
 #ifndef WNDGPU_H
 #define WNDGPU_H
@@ -40,7 +47,9 @@ enum {
   WNDGPU_E_ARG = -1,      /* bad argument (null pointer, size out of range, unknown name) */
   WNDGPU_E_DEVICE = -2,   /* HIP runtime error */
   WNDGPU_E_NOMEM = -3,    /* device allocation failed */
-  WNDGPU_E_IMAGE = -4     /* at least one image of the batch was rejected (see status[]) */
+  WNDGPU_E_IMAGE = -4,    /* at least one image of the batch was rejected (see status[]) */
+  WNDGPU_E_IO = -5,       /* a file could not be opened, read or written (wndgpu_io_error) */
+  WNDGPU_E_FORMAT = -6    /* a file is not in a format this library reads (wndgpu_io_error) */
 };
 
 /* sample types.  RGB8 / RGB16: three interleaved samples per pixel (R, G, B),
@@ -143,6 +152,54 @@ int wndgpu_last_node_times(wndgpu_ctx *ctx, double *ms, const char **names, int 
  * angle a.  Returns the number of entries (<= cap are written).  No
  * counterpart in the reference (Ng is local to CVIPtexture.cpp:400-414). */
 int wndgpu_last_haralick_work(wndgpu_ctx *ctx, int *tones, double *schur_flops, int cap);
+
+/* ---- files (host only: no context, no GPU).  Errors: WNDGPU_E_IO /
+ * WNDGPU_E_FORMAT / WNDGPU_E_ARG with a message in wndgpu_io_error() of the
+ * calling thread. */
+const char *wndgpu_io_error(void);
+void wndgpu_free(void *p);
+
+/* A TIFF image as LoadTIFF reads it: one directory, strips (uncompressed,
+ * LZW, PackBits or Deflate, optional horizontal predictor), 8 or 16 bits per
+ * sample, 1 sample per pixel (dtype WNDGPU_U8 / U16) or 3 (WNDGPU_RGB8 /
+ * RGB16).  *pixels is allocated here (free with wndgpu_free); image->data
+ * points into it, ready for wndgpu_compute. */
+int wndgpu_tiff_read(const char *path, void **pixels, wndgpu_image *image);
+
+/* .sig: "class<TAB>version.type" (or "%f" of sample_value when it is not
+ * NULL), the image path, then "%.8g<TAB>name" per feature ("%.8g" when
+ * names is NULL). */
+int wndgpu_sig_write(const char *path, const char *image_path, int sample_class, const double *sample_value,
+                     int version, int vec_type, const double *values, const char *const *names, int n);
+
+/* Feature tables read from files: one sample (.sig) or a training set (.fit) */
+typedef struct wndgpu_table wndgpu_table;
+/* continuous: read the first field as a value (class 1), as for a continuous training set */
+int wndgpu_sig_read(const char *path, int continuous, wndgpu_table **table);
+int wndgpu_fit_read(const char *path, wndgpu_table **table);
+int wndgpu_table_shape(const wndgpu_table *table, int *n_samples, int *n_features, int *n_classes, int *version,
+                       int *vec_type);
+const double *wndgpu_table_values(const wndgpu_table *table);        /* [n_samples][n_features] */
+const int *wndgpu_table_classes(const wndgpu_table *table);          /* [n_samples] */
+const double *wndgpu_table_sample_values(const wndgpu_table *table); /* [n_samples] */
+const char *wndgpu_table_feature_name(const wndgpu_table *table, int feature);
+const char *wndgpu_table_class_label(const wndgpu_table *table, int cls); /* 0..n_classes */
+const char *wndgpu_table_sample_path(const wndgpu_table *table, int sample);
+void wndgpu_table_destroy(wndgpu_table *table);
+
+/* .fit: header, feature names, n_classes + 1 class labels (label 0 is the
+ * unknown class), then per sample its values ("%ld" when integral, else
+ * "%.8g", space-terminated), its class ("%f" of sample_values when that is
+ * not NULL) and its path (empty when sample_paths is NULL). */
+int wndgpu_fit_write(const char *path, int n_classes, int version, int vec_type, int n_samples, int n_features,
+                     const double *values, const int *classes, const double *sample_values,
+                     const char *const *feature_names, const char *const *class_labels,
+                     const char *const *sample_paths);
+
+/* weights: "%f name" per feature.  The reader returns the number of weights
+ * in *n_read (an error if above cap); negative weights read as 0. */
+int wndgpu_weights_write(const char *path, const double *weights, const char *const *names, int n);
+int wndgpu_weights_read(const char *path, double *weights, int cap, int *n_read);
 
 #ifdef __cplusplus
 }

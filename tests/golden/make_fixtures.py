@@ -5,6 +5,7 @@ expected WND classification table of tests/wndchrm_tests/test01-classify.pl).
 Run once in a container that has /root/reference; the fixtures are committed.
 Only data is copied: pixel arrays, feature values and names, class labels.
 """
+import gzip
 import os
 import re
 import numpy as np
@@ -82,6 +83,21 @@ def main():
                         kat_p=np.array([[float(r[2]), float(r[3])] for r in rows]),
                         kat_pred=np.array([cn.index(r[5]) + 1 for r in rows], np.int32),
                         kat_interp=np.array([float(r[6]) for r in rows]))
+    # data files held verbatim (gzip-compressed) for the file-format tests:
+    # .sig / .fit / weights text as the reference wrote it, and TIFF inputs
+    # (uncompressed big-endian 16-bit, LZW 16-bit, 8-bit strips)
+    raw = os.path.join(OUT, 'raw')
+    os.makedirs(raw, exist_ok=True)
+    for src in [os.path.join(w, 't1_s01_c05_ij-l_precalculated.sig'),
+                os.path.join(w, '010067_301x300-l_precalculated.sig'),
+                os.path.join(w, 'unbalanced.fit'), os.path.join(w, 'test-l.fit'),
+                os.path.join(w, 'test_fit-l.weights'), os.path.join(w, 't1_s01_c05_ij.tif'),
+                os.path.join(w, 'images', 'binuc', 'binuc1.tiff'),
+                os.path.join(p, 'lymphoma_eosin_channel_MCL_test_img_sj-05-3362-R2_001_E-t6x5_5_4-l.tiff'),
+                os.path.join(p, 'test-0032-0008-0008.tif')]:
+        data = open(src, 'rb').read()
+        with gzip.GzipFile(os.path.join(raw, os.path.basename(src) + '.gz'), 'wb', mtime=0) as f:
+            f.write(data)
 
 
 if __name__ == '__main__':

@@ -98,3 +98,21 @@ def test_gray_image_rejected_by_colour_plan():
     assert list(st) == [0, 4, 0]
     assert np.all(np.isnan(out[1]))
     assert np.array_equal(out[0], out[2])
+
+
+def test_open_colour_tiff_through_executor(tmp_path):
+    """ImageMatrix.OpenImage of an RGB TIFF keeps its samples; the executor
+    runs the short colour set on them"""
+    Image = pytest.importorskip('PIL.Image')
+    rgb = rgb_images()['smooth_u8']
+    p = str(tmp_path / 'c.tif')
+    Image.fromarray(rgb).save(p, compression='tiff_lzw')
+    print('open colour tiff', flush=True)
+    im = wndcharm.ImageMatrix()
+    assert im.OpenImage(p) == 1
+    gray, _ = orc_rgb_planes(rgb)
+    assert np.array_equal(im.as_ndarray(), gray)
+    plan = wndcharm.StdFeatureComputationPlans.getFeatureSetColor()
+    vec = wndcharm.DoubleVector(plan.n_features)
+    wndcharm.FeatureComputationPlanExecutor(plan).run(im, vec, 0)
+    compare(vec.as_ndarray(), orc_features_rgb(rgb, 2), exact_mask_for_names(orc_names(2)), 'colour tiff')

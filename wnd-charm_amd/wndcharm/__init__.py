@@ -19,7 +19,9 @@ import numpy as np
 
 __all__ = ['ImageMatrix', 'PyImageMatrix', 'DoubleVector', 'FeatureComputationPlan', 'StdFeatureComputationPlans',
            'FeatureComputationPlanExecutor', 'ImageTransform', 'FeatureAlgorithm', 'compute_features',
-           'normalize_train', 'normalize_test', 'fisher_weights', 'wnd5', 'WndGpuError', 'library_path']
+           'normalize_train', 'normalize_test', 'fisher_weights', 'wnd5', 'WndGpuError', 'library_path',
+           'read_tiff', 'write_sig', 'read_sig', 'write_fit', 'read_fit', 'write_weights', 'read_weights',
+           'FeatureTable']
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.environ.get('WNDGPU_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libwndgpu.so'))
@@ -70,6 +72,23 @@ def _load():
         'wndgpu_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
         'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
         'wndgpu_last_haralick_work': (_I, [_P, _P, _P, _I]),
+        'wndgpu_io_error': (ctypes.c_char_p, []),
+        'wndgpu_free': (None, [_P]),
+        'wndgpu_tiff_read': (_I, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_Image)]),
+        'wndgpu_sig_write': (_I, [ctypes.c_char_p, ctypes.c_char_p, _I, _P, _I, _I, _P, _P, _I]),
+        'wndgpu_sig_read': (_I, [ctypes.c_char_p, _I, ctypes.POINTER(_P)]),
+        'wndgpu_fit_write': (_I, [ctypes.c_char_p, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P]),
+        'wndgpu_fit_read': (_I, [ctypes.c_char_p, ctypes.POINTER(_P)]),
+        'wndgpu_table_shape': (_I, [_P, _P, _P, _P, _P, _P]),
+        'wndgpu_table_values': (_P, [_P]),
+        'wndgpu_table_classes': (_P, [_P]),
+        'wndgpu_table_sample_values': (_P, [_P]),
+        'wndgpu_table_feature_name': (ctypes.c_char_p, [_P, _I]),
+        'wndgpu_table_class_label': (ctypes.c_char_p, [_P, _I]),
+        'wndgpu_table_sample_path': (ctypes.c_char_p, [_P, _I]),
+        'wndgpu_table_destroy': (None, [_P]),
+        'wndgpu_weights_write': (_I, [ctypes.c_char_p, _P, _P, _I]),
+        'wndgpu_weights_read': (_I, [ctypes.c_char_p, _P, _I, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -111,6 +130,7 @@ class ImageMatrix(object):
 
     def __init__(self, pixels=None):
         self._px = None
+        self._rgb = None
         if pixels is not None:
             self.set_pixels(pixels)
 
@@ -122,6 +142,7 @@ class ImageMatrix(object):
         if a.ndim != 2:
             raise ValueError('pixel plane must be 2-D')
         self._px = np.ascontiguousarray(a, dtype=np.float64)
+        self._rgb = None
 
     @property
     def width(self):
@@ -138,20 +159,34 @@ class ImageMatrix(object):
         return self._px
 
     def OpenImage(self, path, downsample=0, bounding_rect=None, mean=0, stdev=0):
-        """Loads an 8/16-bit grayscale TIFF (ImageMatrix::OpenImage, cmatrix.cpp:252-294);
-        returns 1 on success, 0 on failure like the reference."""
+        """Loads an 8/16-bit TIFF with 1 or 3 samples per pixel (ImageMatrix::OpenImage,
+        cmatrix.cpp:252-294, through LoadTIFF 85-203); returns 1 on success, 0 on
+        failure like the reference.  A colour image keeps its RGB samples (the
+        colour plans read them); its pixel plane is RGB2GRAY, computed on the GPU."""
         try:
-            from PIL import Image
-            a = np.array(Image.open(path))
-        except Exception:
-            return 0
-        if a.ndim != 2:
+            a = read_tiff(path)
+        except WndGpuError:
             return 0
         if bounding_rect is not None:
             x, y, w, h = bounding_rect
             a = a[y:y + h, x:x + w]
-        self.set_pixels(a.astype(np.float64))
+        if a.ndim == 3:
+            self.set_rgb(a)
+        else:
+            self.set_pixels(a.astype(np.float64))
         return 1
+
+    def set_rgb(self, a):
+        """an (h, w, 3) uint8 / uint16 colour image: the gray plane is its RGB2GRAY"""
+        a = np.ascontiguousarray(a)
+        if a.ndim != 3 or a.shape[2] != 3 or a.dtype not in (np.uint8, np.uint16):
+            raise ValueError('colour images are (h, w, 3) uint8 or uint16 arrays')
+        self._px = ImageTransform('Color Transform').execute(a)
+        self._rgb = a
+
+    def source(self):
+        """what the engine is given: the RGB samples of a colour image, else the plane"""
+        return self._rgb if self._rgb is not None else self._px
 
 
 class PyImageMatrix(ImageMatrix):
@@ -324,7 +359,7 @@ class FeatureComputationPlanExecutor(object):
         self.plan = plan
 
     def run(self, source_mat, feature_mat, dest_row):
-        px = source_mat.as_ndarray() if hasattr(source_mat, 'as_ndarray') else np.asarray(source_mat)
+        px = source_mat.source() if hasattr(source_mat, 'source') else np.asarray(source_mat)
         if px is None:
             raise WndGpuError('Attempt to execute a FeatureComputationPlan with an empty ImageMatrix')
         nf = self.plan.n_features
@@ -444,3 +479,123 @@ def last_haralick_work():
     flops = np.zeros((max(n, 1), 4))
     lib.wndgpu_last_haralick_work(_context(), _ptr(tones), _ptr(flops), n)
     return tones[:n], flops[:n]
+
+
+# ---------------------------------------------------------------- files
+def _io_check(rc, what):
+    if rc != 0:
+        raise WndGpuError('%s: %s' % (what, _load().wndgpu_io_error().decode(errors='replace')))
+
+
+def _cstrs(items):
+    arr = (ctypes.c_char_p * max(len(items), 1))(*[str(x).encode() for x in items])
+    return arr
+
+
+def read_tiff(path):
+    """A TIFF image as ImageMatrix::LoadTIFF reads it (cmatrix.cpp:85-203):
+    (h, w) uint8 / uint16, or (h, w, 3) for 3 samples per pixel."""
+    lib = _load()
+    buf = _P()
+    im = _Image()
+    _io_check(lib.wndgpu_tiff_read(os.fsencode(path), ctypes.byref(buf), ctypes.byref(im)), 'read_tiff')
+    try:
+        dt = {0: np.uint16, 3: np.uint8, 4: np.uint8, 5: np.uint16}[im.dtype]
+        ch = 3 if im.dtype in (4, 5) else 1
+        n = im.height * im.width * ch
+        raw = ctypes.string_at(buf, n * np.dtype(dt).itemsize)
+        a = np.frombuffer(raw, dt).copy()
+    finally:
+        lib.wndgpu_free(buf)
+    return a.reshape((im.height, im.width, 3) if ch == 3 else (im.height, im.width))
+
+
+def write_sig(path, values, names=None, image_path='', sample_class=0, sample_value=None, version=4, vec_type=0):
+    """signatures::SaveToFile (signatures.cpp:227-261): byte-identical .sig text"""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    sv = ctypes.c_double(sample_value) if sample_value is not None else None
+    nm = _cstrs(names) if names is not None else None
+    _io_check(_load().wndgpu_sig_write(os.fsencode(path), image_path.encode(), int(sample_class),
+                                       ctypes.byref(sv) if sv is not None else None, int(version), int(vec_type),
+                                       _ptr(v), nm, len(v)), 'write_sig')
+
+
+class FeatureTable(object):
+    """a .sig (one sample) or .fit (training set) as read from its file"""
+
+    def __init__(self, handle):
+        lib = _load()
+        shape = [_I() for _ in range(5)]
+        lib.wndgpu_table_shape(handle, *[ctypes.byref(x) for x in shape])
+        ns, nf, nc, self.version, self.feature_vec_type = [x.value for x in shape]
+        self.n_classes = nc
+        vals = lib.wndgpu_table_values(handle)
+        self.values = (np.ctypeslib.as_array(ctypes.cast(vals, ctypes.POINTER(_D)), (ns * nf,)).copy()
+                       if ns * nf else np.zeros(0)).reshape(ns, nf)
+        cl = lib.wndgpu_table_classes(handle)
+        self.classes = np.ctypeslib.as_array(ctypes.cast(cl, ctypes.POINTER(_I)), (ns,)).copy() if ns else \
+            np.zeros(0, np.int32)
+        sv = lib.wndgpu_table_sample_values(handle)
+        self.sample_values = np.ctypeslib.as_array(ctypes.cast(sv, ctypes.POINTER(_D)), (ns,)).copy() if ns else \
+            np.zeros(0)
+        self.names = [lib.wndgpu_table_feature_name(handle, i).decode() for i in range(nf)]
+        labels = []
+        i = 0
+        while True:
+            lab = lib.wndgpu_table_class_label(handle, i)
+            if lab is None:
+                break
+            labels.append(lab.decode())
+            i += 1
+        self.class_labels = labels
+        self.paths = [lib.wndgpu_table_sample_path(handle, i).decode() for i in range(ns)]
+
+
+def read_sig(path, continuous=False):
+    """signatures::LoadFromFilep (signatures.cpp:277-304): a FeatureTable with one sample"""
+    lib = _load()
+    h = _P()
+    _io_check(lib.wndgpu_sig_read(os.fsencode(path), int(bool(continuous)), ctypes.byref(h)), 'read_sig')
+    try:
+        return FeatureTable(h)
+    finally:
+        lib.wndgpu_table_destroy(h)
+
+
+def read_fit(path):
+    """TrainingSet::ReadFromFile (TrainingSet.cpp:402-480)"""
+    lib = _load()
+    h = _P()
+    _io_check(lib.wndgpu_fit_read(os.fsencode(path), ctypes.byref(h)), 'read_fit')
+    try:
+        return FeatureTable(h)
+    finally:
+        lib.wndgpu_table_destroy(h)
+
+
+def write_fit(path, values, classes, feature_names, class_labels, sample_paths=None, sample_values=None,
+              version=4, vec_type=0):
+    """TrainingSet::SaveToFile (TrainingSet.cpp:339-380); class_labels has n_classes + 1
+    entries (entry 0 is the unknown class's label)"""
+    v = np.ascontiguousarray(values, dtype=np.float64)
+    ns, nf = v.shape
+    c = np.ascontiguousarray(classes, dtype=np.int32)
+    sv = np.ascontiguousarray(sample_values, dtype=np.float64) if sample_values is not None else None
+    _io_check(_load().wndgpu_fit_write(os.fsencode(path), len(class_labels) - 1, int(version), int(vec_type), ns, nf,
+                                       _ptr(v), _ptr(c), _ptr(sv) if sv is not None else None,
+                                       _cstrs(feature_names), _cstrs(class_labels),
+                                       _cstrs(sample_paths) if sample_paths is not None else None), 'write_fit')
+
+
+def write_weights(path, weights, names):
+    """TrainingSet::SaveWeightVector (TrainingSet.cpp:594-606)"""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    _io_check(_load().wndgpu_weights_write(os.fsencode(path), _ptr(w), _cstrs(names), len(w)), 'write_weights')
+
+
+def read_weights(path, cap=100000):
+    """TrainingSet::LoadWeightVector (TrainingSet.cpp:613-642) values (negative read as 0)"""
+    w = np.zeros(cap)
+    n = _I()
+    _io_check(_load().wndgpu_weights_read(os.fsencode(path), _ptr(w), cap, ctypes.byref(n)), 'read_weights')
+    return w[:n.value].copy()
