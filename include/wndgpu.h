@@ -26,6 +26,8 @@
  *   wndgpu_normalize_test       signatures::normalize (src/wndchrm_src/signatures.cpp:196-215)
  *   wndgpu_fisher               TrainingSet::SetFisherScores (TrainingSet.cpp:2183-2324)
  *   wndgpu_wnd5                 TrainingSet::classify2 (TrainingSet.cpp:2429-2586)
+ *   wndgpu_compute_sampled      TrainingSet::AddImageFile's sampling (TrainingSet.cpp:1425-1480):
+ *                               OpenImage pre-processing, rotations, tiles
  *   wndgpu_tiff_read            ImageMatrix::LoadTIFF (src/cmatrix.cpp:85-203), the file part
  *   wndgpu_sig_write / _read    signatures::SaveToFile / LoadFromFilep
  *                               (src/wndchrm_src/signatures.cpp:227-304)
@@ -65,7 +67,8 @@ enum {
   WNDGPU_IMG_TOO_SMALL = 1,
   WNDGPU_IMG_TOO_LARGE = 2,
   WNDGPU_IMG_BAD_DTYPE = 3,
-  WNDGPU_IMG_NOT_COLOR = 4 /* the plan reads the colour plane and the image is gray */
+  WNDGPU_IMG_NOT_COLOR = 4, /* the plan reads the colour plane and the image is gray */
+  WNDGPU_IMG_BAD_SAMPLING = 5 /* crop or tile outside the image, or normalisation of a float image */
 };
 
 /* smallest / largest side accepted (the Symlet-5 and Tamura paths need >= 16) */
@@ -107,6 +110,31 @@ int wndgpu_compute(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image 
  * GPU (pointers from hipMalloc / torch tensors); no host round trip. */
 int wndgpu_compute_device(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image *images, int n_images,
                           double *out_device, int *status);
+
+/* Sampling of each image before the features, as wndchrm does it:
+ * ImageMatrix::OpenImage's crop, Downsample and normalisation
+ * (src/cmatrix.cpp:252-294), then rotations x tiles_x x tiles_y samples in
+ * setup_featureset's order (src/wndchrm_src/wndchrm.cpp:89-128: rotation
+ * outer, tile x, tile y inner), each the tile of the image rotated by
+ * 90 * r degrees (Rotate, cmatrix.cpp:727-767, whose "270" is a transpose;
+ * tile sizes as TrainingSet::AddImageFile, TrainingSet.cpp:1455-1480).
+ * The colour plane follows the pixels (the reference does not rotate it).
+ * A crop or tile that falls outside its image rejects the image
+ * (WNDGPU_IMG_BAD_SAMPLING), where the reference's submatrix leaves the
+ * previous tile in place. */
+typedef struct {
+  int roi_x, roi_y, roi_w, roi_h; /* bounding rectangle; roi_x < 0: the whole image */
+  int downsample;                 /* percent, applied when 0 < downsample < 100 */
+  double mean, stdev;             /* normalise to mean (when > 0) and stdev (when > 0): 8/16-bit images */
+  int rotations;                  /* 1..4 */
+  int tiles_x, tiles_y;           /* >= 1 */
+} wndgpu_sampling;
+int wndgpu_sampling_n_samples(const wndgpu_sampling *sampling);
+/* out: host [n_images * n_samples][n_features], sample k of image i at row
+ * i * n_samples + k; status per image.  Images may be up to 32768 a side;
+ * each sample must be within [WNDGPU_MIN_SIDE, WNDGPU_MAX_SIDE]. */
+int wndgpu_compute_sampled(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image *images, int n_images,
+                           const wndgpu_sampling *sampling, double *out, int *status);
 
 /* One transform ("Fourier", "Chebyshev", "Wavelet", "Edge", "Color
  * Transform") on one FP64 host plane.  out must hold the output plane: Fourier/Edge h*w, Chebyshev

@@ -74,6 +74,22 @@ int orc_color_histogram(const void *rgb, int bits, int h, int w, double *out20);
 /* column -> feature-name string of a plan ("Zernike Coefficients (Fourier ()) [3]") */
 const char *orc_feature_name(int set, int column);
 
+/* Sampling of an image before the features: OpenImage's crop (roi_x < 0:
+ * none), downsample (percent, when 0 < downsample < 100) and mean / stdev
+ * normalisation (when mean > 0), then rotations (1..4) x tiles_x x tiles_y
+ * samples, rotation outer, tile y inner.  dtype: 0 uint16, 2 float64,
+ * 3 uint8, 4 RGB8, 5 RGB16 (row-major, RGB interleaved). */
+typedef struct {
+  int roi_x, roi_y, roi_w, roi_h, downsample;
+  double mean, stdev;
+  int rotations, tiles_x, tiles_y;
+} orc_sampling;
+int orc_sample_count(const orc_sampling *s);
+int orc_sample_plane(const void *img, int dtype, int h, int w, const orc_sampling *s, int k, double *plane,
+                     unsigned char *hsv, int *out_h, int *out_w);
+/* out: [n_samples][n_features of set] */
+int orc_features_sampled(int set, const void *img, int dtype, int h, int w, const orc_sampling *s, double *out);
+
 /* Intermediates used by integer-exact parity tests. */
 double orc_otsu(const double *in, int h, int w);                         /* Otsu(dynamic) */
 int orc_radon_proj(const double *in, int h, int w, double *out4xr, int *rsize);

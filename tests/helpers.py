@@ -135,6 +135,29 @@ def orc_find_color_many(hsv):
     return out
 
 
+class OrcSampling(ctypes.Structure):
+    _fields_ = [('roi_x', I), ('roi_y', I), ('roi_w', I), ('roi_h', I), ('downsample', I), ('mean', D),
+                ('stdev', D), ('rotations', I), ('tiles_x', I), ('tiles_y', I)]
+
+
+def orc_features_sampled(img, set_, roi=None, downsample=0, mean=0.0, stdev=0.0, rotations=1, tiles=(1, 1)):
+    """oracle features [n_samples][n_features] of an image (2-D uint8 / uint16 /
+    float64, or (h, w, 3) uint8 / uint16) under the sampling options"""
+    img = np.ascontiguousarray(img)
+    if img.ndim == 3:
+        dt = 4 if img.dtype == np.uint8 else 5
+    else:
+        dt = {np.dtype(np.uint16): 0, np.dtype(np.float64): 2, np.dtype(np.uint8): 3}[img.dtype]
+    x, y, w, h = roi if roi is not None else (-1, -1, 0, 0)
+    s = OrcSampling(x, y, w, h, downsample, mean, stdev, rotations, tiles[0], tiles[1])
+    lib = oracle()
+    lib.orc_features_sampled.argtypes = [I, P, I, I, I, ctypes.POINTER(OrcSampling), P]
+    ns = rotations * tiles[0] * tiles[1]
+    out = np.zeros((ns, SET_NFEAT[set_]))
+    rc = lib.orc_features_sampled(set_, ptr(img), dt, img.shape[0], img.shape[1], ctypes.byref(s), ptr(out))
+    return out if rc == 0 else None
+
+
 def orc_names(long_set=True):
     lib = oracle()
     s = _set_of(long_set)

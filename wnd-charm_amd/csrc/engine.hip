@@ -296,10 +296,30 @@ struct Arena {
 };
 
 // ============================================================ batch execution
+// One input image under sampling: OpenImage's crop, Downsample and
+// normalisation (cmatrix.cpp:252-294), before its samples are cut.
+struct PreImage {
+  LoadDesc full;      // the whole image (the channel range of wide RGB samples)
+  LoadDesc roi;       // the crop
+  int ch, cw;         // crop size
+  int dh, dw;         // size after Downsample
+  bool ds;            // Downsample applies
+  double dx;          // its step, 1 / ratio
+  bool norm;          // normalise to mean / stdev
+  double mean, stdev, max_range;
+};
+// One sample: a rotation and tile of a pre-processed image
+struct SampleRef {
+  int img, rot, x0, y0, h, w;
+};
+
 struct Batch {
-  int n;
+  int n;  // root entries: images, or samples when sampled
   std::vector<int> h0, w0;
   std::vector<LoadDesc> load;
+  bool sampled = false;
+  std::vector<PreImage> pre;
+  std::vector<SampleRef> smp;
   double *out;  // device [n][nf]
   int nf;
 };
@@ -398,6 +418,135 @@ struct LaunchItem {
   bool side;
 };
 
+// Root of a sampled batch: load each image's crop, Downsample it (pixels and
+// colour plane), normalise it, then cut the rotated / tiled samples; the
+// samples are the root entries.  A batch with one unrotated, untiled sample
+// per image uses the pre-processed planes as they are.
+template <class Push>
+static int build_sampled_root(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, Arena &A, NodeRun &R,
+                              Push &push) {
+  hipStream_t s = c->stream;
+  const int ni = (int)B.pre.size(), n = B.n;
+  const bool colour = plan->color;
+  std::vector<Img> cim(ni), dim(ni);
+  std::vector<unsigned char *> chsv(ni, nullptr), dhsv(ni, nullptr);
+  std::vector<LoadDesc> roi(ni), full(ni);
+  bool wide = false;
+  for (int k = 0; k < ni; k++) wide |= B.pre[k].full.dtype == 5;
+  double *range = wide ? A.dalloc_n<double>(2 * (size_t)ni) : nullptr;
+  long maxc = 0;
+  for (int k = 0; k < ni; k++) {
+    const PreImage &P = B.pre[k];
+    cim[k] = Img{A.dalloc_n<double>((size_t)P.ch * P.cw), P.ch, P.cw};
+    if (colour) chsv[k] = A.dalloc_n<unsigned char>(3 * (size_t)P.ch * P.cw);
+    roi[k] = P.roi;
+    roi[k].hsv = chsv[k];
+    roi[k].range = range ? range + 2 * k : nullptr;
+    full[k] = P.full;
+    maxc = std::max(maxc, (long)P.ch * P.cw);
+  }
+  // Downsample: the pixel plane with the Chebyshev-Fourier kernels, the colour plane alongside
+  std::vector<DsDesc> ds;
+  std::vector<HsvDsDesc> hds;
+  int mx_oh = 0, mx_ow = 0, mx_nh = 0, mx_nw = 0;
+  for (int k = 0; k < ni; k++) {
+    const PreImage &P = B.pre[k];
+    if (!P.ds) {
+      dim[k] = cim[k];
+      dhsv[k] = chsv[k];
+      continue;
+    }
+    DsDesc d;
+    d.in = cim[k].p;
+    d.tmp = A.dalloc_n<double>((size_t)P.ch * P.cw);
+    d.out = A.dalloc_n<double>((size_t)P.dh * P.dw);
+    d.oh = P.ch;
+    d.ow = P.cw;
+    d.nh = P.dh;
+    d.nw = P.dw;
+    d.dx = d.dy = P.dx;
+    d.xs = d.ys = nullptr;
+    d.cx = d.cy = 0;
+    if (!A.sizing) {
+      int r;
+      if ((r = get_dspos(c, P.cw, P.dx, &d.xs, &d.cx))) return r;
+      if ((r = get_dspos(c, P.ch, P.dx, &d.ys, &d.cy))) return r;
+    }
+    ds.push_back(d);
+    dim[k] = Img{d.out, P.dh, P.dw};
+    if (colour) {
+      HsvDsDesc h;
+      h.in = chsv[k];
+      h.tmp = A.dalloc_n<unsigned char>(3 * (size_t)P.ch * P.cw);
+      h.out = A.dalloc_n<unsigned char>(3 * (size_t)P.dh * P.dw);
+      h.oh = d.oh; h.ow = d.ow; h.nh = d.nh; h.nw = d.nw; h.cx = d.cx; h.cy = d.cy;
+      h.xs = d.xs; h.ys = d.ys; h.dx = d.dx; h.dy = d.dy;
+      hds.push_back(h);
+      dhsv[k] = h.out;
+    }
+    mx_oh = std::max(mx_oh, P.ch);
+    mx_ow = std::max(mx_ow, P.cw);
+    mx_nh = std::max(mx_nh, P.dh);
+    mx_nw = std::max(mx_nw, P.dw);
+  }
+  std::vector<NormDesc> nd;
+  long maxd = 0;
+  for (int k = 0; k < ni; k++) {
+    const PreImage &P = B.pre[k];
+    if (P.norm) nd.push_back(NormDesc{dim[k].p, (long)P.dh * P.dw, P.mean, P.stdev, P.max_range, 0, 0});
+    maxd = std::max(maxd, (long)P.dh * P.dw);
+  }
+  // samples
+  const bool direct = n == ni && std::all_of(B.smp.begin(), B.smp.end(), [](const SampleRef &q) { return q.rot == 0; }) &&
+                      std::all_of(B.smp.begin(), B.smp.end(), [&](const SampleRef &q) {
+                        return q.x0 == 0 && q.y0 == 0 && q.h == B.pre[q.img].dh && q.w == B.pre[q.img].dw;
+                      });
+  std::vector<Img> sim(n);
+  std::vector<SampleDesc> sd;
+  R.h.resize(n);
+  R.w.resize(n);
+  R.hsv.assign(n, nullptr);
+  long maxs = 0;
+  for (int j = 0; j < n; j++) {
+    const SampleRef &q = B.smp[j];
+    R.h[j] = q.h;
+    R.w[j] = q.w;
+    maxs = std::max(maxs, (long)q.h * q.w);
+    if (direct) {
+      sim[j] = dim[q.img];
+      R.hsv[j] = dhsv[q.img];
+      continue;
+    }
+    sim[j] = Img{A.dalloc_n<double>((size_t)q.h * q.w), q.h, q.w};
+    if (colour) R.hsv[j] = A.dalloc_n<unsigned char>(3 * (size_t)q.h * q.w);
+    sd.push_back(SampleDesc{dim[q.img].p, dhsv[q.img], R.hsv[j], dim[q.img].h, dim[q.img].w, q.rot, q.x0, q.y0});
+  }
+  R.imgs = A.stage(sim);
+  R.stats = A.dalloc_n<Stats>(n);
+  R.planes.resize(n);
+  for (int j = 0; j < n; j++) R.planes[j] = sim[j].p;
+  if (colour) R.hsvd = A.stage(R.hsv);
+  LoadDesc *rd = A.stage(roi), *fd = A.stage(full);
+  Img *cd = A.stage(cim);
+  DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
+  HsvDsDesc *hdd = hds.empty() ? nullptr : A.stage(hds);
+  NormDesc *ndd = nd.empty() ? nullptr : A.stage(nd);
+  SampleDesc *sdd = sd.empty() ? nullptr : A.stage(sd);
+  Img *ri = R.imgs;
+  Stats *rs = R.stats;
+  const int nds = (int)ds.size(), nnd = (int)nd.size(), nsd = (int)sd.size();
+  push("root", [=]() {
+    if (wide) launch_rgb_range(fd, range, ni, s);
+    launch_load(rd, cd, ni, maxc, s);
+    if (nds) launch_downsample(dsd, nds, mx_oh, mx_ow, mx_nh, mx_nw, s);
+    if (hdd) launch_downsample_hsv(hdd, nds, mx_oh, mx_ow, mx_nh, mx_nw, s);
+    if (nnd) launch_normalize(ndd, nnd, maxd, s);
+    if (nsd) launch_sample(sdd, ri, nsd, maxs, s);
+    launch_stats(ri, rs, n, s);
+  });
+  return 0;
+}
+
 // Build (or size) every launch of a batch; when !A.sizing the launches are
 // also appended to `launches` in execution order, and *first_feature is the
 // index of the first feature launch (the ones before it compute the DAG's
@@ -416,7 +565,10 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), true});
   };
   // ---- root: load + stats
-  {
+  if (B.sampled) {
+    int r = build_sampled_root(c, plan, B, A, nodes[{}], push);
+    if (r) return r;
+  } else {
     NodeRun &R = nodes[{}];
     R.h = B.h0;
     R.w = B.w0;
@@ -1088,36 +1240,128 @@ int wndgpu_plan_group_offset(const wndgpu_plan *p, const char *name) {
   return -1;
 }
 
+// Geometry of the samples of one image under `sp`: false (with the reason)
+// if the crop or a tile falls outside the image, as the reference's
+// submatrix refuses it (cmatrix.cpp:414-436).
+static bool sample_geometry(const wndgpu_image &im, const wndgpu_sampling &sp, PreImage &P, std::vector<SampleRef> &out,
+                            int *status) {
+  P.cw = im.width;
+  P.ch = im.height;
+  int x = 0, y = 0;
+  if (sp.roi_x >= 0) {
+    if (sp.roi_y < 0 || sp.roi_w < 1 || sp.roi_h < 1 || (long)sp.roi_x + sp.roi_w > im.width ||
+        (long)sp.roi_y + sp.roi_h > im.height) {
+      *status = WNDGPU_IMG_BAD_SAMPLING;
+      return false;
+    }
+    x = sp.roi_x;
+    y = sp.roi_y;
+    P.cw = sp.roi_w;
+    P.ch = sp.roi_h;
+  }
+  (void)x;
+  (void)y;
+  P.ds = false;
+  P.dx = 1;
+  P.dw = P.cw;
+  P.dh = P.ch;
+  if (sp.downsample > 0 && sp.downsample < 100) {
+    const double ratio = (double)sp.downsample / 100.0;
+    const double dx = 1 / ratio;
+    if (dx != 1) {
+      P.ds = true;
+      P.dx = dx;
+      P.dw = (int)(unsigned)(ratio * P.cw);
+      P.dh = (int)(unsigned)(ratio * P.ch);
+    }
+  }
+  P.norm = sp.mean > 0;
+  P.mean = sp.mean;
+  P.stdev = sp.stdev;
+  if (P.norm) {
+    const int bits = (im.dtype == WNDGPU_U8 || im.dtype == WNDGPU_RGB8) ? 8 : (im.dtype == WNDGPU_U16 || im.dtype == WNDGPU_RGB16) ? 16 : 0;
+    if (!bits) {  // the clamp range comes from the bits per sample
+      *status = WNDGPU_IMG_BAD_SAMPLING;
+      return false;
+    }
+    P.max_range = pow((double)2, bits) - 1;
+  }
+  const int tiles = sp.tiles_x * sp.tiles_y;
+  for (int k = 0; k < sp.rotations * tiles; k++) {
+    const int r = k / tiles, tx = (k % tiles) / sp.tiles_y, ty = k % sp.tiles_y;
+    const int rw = (r == 1 || r == 3) ? P.dh : P.dw, rh = (r == 1 || r == 3) ? P.dw : P.dh;
+    long tw = rw, th = rh;
+    if (tiles != 1) {  // TrainingSet.cpp:1465-1474
+      if (r == 1 || r == 3) { th = rw / sp.tiles_x; tw = rh / sp.tiles_y; }
+      else { tw = rw / sp.tiles_x; th = rh / sp.tiles_y; }
+    }
+    const long x0 = tx * tw, y0 = ty * th;
+    if (tw < 1 || th < 1 || x0 + tw > rw || y0 + th > rh) {
+      *status = WNDGPU_IMG_BAD_SAMPLING;
+      return false;
+    }
+    if (tw < WNDGPU_MIN_SIDE || th < WNDGPU_MIN_SIDE) {
+      *status = WNDGPU_IMG_TOO_SMALL;
+      return false;
+    }
+    if (tw > WNDGPU_MAX_SIDE || th > WNDGPU_MAX_SIDE) {
+      *status = WNDGPU_IMG_TOO_LARGE;
+      return false;
+    }
+    out.push_back(SampleRef{0, r, (int)x0, (int)y0, (int)th, (int)tw});
+  }
+  return true;
+}
+
 static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n, double *out,
-                          int *status, bool device_io) {
+                          int *status, bool device_io, const wndgpu_sampling *samp = nullptr) {
   if (!c || !plan || !images || n <= 0 || !out) return fail(c, WNDGPU_E_ARG, "null argument or empty batch");
+  if (samp && (samp->rotations < 1 || samp->rotations > 4 || samp->tiles_x < 1 || samp->tiles_y < 1))
+    return fail(c, WNDGPU_E_ARG, "sampling: rotations must be 1..4 and tiles at least 1");
   HIPCHK(c, hipSetDevice(c->device));
   Batch B;
   B.nf = plan->n_features;
+  B.sampled = samp != nullptr;
+  const int ns = samp ? samp->rotations * samp->tiles_x * samp->tiles_y : 1;
   std::vector<int> idx;  // accepted images
+  std::vector<PreImage> pre;
+  std::vector<SampleRef> smp;
   bool rejected = false;
   for (int i = 0; i < n; i++) {
     const wndgpu_image &im = images[i];
     int st = WNDGPU_IMG_OK;
+    const int max_in = samp ? 32768 : WNDGPU_MAX_SIDE;  // a sampled image is cut down before the features
     if (im.dtype < 0 || im.dtype > 5 || !im.data || im.stride < im.width) st = WNDGPU_IMG_BAD_DTYPE;
     else if (plan->color && im.dtype < 4) st = WNDGPU_IMG_NOT_COLOR;
-    else if (im.height < WNDGPU_MIN_SIDE || im.width < WNDGPU_MIN_SIDE) st = WNDGPU_IMG_TOO_SMALL;
-    else if (im.height > WNDGPU_MAX_SIDE || im.width > WNDGPU_MAX_SIDE) st = WNDGPU_IMG_TOO_LARGE;
+    else if (!samp && (im.height < WNDGPU_MIN_SIDE || im.width < WNDGPU_MIN_SIDE)) st = WNDGPU_IMG_TOO_SMALL;
+    else if (im.height < 1 || im.width < 1) st = WNDGPU_IMG_TOO_SMALL;
+    else if (im.height > max_in || im.width > max_in) st = WNDGPU_IMG_TOO_LARGE;
+    if (st == WNDGPU_IMG_OK && samp) {
+      PreImage P;
+      std::vector<SampleRef> g;
+      if (sample_geometry(im, *samp, P, g, &st)) {
+        for (SampleRef &q : g) q.img = (int)pre.size();
+        pre.push_back(P);
+        smp.insert(smp.end(), g.begin(), g.end());
+      }
+    }
     if (status) status[i] = st;
     if (st == WNDGPU_IMG_OK) idx.push_back(i);
     else rejected = true;
   }
   if (idx.empty()) return fail(c, WNDGPU_E_IMAGE, "no valid image in the batch");
-  B.n = (int)idx.size();
+  B.n = (int)idx.size() * ns;
   // host inputs are staged into one device buffer
   static const size_t esz[6] = {2, 4, 8, 1, 3, 6};  // bytes per pixel
   std::vector<void *> tmp;
   size_t in_bytes = 0;
-  std::vector<size_t> in_off(B.n);
-  for (int k = 0; k < B.n; k++) {
+  std::vector<size_t> in_off(idx.size());
+  for (size_t k = 0; k < idx.size(); k++) {
     const wndgpu_image &im = images[idx[k]];
-    B.h0.push_back(im.height);
-    B.w0.push_back(im.width);
+    if (!samp) {
+      B.h0.push_back(im.height);
+      B.w0.push_back(im.width);
+    }
     in_off[k] = in_bytes;
     in_bytes += ((size_t)im.height * im.stride * esz[im.dtype] + 255) & ~(size_t)255;
   }
@@ -1125,16 +1369,14 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   double *dout = nullptr;
   if (!device_io) {
     HIPCHK(c, hipMallocAsync((void **)&din, in_bytes, c->stream));
-    for (int k = 0; k < B.n; k++) {
+    for (size_t k = 0; k < idx.size(); k++) {
       const wndgpu_image &im = images[idx[k]];
       HIPCHK(c, hipMemcpyAsync(din + in_off[k], im.data, (size_t)im.height * im.stride * esz[im.dtype],
                                hipMemcpyHostToDevice, c->stream));
     }
-    HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)B.n * B.nf * sizeof(double), c->stream));
-  } else {
-    HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)B.n * B.nf * sizeof(double), c->stream));
   }
-  for (int k = 0; k < B.n; k++) {
+  HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)B.n * B.nf * sizeof(double), c->stream));
+  for (size_t k = 0; k < idx.size(); k++) {
     const wndgpu_image &im = images[idx[k]];
     LoadDesc L;
     L.src = device_io ? im.data : (const void *)(din + in_off[k]);
@@ -1142,21 +1384,38 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
     L.w = im.width;
     L.stride = im.stride;
     L.dtype = im.dtype;
-    B.load.push_back(L);
+    L.hsv = nullptr;
+    L.range = nullptr;
+    if (!samp) {
+      B.load.push_back(L);
+      continue;
+    }
+    PreImage &P = pre[k];
+    P.full = L;
+    P.roi = L;
+    if (samp->roi_x >= 0) {
+      P.roi.src = (const char *)L.src + ((size_t)samp->roi_y * im.stride + samp->roi_x) * esz[im.dtype];
+      P.roi.h = samp->roi_h;
+      P.roi.w = samp->roi_w;
+    }
   }
+  B.pre = std::move(pre);
+  B.smp = std::move(smp);
   B.out = dout;
   int r = run_batch(c, plan, B, getenv("WNDGPU_TIMING") != nullptr);
   if (r) return r;
-  // scatter accepted rows
+  // scatter accepted rows: image idx[k] owns rows idx[k]*ns .. +ns-1
+  const size_t rowb = (size_t)ns * B.nf;
   if (!device_io) {
     std::vector<double> h((size_t)B.n * B.nf);
     HIPCHK(c, hipMemcpyAsync(h.data(), dout, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int k = 0; k < B.n; k++) memcpy(out + (size_t)idx[k] * B.nf, h.data() + (size_t)k * B.nf, B.nf * sizeof(double));
+    for (size_t k = 0; k < idx.size(); k++)
+      memcpy(out + (size_t)idx[k] * rowb, h.data() + k * rowb, rowb * sizeof(double));
     hipFreeAsync(din, c->stream);
   } else {
-    for (int k = 0; k < B.n; k++)
-      HIPCHK(c, hipMemcpyAsync(out + (size_t)idx[k] * B.nf, dout + (size_t)k * B.nf, B.nf * sizeof(double),
+    for (size_t k = 0; k < idx.size(); k++)
+      HIPCHK(c, hipMemcpyAsync(out + (size_t)idx[k] * rowb, dout + k * rowb, rowb * sizeof(double),
                                hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
@@ -1179,6 +1438,17 @@ int wndgpu_compute(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *i
 int wndgpu_compute_device(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n, double *out,
                           int *status) {
   return compute_common(c, plan, images, n, out, status, true);
+}
+
+int wndgpu_sampling_n_samples(const wndgpu_sampling *s) {
+  if (!s || s->rotations < 1 || s->rotations > 4 || s->tiles_x < 1 || s->tiles_y < 1) return -1;
+  return s->rotations * s->tiles_x * s->tiles_y;
+}
+
+int wndgpu_compute_sampled(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n,
+                           const wndgpu_sampling *sampling, double *out, int *status) {
+  if (!sampling) return fail(c, WNDGPU_E_ARG, "null sampling");
+  return compute_common(c, plan, images, n, out, status, false, sampling);
 }
 
 int wndgpu_algorithm_n_features(const char *name) {

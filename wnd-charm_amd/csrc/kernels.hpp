@@ -51,6 +51,30 @@ struct CfGroup {
   double *sum[CF_GROUP];
 };
 
+struct HsvDsDesc {  // Downsample of a colour plane (3 bytes per pixel), positions as DsDesc
+  const unsigned char *in;
+  unsigned char *tmp;  // [oh][ow] after the x pass
+  unsigned char *out;  // [nh][nw]
+  int oh, ow, nh, nw, cx, cy;
+  const double *xs;
+  const double *ys;
+  double dx, dy;
+};
+
+struct NormDesc {  // normalize to a mean / stdev (cmatrix.cpp:836-888), in place
+  double *p;
+  long n;
+  double mean, stdev, max_range;
+  double d_mean, std_fact;  // written by the statistics pass
+};
+
+struct SampleDesc {  // one rotated, tiled sample of a pre-processed plane
+  const double *src;
+  const unsigned char *hsrc;  // colour plane of the source, or null
+  unsigned char *hdst;        // colour plane of the sample, or null
+  int sh, sw, rot, x0, y0;
+};
+
 struct DsDesc {  // area-weighted Downsample (cmatrix.cpp:579-720)
   const double *in;
   double *tmp;          // [oh][ow] after the x pass
@@ -129,5 +153,8 @@ void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *
                     hipStream_t s);
 void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s);
 void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
+void launch_downsample_hsv(const HsvDsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
+void launch_normalize(NormDesc *nd, int nimg, long maxn, hipStream_t s);
+void launch_sample(const SampleDesc *sd, const Img *dst, int nsamples, long maxn, hipStream_t s);
 
 }  // namespace wnd

@@ -21,7 +21,7 @@ __all__ = ['ImageMatrix', 'PyImageMatrix', 'DoubleVector', 'FeatureComputationPl
            'FeatureComputationPlanExecutor', 'ImageTransform', 'FeatureAlgorithm', 'compute_features',
            'normalize_train', 'normalize_test', 'fisher_weights', 'wnd5', 'WndGpuError', 'library_path',
            'read_tiff', 'write_sig', 'read_sig', 'write_fit', 'read_fit', 'write_weights', 'read_weights',
-           'FeatureTable']
+           'FeatureTable', 'Sampling']
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.environ.get('WNDGPU_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libwndgpu.so'))
@@ -72,6 +72,8 @@ def _load():
         'wndgpu_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
         'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
         'wndgpu_last_haralick_work': (_I, [_P, _P, _P, _I]),
+        'wndgpu_sampling_n_samples': (_I, [_P]),
+        'wndgpu_compute_sampled': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P, _P]),
         'wndgpu_io_error': (ctypes.c_char_p, []),
         'wndgpu_free': (None, [_P]),
         'wndgpu_tiff_read': (_I, [ctypes.c_char_p, ctypes.POINTER(_P), ctypes.POINTER(_Image)]),
@@ -326,11 +328,33 @@ def _as_image_struct(a):
     return a, _Image(ctypes.c_void_p(a.ctypes.data), a.shape[0], a.shape[1], a.shape[1], dt)
 
 
-def compute_features(images, plan=None, return_status=False):
+class Sampling(ctypes.Structure):
+    """How each image is sampled before the features (wndchrm's pre-processing
+    and sampling options): crop to roi = (x, y, w, h), downsample in percent,
+    normalise to mean (and stdev), then rotations x tiles = (tiles_x, tiles_y)
+    samples, rotation outer, tile y inner (TrainingSet.cpp:1425-1480)."""
+    _fields_ = [('roi_x', _I), ('roi_y', _I), ('roi_w', _I), ('roi_h', _I), ('downsample', _I), ('mean', _D),
+                ('stdev', _D), ('rotations', _I), ('tiles_x', _I), ('tiles_y', _I)]
+
+    def __init__(self, roi=None, downsample=0, mean=0.0, stdev=0.0, rotations=1, tiles=(1, 1)):
+        x, y, w, h = roi if roi is not None else (-1, -1, 0, 0)
+        super(Sampling, self).__init__(x, y, w, h, int(downsample), float(mean), float(stdev), int(rotations),
+                                       int(tiles[0]), int(tiles[1]))
+
+    @property
+    def n_samples(self):
+        n = _load().wndgpu_sampling_n_samples(ctypes.byref(self))
+        if n < 0:
+            raise WndGpuError('rotations must be 1..4 and tiles at least 1')
+        return n
+
+
+def compute_features(images, plan=None, return_status=False, sampling=None):
     """Features of a batch of 2-D arrays (uint8/uint16/float32/float64) or
     (h, w, 3) RGB arrays (uint8/uint16; the colour plans need these), sizes
     may differ, in one pass of the DAG on the GPU.  Returns [n][n_features]
-    float64; rows of rejected images are NaN (status != 0)."""
+    float64; rows of rejected images are NaN (status != 0).  With a Sampling,
+    each image gives n_samples rows (sample k of image i at i * n_samples + k)."""
     if plan is None:
         plan = StdFeatureComputationPlans.getFeatureSetLong()
     keep = []
@@ -340,9 +364,15 @@ def compute_features(images, plan=None, return_status=False):
         keep.append(a)
         structs[i] = s
     nf = plan.n_features
-    out = np.full((len(images), nf), np.nan)
     status = np.zeros(len(images), np.int32)
-    rc = _load().wndgpu_compute(_context(), plan.handle, structs, len(images), _ptr(out), _ptr(status))
+    if sampling is not None:
+        # rows: sample k of image i at i * n_samples + k
+        out = np.full((len(images) * sampling.n_samples, nf), np.nan)
+        rc = _load().wndgpu_compute_sampled(_context(), plan.handle, structs, len(images), ctypes.byref(sampling),
+                                            _ptr(out), _ptr(status))
+    else:
+        out = np.full((len(images), nf), np.nan)
+        rc = _load().wndgpu_compute(_context(), plan.handle, structs, len(images), _ptr(out), _ptr(status))
     if rc not in (0, -4):
         _check(rc, 'wndgpu_compute')
     if return_status:
