@@ -18,6 +18,15 @@ Also reported:
                 when present.
   cpu_baseline  the FP64 C++ oracle, one image per thread on the host
                 cores, on a bounded sample of the same images.
+
+--mixed: the mixed-precision configuration instead -- the 20 short-set
+128x128 images (2 classes, periods 8 / 20 px, seed 1001) through the
+Chebyshev (order 20), Zernike (order 15, 72 moments), Chebyshev-Fourier
+(order 23) and Gabor (7 frequencies) contractions in FP64, FP32 and BF16x3:
+images/s of the batched forms against the plain per-image FP64 form (one
+call per image), the max / mean relative error of each feature family
+against the batched FP64 form (which equals the FP64 oracle, see
+tests/test_gpu_parity.py), and whether 5-fold WND predictions agree.
 """
 import argparse
 import ctypes
@@ -94,6 +103,82 @@ def profiled_traffic(kernel):
     return float(m.group(1)) if m else None
 
 
+MIXED_GROUPS = [('Chebyshev Coefficients ()', 32), ('Zernike Coefficients ()', 72),
+                ('Chebyshev-Fourier Coefficients ()', 32), ('Gabor Textures ()', 7)]
+
+
+def mixed_main(args):
+    import torch
+    import wndcharm
+    from wndcharm import synthetic
+    imgs, cls = synthetic.config_images('short_128')
+
+    def plan_of(prec):
+        p = wndcharm.FeatureComputationPlan('contractions')
+        for g, _ in MIXED_GROUPS:
+            p.add(g)
+        p.finalize()
+        p.set_precision(prec)
+        return p
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / reps
+
+    big = imgs * 50  # 1000 images: the batched forms at a size that fills the GPU
+    forms, feats = {}, {}
+    p64 = plan_of(0)
+    _, dt = timed(lambda: [wndcharm.compute_features([im], p64) for im in imgs], args.steps)
+    forms['fp64_per_image'] = len(imgs) / dt
+    for name, prec in (('fp64', 0), ('fp32', 1), ('bf16x3', 2)):
+        pl = plan_of(prec)
+        feats[name], dt = timed(lambda: wndcharm.compute_features(imgs, pl), args.steps)
+        forms[name + '_batched_20'] = len(imgs) / dt
+        _, dt = timed(lambda: wndcharm.compute_features(big, pl), args.steps)
+        forms[name + '_batched_1000'] = len(big) / dt
+    errors, off = {}, 0
+    for g, n in MIXED_GROUPS:
+        ref = feats['fp64'][:, off:off + n]
+        for name in ('fp32', 'bf16x3'):
+            got = feats[name][:, off:off + n]
+            rel = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+            errors.setdefault(g.split(' (')[0], {})[name] = {'max_rel': float('%.3e' % rel.max()),
+                                                           'mean_rel': float('%.3e' % rel.mean()),
+                                                           'values_changed': int((got != ref).sum())}
+        off += n
+    # 5 folds of 2 images per class held out: WND5 on Fisher weights (15% kept)
+    preds = {}
+    for name in ('fp64', 'fp32', 'bf16x3'):
+        f = feats[name]
+        out = []
+        for k in range(5):
+            test = np.r_[2 * k:2 * k + 2, 10 + 2 * k:10 + 2 * k + 2]
+            train = np.setdiff1d(np.arange(20), test)
+            tr, mins, maxs = wndcharm.normalize_train(f[train])
+            w = wndcharm.fisher_weights(tr, cls[train], 2, 0.15)
+            te = np.stack([wndcharm.normalize_test(f[t], mins, maxs) for t in test])
+            out += [int(p) for p in wndcharm.wnd5(tr, cls[train], 2, w, te)[2]]
+        preds[name] = out
+    line = {
+        'metric': 'feature vectors/sec, batched-mixed vs plain GPU (basis contractions)',
+        'value': round(forms['bf16x3_batched_1000'], 3), 'unit': 'feature vectors/s', 'n_gpus': 1,
+        'steps': args.steps, 'higher_is_better': True, 'dtype': 'bf16x3 (fp32 accumulation)',
+        'data': 'synthetic (seeded grating/blob/Poisson generator, 128x128 uint16)',
+        'config': {'workload': 'Chebyshev-20, Zernike-15, Chebyshev-Fourier-23 and Gabor-7 contractions of '
+                               '128x128 images (143 features)', 'images': len(imgs)},
+        'images_per_s': {k: round(v, 2) for k, v in forms.items()},
+        'vs_plain_per_image': {k: round(v / forms['fp64_per_image'], 2) for k, v in forms.items()},
+        'rel_error_vs_fp64': errors,
+        'wnd_predictions_equal': {k: preds[k] == preds['fp64'] for k in ('fp32', 'bf16x3')},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -103,6 +188,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--host-io', action='store_true',
                     help='only time wndgpu_compute with host image and feature buffers (PCIe-inclusive rate)')
+    ap.add_argument('--mixed', action='store_true',
+                    help='the mixed-precision configuration (FP64 / FP32 / BF16x3 basis contractions)')
     ap.add_argument('--copy-probe', action='store_true',
                     help='only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
@@ -114,6 +201,9 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({'copy_probe_bytes': src.numel() * 8, 'launches': 3}))
+        return
+    if args.mixed:
+        mixed_main(args)
         return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs and reports a single MI355X '

@@ -98,6 +98,20 @@ int wndgpu_plan_from_groups(const char *const *group_names, int n_groups, wndgpu
 void wndgpu_plan_destroy(wndgpu_plan *plan);
 int wndgpu_plan_n_features(const wndgpu_plan *plan);
 const char *wndgpu_plan_feature_name(const wndgpu_plan *plan, int column);
+/* Arithmetic of the basis contractions of "Chebyshev Coefficients",
+ * "Zernike Coefficients", "Chebyshev-Fourier Coefficients" and "Gabor
+ * Textures" (the mixed-precision form; no counterpart in the reference,
+ * which computes in FP64): WNDGPU_FP64 (default, the plain form),
+ * WNDGPU_FP32, or WNDGPU_BF16X3.  BF16X3 splits each operand of the
+ * GEMM-shaped contractions (Chebyshev, Gabor) into bf16 hi + lo and takes
+ * hi*hi + hi*lo + lo*hi on the bf16 matrix cores with FP32 accumulation;
+ * Zernike and Chebyshev-Fourier accumulate in FP32 in both reduced forms
+ * (Zernike's radial polynomials stay FP64).  Histogram and Otsu
+ * post-processing stays FP64; its counts can move where a value lands on a
+ * bin edge.  Set the precision on a plan of your own
+ * (wndgpu_plan_from_groups): the standard plans are meant to stay plain. */
+enum { WNDGPU_FP64 = 0, WNDGPU_FP32 = 1, WNDGPU_BF16X3 = 2 };
+int wndgpu_plan_set_precision(wndgpu_plan *plan, int precision);
 /* column offset of a feature group in the plan, or -1 */
 int wndgpu_plan_group_offset(const wndgpu_plan *plan, const char *group_name);
 

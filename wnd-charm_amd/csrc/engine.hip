@@ -84,6 +84,9 @@ struct wndgpu_plan {
   std::vector<Group> groups;
   int n_features = 0;
   bool color = false;  // some group reads the colour plane (RGB images only)
+  // arithmetic of the basis contractions of Chebyshev Coefficients and the
+  // Gabor filter bank: 0 FP64 (the plain form), 1 FP32, 2 BF16x3 on MFMA
+  int precision = 0;
   std::vector<std::string> names;
   std::vector<std::vector<int>> nodes;  // DAG nodes, parents before children
 };
@@ -235,6 +238,7 @@ struct wndgpu_ctx {
   std::map<int, DevTable> comb;
   DevTable gabor, zern, rtrig;
   DevTable color_lut;  // fuzzy colour class of every (h, s, v) byte triple, built on first use
+  DevTable gabor_f32, gabor_bhi, gabor_blo;  // Gabor kernels for the reduced-precision forms
   std::vector<std::pair<std::string, double>> times;
   // per-launch timing (WNDGPU_TIMING): an event pair around every launch,
   // read after the compute call has synchronised
@@ -838,7 +842,8 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
         double *part = A.dalloc_n<double>((size_t)ne * (4 + nchunk * 144));
         const double *ht = (const double *)c->zern.p;
-        push(gn, [=]() { launch_zernike(ni, part, nchunk, ht, o, ne, s); });
+        const int prec = plan->precision;
+        push(gn, [=]() { launch_zernike(ni, part, nchunk, ht, o, ne, s, prec); });
       } break;
       case A_CHEBYSHEV: {
         std::vector<GemmDesc> g1(ne), g2(ne);
@@ -862,9 +867,15 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         }
         GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
         Img *vd = A.stage(vals);
+        const int prec = plan->precision;
         push(gn, [=]() {
-          launch_cheb_gemm(d1, ne, maxH, maxN, s);
-          launch_cheb_gemm(d2, ne, maxN, maxN, s);
+          if (prec) {
+            launch_gemm_mixed(prec, d1, ne, maxH, maxN, s);
+            launch_gemm_mixed(prec, d2, ne, maxN, maxN, s);
+          } else {
+            launch_cheb_gemm(d1, ne, maxH, maxN, s);
+            launch_cheb_gemm(d2, ne, maxN, maxN, s);
+          }
           launch_hist_values(vd, 32, o, ne, s);
         });
       } break;
@@ -947,9 +958,10 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         CfDesc *cdd = A.stage(cd);
         DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
         const int nds = (int)ds.size();
+        const int prec = plan->precision;
         push(gn, [=]() {
           if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
-          launch_cf(grpd, ngrp, cdd, ne, o, s);
+          launch_cf(grpd, ngrp, cdd, ne, o, s, prec);
         });
       } break;
       case A_EDGE: {
@@ -1003,7 +1015,12 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         for (int k = 0; k < ne; k++) en[k] = A.dalloc_n<double>(8 * (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i]);
         double **end_ = A.stage(en);
         const double *gk = (const double *)c->gabor.p;
-        push(gn, [=]() { launch_gabor(ni, gk, end_, o, ne, mh, mw, s); });
+        const int prec = plan->precision;
+        const void *k0 = prec == 1 ? c->gabor_f32.p : c->gabor_bhi.p, *k1 = c->gabor_blo.p;
+        push(gn, [=]() {
+          if (prec) launch_gabor_mixed(prec, ni, k0, k1, end_, o, ne, mh, mw, s);
+          else launch_gabor(ni, gk, end_, o, ne, mh, mw, s);
+        });
       } break;
       default: break;
     }
@@ -1059,6 +1076,45 @@ static int ensure_tables(wndgpu_ctx *c) {
   return 0;
 }
 
+// float -> bf16 bits, round to nearest even
+static unsigned short host_bf16(float f) {
+  unsigned u;
+  memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// Gabor kernels for the reduced-precision forms: FP32 (re, im) as the FP64
+// table; BF16x3 as hi and lo planes [j][i-block of 16][n = 2 f + part][16],
+// taps 38..47 zero
+static int ensure_mixed_tables(wndgpu_ctx *c) {
+  if (c->gabor_f32.p) return 0;
+  const std::vector<double> g = gabor_kernels();
+  const int N = 38;
+  std::vector<float> f32(g.begin(), g.end());
+  std::vector<unsigned short> hi((size_t)N * 3 * 16 * 16, 0), lo(hi.size(), 0);
+  for (int j = 0; j < N; j++)
+    for (int ib = 0; ib < 3; ib++)
+      for (int n = 0; n < 16; n++)
+        for (int k = 0; k < 16; k++) {
+          const int i = 16 * ib + k;
+          if (i >= N) continue;
+          const float w = (float)g[(size_t)(n >> 1) * N * N * 2 + 2 * ((size_t)j * N + i) + (n & 1)];
+          const unsigned short h = host_bf16(w);
+          const unsigned hb = (unsigned)h << 16;
+          float hf;
+          memcpy(&hf, &hb, 4);
+          const size_t o = (((size_t)(j * 3 + ib) * 16 + n) * 16) + k;
+          hi[o] = h;
+          lo[o] = host_bf16(w - hf);
+        }
+  int r;
+  if ((r = upload(c, c->gabor_f32, f32.data(), f32.size() * sizeof(float)))) return r;
+  if ((r = upload(c, c->gabor_bhi, hi.data(), hi.size() * 2))) return r;
+  if ((r = upload(c, c->gabor_blo, lo.data(), lo.size() * 2))) return r;
+  return 0;
+}
+
 // the colour class of every (h, s, v) byte triple (16 MiB), computed on the
 // device once per context
 static int ensure_color_lut(wndgpu_ctx *c) {
@@ -1074,6 +1130,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   int r;
   if ((r = ensure_tables(c))) return r;
   if (plan->color && (r = ensure_color_lut(c))) return r;
+  if (plan->precision && (r = ensure_mixed_tables(c))) return r;
   std::vector<LaunchItem> launches;
   size_t first_feature = 0;
   // pass 1: sizes
@@ -1185,6 +1242,7 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   for (auto &kv : c->dspos) fr(kv.second);
   for (auto &kv : c->comb) fr(kv.second);
   fr(c->gabor); fr(c->zern); fr(c->rtrig); fr(c->color_lut);
+  fr(c->gabor_f32); fr(c->gabor_bhi); fr(c->gabor_blo);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->dev) hipFree(c->dev);
   if (c->hdesc) hipFree(c->hdesc);
@@ -1226,6 +1284,12 @@ int wndgpu_plan_from_groups(const char *const *names, int ng, wndgpu_plan **plan
 
 void wndgpu_plan_destroy(wndgpu_plan *p) { delete p; }
 int wndgpu_plan_n_features(const wndgpu_plan *p) { return p ? p->n_features : -1; }
+
+int wndgpu_plan_set_precision(wndgpu_plan *p, int precision) {
+  if (!p || precision < WNDGPU_FP64 || precision > WNDGPU_BF16X3) return WNDGPU_E_ARG;
+  p->precision = precision;
+  return WNDGPU_OK;
+}
 const char *wndgpu_plan_feature_name(const wndgpu_plan *p, int col) {
   if (!p || col < 0 || col >= p->n_features) return "";
   return p->names[col].c_str();

@@ -1,0 +1,269 @@
+// Reduced-precision forms of the basis contractions (the mixed-precision
+// configuration): the Gabor filter bank (GaborEnergy's conv2comp,
+// src/textures/gabor.cpp:19-52, 166-195) and the Chebyshev coefficient
+// contractions (Chebyshev2D, src/transforms/chebyshev.cpp:118-165) in
+//   * FP32: the FP64 kernels' tiling with FP32 fused multiply-adds;
+//   * BF16x3: each operand split into bf16 hi + lo (x = hi + lo to ~16 bits),
+//     the product taken as hi*hi + hi*lo + lo*hi on the bf16 matrix cores
+//     (v_mfma_f32_16x16x16_bf16) with FP32 accumulation.
+// Inputs and outputs stay FP64 planes; the energies / coefficients go on to
+// the same FP64 post-processing as the plain form.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace wnd {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16, round to nearest even (finite inputs)
+__device__ __forceinline__ unsigned bf16_bits(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+// hi | lo << 16 with f ~= hi + lo
+__device__ __forceinline__ unsigned bf16_split(float f) {
+  const unsigned hi = bf16_bits(f);
+  const float rest = f - __uint_as_float(hi << 16);
+  return hi | (bf16_bits(rest) << 16);
+}
+
+// ---------------------------------------------------------------- Gabor FP32
+#define GM_N 38
+#define GM_TR 32
+#define GM_TC 64
+#define GM_LD (GM_TC + GM_N - 1 + 1)
+
+// the FP64 kernel's layout (32 x 64 outputs of one filter per block, 8
+// outputs per thread), in FP32 with fused multiply-adds
+__global__ void __launch_bounds__(256) k_gabor_conv_f32(const Img *imgs, const float *kernels, double *const *energy) {
+  __shared__ float tile[GM_TR + GM_N - 1][GM_LD];
+  __shared__ float2 wk[GM_N * GM_N];
+  const Img im = imgs[blockIdx.z];
+  const int f = blockIdx.y % 8;
+  const int tilesx = (im.w + GM_TC - 1) / GM_TC, tilesy = (im.h + GM_TR - 1) / GM_TR;
+  if ((int)blockIdx.x >= tilesx * tilesy) return;
+  const int b0 = (blockIdx.x / tilesx) * GM_TR, a0 = (blockIdx.x % tilesx) * GM_TC;
+  const int t = threadIdx.x;
+  const float *K = kernels + (long)f * GM_N * GM_N * 2;
+  for (int e = t; e < GM_N * GM_N; e += 256) wk[e] = make_float2(K[2 * e], K[2 * e + 1]);
+  for (int r = t / 64; r < GM_TR + GM_N - 1; r += 4) {
+    const int y = b0 - 18 + r;
+    for (int c = t % 64; c < GM_TC + GM_N - 1; c += 64) {
+      const int x = a0 - 18 + c;
+      tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? (float)im.p[(long)y * im.w + x] : 0.0f;
+    }
+  }
+  __syncthreads();
+  const int ty = t / 8, tx = t % 8;
+  const int b = b0 + ty, a = a0 + tx * 8;
+  float re[8], imv[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) re[q] = imv[q] = 0.0f;
+  const int rb = ty + 37, cb = tx * 8 + 37;
+  for (int j = 0; j < GM_N; j++) {
+    const float *row = &tile[rb - j][0];
+    const float2 *wr = &wk[j * GM_N];
+#pragma unroll
+    for (int i0 = 0; i0 < GM_N; i0 += 8) {
+      const int D = (GM_N - i0 < 8) ? GM_N - i0 : 8;
+      float u[15];
+#pragma unroll
+      for (int k = 0; k < 15; k++) u[k] = (k >= 8 - D) ? row[cb - i0 - 7 + k] : 0.0f;
+#pragma unroll
+      for (int di = 0; di < 8; di++) {
+        if (i0 + di < GM_N) {
+          const float2 w = wr[i0 + di];
+#pragma unroll
+          for (int q = 0; q < 8; q++) {
+            const float v = u[q - di + 7];
+            re[q] = fmaf(v, w.x, re[q]);
+            imv[q] = fmaf(v, w.y, imv[q]);
+          }
+        }
+      }
+    }
+  }
+  double *E = energy[blockIdx.z] + (long)f * im.h * im.w;
+  if (b < im.h)
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (a + q < im.w) E[(long)b * im.w + a + q] = (double)sqrtf(re[q] * re[q] + imv[q] * imv[q]);
+}
+
+// ---------------------------------------------------------------- Gabor BF16x3
+// Implicit GEMM: M = 16 consecutive output pixels of a row, N = 16 = the 8
+// filters' real and imaginary parts, K = the taps of one kernel row, padded
+// from 38 to 48 with zero weights.  E(b, a) = |sum_{j,i} I(b+19-j, a+19-i) w(j,i)|.
+// Block = 16 rows x 32 columns of outputs, all 8 filters; wave w owns rows
+// 4w..4w+3 (8 M-tiles).  The input tile holds each pixel as bf16 hi | lo in
+// one LDS word, from 28 columns left of the block (taps up to 47) to 18
+// right; weights (hi and lo planes, [j][i-block][n][16]) come from L2.
+#define GX_TR 16
+#define GX_TC 32
+#define GX_W (GX_TC + 47)      // tile columns: x from a0 - 28 to a0 + 50
+#define GX_LD (GX_W + 1)
+#define GX_H (GX_TR + GM_N - 1)
+
+__global__ void __launch_bounds__(256) k_gabor_conv_bf16x3(const Img *imgs, const unsigned short *w_hi,
+                                                           const unsigned short *w_lo, double *const *energy) {
+  __shared__ unsigned tile[GX_H][GX_LD];
+  const Img im = imgs[blockIdx.z];
+  const int tilesx = (im.w + GX_TC - 1) / GX_TC, tilesy = (im.h + GX_TR - 1) / GX_TR;
+  if ((int)blockIdx.x >= tilesx * tilesy) return;
+  const int b0 = (blockIdx.x / tilesx) * GX_TR, a0 = (blockIdx.x % tilesx) * GX_TC;
+  const int t = threadIdx.x;
+  for (int e = t; e < GX_H * GX_W; e += 256) {
+    const int r = e / GX_W, c = e - r * GX_W;
+    const int y = b0 - 18 + r, x = a0 - 28 + c;
+    const float v = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? (float)im.p[(long)y * im.w + x] : 0.0f;
+    tile[r][c] = bf16_split(v);
+  }
+  __syncthreads();
+  const int wave = t >> 6, lane = t & 63;
+  const int m = lane & 15, g = lane >> 4;
+  f32x4 acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) acc[q] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int j = 0; j < GM_N; j++) {
+#pragma unroll
+    for (int ib = 0; ib < 3; ib++) {
+      // B: rows k = 4g..4g+3 of column n = m
+      const long wo = (((long)(j * 3 + ib) * 16 + m) * 16) + 4 * g;
+      const uint2 bh = *(const uint2 *)(w_hi + wo), bl = *(const uint2 *)(w_lo + wo);
+      bf16x4 Bh, Bl;
+      Bh[0] = (short)(bh.x & 0xffff); Bh[1] = (short)(bh.x >> 16); Bh[2] = (short)(bh.y & 0xffff); Bh[3] = (short)(bh.y >> 16);
+      Bl[0] = (short)(bl.x & 0xffff); Bl[1] = (short)(bl.x >> 16); Bl[2] = (short)(bl.y & 0xffff); Bl[3] = (short)(bl.y >> 16);
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int rr = wave * 4 + (q >> 1), seg = q & 1;
+        // A: row m of the M-tile, taps i = 16 ib + 4g + u: tile column 16 seg + m + 47 - i
+        const unsigned *src = &tile[rr + 37 - j][16 * seg + m + 47 - 16 * ib - 4 * g];
+        bf16x4 Ah, Al;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const unsigned v = src[-u];
+          Ah[u] = (short)(v & 0xffff);
+          Al[u] = (short)(v >> 16);
+        }
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, acc[q], 0, 0, 0);
+      }
+    }
+  }
+  // D[i][n]: lane holds n = m, pixels i = 4g..4g+3; the partner lane m ^ 1
+  // holds the other part of the same filter
+  double *E0 = energy[blockIdx.z];
+  const long plane = (long)im.h * im.w;
+  const int f = m >> 1;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int b = b0 + wave * 4 + (q >> 1);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const float mine = acc[q][r];
+      const float other = __shfl_xor(mine, 1);
+      const int a = a0 + 16 * (q & 1) + 4 * g + r;
+      if (!(m & 1) && b < im.h && a < im.w) E0[f * plane + (long)b * im.w + a] = (double)sqrtf(mine * mine + other * other);
+    }
+  }
+}
+
+void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const void *k1, double *const *energy,
+                        OutRef o, int nimg, int maxh, int maxw, hipStream_t s) {
+  if (precision == 1) {
+    const int tiles = ((maxw + GM_TC - 1) / GM_TC) * ((maxh + GM_TR - 1) / GM_TR);
+    k_gabor_conv_f32<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, (const float *)k0, energy);
+  } else {
+    const int tiles = ((maxw + GX_TC - 1) / GX_TC) * ((maxh + GX_TR - 1) / GX_TR);
+    k_gabor_conv_bf16x3<<<dim3(tiles, 1, nimg), 256, 0, s>>>(imgs, (const unsigned short *)k0,
+                                                             (const unsigned short *)k1, energy);
+  }
+  launch_gabor_post(imgs, energy, o, nimg, s);
+}
+
+// ---------------------------------------------------------------- Chebyshev GEMMs
+// C[r][c] = sum_a A[r*sar + a*sak] B[a*N + c] / 2
+__global__ void __launch_bounds__(256) k_gemm_f32(const GemmDesc *ds) {
+  const GemmDesc d = ds[blockIdx.z];
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  if (r0 >= d.M || c0 >= d.N) return;
+  __shared__ float As[16][65];
+  __shared__ float Bs[16][65];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
+  for (int k0 = 0; k0 < d.K; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, rr = e & 63;
+      const int k = k0 + kk, r = r0 + rr, c = c0 + rr;
+      As[kk][rr] = (k < d.K && r < d.M) ? (float)d.A[(long)r * d.sar + (long)k * d.sak] : 0.f;
+      Bs[kk][rr] = (k < d.K && c < d.N) ? (float)d.B[(long)k * d.N + c] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; kk++) {
+      float av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
+      if (r < d.M && c < d.N) d.C[(long)r * d.N + c] = (double)(acc[i][j] * 0.5f);
+    }
+}
+
+// one wavefront per 16 x 16 tile of C, K in steps of 16 on the bf16 matrix cores
+__global__ void __launch_bounds__(64) k_gemm_bf16x3(const GemmDesc *ds) {
+  const GemmDesc d = ds[blockIdx.z];
+  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
+  if (r0 >= d.M || c0 >= d.N) return;
+  const int lane = threadIdx.x, m = lane & 15, g = lane >> 4;
+  const int r = r0 + m, c = c0 + m;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < d.K; k0 += 16) {
+    bf16x4 Ah, Al, Bh, Bl;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k = k0 + 4 * g + u;
+      const float a = (k < d.K && r < d.M) ? (float)d.A[(long)r * d.sar + (long)k * d.sak] : 0.f;
+      const float b = (k < d.K && c < d.N) ? (float)d.B[(long)k * d.N + c] : 0.f;
+      const unsigned sa = bf16_split(a), sb = bf16_split(b);
+      Ah[u] = (short)(sa & 0xffff);
+      Al[u] = (short)(sa >> 16);
+      Bh[u] = (short)(sb & 0xffff);
+      Bl[u] = (short)(sb >> 16);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int rr = r0 + 4 * g + u;
+    if (rr < d.M && c < d.N) d.C[(long)rr * d.N + c] = (double)(acc[u] * 0.5f);
+  }
+}
+
+void launch_gemm_mixed(int precision, const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s) {
+  if (precision == 1) k_gemm_f32<<<dim3((maxN + 63) / 64, (maxM + 63) / 64, nimg), 256, 0, s>>>(d);
+  else k_gemm_bf16x3<<<dim3((maxN + 15) / 16, (maxM + 15) / 16, nimg), 64, 0, s>>>(d);
+}
+
+}  // namespace wnd

@@ -449,5 +449,8 @@ void launch_gabor(const Img *imgs, const double *kernels, double *const *energy,
   k_gabor_conv<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, kernels, energy);
   k_gabor_post<<<nimg, 1024, 0, s>>>(imgs, energy, o);
 }
+void launch_gabor_post(const Img *imgs, double *const *energy, OutRef o, int nimg, hipStream_t s) {
+  k_gabor_post<<<nimg, 1024, 0, s>>>(imgs, energy, o);
+}
 
 }  // namespace wnd

@@ -47,7 +47,9 @@ __global__ void __launch_bounds__(1024) k_zern_moments(const Img *imgs, double *
 // One wavefront per group of orders [N0, N1] (groups balanced by work),
 // lanes over pixels; the group's accumulators stay in registers (every loop
 // is unrolled at compile time) and are reduced across lanes at the end.
-template <int N0, int N1>
+// T: the accumulators' type (double for the plain form; float for the
+// reduced-precision forms, where the radial polynomials stay FP64).
+template <int N0, int N1, class T>
 __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int lane, double cxm, double cym, double sum,
                                            const double *H1, const double *H2, const double *H3, double *dst) {
   constexpr int NP = [] {
@@ -57,9 +59,9 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
   }();
   const double rad = (double)(im.w < im.h ? im.w : im.h);
   const double PI = 3.14159265358979323846264338328;
-  double AR[NP], AI[NP];
+  T AR[NP], AI[NP];
 #pragma unroll
-  for (int k = 0; k < NP; k++) AR[k] = AI[k] = 0.0;
+  for (int k = 0; k < NP; k++) AR[k] = AI[k] = (T)0;
   for (long p = p0 + lane; p < p1; p += 64) {
     const int j = p / im.w, i = p - (long)j * im.w;  // i: column, j: row
     const double x = (i + 1 - cxm) / rad;
@@ -96,8 +98,8 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
           Rnmp4 = Rnmp2;
           Rnmp2 = Rnm;
         }
-        AR[base + (m >> 1)] += const_t * Rnm * COST[m];
-        AI[base + (m >> 1)] -= const_t * Rnm * SINT[m];
+        AR[base + (m >> 1)] += (T)(const_t * Rnm * COST[m]);
+        AI[base + (m >> 1)] -= (T)(const_t * Rnm * SINT[m]);
       }
       base += n / 2 + 1;
     }
@@ -105,7 +107,7 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
   const int off = zoff(N0);
 #pragma unroll
   for (int k = 0; k < NP; k++) {
-    double vr = AR[k], vi = AI[k];
+    T vr = AR[k], vi = AI[k];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       vr += __shfl_xor(vr, o, 64);
@@ -119,6 +121,7 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
 }
 
 // block = one pixel chunk of one image; wave w handles order group w
+template <class T>
 __global__ void __launch_bounds__(256) k_zern_accum(const Img *imgs, double *partial, int nchunk, const double *htab) {
   const Img im = imgs[blockIdx.y];
   const long P = (long)im.h * im.w;
@@ -130,10 +133,10 @@ __global__ void __launch_bounds__(256) k_zern_accum(const Img *imgs, double *par
   double *dst = hdr + 4 + (long)blockIdx.x * 2 * ZPAIRS;
   const int lane = threadIdx.x & 63;
   switch (threadIdx.x >> 6) {
-    case 0: zern_group<0, 7>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
-    case 1: zern_group<8, 10>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
-    case 2: zern_group<11, 13>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
-    default: zern_group<14, 15>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    case 0: zern_group<0, 7, T>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    case 1: zern_group<8, 10, T>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    case 2: zern_group<11, 13, T>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
+    default: zern_group<14, 15, T>(im, p0, p1, lane, cxm, cym, sum, H1, H2, H3, dst); break;
   }
 }
 
@@ -152,9 +155,10 @@ __global__ void k_zern_final(const Img *imgs, const double *partial, int nchunk,
 }
 
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
-                    hipStream_t s) {
+                    hipStream_t s, int precision) {
   k_zern_moments<<<nimg, 1024, 0, s>>>(imgs, partial, nchunk);
-  k_zern_accum<<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
+  if (precision) k_zern_accum<float><<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
+  else k_zern_accum<double><<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
   k_zern_final<<<nimg, 128, 0, s>>>(imgs, partial, nchunk, o);
 }
 
@@ -166,14 +170,20 @@ void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *
 constexpr int CF_CHUNK = 64;
 constexpr int CF_NN = 23;
 
+// acc + a * b: unfused in FP64 (the reference's rounding), fused in FP32
+__device__ __forceinline__ double mac(double acc, double a, double b) { return acc + a * b; }
+__device__ __forceinline__ float mac(float acc, float a, float b) { return fmaf(a, b, acc); }
+
 // Block = one group of entries sharing the basis tables.  Each thread owns
 // up to five of the 2*NN*NN coefficients and keeps one running sum per
 // coefficient and entry across all pixel chunks, so every sum is taken in
 // the reference's pixel order (ChebyshevFourier.cpp:112-135); the tables
 // of a chunk are staged in LDS once for all entries of the group.
+// T: the arithmetic (double, the plain form; float, the reduced forms).
+template <class T>
 __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
-  __shared__ double tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN];
-  __shared__ double iv[CF_GROUP][CF_CHUNK];
+  __shared__ T tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN];
+  __shared__ T iv[CF_GROUP][CF_CHUNK];
   const CfGroup G = groups[blockIdx.x];
   const int NN = G.NN, NN2 = NN * NN, ne = G.ne;
   const int t = threadIdx.x;
@@ -189,11 +199,11 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
     oim[q] = on[q] ? k / NN : 0;
     oa[q] = on[q] ? k - (k / NN) * NN : 0;
   }
-  double acc[CF_GROUP][5];
+  T acc[CF_GROUP][5];
 #pragma unroll
   for (int g = 0; g < CF_GROUP; g++)
 #pragma unroll
-    for (int q = 0; q < 5; q++) acc[g][q] = 0.0;
+    for (int q = 0; q < 5; q++) acc[g][q] = (T)0;
   // the next chunk's table entries and pixels are fetched into registers
   // while the current chunk is consumed from LDS
   constexpr int CF_PER = (CF_CHUNK * CF_NN + 255) / 256;
@@ -222,14 +232,14 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
       const int e = t + 256 * k;
       if (e < np * NN) {
         const int p = e / NN, a = e - p * NN;
-        tn[p][a] = rt[k];
-        fc[p][a] = rc[k];
-        fs[p][a] = rs[k];
+        tn[p][a] = (T)rt[k];
+        fc[p][a] = (T)rc[k];
+        fs[p][a] = (T)rs[k];
       }
     }
     if (t < np) {
 #pragma unroll
-      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = rv[g];
+      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = (T)rv[g];
     }
     __syncthreads();
     if (p0 + CF_CHUNK < G.np) fetch(p0 + CF_CHUNK);
@@ -237,10 +247,10 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
 #pragma unroll
       for (int q = 0; q < 5; q++) {
         if (!on[q]) continue;
-        const double f = ore[q] ? fc[p][oim[q]] : fs[p][oim[q]];
-        const double ft = f * tn[p][oa[q]];
+        const T f = ore[q] ? fc[p][oim[q]] : fs[p][oim[q]];
+        const T ft = f * tn[p][oa[q]];
 #pragma unroll
-        for (int g = 0; g < CF_GROUP; g++) acc[g][q] += ft * iv[g][p];
+        for (int g = 0; g < CF_GROUP; g++) acc[g][q] = mac(acc[g][q], ft, iv[g][p]);
       }
     }
   }
@@ -249,7 +259,7 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
     if (g < ne)
 #pragma unroll
       for (int q = 0; q < 5; q++)
-        if (on[q]) G.sum[g][t + 256 * q] = acc[g][q];
+        if (on[q]) G.sum[g][t + 256 * q] = (double)acc[g][q];
 }
 
 __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) {
@@ -278,8 +288,10 @@ __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) 
   if (threadIdx.x < 32) o.rows[blockIdx.x][threadIdx.x] = bins[threadIdx.x];
 }
 
-void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s) {
-  k_cf_accum<<<ngroups, 256, 0, s>>>(groups);
+void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s,
+               int precision) {
+  if (precision) k_cf_accum<float><<<ngroups, 256, 0, s>>>(groups);
+  else k_cf_accum<double><<<ngroups, 256, 0, s>>>(groups);
   k_cf_final<<<nimg, 1024, 0, s>>>(cd, o);
 }
 

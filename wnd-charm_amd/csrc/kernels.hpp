@@ -147,11 +147,17 @@ void launch_objects(const Img *imgs, const Stats *st, int invert, const ObjDesc 
 void launch_prewitt(const Img *imgs, const Img *mag, const Img *dir, int nimg, long maxn, hipStream_t s);
 void launch_edge_final(const Img *mag, const Img *dir, const Stats *st2, const double *medians, OutRef o, int nimg,
                        hipStream_t s);
+void launch_gabor_post(const Img *imgs, double *const *energy, OutRef o, int nimg, hipStream_t s);
+// reduced-precision contractions (k_mixed.hip): precision 1 = FP32, 2 = BF16x3 on MFMA.
+// Gabor: FP32 takes float kernels (k0); BF16x3 takes the hi / lo bf16 planes (k0, k1)
+void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const void *k1, double *const *energy,
+                        OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
+void launch_gemm_mixed(int precision, const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s);
 void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
                   int maxw, hipStream_t s);
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
-                    hipStream_t s);
-void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s);
+                    hipStream_t s, int precision = 0);
+void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s, int precision = 0);
 void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
 void launch_downsample_hsv(const HsvDsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
 void launch_normalize(NormDesc *nd, int nimg, long maxn, hipStream_t s);

@@ -21,7 +21,7 @@ __all__ = ['ImageMatrix', 'PyImageMatrix', 'DoubleVector', 'FeatureComputationPl
            'FeatureComputationPlanExecutor', 'ImageTransform', 'FeatureAlgorithm', 'compute_features',
            'normalize_train', 'normalize_test', 'fisher_weights', 'wnd5', 'WndGpuError', 'library_path',
            'read_tiff', 'write_sig', 'read_sig', 'write_fit', 'read_fit', 'write_weights', 'read_weights',
-           'FeatureTable', 'Sampling']
+           'FeatureTable', 'Sampling', 'FP64', 'FP32', 'BF16X3']
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.environ.get('WNDGPU_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libwndgpu.so'))
@@ -29,6 +29,9 @@ library_path = os.environ.get('WNDGPU_LIB', os.path.join(os.path.dirname(_HERE),
 
 class WndGpuError(RuntimeError):
     pass
+
+
+FP64, FP32, BF16X3 = 0, 1, 2  # precisions of the basis contractions (FeatureComputationPlan.set_precision)
 
 
 _P = ctypes.c_void_p
@@ -59,6 +62,7 @@ def _load():
         'wndgpu_plan_n_features': (_I, [_P]),
         'wndgpu_plan_feature_name': (ctypes.c_char_p, [_P, _I]),
         'wndgpu_plan_group_offset': (_I, [_P, ctypes.c_char_p]),
+        'wndgpu_plan_set_precision': (_I, [_P, _I]),
         'wndgpu_compute': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P]),
         'wndgpu_compute_device': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P]),
         'wndgpu_transform': (_I, [_P, ctypes.c_char_p, _P, _I, _I, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
@@ -273,6 +277,17 @@ class FeatureComputationPlan(object):
 
     def getFGoffset(self, group_name):
         return _load().wndgpu_plan_group_offset(self.handle, group_name.encode())
+
+    def set_precision(self, precision):
+        """FP64 (0, the plain form), FP32 (1) or BF16X3 (2) for the basis contractions
+        of Chebyshev, Zernike, Chebyshev-Fourier Coefficients and Gabor Textures
+        (BF16X3 on the matrix cores for the GEMM-shaped Chebyshev and Gabor,
+        FP32 for the other two); the standard plans are shared, so set it on a
+        plan of your own"""
+        if self._handle is not None and any(self is p for p in StdFeatureComputationPlans._cache.values()):
+            raise WndGpuError('set the precision on a plan of your own, not a shared standard plan')
+        _check(_load().wndgpu_plan_set_precision(self.handle, int(precision)), 'set_precision')
+        self.precision = int(precision)
 
 
 class StdFeatureComputationPlans(object):
