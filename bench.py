@@ -19,6 +19,17 @@ Also reported:
   cpu_baseline  the FP64 C++ oracle, one image per thread on the host
                 cores, on a bounded sample of the same images.
 
+forms         the metric's comparison: the same batch with the basis
+                contractions in BF16x3 (batched-mixed; with how many
+                columns moved against FP64), and the plain form, one call
+                per image, on the first 40 images.
+
+--tiles N: the uneven-tiles configuration instead -- the first N tiles of
+the 8-class tiles configuration (seed 4004, sides independently uniform in
+[64, 1024]) through the long set in batches of --batch tiles, inputs
+resident in HBM; reported next to the short-set plain per-image form
+(128x128 images, one call per image) on the same GPU.
+
 --mixed: the mixed-precision configuration instead -- the 20 short-set
 128x128 images (2 classes, periods 8 / 20 px, seed 1001) through the
 Chebyshev (order 20), Zernike (order 15, 72 moments), Chebyshev-Fourier
@@ -60,6 +71,25 @@ def hess_flops(n):
         line = 2 * (rem - 1) + 1 + 2 + 3 * (rem - 1)
         f += 3 * (rem - 1) + 4 + rem * line + n * line
     return f
+
+
+def exact_columns(plan):
+    """columns whose features are integer counts or ratios of counts (the
+    histogram, threshold and object features), from the group layout"""
+    exact = {0: range(32), 1: range(32), 4: range(24), 5: (0, 1, 2, 5), 6: range(48), 7: range(12),
+             9: (1, 3, 4), 10: [0] + list(range(1, 14)) + list(range(17, 25)) + [26],
+             11: list(range(0, 14)) + [15, 16, 17] + list(range(18, 29)) + [30, 31, 33],
+             12: list(range(0, 14)) + [15, 16, 17] + list(range(18, 29)) + [30, 31, 33], 13: range(7)}
+    algs = ['Chebyshev-Fourier Coefficients', 'Chebyshev Coefficients', 'Zernike Coefficients', 'Haralick Textures',
+            'Multiscale Histograms', 'Tamura Textures', 'Comb Moments', 'Radon Coefficients', 'Fractal Features',
+            'Pixel Intensity Statistics', 'Edge Features', 'Otsu Object Features', 'Inverse-Otsu Object Features',
+            'Gabor Textures', 'Gini Coefficient']
+    m = np.zeros(plan.n_features, bool)
+    for c, nm in enumerate(plan.feature_names()):
+        g, k = nm.rsplit(' [', 1)
+        a = algs.index(g[:g.index(' (')])
+        m[c] = int(k[:-1]) in exact.get(a, ())
+    return m
 
 
 def cpu_baseline(imgs, seconds=12.0):
@@ -105,6 +135,69 @@ def profiled_traffic(kernel):
 
 MIXED_GROUPS = [('Chebyshev Coefficients ()', 32), ('Zernike Coefficients ()', 72),
                 ('Chebyshev-Fourier Coefficients ()', 32), ('Gabor Textures ()', 7)]
+
+
+def _tile(k):
+    from wndcharm import synthetic
+    base, periods, per_class, _ = synthetic.CONFIGS['tiles_mixed']
+    r = np.random.Generator(np.random.PCG64(base * 1000003 + k))
+    shp = (int(r.integers(64, 1025)), int(r.integers(64, 1025)))
+    return synthetic.image(base + k, periods[k // per_class], shp)
+
+
+def tiles_main(args):
+    import multiprocessing as mp
+    n = args.tiles
+    # the generator is numpy on the host: the tiles are made in parallel
+    # before anything touches the GPU; the class of tile k is k // 1250 as in
+    # the configuration's global order
+    with mp.get_context('fork').Pool(min(16, os.cpu_count() or 1)) as pool:
+        tiles = pool.map(_tile, range(n), chunksize=8)
+    import torch
+    import wndcharm
+    from wndcharm import synthetic
+    pix = sum(t.size for t in tiles)
+    plan = wndcharm.StdFeatureComputationPlans.getFeatureSetLong()
+    nf = plan.n_features
+    lib = wndcharm._load()
+    ctx = wndcharm._context()
+    dev = [torch.from_numpy(t.view(np.int16)).cuda() for t in tiles]
+    out = torch.zeros((args.batch, nf), dtype=torch.float64, device='cuda')
+    status = np.zeros(args.batch, np.int32)
+
+    def run():
+        for b0 in range(0, n, args.batch):
+            chunk = dev[b0:b0 + args.batch]
+            st = (wndcharm._Image * len(chunk))()
+            for i, t in enumerate(chunk):
+                st[i] = wndcharm._Image(ctypes.c_void_p(t.data_ptr()), t.shape[0], t.shape[1], t.shape[1], 0)
+            rc = lib.wndgpu_compute_device(ctx, plan.handle, st, len(chunk), ctypes.c_void_p(out.data_ptr()),
+                                           ctypes.c_void_p(status.ctypes.data))
+            if rc != 0:
+                raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
+            print('tiles %d..%d done' % (b0, b0 + len(chunk) - 1), file=sys.stderr, flush=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    # the short-set plain per-image form on the same GPU, for comparison
+    imgs, _ = synthetic.config_images('short_128')
+    short = wndcharm.StdFeatureComputationPlans.getFeatureSet()
+    wndcharm.compute_features([imgs[0]], short)
+    t1 = time.perf_counter()
+    for im in imgs:
+        wndcharm.compute_features([im], short)
+    plain = len(imgs) / (time.perf_counter() - t1)
+    print(json.dumps({
+        'metric': 'feature vectors/sec (2919-feat), uneven tiles batched', 'value': round(n / el, 3),
+        'unit': 'feature vectors/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f64',
+        'data': 'synthetic (seeded grating/blob/Poisson generator, uint16 tiles, sides uniform in [64, 1024])',
+        'config': {'workload': 'long set over %d uneven tiles, %d per call' % (n, args.batch), 'tiles': n,
+                   'mean_pixels': round(pix / n), 'seconds': round(el, 3)},
+        'megapixels_per_s': round(pix / el / 1e6, 3),
+        'short_set_plain_per_image_128': round(plain, 3)}))
 
 
 def mixed_main(args):
@@ -188,6 +281,8 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--host-io', action='store_true',
                     help='only time wndgpu_compute with host image and feature buffers (PCIe-inclusive rate)')
+    ap.add_argument('--tiles', type=int, default=0,
+                    help='the uneven-tiles configuration over this many tiles (batches of --batch)')
     ap.add_argument('--mixed', action='store_true',
                     help='the mixed-precision configuration (FP64 / FP32 / BF16x3 basis contractions)')
     ap.add_argument('--copy-probe', action='store_true',
@@ -204,6 +299,9 @@ def main():
         return
     if args.mixed:
         mixed_main(args)
+        return
+    if args.tiles:
+        tiles_main(args)
         return
     if args.gpus != 1:
         print('bench.py measures one GPU: this build runs and reports a single MI355X '
@@ -273,6 +371,50 @@ def main():
             'unit': 'TFLOP/s', 'frac': round(achieved / FP64_PEAK_TFLOPS, 5), 'traffic': profiled_traffic(kname),
             'launch_ms': round(launch_ms, 3), 'launches_per_step': per_step, 'flops_per_launch': flops / per_step,
             'matrices_per_launch': 4 * len(tones) / per_step}
+    # the metric's comparison: the batched form with BF16x3 basis contractions,
+    # and the plain per-image form (one call per image) on the first 40 images
+    forms = {'batched_fp64': round(value, 3)}
+    hp = ctypes.c_void_p()
+    lib.wndgpu_plan_standard(1, ctypes.byref(hp))
+    if lib.wndgpu_plan_set_precision(hp, 2) != 0:
+        raise RuntimeError('wndgpu_plan_set_precision failed')
+    mixed = wndcharm.FeatureComputationPlan('long, bf16x3 contractions', _handle=hp)
+    out_m = torch.zeros_like(out)
+
+    def step_mixed():
+        rc = lib.wndgpu_compute_device(ctx, mixed.handle, structs, len(imgs), ctypes.c_void_p(out_m.data_ptr()),
+                                       ctypes.c_void_p(status.ctypes.data))
+        if rc != 0:
+            raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
+
+    os.environ.pop('WNDGPU_TIMING', None)
+    step_mixed()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(2):
+        step_mixed()
+    torch.cuda.synchronize()
+    forms['batched_bf16x3'] = round(2 * len(imgs) / (time.perf_counter() - t0), 3)
+    exact = exact_columns(plan)
+    diff = (out_m != out).cpu().numpy()
+    forms['bf16x3_columns_changed'] = int(diff.any(0).sum())
+    forms['bf16x3_exact_columns_changed'] = int((diff.any(0) & exact).sum())
+    npi = min(40, len(imgs))
+    st1 = np.zeros(1, np.int32)
+
+    def per_image():
+        for i in range(npi):
+            rc = lib.wndgpu_compute_device(ctx, plan.handle, ctypes.byref(structs[i]), 1,
+                                           ctypes.c_void_p(out[i].data_ptr()), ctypes.c_void_p(st1.ctypes.data))
+            if rc != 0:
+                raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
+    per_image()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    per_image()
+    torch.cuda.synchronize()
+    forms['plain_per_image_fp64'] = round(npi / (time.perf_counter() - t0), 3)
+    forms['batched_bf16x3_vs_plain_per_image'] = round(forms['batched_bf16x3'] / forms['plain_per_image_fp64'], 2)
     stage = {}
     for k, v in mean.items():
         key = k if '[' in k else k.split(' (')[0]
@@ -286,6 +428,7 @@ def main():
         'config': {'workload': 'long-set 2919 features, %d images 256x256 per step, plain FP64' % len(imgs),
                    'images_per_step': len(imgs), 'set': 'long', 'precision': 'fp64'},
         'roofline': roof,
+        'forms': forms,
         'stage_ms': {k: round(v, 3) for k, v in top[:14]},
     }
     if not args.no_cpu:

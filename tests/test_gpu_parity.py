@@ -97,6 +97,18 @@ def test_long_set_batch_uneven_sizes():
         compare(got[k], want, mask, 'long set image %d %s' % (k, im.shape))
 
 
+def test_long_set_uneven_tiles():
+    """tiles of the uneven-tiles configuration (sides uniform in [64, 1024]),
+    batched with a square image"""
+    tiles, _ = synthetic.config_images('tiles_mixed', limit=7)
+    imgs = [tiles[6], tiles[2], synthetic.image(2004, 12, (64, 64))]
+    print('long set uneven tiles', [i.shape for i in imgs], flush=True)
+    got = wndcharm.compute_features(imgs, wndcharm.StdFeatureComputationPlans.getFeatureSetLong())
+    mask = exact_mask_for_names(orc_names(True))
+    for k, im in enumerate(imgs):
+        compare(got[k], orc_features(im.astype(np.float64), True), mask, 'tile %d %s' % (k, im.shape))
+
+
 def test_short_set_golden_tile():
     d = load_npz('img_lymphoma_tile.npz')
     print('short set lymphoma tile', d['pixels'].shape, flush=True)

@@ -18,8 +18,11 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
+#include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/wndgpu.h"
@@ -194,11 +197,14 @@ static wndgpu_plan *make_standard(int set) {
 struct DevTable {
   void *p = nullptr;
   size_t bytes = 0;
+  long used = 0;  // last batch that read it
 };
 
 struct CfDev {
   DevTable kk, tn, fc, fs;
   int np = 0, NN = 0;
+  long used = 0;
+  size_t bytes() const { return kk.bytes + tn.bytes + fc.bytes + fs.bytes; }
 };
 
 struct NodeRun {
@@ -238,6 +244,14 @@ struct wndgpu_ctx {
   std::map<int, DevTable> comb;
   DevTable gabor, zern, rtrig;
   DevTable color_lut;  // fuzzy colour class of every (h, s, v) byte triple, built on first use
+  // Size-dependent bases (Chebyshev, Chebyshev-Fourier geometry) take libm
+  // calls on the host per new image size: the sizing pass of a batch lists
+  // the ones it needs, the missing ones are computed on host threads
+  // together, and least recently used ones beyond table_budget bytes are
+  // freed between batches.
+  std::set<std::pair<int, int>> need_cheb, need_cf;
+  long batch_id = 0;
+  size_t table_budget = (size_t)4 << 30;
   DevTable gabor_f32, gabor_bhi, gabor_blo;  // Gabor kernels for the reduced-precision forms
   std::vector<std::pair<std::string, double>> times;
   // per-launch timing (WNDGPU_TIMING): an event pair around every launch,
@@ -338,6 +352,7 @@ static int get_cheb(wndgpu_ctx *c, int len, int N, const double **p) {
     if (r) return r;
     it = c->cheb.emplace(key, t).first;
   }
+  it->second.used = c->batch_id;
   *p = (const double *)it->second.p;
   return 0;
 }
@@ -376,7 +391,86 @@ static int get_cf(wndgpu_ctx *c, int w, int h, CfDev **out) {
     if ((r = upload(c, d.fs, t.fs.data(), t.fs.size() * sizeof(double)))) return r;
     it = c->cf.emplace(key, d).first;
   }
+  it->second.used = c->batch_id;
   *out = &it->second;
+  return 0;
+}
+
+// The size-dependent tables a batch's sizing pass listed: missing ones are
+// computed on up to 16 host threads (glibc libm, as the reference), then
+// uploaded; afterwards tables no batch has read recently are freed while the
+// cache holds more than table_budget bytes.
+static int prepare_tables(wndgpu_ctx *c) {
+  c->batch_id++;
+  std::vector<std::pair<int, int>> cheb_keys, cf_keys;
+  for (const auto &k : c->need_cheb)
+    if (!c->cheb.count(k)) cheb_keys.push_back(k);
+  for (const auto &k : c->need_cf)
+    if (!c->cf.count(k)) cf_keys.push_back(k);
+  const size_t nwork = cheb_keys.size() + cf_keys.size();
+  if (nwork) {
+    std::vector<std::vector<double>> cheb_v(cheb_keys.size());
+    std::vector<CfTable> cf_v(cf_keys.size());
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+      for (size_t j = next++; j < nwork; j = next++) {
+        if (j < cheb_keys.size()) cheb_v[j] = cheb_basis(cheb_keys[j].first, cheb_keys[j].second);
+        else cf_v[j - cheb_keys.size()] = cf_table(cf_keys[j - cheb_keys.size()].first, cf_keys[j - cheb_keys.size()].second);
+      }
+    };
+    const size_t nt = std::min<size_t>(nwork, std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nt; t++) th.emplace_back(worker);
+    worker();
+    for (auto &t : th) t.join();
+    int r;
+    for (size_t j = 0; j < cheb_keys.size(); j++) {
+      DevTable t;
+      if ((r = upload(c, t, cheb_v[j].data(), cheb_v[j].size() * sizeof(double)))) return r;
+      c->cheb.emplace(cheb_keys[j], t);
+    }
+    for (size_t j = 0; j < cf_keys.size(); j++) {
+      const CfTable &t = cf_v[j];
+      CfDev d;
+      d.np = (int)t.kk.size();
+      d.NN = t.NN;
+      if ((r = upload(c, d.kk, t.kk.data(), t.kk.size() * sizeof(int)))) return r;
+      if ((r = upload(c, d.tn, t.tn.data(), t.tn.size() * sizeof(double)))) return r;
+      if ((r = upload(c, d.fc, t.fc.data(), t.fc.size() * sizeof(double)))) return r;
+      if ((r = upload(c, d.fs, t.fs.data(), t.fs.size() * sizeof(double)))) return r;
+      c->cf.emplace(cf_keys[j], d);
+    }
+  }
+  for (const auto &k : c->need_cheb) c->cheb[k].used = c->batch_id;
+  for (const auto &k : c->need_cf) c->cf[k].used = c->batch_id;
+  c->need_cheb.clear();
+  c->need_cf.clear();
+  // eviction: previous batches have completed (every compute call waits for
+  // its streams), so tables this batch does not read can be freed
+  size_t total = 0;
+  for (auto &kv : c->cheb) total += kv.second.bytes;
+  for (auto &kv : c->cf) total += kv.second.bytes();
+  while (total > c->table_budget) {
+    long oldest = c->batch_id;
+    int which = -1;
+    std::pair<int, int> key;
+    for (auto &kv : c->cheb)
+      if (kv.second.used < oldest) { oldest = kv.second.used; which = 0; key = kv.first; }
+    for (auto &kv : c->cf)
+      if (kv.second.used < oldest) { oldest = kv.second.used; which = 1; key = kv.first; }
+    if (which < 0) break;  // everything left is read by this batch
+    if (which == 0) {
+      DevTable &t = c->cheb[key];
+      total -= t.bytes;
+      hipFree(t.p);
+      c->cheb.erase(key);
+    } else {
+      CfDev &d = c->cf[key];
+      total -= d.bytes();
+      for (DevTable *t : {&d.kk, &d.tn, &d.fc, &d.fs}) hipFree(t->p);
+      c->cf.erase(key);
+    }
+  }
   return 0;
 }
 
@@ -670,6 +764,9 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           int r;
           if ((r = get_cheb(c, w, NN, &bx))) return r;
           if ((r = get_cheb(c, h, NN, &by))) return r;
+        } else {
+          c->need_cheb.insert({w, NN});
+          c->need_cheb.insert({h, NN});
         }
         double *c1 = A.dalloc_n<double>((size_t)h * NN);
         g1[i] = GemmDesc{P.planes[i], bx, c1, h, w, NN, (long)w, 1};
@@ -856,6 +953,9 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
             int r;
             if ((r = get_cheb(c, w, NN, &bx))) return r;
             if ((r = get_cheb(c, h, NN, &by))) return r;
+          } else {
+            c->need_cheb.insert({w, NN});
+            c->need_cheb.insert({h, NN});
           }
           double *c1 = A.dalloc_n<double>((size_t)h * NN);
           double *c2 = A.dalloc_n<double>((size_t)NN * NN);
@@ -915,6 +1015,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           C.img = plane;
           C.w = w;
           C.h = h;
+          if (A.sizing) c->need_cf.insert({w, h});
           if (!A.sizing) {
             CfDev *t = nullptr;
             int r;
@@ -1136,6 +1237,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   // pass 1: sizes
   Arena A1{c, true};
   if ((r = build_batch(c, plan, B, A1, launches, &first_feature))) return r;
+  if ((r = prepare_tables(c))) return r;
   if (A1.d + 4096 > c->dev_cap) {
     if (c->dev) hipFree(c->dev);
     c->dev = nullptr;
@@ -1225,6 +1327,7 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     return WNDGPU_E_DEVICE;
   }
   c->serial = getenv("WNDGPU_SERIAL") != nullptr;
+  if (const char *mb = getenv("WNDGPU_TABLE_CACHE_MB")) c->table_budget = (size_t)std::max(0L, atol(mb)) << 20;
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS
   *out = c;
   return WNDGPU_OK;
@@ -1429,17 +1532,25 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
     in_off[k] = in_bytes;
     in_bytes += ((size_t)im.height * im.stride * esz[im.dtype] + 255) & ~(size_t)255;
   }
+  // stream-ordered staging buffers, released on every return
+  struct StreamBuf {
+    hipStream_t s;
+    void *p = nullptr;
+    ~StreamBuf() { if (p) (void)hipFreeAsync(p, s); }
+  } in_buf{c->stream}, out_buf{c->stream};
   char *din = nullptr;
   double *dout = nullptr;
   if (!device_io) {
-    HIPCHK(c, hipMallocAsync((void **)&din, in_bytes, c->stream));
+    HIPCHK(c, hipMallocAsync(&in_buf.p, in_bytes, c->stream));
+    din = (char *)in_buf.p;
     for (size_t k = 0; k < idx.size(); k++) {
       const wndgpu_image &im = images[idx[k]];
       HIPCHK(c, hipMemcpyAsync(din + in_off[k], im.data, (size_t)im.height * im.stride * esz[im.dtype],
                                hipMemcpyHostToDevice, c->stream));
     }
   }
-  HIPCHK(c, hipMallocAsync((void **)&dout, (size_t)B.n * B.nf * sizeof(double), c->stream));
+  HIPCHK(c, hipMallocAsync(&out_buf.p, (size_t)B.n * B.nf * sizeof(double), c->stream));
+  dout = (double *)out_buf.p;
   for (size_t k = 0; k < idx.size(); k++) {
     const wndgpu_image &im = images[idx[k]];
     LoadDesc L;
@@ -1467,7 +1578,12 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   B.smp = std::move(smp);
   B.out = dout;
   int r = run_batch(c, plan, B, getenv("WNDGPU_TIMING") != nullptr);
-  if (r) return r;
+  if (r) {
+    // work already issued on either stream may still read the staging buffers
+    (void)hipStreamSynchronize(c->side);
+    (void)hipStreamSynchronize(c->stream);
+    return r;
+  }
   // scatter accepted rows: image idx[k] owns rows idx[k]*ns .. +ns-1
   const size_t rowb = (size_t)ns * B.nf;
   if (!device_io) {
@@ -1476,14 +1592,12 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t k = 0; k < idx.size(); k++)
       memcpy(out + (size_t)idx[k] * rowb, h.data() + k * rowb, rowb * sizeof(double));
-    hipFreeAsync(din, c->stream);
   } else {
     for (size_t k = 0; k < idx.size(); k++)
       HIPCHK(c, hipMemcpyAsync(out + (size_t)idx[k] * rowb, dout + k * rowb, rowb * sizeof(double),
                                hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
-  hipFreeAsync(dout, c->stream);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (size_t k = 0; k < c->tnames.size(); k++) {
     float ms = 0;
