@@ -77,11 +77,24 @@ CfTable cf_table(int w, int h) {
       for (unsigned long a = 0; a < NN; a++) Tn[a] = cos(a * acos_x);
       Tn[0] = 0.5;
       for (unsigned long a = 0; a < NN; a++) t.tn.push_back(Tn[a]);
+      // Ftrm*cos(mf*phi), Ftrm*sin(-mf*phi) for mf = -N..N.  (-k)*phi is
+      // -(k*phi) exactly and libm's cos is even and sin odd bit for bit, so
+      // each |mf| takes one cos and one sin call.
+      double ck[12], sk[12];
+      for (unsigned long k = 1; k <= N; k++) {
+        ck[k] = cos((double)k * f);
+        sk[k] = sin((double)k * f);
+      }
       for (unsigned long im = 1; im <= NN; im++) {
         const long mf = (long)im - 1 - (long)N;
-        const double Ftrm = (mf == 0) ? 0.5 : 1.0;
-        t.fc.push_back(Ftrm * cos(mf * f));
-        t.fs.push_back(Ftrm * sin(-1 * mf * f));
+        if (mf == 0) {
+          t.fc.push_back(0.5 * cos(mf * f));
+          t.fs.push_back(0.5 * sin(-1 * mf * f));
+        } else {
+          const unsigned long k = (unsigned long)(mf < 0 ? -mf : mf);
+          t.fc.push_back(1.0 * ck[k]);
+          t.fs.push_back(1.0 * (mf < 0 ? sk[k] : -sk[k]));
+        }
       }
     }
   }
