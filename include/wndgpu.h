@@ -181,6 +181,48 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
                 int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
                 double *norm_factor, long *predicted);
 
+/* ---- Differentiable form: forward and adjoint (reverse-mode) passes of the
+ * smooth transforms and features, and of the WND similarity, for gradients
+ * of class probabilities with respect to pixels.  No counterpart in the
+ * reference, which has no derivatives; the forward values follow:
+ *   WNDGPU_AD_FOURIER          FourierTransform::execute -> ImageMatrix::fft2
+ *                              (src/ImageTransforms.cpp:161-165, src/cmatrix.cpp:1107-1200): h x w -> h x w
+ *   WNDGPU_AD_CHEBYSHEV        ChebyshevTransform::execute (src/ImageTransforms.cpp:175-179,
+ *                              src/cmatrix.cpp:1203-1222), order min(h,w): h x w -> N x N
+ *   WNDGPU_AD_CHEBYSHEV_COEFFS the order-20 coefficients that ChebyshevCoefficients
+ *                              histograms (src/FeatureAlgorithms.cpp:73-93, cmatrix.cpp:1274-1281): -> 20 x 20
+ *   WNDGPU_AD_ZERNIKE          ZernikeCoefficients::execute (src/FeatureAlgorithms.cpp:98-116,
+ *                              src/textures/zernike/zernike.cpp:207-341): -> 72 values
+ * All pointers are device memory of the context's GPU: in [n][h][w] FP64
+ * planes of one size, out / grad_out [n][out_h][out_w], grad_in [n][h][w]
+ * (overwritten).  work: caller-owned device scratch of
+ * wndgpu_ad_work_bytes(op, n, h, w) bytes.  stream: the hipStream_t to issue
+ * on (NULL: the default stream), ordered after the caller's earlier work on
+ * it; calls return without waiting.  The backward pass recomputes what it
+ * needs from `in`. */
+enum { WNDGPU_AD_FOURIER = 0, WNDGPU_AD_CHEBYSHEV = 1, WNDGPU_AD_ZERNIKE = 2, WNDGPU_AD_CHEBYSHEV_COEFFS = 3 };
+int wndgpu_ad_output_shape(int op, int h, int w, int *out_h, int *out_w);
+size_t wndgpu_ad_work_bytes(int op, int n, int h, int w);
+int wndgpu_ad_forward(wndgpu_ctx *ctx, int op, const double *in, int n, int h, int w, double *out, void *work,
+                      void *stream);
+int wndgpu_ad_backward(wndgpu_ctx *ctx, int op, const double *in, const double *grad_out, int n, int h, int w,
+                       double *grad_in, void *work, void *stream);
+/* WND5 marginal probabilities of n_test raw feature vectors x [n_test][F]
+ * (normalised inside with the training mins / maxs, as
+ * signatures::normalize, src/wndchrm_src/signatures.cpp:196-215, then
+ * TrainingSet::classify2, TrainingSet.cpp:2429-2586) and their adjoint:
+ * grad_x = (dprob/dx)^T grad_prob.  train [n_samples][F] normalised,
+ * classes 1..n_classes (int32), weights / mins / maxs [F]; all device
+ * memory; work of wndgpu_ad_wnd_work_bytes bytes. */
+size_t wndgpu_ad_wnd_work_bytes(int n_samples, int n_features, int n_classes, int n_test);
+int wndgpu_ad_wnd_forward(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
+                          int n_classes, const double *weights, const double *mins, const double *maxs,
+                          const double *x, int n_test, double *prob, void *work, void *stream);
+int wndgpu_ad_wnd_backward(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
+                           int n_classes, const double *weights, const double *mins, const double *maxs,
+                           const double *x, int n_test, const double *grad_prob, double *grad_x, void *work,
+                           void *stream);
+
 /* Per-launch timings of the last wndgpu_compute / wndgpu_compute_device made
  * with WNDGPU_TIMING set in the environment (ms from a HIP event pair around
  * each launch, in execution order; launches are named by feature group, and

@@ -1904,4 +1904,162 @@ int wndgpu_last_node_times(wndgpu_ctx *c, double *ms, const char **names, int ca
   return k;
 }
 
+// ------------------------------------------------------- differentiable form
+// Forward and adjoint passes of the smooth transforms and features on
+// batches of device planes; see wndgpu.h.
+static int ad_dims(int op, int h, int w, int *oh, int *ow) {
+  const int N = h < w ? h : w;
+  switch (op) {
+    case WNDGPU_AD_FOURIER: *oh = h; *ow = w; return 0;
+    case WNDGPU_AD_CHEBYSHEV: *oh = N; *ow = N; return 0;
+    case WNDGPU_AD_CHEBYSHEV_COEFFS: *oh = N < 20 ? N : 20; *ow = *oh; return 0;
+    case WNDGPU_AD_ZERNIKE: *oh = 1; *ow = 72; return 0;
+    default: return -1;
+  }
+}
+
+int wndgpu_ad_output_shape(int op, int h, int w, int *out_h, int *out_w) {
+  if (!out_h || !out_w || h < WNDGPU_MIN_SIDE || w < WNDGPU_MIN_SIDE || h > WNDGPU_MAX_SIDE || w > WNDGPU_MAX_SIDE)
+    return WNDGPU_E_ARG;
+  return ad_dims(op, h, w, out_h, out_w) ? WNDGPU_E_ARG : WNDGPU_OK;
+}
+
+size_t wndgpu_ad_work_bytes(int op, int n, int h, int w) {
+  int oh, ow;
+  if (n <= 0 || ad_dims(op, h, w, &oh, &ow)) return 0;
+  switch (op) {
+    case WNDGPU_AD_FOURIER: return (size_t)n * h * w * sizeof(double2);
+    case WNDGPU_AD_CHEBYSHEV:
+    case WNDGPU_AD_CHEBYSHEV_COEFFS: return (size_t)n * h * oh * sizeof(double);
+    default: return zernike_ad_work_doubles(n, h, w) * sizeof(double);
+  }
+}
+
+static int ad_check(wndgpu_ctx *c, int op, const void *in, const void *out, int n, int h, int w, const void *work) {
+  int oh, ow;
+  if (!in || !out || !work || n <= 0) return fail(c, WNDGPU_E_ARG, "null argument or empty batch");
+  if (h < WNDGPU_MIN_SIDE || w < WNDGPU_MIN_SIDE || h > WNDGPU_MAX_SIDE || w > WNDGPU_MAX_SIDE)
+    return fail(c, WNDGPU_E_ARG, "image size %dx%d out of range", w, h);
+  if (ad_dims(op, h, w, &oh, &ow)) return fail(c, WNDGPU_E_ARG, "unknown differentiable operation %d", op);
+  return 0;
+}
+
+// Chebyshev transform of order N as two strided products (forward) or
+// their transposes (adjoint); bx [w][N], by [h][N] are the host-built bases.
+static int ad_cheb_tables(wndgpu_ctx *c, int h, int w, int N, const double **bx, const double **by) {
+  int r;
+  if ((r = get_cheb(c, w, N, bx))) return r;
+  return get_cheb(c, h, N, by);
+}
+
+int wndgpu_ad_forward(wndgpu_ctx *c, int op, const double *in, int n, int h, int w, double *out, void *work,
+                      void *stream) {
+  if (!c) return WNDGPU_E_ARG;
+  int r = ad_check(c, op, in, out, n, h, w, work);
+  if (r) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  int oh, ow;
+  ad_dims(op, h, w, &oh, &ow);
+  if (op == WNDGPU_AD_FOURIER) {
+    FftPlan pw, ph;
+    if ((r = get_fft(c, w, &pw)) || (r = get_fft(c, h, &ph))) return r;
+    launch_ad_fourier_fwd(in, out, (double2 *)work, n, h, w, pw, ph, s);
+  } else if (op == WNDGPU_AD_ZERNIKE) {
+    launch_ad_zernike_fwd(in, out, (double *)work, n, h, w, s);
+  } else {
+    const int N = oh;
+    const double *bx, *by;
+    if ((r = ad_cheb_tables(c, h, w, N, &bx, &by))) return r;
+    double *t1 = (double *)work;
+    launch_ad_gemm(AdGemm{in, (long)h * w, w, 1, bx, 0, N, 1, t1, (long)h * N, h, w, N, 0.5}, n, s);
+    launch_ad_gemm(AdGemm{t1, (long)h * N, 1, N, by, 0, N, 1, out, (long)N * N, N, h, N, 0.5}, n, s);
+  }
+  HIPCHK(c, hipGetLastError());
+  return WNDGPU_OK;
+}
+
+int wndgpu_ad_backward(wndgpu_ctx *c, int op, const double *in, const double *grad_out, int n, int h, int w,
+                       double *grad_in, void *work, void *stream) {
+  if (!c) return WNDGPU_E_ARG;
+  int r = ad_check(c, op, in, grad_in, n, h, w, work);
+  if (r) return r;
+  if (!grad_out) return fail(c, WNDGPU_E_ARG, "null argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  hipStream_t s = (hipStream_t)stream;
+  int oh, ow;
+  ad_dims(op, h, w, &oh, &ow);
+  if (op == WNDGPU_AD_FOURIER) {
+    FftPlan pw, ph;
+    if ((r = get_fft(c, w, &pw)) || (r = get_fft(c, h, &ph))) return r;
+    launch_ad_fourier_bwd(in, grad_out, grad_in, (double2 *)work, n, h, w, pw, ph, s);
+  } else if (op == WNDGPU_AD_ZERNIKE) {
+    launch_ad_zernike_bwd(in, grad_out, grad_in, (double *)work, n, h, w, s);
+  } else {
+    const int N = oh;
+    const double *bx, *by;
+    if ((r = ad_cheb_tables(c, h, w, N, &bx, &by))) return r;
+    double *u = (double *)work;
+    // U[y][j] = 1/2 sum_k by[y][k] G[j][k];  grad[y][x] = 1/2 sum_j U[y][j] bx[x][j]
+    launch_ad_gemm(AdGemm{by, 0, N, 1, grad_out, (long)N * N, 1, N, u, (long)h * N, h, N, N, 0.5}, n, s);
+    launch_ad_gemm(AdGemm{u, (long)h * N, N, 1, bx, 0, 1, N, grad_in, (long)h * w, h, N, w, 0.5}, n, s);
+  }
+  HIPCHK(c, hipGetLastError());
+  return WNDGPU_OK;
+}
+
+// WND work layout: v [nt][F], dist [nt][T], prob [nt][C], norm [nt], pred [nt], gsn [nt][C+1], gd [nt][T]
+size_t wndgpu_ad_wnd_work_bytes(int n_samples, int n_features, int n_classes, int n_test) {
+  if (n_samples <= 0 || n_features <= 0 || n_classes <= 0 || n_test <= 0) return 0;
+  const size_t nt = n_test;
+  return 8 * (nt * n_features + 2 * nt * n_samples + nt * n_classes + 2 * nt + nt * (n_classes + 1));
+}
+
+static int ad_wnd_common(wndgpu_ctx *c, const double *train, const int *cls, int n, int f, int C, const double *wts,
+                         const double *mins, const double *maxs, const double *x, int nt, void *work, hipStream_t s,
+                         double *prob_out) {
+  if (!train || !cls || !wts || !mins || !maxs || !x || !work || n <= 0 || f <= 0 || C <= 0 || C > 63 || nt <= 0)
+    return fail(c, WNDGPU_E_ARG, "bad argument");
+  HIPCHK(c, hipSetDevice(c->device));
+  double *v = (double *)work, *dist = v + (size_t)nt * f, *prob = dist + (size_t)nt * n, *norm = prob + (size_t)nt * C;
+  long *pred = (long *)(norm + nt);
+  HIPCHK(c, hipMemcpyAsync(v, x, (size_t)nt * f * 8, hipMemcpyDeviceToDevice, s));
+  launch_norm_test(v, nt, f, mins, maxs, s);
+  launch_wnd5(train, cls, wts, v, n, f, C, nt, dist, prob, norm, pred, s);
+  if (prob_out) HIPCHK(c, hipMemcpyAsync(prob_out, prob, (size_t)nt * C * 8, hipMemcpyDeviceToDevice, s));
+  return 0;
+}
+
+int wndgpu_ad_wnd_forward(wndgpu_ctx *c, const double *train, const int *classes, int n_samples, int n_features,
+                          int n_classes, const double *weights, const double *mins, const double *maxs,
+                          const double *x, int n_test, double *prob, void *work, void *stream) {
+  if (!c) return WNDGPU_E_ARG;
+  if (!prob) return fail(c, WNDGPU_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int r = ad_wnd_common(c, train, classes, n_samples, n_features, n_classes, weights, mins, maxs, x, n_test, work, s,
+                        prob);
+  if (r) return r;
+  HIPCHK(c, hipGetLastError());
+  return WNDGPU_OK;
+}
+
+int wndgpu_ad_wnd_backward(wndgpu_ctx *c, const double *train, const int *classes, int n_samples, int n_features,
+                           int n_classes, const double *weights, const double *mins, const double *maxs,
+                           const double *x, int n_test, const double *grad_prob, double *grad_x, void *work,
+                           void *stream) {
+  if (!c) return WNDGPU_E_ARG;
+  if (!grad_prob || !grad_x) return fail(c, WNDGPU_E_ARG, "null argument");
+  hipStream_t s = (hipStream_t)stream;
+  int r = ad_wnd_common(c, train, classes, n_samples, n_features, n_classes, weights, mins, maxs, x, n_test, work, s,
+                        nullptr);
+  if (r) return r;
+  const size_t nt = n_test, f = n_features, n = n_samples, C = n_classes;
+  double *v = (double *)work, *dist = v + nt * f, *prob = dist + nt * n, *norm = prob + nt * C;
+  double *gsn = norm + 2 * nt, *gd = gsn + nt * (C + 1);
+  launch_ad_wnd_bwd(train, classes, weights, x, v, mins, maxs, dist, prob, norm, grad_prob, gsn, gd, n_samples,
+                    n_features, n_classes, n_test, grad_x, s);
+  HIPCHK(c, hipGetLastError());
+  return WNDGPU_OK;
+}
+
 }  // extern "C"

@@ -1,0 +1,535 @@
+// Adjoint (reverse-mode) kernels of the smooth transforms and features and
+// of the WND similarity: the differentiable form of the feature path.
+//
+// Every kernel works on a batch of n planes of one size, FP64 row-major
+// [n][h][w], with plain device pointers (the caller owns all buffers).  The
+// forward values follow the feature path's definitions; the backward kernels
+// recompute what they need from the input plane instead of keeping
+// intermediate results (complex spectra, moment sums) between the passes.
+#include "common.hpp"
+#include "kernels.hpp"
+#include "fft.hpp"
+
+namespace wnd {
+
+// ============================================================ Fourier |F|
+// ImageMatrix::fft2 (src/cmatrix.cpp:1107-1160) is the unnormalised 2-D DFT
+// magnitude, |F(ky,kx)| for ky < H/2+1 and the lower rows copied from
+// |F(H-ky, (W-kx) mod W)|.  As a function of the pixels every output is
+// |F| at its own frequency (Hermitian symmetry of a real plane), so
+//   dL/dI(y,x) = Re sum_k Z(k) exp(-2 pi i (kx x / W + ky y / H)),
+//   Z(k) = G(k) conj(F(k)) / |F(k)|   (0 where F(k) = 0),
+// one more forward 2-D DFT of the complex plane Z.
+
+// row spectra of real planes: buf[b][y][kx] = DFT_x(in[b][y][:])
+__global__ void __launch_bounds__(256) k_ad_fft_rows_real(const double *in, double2 *buf, int h, int w,
+                                                          FftPlan pl) {
+  extern __shared__ double2 sh[];
+  const long row = (long)blockIdx.y * h + blockIdx.x;
+  const double *src = in + row * w;
+  for (int x = threadIdx.x; x < w; x += blockDim.x) sh[x] = make_double2(src[x], 0.0);
+  __syncthreads();
+  double2 *r;
+  fft_lds(sh, sh + w, pl, &r);
+  double2 *o = buf + row * w;
+  for (int x = threadIdx.x; x < w; x += blockDim.x) o[x] = r[x];
+}
+
+// column spectra -> magnitudes, upper rows computed and lower rows written
+// from them exactly as the reference's fill
+__global__ void __launch_bounds__(256) k_ad_fft_cols_mag(const double2 *buf, double *out, int h, int w,
+                                                         FftPlan pl) {
+  extern __shared__ double2 sh[];
+  const int x = blockIdx.x;
+  const double2 *src = buf + (long)blockIdx.y * h * w;
+  for (int y = threadIdx.x; y < h; y += blockDim.x) sh[y] = src[(long)y * w + x];
+  __syncthreads();
+  double2 *r;
+  fft_lds(sh, sh + h, pl, &r);
+  double *o = out + (long)blockIdx.y * h * w;
+  const int hh = h / 2 + 1, xm = x == 0 ? 0 : w - x;
+  for (int y = threadIdx.x; y < hh; y += blockDim.x) {
+    const double2 c = r[y];
+    const double m = sqrt(c.x * c.x + c.y * c.y);
+    o[(long)y * w + x] = m;
+    if (y > 0 && h - y >= hh) o[(long)(h - y) * w + xm] = m;
+  }
+}
+
+// backward, columns: F = DFT_y(column of the row spectra), Z = G conj(F)/|F|,
+// then DFT_y(Z) written back over the same column
+__global__ void __launch_bounds__(256) k_ad_fft_cols_adj(double2 *buf, const double *g, int h, int w, FftPlan pl) {
+  extern __shared__ double2 sh[];
+  const int x = blockIdx.x;
+  double2 *col = buf + (long)blockIdx.y * h * w;
+  const double *gp = g + (long)blockIdx.y * h * w;
+  for (int y = threadIdx.x; y < h; y += blockDim.x) sh[y] = col[(long)y * w + x];
+  __syncthreads();
+  double2 *r;
+  fft_lds(sh, sh + h, pl, &r);
+  for (int y = threadIdx.x; y < h; y += blockDim.x) {
+    const double2 c = r[y];
+    const double m = sqrt(c.x * c.x + c.y * c.y);
+    const double s = m > 0 ? gp[(long)y * w + x] / m : 0.0;
+    r[y] = make_double2(s * c.x, -s * c.y);
+  }
+  __syncthreads();
+  double2 *r2;
+  fft_lds(r, r == sh ? sh + h : sh, pl, &r2);
+  for (int y = threadIdx.x; y < h; y += blockDim.x) col[(long)y * w + x] = r2[y];
+}
+
+// backward, rows: grad[b][y][x] = Re DFT_x(buf[b][y][:])
+__global__ void __launch_bounds__(256) k_ad_fft_rows_re(const double2 *buf, double *grad, int h, int w,
+                                                        FftPlan pl) {
+  extern __shared__ double2 sh[];
+  const long row = (long)blockIdx.y * h + blockIdx.x;
+  const double2 *src = buf + row * w;
+  for (int x = threadIdx.x; x < w; x += blockDim.x) sh[x] = src[x];
+  __syncthreads();
+  double2 *r;
+  fft_lds(sh, sh + w, pl, &r);
+  double *o = grad + row * w;
+  for (int x = threadIdx.x; x < w; x += blockDim.x) o[x] = r[x].x;
+}
+
+// lines up to 4096 points keep two complex buffers of 64 KiB each in LDS
+static void fft_lds_attr() {
+  static const bool done = [] {
+    const int lds = 2 * 4096 * (int)sizeof(double2);
+    (void)hipFuncSetAttribute((const void *)k_ad_fft_rows_real, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_ad_fft_cols_mag, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_ad_fft_cols_adj, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    (void)hipFuncSetAttribute((const void *)k_ad_fft_rows_re, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    return true;
+  }();
+  (void)done;
+}
+
+void launch_ad_fourier_fwd(const double *in, double *out, double2 *buf, int n, int h, int w, const FftPlan &pw,
+                           const FftPlan &ph, hipStream_t s) {
+  fft_lds_attr();
+  k_ad_fft_rows_real<<<dim3(h, n), 256, 2 * (size_t)w * sizeof(double2), s>>>(in, buf, h, w, pw);
+  k_ad_fft_cols_mag<<<dim3(w, n), 256, 2 * (size_t)h * sizeof(double2), s>>>(buf, out, h, w, ph);
+}
+
+void launch_ad_fourier_bwd(const double *in, const double *g, double *grad, double2 *buf, int n, int h, int w,
+                           const FftPlan &pw, const FftPlan &ph, hipStream_t s) {
+  fft_lds_attr();
+  k_ad_fft_rows_real<<<dim3(h, n), 256, 2 * (size_t)w * sizeof(double2), s>>>(in, buf, h, w, pw);
+  k_ad_fft_cols_adj<<<dim3(w, n), 256, 2 * (size_t)h * sizeof(double2), s>>>(buf, g, h, w, ph);
+  k_ad_fft_rows_re<<<dim3(h, n), 256, 2 * (size_t)w * sizeof(double2), s>>>(buf, grad, h, w, pw);
+}
+
+// ============================================================ strided GEMM
+// C[b][r][c] = alpha * sum_a A[b](r, a) * B[b](a, c), C row-major M x N per
+// batch entry.  The Chebyshev transform (src/cmatrix.cpp:1203-1222,
+// chebyshev.cpp:118-165) is two such products against the basis tables and
+// its adjoint is the two transposed ones.  64 x 64 tile per block, 4 x 4
+// outputs per thread, K staged through LDS 16 at a time.
+__global__ void __launch_bounds__(256) k_ad_gemm(AdGemm d) {
+  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const double *A = d.A + (long)blockIdx.z * d.sab;
+  const double *B = d.B + (long)blockIdx.z * d.sbb;
+  double *C = d.C + (long)blockIdx.z * d.scb;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+  for (int k0 = 0; k0 < d.K; k0 += 16) {
+    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
+      const int kk = e >> 6, rr = e & 63;
+      const int k = k0 + kk, r = r0 + rr, c = c0 + rr;
+      As[kk][rr] = (k < d.K && r < d.M) ? A[(long)r * d.sar + (long)k * d.sak] : 0.0;
+      Bs[kk][rr] = (k < d.K && c < d.N) ? B[(long)k * d.sbk + (long)c * d.sbc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; kk++) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
+      if (r < d.M && c < d.N) C[(long)r * d.N + c] = d.alpha * acc[i][j];
+    }
+}
+
+void launch_ad_gemm(const AdGemm &d, int nbatch, hipStream_t s) {
+  k_ad_gemm<<<dim3((d.N + 63) / 64, (d.M + 63) / 64, nbatch), 256, 0, s>>>(d);
+}
+
+// ============================================================ Zernike
+// mb_zernike2D (src/textures/zernike/zernike.cpp:207-341): with the centroid
+// (xc, yc) = (sum (i+1) f, sum (j+1) f) / S, S = sum f, and
+// x = (i+1-xc)/rad, y = (j+1-yc)/rad, rad = min(W,H), each pixel of the
+// closed unit disk (r >= DBL_EPSILON) adds
+//   A_nm += (n+1)/pi * f/S * R_nm(r) * exp(-i (m+1) theta)
+// (the reference's COST[m] / SINT[m] tables hold the angle (m+1) theta), and
+// the feature is |A_nm|.  Writing R_nm(r) = r^m P_nm(r^2) and
+// exp(-i q theta) = (x - i y)^q / r^q, q = m+1, the basis is
+//   Z_nm(x, y) = c_n P_nm(t) (x - i y)^q / r,  t = r^2,  c_n = (n+1)/pi,
+// a closed form whose x / y derivatives the adjoint needs:
+//   dZ/dx = c_n [Q'(t) 2x w^q + Q(t) q w^(q-1)],
+//   dZ/dy = c_n [Q'(t) 2y w^q - i Q(t) q w^(q-1)],
+// with w = x - i y, Q = P / r, Q' = (P' - P / (2t)) / r.
+// Because the centroid and S depend on every pixel,
+//   dA/df_p = [Z(u_p) - A - x_p Gx - y_p Gy] / S,  Gx,y = sum_q f_q dZ/dx,y(u_q) / S.
+// Pairs (n, m) are ordered n ascending, m ascending, n - m even (the
+// feature order, zernike.cpp:329-337); pair index = zoff(n) + m/2.
+constexpr int ZAD_L = 15;
+constexpr int ZAD_PAIRS = 72;
+constexpr int ZAD_NCO = 8;  // polynomial coefficients per pair (degree (n-m)/2 <= 7 in t)
+
+__device__ __forceinline__ int zad_off(int n) { return (n / 2) * (n / 2 + 1) + ((n & 1) ? n / 2 + 1 : 0); }
+
+// P_nm coefficients, highest power of t first: c_s = (-1)^s (n-s)! /
+// (s! ((n+m)/2-s)! ((n-m)/2-s)!), s = 0..(n-m)/2.  Fills co[72][8] and
+// nm[72] = n | m << 8.
+__device__ void zad_tables(double *co, int *nm) {
+  for (int k = threadIdx.x; k < ZAD_PAIRS; k += blockDim.x) {
+    int n = 0;
+    while (n < ZAD_L && zad_off(n + 1) <= k) n++;
+    const int m = 2 * (k - zad_off(n)) + (n & 1);
+    auto fact = [](int a) {
+      double r = 1;
+      for (int i = 2; i <= a; i++) r *= i;
+      return r;
+    };
+    const int K = (n - m) / 2;
+    for (int s = 0; s < ZAD_NCO; s++) {
+      double v = 0;
+      if (s <= K) {
+        v = fact(n - s) / (fact(s) * fact((n + m) / 2 - s) * fact(K - s));
+        if (s & 1) v = -v;
+      }
+      co[k * ZAD_NCO + s] = v;
+    }
+    nm[k] = n | (m << 8);
+  }
+}
+
+// per image: hdr[b] = {S, xc, yc, rad}
+__global__ void __launch_bounds__(1024) k_ad_zern_moments(const double *in, int h, int w, double *hdr) {
+  __shared__ double shd[16];
+  const double *p = in + (long)blockIdx.x * h * w;
+  const long np = (long)h * w;
+  double s = 0, m10 = 0, m01 = 0;
+  for (long i = threadIdx.x; i < np; i += blockDim.x) {
+    const int y = i / w, x = i - (long)y * w;
+    const double v = p[i];
+    s += v;
+    m10 += (x + 1) * v;
+    m01 += (y + 1) * v;
+  }
+  s = block_reduce(s, SumOp(), shd);
+  m10 = block_reduce(m10, SumOp(), shd);
+  m01 = block_reduce(m01, SumOp(), shd);
+  if (threadIdx.x == 0) {
+    double *o = hdr + 4 * (long)blockIdx.x;
+    o[0] = s;
+    o[1] = m10 / s;
+    o[2] = m01 / s;
+    o[3] = (double)(w < h ? w : h);
+  }
+}
+
+// Moment sums over one pixel chunk for one group of nine pairs: the pairs
+// of m = g and of m = 15 - g (8-g/2.. pairs each side, nine in all).
+// part[((b * nchunk + chunk) * 72 + pair) * 6 + {Re A, Im A, Re Gx, Im Gx,
+// Re Gy, Im Gy}] (unscaled sums of f * Z and f * dZ).
+template <bool DERIV>
+__global__ void __launch_bounds__(256) k_ad_zern_accum(const double *in, int h, int w, const double *hdr,
+                                                       double *part, int nchunk) {
+  __shared__ double co[ZAD_PAIRS * ZAD_NCO];
+  __shared__ int nm[ZAD_PAIRS];
+  __shared__ double red[4][9 * 6];
+  zad_tables(co, nm);
+  __syncthreads();
+  const int b = blockIdx.z, g = blockIdx.y, chunk = blockIdx.x;
+  const double *p = in + (long)b * h * w;
+  const double S = hdr[4 * b], xc = hdr[4 * b + 1], yc = hdr[4 * b + 2], rad = hdr[4 * b + 3];
+  const long np = (long)h * w, csz = (np + nchunk - 1) / nchunk;
+  const long p0 = chunk * csz, p1 = p0 + csz < np ? p0 + csz : np;
+  const int m1 = g, m2 = ZAD_L - g, cnt1 = (ZAD_L - m1) / 2 + 1;
+  constexpr int NV = DERIV ? 6 : 2;
+  double acc[9][NV];
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+#pragma unroll
+    for (int v = 0; v < NV; v++) acc[k][v] = 0;
+  const double PI = 3.14159265358979323846264338328;
+  for (long q = p0 + threadIdx.x; q < p1; q += blockDim.x) {
+    const int j = q / w, i = q - (long)j * w;
+    const double x = (i + 1 - xc) / rad, y = (j + 1 - yc) / rad;
+    const double t = x * x + y * y, r = sqrt(t);
+    if (r < kDblEps || r > 1.0) continue;
+    const double f = p[q], ir = 1.0 / r;
+    // w^(q-1) and w^q for q = m1+1 and q = m2+1
+    double a1r = 1, a1i = 0, a2r = 1, a2i = 0;
+    for (int e = 0; e < m1; e++) {
+      const double tr = a1r * x + a1i * y, ti = a1i * x - a1r * y;
+      a1r = tr; a1i = ti;
+    }
+    for (int e = 0; e < m2; e++) {
+      const double tr = a2r * x + a2i * y, ti = a2i * x - a2r * y;
+      a2r = tr; a2i = ti;
+    }
+    const double b1r = a1r * x + a1i * y, b1i = a1i * x - a1r * y;
+    const double b2r = a2r * x + a2i * y, b2i = a2i * x - a2r * y;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const bool first = k < cnt1;
+      const int m = first ? m1 : m2;
+      const int n = first ? m1 + 2 * k : m2 + 2 * (k - cnt1);
+      const int pk = zad_off(n) + (m >> 1);
+      const int K = (n - m) >> 1;
+      const double *c = co + pk * ZAD_NCO;
+      double P = c[0], dP = 0;
+      for (int s = 1; s <= K; s++) {
+        dP = dP * t + P;
+        P = P * t + c[s];
+      }
+      const double cn = (n + 1) / PI;
+      const double wr = first ? b1r : b2r, wi = first ? b1i : b2i;   // w^q
+      const double Q = P * ir;
+      acc[k][0] += f * cn * Q * wr;
+      acc[k][1] += f * cn * Q * wi;
+      if constexpr (DERIV) {
+        const double vr = first ? a1r : a2r, vi = first ? a1i : a2i;  // w^(q-1)
+        const double dQ = (dP - P / (2 * t)) * ir;
+        const double qq = (double)(m + 1) * Q;
+        const double ex = 2 * x * dQ, ey = 2 * y * dQ;
+        acc[k][2] += f * cn * (ex * wr + qq * vr);
+        acc[k][3] += f * cn * (ex * wi + qq * vi);
+        acc[k][4] += f * cn * (ey * wr + qq * vi);
+        acc[k][5] += f * cn * (ey * wi - qq * vr);
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      double s = acc[k][v];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) red[wv][k * NV + v] = s;
+    }
+  __syncthreads();
+  if (threadIdx.x < 9 * NV) {
+    const int k = threadIdx.x / NV, v = threadIdx.x - k * NV;
+    const double s = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    const bool first = k < cnt1;
+    const int m = first ? m1 : m2;
+    const int n = first ? m1 + 2 * k : m2 + 2 * (k - cnt1);
+    const int pk = zad_off(n) + (m >> 1);
+    part[(((long)b * nchunk + chunk) * ZAD_PAIRS + pk) * 6 + v] = s;
+  }
+}
+
+// Per image, one thread per pair: sums of the chunk partials in chunk
+// order.  Forward: out[b][k] = |A_k|.  Backward (gout != null):
+// coef[b] = {B_k = g_k conj(A_k)/|A_k| (72 complex), K0 = sum g|A|,
+// Kx = Re sum B Gx, Ky = Re sum B Gy}.
+__global__ void __launch_bounds__(128) k_ad_zern_final(const double *hdr, const double *part, int nchunk,
+                                                       double *out, const double *gout, double *coef) {
+  __shared__ double shd[16];
+  const int b = blockIdx.x, k = threadIdx.x;
+  const double S = hdr[4 * b];
+  double e[6] = {0, 0, 0, 0, 0, 0};
+  if (k < ZAD_PAIRS)
+    for (int c = 0; c < nchunk; c++) {
+      const double *pp = part + (((long)b * nchunk + c) * ZAD_PAIRS + k) * 6;
+#pragma unroll
+      for (int v = 0; v < 6; v++) e[v] += pp[v];
+    }
+  const double Ar = e[0] / S, Ai = e[1] / S;
+  const double mag = sqrt(Ar * Ar + Ai * Ai);
+  if (!gout) {
+    if (k < ZAD_PAIRS) out[(long)b * ZAD_PAIRS + k] = mag;
+    return;
+  }
+  double k0 = 0, kx = 0, ky = 0;
+  if (k < ZAD_PAIRS) {
+    const double g = gout[(long)b * ZAD_PAIRS + k];
+    const double wgt = mag > 0 ? g / mag : 0.0;
+    const double Br = wgt * Ar, Bi = -wgt * Ai;
+    coef[(long)b * (2 * ZAD_PAIRS + 3) + 2 * k] = Br;
+    coef[(long)b * (2 * ZAD_PAIRS + 3) + 2 * k + 1] = Bi;
+    k0 = wgt * mag * mag;
+    kx = Br * (e[2] / S) - Bi * (e[3] / S);
+    ky = Br * (e[4] / S) - Bi * (e[5] / S);
+  }
+  k0 = block_reduce(k0, SumOp(), shd);
+  kx = block_reduce(kx, SumOp(), shd);
+  ky = block_reduce(ky, SumOp(), shd);
+  if (k == 0) {
+    double *cf = coef + (long)b * (2 * ZAD_PAIRS + 3) + 2 * ZAD_PAIRS;
+    cf[0] = k0;
+    cf[1] = kx;
+    cf[2] = ky;
+  }
+}
+
+// grad[b][p] = (sum_k Re(B_k Z_k(u_p)) - K0 - x_p Kx - y_p Ky) / S
+__global__ void __launch_bounds__(256) k_ad_zern_pixels(int h, int w, const double *hdr, const double *coef,
+                                                        double *grad) {
+  __shared__ double co[ZAD_PAIRS * ZAD_NCO];
+  __shared__ int nm[ZAD_PAIRS];
+  __shared__ double B[2 * ZAD_PAIRS + 3];
+  zad_tables(co, nm);
+  const int b = blockIdx.y;
+  for (int i = threadIdx.x; i < 2 * ZAD_PAIRS + 3; i += blockDim.x) B[i] = coef[(long)b * (2 * ZAD_PAIRS + 3) + i];
+  __syncthreads();
+  const double S = hdr[4 * b], xc = hdr[4 * b + 1], yc = hdr[4 * b + 2], rad = hdr[4 * b + 3];
+  const double K0 = B[2 * ZAD_PAIRS], Kx = B[2 * ZAD_PAIRS + 1], Ky = B[2 * ZAD_PAIRS + 2];
+  const long np = (long)h * w;
+  const double PI = 3.14159265358979323846264338328;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < np; q += (long)gridDim.x * blockDim.x) {
+    const int j = q / w, i = q - (long)j * w;
+    const double x = (i + 1 - xc) / rad, y = (j + 1 - yc) / rad;
+    const double t = x * x + y * y, r = sqrt(t);
+    double acc = 0;
+    if (!(r < kDblEps || r > 1.0)) {
+      const double ir = 1.0 / r;
+      double wr = x, wi = -y;  // w^q, q = m + 1
+      for (int m = 0; m <= ZAD_L; m++) {
+        for (int n = m; n <= ZAD_L; n += 2) {
+          const int pk = zad_off(n) + (m >> 1);
+          const int K = (n - m) >> 1;
+          const double *c = co + pk * ZAD_NCO;
+          double P = c[0];
+          for (int s = 1; s <= K; s++) P = P * t + c[s];
+          acc += (n + 1) / PI * P * ir * (B[2 * pk] * wr - B[2 * pk + 1] * wi);
+        }
+        const double tr = wr * x + wi * y, ti = wi * x - wr * y;
+        wr = tr; wi = ti;
+      }
+    }
+    grad[(long)b * np + q] = (acc - K0 - x * Kx - y * Ky) / S;
+  }
+}
+
+int zernike_ad_chunks(int h, int w) {
+  const long np = (long)h * w;
+  int c = (int)((np + 16383) / 16384);
+  return c < 1 ? 1 : (c > 64 ? 64 : c);
+}
+
+void launch_ad_zernike_fwd(const double *in, double *out, double *work, int n, int h, int w, hipStream_t s) {
+  const int nchunk = zernike_ad_chunks(h, w);
+  double *hdr = work, *part = work + 4 * (long)n;
+  k_ad_zern_moments<<<n, 1024, 0, s>>>(in, h, w, hdr);
+  k_ad_zern_accum<false><<<dim3(nchunk, 8, n), 256, 0, s>>>(in, h, w, hdr, part, nchunk);
+  k_ad_zern_final<<<n, 128, 0, s>>>(hdr, part, nchunk, out, nullptr, nullptr);
+}
+
+void launch_ad_zernike_bwd(const double *in, const double *gout, double *grad, double *work, int n, int h, int w,
+                           hipStream_t s) {
+  const int nchunk = zernike_ad_chunks(h, w);
+  double *hdr = work, *part = work + 4 * (long)n;
+  double *coef = part + (long)n * nchunk * ZAD_PAIRS * 6;
+  k_ad_zern_moments<<<n, 1024, 0, s>>>(in, h, w, hdr);
+  k_ad_zern_accum<true><<<dim3(nchunk, 8, n), 256, 0, s>>>(in, h, w, hdr, part, nchunk);
+  k_ad_zern_final<<<n, 128, 0, s>>>(hdr, part, nchunk, nullptr, gout, coef);
+  const long np = (long)h * w;
+  int gx = (int)((np + 255) / 256);
+  if (gx > 1024) gx = 1024;
+  k_ad_zern_pixels<<<dim3(gx, n), 256, 0, s>>>(h, w, hdr, coef, grad);
+}
+
+size_t zernike_ad_work_doubles(int n, int h, int w) {
+  return 4 * (size_t)n + (size_t)n * zernike_ad_chunks(h, w) * ZAD_PAIRS * 6 + (size_t)n * (2 * ZAD_PAIRS + 3);
+}
+
+// ============================================================ WND similarity
+// classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586) after
+// signatures::normalize (signatures.cpp:196-215):
+//   v_f = 100 (x_f - min_f) / (max_f - min_f) (0 / 100 outside, 0 for NaN),
+//   d_t = sum_f w_f^2 (v_f - T_tf)^2 over terms with w_f, |v_f - T_tf| >= eps,
+//   s_t = d_t^-5 (d_t >= eps), sbar_c = mean of s_t over class c, p_c = sbar_c / sum sbar.
+// With upstream G = dL/dp:
+//   dL/dsbar_c = (G_c - sum_k G_k p_k) / sum sbar,
+//   dL/dd_t = -5 d_t^-6 dL/dsbar_c(t) / n_c(t),
+//   dL/dv_f = sum_t dL/dd_t 2 w_f^2 (v_f - T_tf),  dL/dx_f = dL/dv_f * 100 / (max_f - min_f).
+
+// per test vector: gsn[q][c] = dL/dsbar_c / n_c (0 for empty classes)
+__global__ void k_ad_wnd_cls(const double *dist, const int *cls, int n, int C, int ntest, const double *prob,
+                             const double *norm, const double *gp, double *gsn) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ntest) return;
+  int nper[64];
+  for (int c = 0; c <= C; c++) nper[c] = 0;
+  for (int i = 0; i < n; i++)
+    if (!(dist[(long)q * n + i] < kDblEps)) nper[cls[i]]++;
+  double dot = 0;
+  for (int c = 0; c < C; c++) dot += gp[(long)q * C + c] * prob[(long)q * C + c];
+  for (int c = 1; c <= C; c++) {
+    const double gs = (gp[(long)q * C + c - 1] - dot) / norm[q];
+    gsn[(long)q * (C + 1) + c] = nper[c] ? gs / nper[c] : 0.0;
+  }
+}
+
+// gd[q][t] = dL/dd_t
+__global__ void k_ad_wnd_gd(const double *dist, const int *cls, int n, int C, int ntest, const double *gsn,
+                            double *gd) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * ntest) return;
+  const int q = e / n, i = e - (long)q * n;
+  const double d = dist[e];
+  gd[e] = d < kDblEps ? 0.0 : -5.0 * pow(d, -6.0) * gsn[(long)q * (C + 1) + cls[i]];
+}
+
+// gx[q][f]: one thread per (q, f), training rows in order
+__global__ void k_ad_wnd_gx(const double *train, const double *wts, const double *x, const double *v,
+                            const double *mins, const double *maxs, const double *gd, int n, int f, int ntest,
+                            double *gx) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)f * ntest) return;
+  const int q = e / f, s = e - (long)q * f;
+  const double ws = wts[s];
+  double acc = 0;
+  if (!(ws < kDblEps)) {
+    const double vs = v[e];
+    for (int i = 0; i < n; i++) {
+      const double dlt = vs - train[(long)i * f + s];
+      if (fabs(dlt) < kDblEps) continue;
+      acc += gd[(long)q * n + i] * dlt;
+    }
+    acc *= 2 * ws * ws;
+  }
+  const double xv = x[e], mn = mins[s], mx = maxs[s];
+  const bool lin = !(xv != xv || xv < mn || (mx - mn) < kDblEps || xv > mx);
+  gx[e] = lin ? acc * (100.0 / (mx - mn)) : 0.0;
+}
+
+void launch_ad_wnd_bwd(const double *train, const int *cls, const double *wts, const double *x, const double *v,
+                       const double *mins, const double *maxs, const double *dist, const double *prob,
+                       const double *norm, const double *gp, double *gsn, double *gd, int n, int f, int C, int ntest,
+                       double *gx, hipStream_t s) {
+  k_ad_wnd_cls<<<(ntest + 63) / 64, 64, 0, s>>>(dist, cls, n, C, ntest, prob, norm, gp, gsn);
+  const long e = (long)n * ntest;
+  k_ad_wnd_gd<<<(e + 255) / 256, 256, 0, s>>>(dist, cls, n, C, ntest, gsn, gd);
+  const long e2 = (long)f * ntest;
+  k_ad_wnd_gx<<<(e2 + 255) / 256, 256, 0, s>>>(train, wts, x, v, mins, maxs, gd, n, f, ntest, gx);
+}
+
+}  // namespace wnd

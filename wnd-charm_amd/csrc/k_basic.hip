@@ -7,6 +7,7 @@
 #include "color.hpp"
 #include "kernels.hpp"
 #include "blocksort.hpp"
+#include "fft.hpp"
 
 namespace wnd {
 
@@ -223,37 +224,7 @@ __global__ void __launch_bounds__(256) k_cheb_gemm(const GemmDesc *ds) {
 // Fourier magnitude (ImageMatrix::fft2, src/cmatrix.cpp:1107-1160): the
 // unnormalised 2-D DFT, |F(ky,kx)| for ky < H/2+1, the lower rows filled by
 // |F(H-ky, (W-kx) mod W)| as the reference does.  Mixed-radix Stockham FFT
-// in LDS, one work-group per line; any radix is evaluated as a direct
-// R-point DFT with combined twiddles from a length-n table.
-__device__ void fft_lds(double2 *a, double2 *b, const FftPlan &pl, double2 **res) {
-  const int n = pl.n;
-  int Ns = 1;
-  double2 *src = a, *dst = b;
-  for (int s = 0; s < pl.nst; s++) {
-    const int R = pl.radix[s];
-    const int nr = n / R, step = n / (Ns * R);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-      const int j = t % nr, q = t / nr;
-      const int k = j % Ns;
-      const int base = k * step + q * nr;
-      double re = 0, im = 0;
-      long idx = 0;
-      for (int r = 0; r < R; r++) {
-        const double2 v = src[j + r * nr];
-        const double2 w = pl.tw[idx];
-        re += v.x * w.x - v.y * w.y;
-        im += v.x * w.y + v.y * w.x;
-        idx += base;
-        if (idx >= n) idx -= n;
-      }
-      dst[(j / Ns) * Ns * R + k + q * Ns] = make_double2(re, im);
-    }
-    __syncthreads();
-    double2 *tt = src; src = dst; dst = tt;
-    Ns *= R;
-  }
-  *res = src;
-}
+// in LDS (fft.hpp), one work-group per line.
 
 // rows: F1[y][x] = DFT_x(img[y][:]), complex, row-major H x W
 __global__ void __launch_bounds__(256) k_fft_rows(const Img *in, double2 *const *f1, const FftPlan *plw) {

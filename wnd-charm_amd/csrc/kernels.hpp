@@ -23,6 +23,19 @@ struct GemmDesc {
   long sar, sak;
 };
 
+// C[b][r][c] = alpha * sum_a A[b*sab + r*sar + a*sak] * B[b*sbb + a*sbk + c*sbc],
+// C row-major M x N at C + b*scb (adjoint path, k_adjoint.hip)
+struct AdGemm {
+  const double *A;
+  long sab, sar, sak;
+  const double *B;
+  long sbb, sbk, sbc;
+  double *C;
+  long scb;
+  int M, K, N;
+  double alpha;
+};
+
 struct FftPlan {
   const double2 *tw;  // tw[k] = exp(-2 pi i k / n)
   int n, nst;
@@ -162,5 +175,20 @@ void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int max
 void launch_downsample_hsv(const HsvDsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
 void launch_normalize(NormDesc *nd, int nimg, long maxn, hipStream_t s);
 void launch_sample(const SampleDesc *sd, const Img *dst, int nsamples, long maxn, hipStream_t s);
+
+// differentiable form (k_adjoint.hip): batches of n planes [n][h][w] of one size
+void launch_ad_fourier_fwd(const double *in, double *out, double2 *buf, int n, int h, int w, const FftPlan &pw,
+                           const FftPlan &ph, hipStream_t s);
+void launch_ad_fourier_bwd(const double *in, const double *g, double *grad, double2 *buf, int n, int h, int w,
+                           const FftPlan &pw, const FftPlan &ph, hipStream_t s);
+void launch_ad_gemm(const AdGemm &d, int nbatch, hipStream_t s);
+size_t zernike_ad_work_doubles(int n, int h, int w);
+void launch_ad_zernike_fwd(const double *in, double *out, double *work, int n, int h, int w, hipStream_t s);
+void launch_ad_zernike_bwd(const double *in, const double *gout, double *grad, double *work, int n, int h, int w,
+                           hipStream_t s);
+void launch_ad_wnd_bwd(const double *train, const int *cls, const double *wts, const double *x, const double *v,
+                       const double *mins, const double *maxs, const double *dist, const double *prob,
+                       const double *norm, const double *gp, double *gsn, double *gd, int n, int f, int C, int ntest,
+                       double *gx, hipStream_t s);
 
 }  // namespace wnd
