@@ -146,13 +146,13 @@ def _tile(k):
 
 
 def tiles_main(args):
-    import multiprocessing as mp
+    from concurrent.futures import ThreadPoolExecutor
     n = args.tiles
-    # the generator is numpy on the host: the tiles are made in parallel
-    # before anything touches the GPU; the class of tile k is k // 1250 as in
-    # the configuration's global order
-    with mp.get_context('fork').Pool(min(16, os.cpu_count() or 1)) as pool:
-        tiles = pool.map(_tile, range(n), chunksize=8)
+    # the generator is numpy on the host (it releases the GIL): the tiles are
+    # made on 16 threads, no child processes; the class of tile k is k // 1250
+    # as in the configuration's global order
+    with ThreadPoolExecutor(16) as ex:
+        tiles = list(ex.map(_tile, range(n)))
     import torch
     import wndcharm
     from wndcharm import synthetic
@@ -272,6 +272,98 @@ def mixed_main(args):
     print(json.dumps(line))
 
 
+GRAD_GROUPS = ['Zernike Coefficients ()', 'Zernike Coefficients (Fourier ())', 'Zernike Coefficients (Chebyshev ())',
+               'Zernike Coefficients (Fourier (Chebyshev ()))', 'Zernike Coefficients (Chebyshev (Fourier ()))',
+               'Chebyshev Coefficients (raw) ()', 'Chebyshev Coefficients (raw) (Fourier ())']
+
+
+def _grad_image(k):
+    """image k of the differentiation configuration, scaled to [0, 1] as float32:
+    k < 4000 training (class k // 500 + 1), k >= 4000 test (class k % 8 + 1)"""
+    from wndcharm import synthetic
+    base, periods, per_class, shape = synthetic.CONFIGS['grad_512']
+    c = k // per_class if k < 8 * per_class else k % 8
+    return (synthetic.image(base + k, periods[c], shape).astype(np.float32) / np.float32(4095)), c + 1
+
+
+def grad_main(args):
+    """pixel gradients of the WND5 marginal probability of the predicted class"""
+    from concurrent.futures import ThreadPoolExecutor
+    ntrain, ntest = 8 * args.train_per_class, 64
+    # images are generated on 16 host threads (numpy releases the GIL); no child processes
+    t0 = time.time()
+    ks = [(k // args.train_per_class) * 500 + k % args.train_per_class if k < ntrain else 4000 + k - ntrain
+          for k in range(ntrain + ntest)]
+    with ThreadPoolExecutor(16) as ex:
+        imgs = list(ex.map(_grad_image, ks))
+    t_gen = time.time() - t0
+    print('grad: %d images generated in %.1f s' % (len(imgs), t_gen), file=sys.stderr, flush=True)
+    import torch
+    from wndcharm import autograd as ad
+    dev = torch.device('cuda', 0)
+    plan = ad.DifferentiablePlan(GRAD_GROUPS)
+    cls = np.array([c for _, c in imgs[:ntrain]], np.int32)
+    feats = []
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for i in range(0, ntrain, 128):
+        x = torch.from_numpy(np.stack([im for im, _ in imgs[i:min(i + 128, ntrain)]])).to(dev).double()
+        with torch.no_grad():
+            feats.append(plan(x).cpu().numpy())
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    print('grad: training features %s in %.1f s' % (str((ntrain, plan.n_features)), t_train), file=sys.stderr,
+          flush=True)
+    model = ad.WNDModel(np.concatenate(feats), cls, 8, used_fraction=0.15)
+    print('grad: WND5 model built', file=sys.stderr, flush=True)
+    test = torch.from_numpy(np.stack([im for im, _ in imgs[ntrain:]])).to(dev)
+    xd = test.double()
+
+    def step():
+        xg = xd.clone().requires_grad_(True)
+        p = model.probabilities(plan(xg))
+        pred = torch.argmax(p.detach(), 1)
+        p.gather(1, pred[:, None]).sum().backward()
+        return xg.grad, pred
+
+    def forward_only():
+        with torch.no_grad():
+            return model.probabilities(plan(xd))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    print('grad: warmup done', file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g, pred = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    forward_only()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        forward_only()
+    torch.cuda.synchronize()
+    dtf = (time.perf_counter() - t0) / args.steps
+    line = {
+        'metric': 'pixel-gradient images/s (WND5 probability of the predicted class, adjoint kernels)',
+        'value': round(ntest / dt, 3), 'unit': 'images/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(dt * 1e3, 3), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
+        'data': 'synthetic (seeded grating/blob/Poisson generator, 512x512, scaled to [0, 1] float32)',
+        'config': {'workload': 'differentiation: %d test images 512x512, forward + adjoint through %d features '
+                               '(Zernike on 5 nodes, raw Chebyshev-20 on 2) and WND5 against %d training '
+                               'samples (8 classes)' % (ntest, plan.n_features, ntrain),
+                   'test_images': ntest, 'train_samples': ntrain, 'features': plan.n_features},
+        'forward_only_images_per_s': round(ntest / dtf, 3),
+        'training_features_images_per_s': round(ntrain / t_train, 2),
+        'generator_s': round(t_gen, 1),
+        'grad_finite': bool(torch.isfinite(g).all().item()),
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -285,6 +377,9 @@ def main():
                     help='the uneven-tiles configuration over this many tiles (batches of --batch)')
     ap.add_argument('--mixed', action='store_true',
                     help='the mixed-precision configuration (FP64 / FP32 / BF16x3 basis contractions)')
+    ap.add_argument('--grad', action='store_true',
+                    help='the differentiation configuration (pixel gradients through the adjoint kernels)')
+    ap.add_argument('--train-per-class', type=int, default=500)
     ap.add_argument('--copy-probe', action='store_true',
                     help='only run a plain 1 GiB device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)')
     args = ap.parse_args()
@@ -297,6 +392,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({'copy_probe_bytes': src.numel() * 8, 'launches': 3}))
         return
+    if args.grad:
+        return grad_main(args)
     if args.mixed:
         mixed_main(args)
         return

@@ -169,7 +169,8 @@ int wndgpu_algorithm(wndgpu_ctx *ctx, const char *name, const double *in, int h,
 int wndgpu_algorithm_n_features(const char *name);
 
 /* Classifier (host arrays, FP64, row-major [n_samples][n_features];
- * classes are 1..n_classes). */
+ * class labels 1..n_classes, 0 for a sample of no class; a label outside
+ * 0..n_classes is WNDGPU_E_ARG). */
 int wndgpu_normalize_train(wndgpu_ctx *ctx, double *data, int n_samples, int n_features, double *mins,
                            double *maxs);
 int wndgpu_normalize_test(wndgpu_ctx *ctx, double *vec, int n_features, const double *mins, const double *maxs);
@@ -212,8 +213,9 @@ int wndgpu_ad_backward(wndgpu_ctx *ctx, int op, const double *in, const double *
  * signatures::normalize, src/wndchrm_src/signatures.cpp:196-215, then
  * TrainingSet::classify2, TrainingSet.cpp:2429-2586) and their adjoint:
  * grad_x = (dprob/dx)^T grad_prob.  train [n_samples][F] normalised,
- * classes 1..n_classes (int32), weights / mins / maxs [F]; all device
- * memory; work of wndgpu_ad_wnd_work_bytes bytes. */
+ * classes 1..n_classes (int32; the kernels skip a sample whose label lies
+ * outside 0..n_classes, which the caller is to prevent), weights / mins /
+ * maxs [F]; all device memory; work of wndgpu_ad_wnd_work_bytes bytes. */
 size_t wndgpu_ad_wnd_work_bytes(int n_samples, int n_features, int n_classes, int n_test);
 int wndgpu_ad_wnd_forward(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
                           int n_classes, const double *weights, const double *mins, const double *maxs,

@@ -168,3 +168,17 @@ def test_reference_wnd_gradient_matches_oracle_differences():
         pb = orc_probabilities(orc_features(b, GROUPS), data, cls, w, mins, maxs, 3)[cls_k]
         fd = (pa - pb) / (2 * h)
         assert abs(fd - g[q]) <= 1e-4 * np.max(np.abs(g)), (q, fd, g[q])
+
+
+def test_wnd_model_rejects_mismatched_training_set():
+    """feature rows and class labels must agree before anything reaches the GPU"""
+    import wndcharm
+    from wndcharm import autograd
+    with pytest.raises(wndcharm.WndGpuError):
+        autograd.WNDModel(np.zeros((10, 4)), np.ones(8, np.int32), 2)
+    with pytest.raises(wndcharm.WndGpuError):
+        autograd.WNDModel(np.zeros((3, 4)), np.array([1, 2, 3], np.int32), 2)
+    with pytest.raises(wndcharm.WndGpuError):
+        wndcharm.fisher_weights(np.zeros((10, 4)), np.ones(9, np.int32), 2)
+    with pytest.raises(wndcharm.WndGpuError):
+        wndcharm.wnd5(np.zeros((10, 4)), np.ones(10, np.int32), 2, np.ones(4), np.zeros((2, 5)))

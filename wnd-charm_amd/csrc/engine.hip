@@ -1840,8 +1840,16 @@ int wndgpu_normalize_test(wndgpu_ctx *c, double *vec, int f, const double *mins,
   return WNDGPU_OK;
 }
 
+// class labels must lie in 0..C (0: a sample of no class)
+static int check_labels(wndgpu_ctx *c, const int *cls, int n, int C) {
+  for (int i = 0; i < n; i++)
+    if (cls[i] < 0 || cls[i] > C) return fail(c, WNDGPU_E_ARG, "class label %d of sample %d outside 0..%d", cls[i], i, C);
+  return 0;
+}
+
 int wndgpu_fisher(wndgpu_ctx *c, const double *data, const int *cls, int n, int f, int C, double used, double *w) {
   if (!c || !data || !cls || !w || n <= 0 || f <= 0 || C <= 0) return fail(c, WNDGPU_E_ARG, "bad argument");
+  if (int r = check_labels(c, cls, n, C)) return r;
   HIPCHK(c, hipSetDevice(c->device));
   DBuf d, k, wd;
   DALLOC(c, d, (size_t)n * f * 8);
@@ -1860,6 +1868,7 @@ int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f
   if (!c || !train || !cls || !w || !test || !prob || !norm || !pred || n <= 0 || f <= 0 || C <= 0 || C > 63 ||
       ntest <= 0)
     return fail(c, WNDGPU_E_ARG, "bad argument");
+  if (int r = check_labels(c, cls, n, C)) return r;
   HIPCHK(c, hipSetDevice(c->device));
   DBuf tr, k, wd, te, dist, pr, nm, pd;
   DALLOC(c, tr, (size_t)n * f * 8);

@@ -478,8 +478,10 @@ __global__ void k_ad_wnd_cls(const double *dist, const int *cls, int n, int C, i
   if (q >= ntest) return;
   int nper[64];
   for (int c = 0; c <= C; c++) nper[c] = 0;
-  for (int i = 0; i < n; i++)
-    if (!(dist[(long)q * n + i] < kDblEps)) nper[cls[i]]++;
+  for (int i = 0; i < n; i++) {
+    const int ci = cls[i];
+    if (!(dist[(long)q * n + i] < kDblEps) && ci >= 0 && ci <= C) nper[ci]++;
+  }
   double dot = 0;
   for (int c = 0; c < C; c++) dot += gp[(long)q * C + c] * prob[(long)q * C + c];
   for (int c = 1; c <= C; c++) {
@@ -495,7 +497,8 @@ __global__ void k_ad_wnd_gd(const double *dist, const int *cls, int n, int C, in
   if (e >= (long)n * ntest) return;
   const int q = e / n, i = e - (long)q * n;
   const double d = dist[e];
-  gd[e] = d < kDblEps ? 0.0 : -5.0 * pow(d, -6.0) * gsn[(long)q * (C + 1) + cls[i]];
+  const int ci = cls[i];
+  gd[e] = (d < kDblEps || ci < 1 || ci > C) ? 0.0 : -5.0 * pow(d, -6.0) * gsn[(long)q * (C + 1) + ci];
 }
 
 // gx[q][f]: one thread per (q, f), training rows in order

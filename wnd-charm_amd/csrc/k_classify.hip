@@ -136,9 +136,10 @@ __global__ void k_wnd_classes(const double *dist, const int *cls, int n, int C, 
   for (int c = 0; c <= C; c++) { csim[c] = 0; nper[c] = 0; }
   for (int i = 0; i < n; i++) {
     const double ds = dist[(long)q * n + i];
-    if (ds < kDblEps) continue;
-    csim[cls[i]] += pow(ds, -5.0);
-    nper[cls[i]]++;
+    const int ci = cls[i];
+    if (ds < kDblEps || ci < 0 || ci > C) continue;  // labels are validated on the host; never index outside
+    csim[ci] += pow(ds, -5.0);
+    nper[ci]++;
   }
   long best = -1;
   double maxs = 0;

@@ -485,6 +485,8 @@ def normalize_test(vec, mins, maxs):
 def fisher_weights(data, classes, n_classes, used_fraction=0.15):
     d = np.ascontiguousarray(data, np.float64)
     c = np.ascontiguousarray(classes, np.int32)
+    if d.ndim != 2 or c.shape != (d.shape[0],):
+        raise WndGpuError('fisher_weights: %d samples and %d class labels' % (d.shape[0], c.size))
     w = np.zeros(d.shape[1])
     _check(_load().wndgpu_fisher(_context(), _ptr(d), _ptr(c), d.shape[0], d.shape[1], int(n_classes),
                                  float(used_fraction), _ptr(w)), 'fisher')
@@ -498,6 +500,9 @@ def wnd5(train, classes, n_classes, weights, test):
     c = np.ascontiguousarray(classes, np.int32)
     w = np.ascontiguousarray(weights, np.float64)
     te = np.ascontiguousarray(np.atleast_2d(test), np.float64)
+    if tr.ndim != 2 or c.shape != (tr.shape[0],) or w.shape != (tr.shape[1],) or te.shape[1] != tr.shape[1]:
+        raise WndGpuError('wnd5: shapes of train %s, classes %s, weights %s and test %s disagree'
+                          % (tr.shape, c.shape, w.shape, te.shape))
     nt = te.shape[0]
     prob = np.zeros((nt, n_classes))
     norm = np.zeros(nt)

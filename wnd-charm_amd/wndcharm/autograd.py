@@ -312,8 +312,12 @@ class WNDModel(object):
 
     def __init__(self, train_features, classes, n_classes, used_fraction=0.15, device=0):
         raw = np.ascontiguousarray(np.asarray(train_features, np.float64))
-        norm, mins, maxs = normalize_train(raw)
         cls = np.ascontiguousarray(np.asarray(classes, np.int32))
+        if raw.ndim != 2 or cls.shape != (raw.shape[0],):
+            raise WndGpuError('WND5 training set: %s feature rows and %s class labels' % (raw.shape[:1], cls.shape))
+        if n_classes < 1 or n_classes > 63 or cls.min() < 0 or cls.max() > n_classes:
+            raise WndGpuError('WND5 training set: class labels must lie in 0..n_classes (n_classes <= 63)')
+        norm, mins, maxs = normalize_train(raw)
         w = fisher_weights(norm, cls, n_classes, used_fraction)
         dev = torch.device('cuda', device)
         self.n_classes = int(n_classes)

@@ -15,6 +15,8 @@ CONFIGS = {
     'short_128': (1001, (8, 20), 10, (128, 128)),
     'long_256': (2002, (6, 12, 24, 48), 100, (256, 256)),
     'tiles_mixed': (4004, (5, 7, 10, 14, 20, 28, 40, 56), 1250, None),
+    # differentiation: 8 classes x 500 training images of 512 x 512, scaled to [0, 1] as float32
+    'grad_512': (5005, (4, 6, 8, 12, 16, 24, 32, 48), 500, (512, 512)),
 }
 
 
