@@ -194,6 +194,10 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
  *                              histograms (src/FeatureAlgorithms.cpp:73-93, cmatrix.cpp:1274-1281): -> 20 x 20
  *   WNDGPU_AD_ZERNIKE          ZernikeCoefficients::execute (src/FeatureAlgorithms.cpp:98-116,
  *                              src/textures/zernike/zernike.cpp:207-341): -> 72 values
+ *   WNDGPU_AD_CHEB_FOURIER     the NN x NN coefficient magnitudes that ChebyshevFourierCoefficients
+ *                              histograms, after its Downsample above 300 x 300 pixels
+ *                              (src/FeatureAlgorithms.cpp:54-69, src/cmatrix.cpp:579-720,1227-1240,
+ *                              src/transforms/ChebyshevFourier.cpp:59-160): -> NN*NN values (NN <= 23)
  * All pointers are device memory of the context's GPU: in [n][h][w] FP64
  * planes of one size, out / grad_out [n][out_h][out_w], grad_in [n][h][w]
  * (overwritten).  work: caller-owned device scratch of
@@ -201,7 +205,8 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
  * on (NULL: the default stream), ordered after the caller's earlier work on
  * it; calls return without waiting.  The backward pass recomputes what it
  * needs from `in`. */
-enum { WNDGPU_AD_FOURIER = 0, WNDGPU_AD_CHEBYSHEV = 1, WNDGPU_AD_ZERNIKE = 2, WNDGPU_AD_CHEBYSHEV_COEFFS = 3 };
+enum { WNDGPU_AD_FOURIER = 0, WNDGPU_AD_CHEBYSHEV = 1, WNDGPU_AD_ZERNIKE = 2, WNDGPU_AD_CHEBYSHEV_COEFFS = 3,
+       WNDGPU_AD_CHEB_FOURIER = 4 };
 int wndgpu_ad_output_shape(int op, int h, int w, int *out_h, int *out_w);
 size_t wndgpu_ad_work_bytes(int op, int n, int h, int w);
 int wndgpu_ad_forward(wndgpu_ctx *ctx, int op, const double *in, int n, int h, int w, double *out, void *work,

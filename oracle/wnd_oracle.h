@@ -91,6 +91,9 @@ int orc_sample_plane(const void *img, int dtype, int h, int w, const orc_samplin
 int orc_features_sampled(int set, const void *img, int dtype, int h, int w, const orc_sampling *s, double *out);
 
 /* Intermediates used by integer-exact parity tests. */
+/* the NN x NN magnitudes ChebyshevFourierCoefficients histograms (after its
+ * Downsample above 300 x 300); out: 529 doubles, *nn = NN */
+int orc_cf_magnitudes(const double *in, int h, int w, double *out, int *nn);
 double orc_otsu(const double *in, int h, int w);                         /* Otsu(dynamic) */
 int orc_radon_proj(const double *in, int h, int w, double *out4xr, int *rsize);
 int orc_haralick_grays(const double *in, int h, int w, unsigned char *out);

@@ -13,11 +13,16 @@ Definitions follow the reference sources:
 * zernike   mb_zernike2D (src/textures/zernike/zernike.cpp:207-341): order 15,
             the angle of repetition m taken as (m+1) theta as the reference's
             COST / SINT tables hold it.
+* cf        ChebyshevFourierCoefficients before its histogram (Downsample above
+            300 x 300, src/cmatrix.cpp:579-720 and 1227-1240; ChebyshevFourier2D,
+            src/transforms/ChebyshevFourier.cpp:59-160): the NN x NN magnitudes.
 * wnd5      signatures::normalize (src/wndchrm_src/signatures.cpp:196-215) and
             TrainingSet::classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586).
 """
+import functools
 import math
 
+import numpy as np
 import torch
 
 F64 = torch.float64
@@ -109,6 +114,77 @@ def wnd5(x, train, classes, weights, mins, maxs, n_classes):
     return sb / sb.sum(1, keepdim=True)
 
 
+def downsample_matrix(L, ratio):
+    """dense weights [int(ratio L)][L] of one pass of ImageMatrix::Downsample
+    (src/cmatrix.cpp:579-720): output k averages [x_k, x_k + d) with
+    fractional end weights, x_k by the sequential x += d, d = 1 / ratio"""
+    r = min(ratio, 1.0)
+    d = 1 / r
+    rows = []
+    x = 0.0
+    while x < L:
+        row = np.zeros(L)
+        a = int(math.floor(x))
+        frac = math.ceil(x) - x
+        if frac > 0 and a < L:
+            row[a] += frac
+        a = int(math.ceil(x))
+        while a < math.floor(x + d):
+            if a < L:
+                row[a] += 1
+            a += 1
+        frac = x + d - math.floor(x + d)
+        if frac > 0 and a < L:
+            row[a] += frac
+        rows.append(row / d)
+        x += d
+    return torch.from_numpy(np.stack(rows[:int(r * L)]))
+
+
+@functools.lru_cache(maxsize=8)
+def cf_geometry(h, w):
+    """in-disk pixels (row-major indices, x-outer order) and the basis tables
+    of ChebyshevFourier2D (src/transforms/ChebyshevFourier.cpp:59-127)"""
+    m, n = h, w
+    kk, tn, fc, fs = [], [], [], []
+    N = min(11, (min(m, n) - 1) // 2)
+    NN = 2 * N + 1
+    for x in range(n):
+        xi = -1.0 + x * (2.0 / (n - 1))
+        for y in range(m):
+            yi = -1.0 + y * (2.0 / (m - 1))
+            r = math.sqrt(xi * xi + yi * yi)
+            f = -1 * math.atan2(yi, xi)
+            if not r < 1:
+                continue
+            kk.append(y * n + x)
+            xx = min(max(r * 2 - 1, -1.0), 1.0)
+            t = [math.cos(a * math.acos(xx)) for a in range(NN)]
+            t[0] = 0.5
+            tn.append(t)
+            mf = np.arange(NN) - N
+            ft = np.where(mf == 0, 0.5, 1.0)
+            fc.append(ft * np.cos(mf * f))
+            fs.append(ft * np.sin(-1 * mf * f))
+    t = lambda a: torch.from_numpy(np.asarray(a, np.float64))
+    return torch.tensor(kk), t(tn), t(fc), t(fs), NN
+
+
+def cf_magnitudes(img):
+    """the NN x NN magnitudes ChebyshevFourierCoefficients histograms
+    (src/FeatureAlgorithms.cpp:54-69, src/cmatrix.cpp:1227-1240)"""
+    h, w = img.shape
+    if w * h > 300 * 300:
+        rr = min(300.0 / w, 300.0 / h)
+        img = downsample_matrix(h, rr) @ img @ downsample_matrix(w, rr).t()
+        h, w = img.shape
+    kk, tn, fc, fs, NN = cf_geometry(h, w)
+    v = img.reshape(-1)[kk]
+    cr = (fc * v[:, None]).t() @ tn
+    ci = (fs * v[:, None]).t() @ tn
+    return torch.sqrt(cr * cr + ci * ci).reshape(-1)
+
+
 TRANSFORMS = {'Fourier': fourier, 'Chebyshev': chebyshev}
 
 
@@ -127,6 +203,8 @@ def features(img, groups):
             out.append(zernike(p))
         elif alg == 'Chebyshev Coefficients (raw)':
             out.append(chebyshev(p, min(20, *p.shape)).reshape(-1))
+        elif alg == 'Chebyshev-Fourier Coefficients (raw)':
+            out.append(cf_magnitudes(p))
         else:
             raise ValueError(alg)
     return torch.cat(out)

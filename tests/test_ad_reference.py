@@ -28,6 +28,14 @@ def orc_zernike(img):
     return out
 
 
+def orc_cf(img):
+    img = np.ascontiguousarray(img, np.float64)
+    out = np.zeros(529)
+    nn = np.zeros(1, np.int32)
+    assert oracle().orc_cf_magnitudes(ptr(img), img.shape[0], img.shape[1], ptr(out), ptr(nn)) == 0
+    return out[:nn[0] * nn[0]]
+
+
 def orc_node(img, chain):
     for t in chain:
         img = orc_transform({'Fourier': T_FOURIER, 'Chebyshev': T_CHEBYSHEV}[t], img)
@@ -40,6 +48,8 @@ def orc_features(img, groups):
         p = orc_node(img, chain)
         if alg == 'Zernike Coefficients':
             out.append(orc_zernike(p))
+        elif alg == 'Chebyshev-Fourier Coefficients (raw)':
+            out.append(orc_cf(p))
         else:  # raw order-20 Chebyshev coefficients = leading block of the full transform
             c = orc_transform(T_CHEBYSHEV, p)
             n = min(20, *p.shape)
@@ -79,6 +89,16 @@ def test_zernike_reference_matches_oracle(shape):
     want = orc_zernike(img)
     got = R.zernike(torch.from_numpy(img)).numpy()
     assert np.all(np.abs(got - want) <= 1e-10 * np.abs(want) + 1e-14)
+
+
+@pytest.mark.parametrize('shape', [(40, 48), (33, 29), (301, 300), (320, 352)])
+def test_cf_reference_matches_oracle(shape):
+    """raw Chebyshev-Fourier magnitudes, with the Downsample above 300 x 300"""
+    img = plane(14, *shape)
+    want = orc_cf(img)
+    got = R.cf_magnitudes(torch.from_numpy(img)).numpy()
+    assert got.shape == want.shape
+    assert rel(got, want) < 1e-12
 
 
 def _training_set(n_classes, per_class, groups, shape, seed):
@@ -124,7 +144,9 @@ def test_wnd_reference_matches_oracle():
     assert rel(got, want) < 1e-10
 
 
-@pytest.mark.parametrize('group', GROUPS + [('Zernike Coefficients', ['Chebyshev', 'Fourier'])],
+@pytest.mark.parametrize('group', GROUPS + [('Zernike Coefficients', ['Chebyshev', 'Fourier']),
+                                   ('Chebyshev-Fourier Coefficients (raw)', []),
+                                   ('Chebyshev-Fourier Coefficients (raw)', ['Fourier'])],
                          ids=lambda g: g[0].split()[0] + '(' + ','.join(g[1]) + ')')
 def test_reference_gradient_matches_oracle_differences(group):
     """d(c . features)/dI from torch autograd = central differences of the oracle"""

@@ -241,6 +241,7 @@ struct wndgpu_ctx {
   std::map<int, std::pair<DevTable, std::vector<int>>> fft;
   std::map<std::pair<int, int>, CfDev> cf;
   std::map<std::pair<int, double>, DevTable> dspos;
+  std::map<std::tuple<int, double, int>, DevTable> dsdense;  // (len, dx, nout): dense Downsample passes
   std::map<int, DevTable> comb;
   DevTable gabor, zern, rtrig;
   DevTable color_lut;  // fuzzy colour class of every (h, s, v) byte triple, built on first use
@@ -1916,9 +1917,75 @@ int wndgpu_last_node_times(wndgpu_ctx *c, double *ms, const char **names, int ca
 // ------------------------------------------------------- differentiable form
 // Forward and adjoint passes of the smooth transforms and features on
 // batches of device planes; see wndgpu.h.
+// Chebyshev-Fourier geometry of the differentiable form: the plane the
+// coefficients are taken from (downsampled above 300 x 300 pixels, as
+// FeatureAlgorithms.cpp:54-69 / cmatrix.cpp:1227-1240) and NN
+struct CfAdGeom {
+  bool ds;
+  int nh, nw, NN, nchunk;
+  double dx;
+};
+static CfAdGeom cf_ad_geom(int h, int w) {
+  CfAdGeom g{false, h, w, 0, 0, 1.0};
+  if ((long)w * h > 300 * 300) {
+    double rr = std::min(300.0 / (double)w, 300.0 / (double)h);
+    if (rr > 1) rr = 1;
+    g.ds = true;
+    g.dx = 1 / rr;
+    g.nw = (int)(unsigned)(rr * w);
+    g.nh = (int)(unsigned)(rr * h);
+  }
+  int N = 11;
+  const int nm = (std::min(g.nh, g.nw) - 1) / 2;
+  if (N > nm) N = nm;
+  g.NN = 2 * N + 1;
+  g.nchunk = cf_ad_chunks(g.nh * g.nw);
+  return g;
+}
+
+static int get_dsdense(wndgpu_ctx *c, int len, double dx, int nout, const double **p) {
+  auto key = std::make_tuple(len, dx, nout);
+  auto it = c->dsdense.find(key);
+  if (it == c->dsdense.end()) {
+    std::vector<double> d = ds_dense((unsigned)len, dx, (unsigned)nout);
+    DevTable t;
+    int r = upload(c, t, d.data(), d.size() * sizeof(double));
+    if (r) return r;
+    it = c->dsdense.emplace(key, t).first;
+  }
+  *p = (const double *)it->second.p;
+  return 0;
+}
+
+// work layout of the Chebyshev-Fourier op, in doubles: tmp (n h nw, when
+// downsampled), the downsampled planes (n nh nw), chunk partials, W,
+// the downsampled gradient (n nh nw), then nh nw int slots
+struct CfAdWork {
+  size_t tmp, dsp, part, coef, gds, slot, total_bytes;
+};
+static CfAdWork cf_ad_work(const CfAdGeom &g, int n, int h) {
+  CfAdWork L;
+  const size_t dsn = g.ds ? (size_t)n * g.nh * g.nw : 0, nn2 = (size_t)g.NN * g.NN;
+  size_t o = 0;
+  L.tmp = o; o += g.ds ? (size_t)n * h * g.nw : 0;
+  L.dsp = o; o += dsn;
+  L.part = o; o += (size_t)n * g.nchunk * 2 * nn2;
+  L.coef = o; o += (size_t)n * 2 * nn2;
+  L.gds = o; o += dsn;
+  L.slot = o;
+  L.total_bytes = o * sizeof(double) + (size_t)g.nh * g.nw * sizeof(int);
+  return L;
+}
+
 static int ad_dims(int op, int h, int w, int *oh, int *ow) {
   const int N = h < w ? h : w;
   switch (op) {
+    case WNDGPU_AD_CHEB_FOURIER: {
+      const CfAdGeom g = cf_ad_geom(h, w);
+      *oh = 1;
+      *ow = g.NN * g.NN;
+      return g.NN >= 1 ? 0 : -1;
+    }
     case WNDGPU_AD_FOURIER: *oh = h; *ow = w; return 0;
     case WNDGPU_AD_CHEBYSHEV: *oh = N; *ow = N; return 0;
     case WNDGPU_AD_CHEBYSHEV_COEFFS: *oh = N < 20 ? N : 20; *ow = *oh; return 0;
@@ -1938,6 +2005,7 @@ size_t wndgpu_ad_work_bytes(int op, int n, int h, int w) {
   if (n <= 0 || ad_dims(op, h, w, &oh, &ow)) return 0;
   switch (op) {
     case WNDGPU_AD_FOURIER: return (size_t)n * h * w * sizeof(double2);
+    case WNDGPU_AD_CHEB_FOURIER: return cf_ad_work(cf_ad_geom(h, w), n, h).total_bytes;
     case WNDGPU_AD_CHEBYSHEV:
     case WNDGPU_AD_CHEBYSHEV_COEFFS: return (size_t)n * h * oh * sizeof(double);
     default: return zernike_ad_work_doubles(n, h, w) * sizeof(double);
@@ -1961,6 +2029,32 @@ static int ad_cheb_tables(wndgpu_ctx *c, int h, int w, int N, const double **bx,
   return get_cheb(c, h, N, by);
 }
 
+// Downsample (when the plane is above 300 x 300) into the work buffer and
+// fetch the coefficient tables of the resulting size
+static int ad_cf_prepare(wndgpu_ctx *c, const double *in, int n, int h, int w, double *wk, hipStream_t s,
+                         const double **plane, const CfDev **tab, CfAdGeom *geom) {
+  const CfAdGeom g = cf_ad_geom(h, w);
+  const CfAdWork L = cf_ad_work(g, n, h);
+  int r;
+  *plane = in;
+  if (g.ds) {
+    // tmp[y][k] = sum_x in[y][x] Dx[k][x];  planes[j][k] = sum_y Dy[j][y] tmp[y][k]
+    const double *Dx, *Dy;
+    if ((r = get_dsdense(c, w, g.dx, g.nw, &Dx)) || (r = get_dsdense(c, h, g.dx, g.nh, &Dy))) return r;
+    double *tmp = wk + L.tmp, *dsp = wk + L.dsp;
+    launch_ad_gemm(AdGemm{in, (long)h * w, w, 1, Dx, 0, 1, w, tmp, (long)h * g.nw, h, w, g.nw, 1.0}, n, s);
+    launch_ad_gemm(AdGemm{Dy, 0, h, 1, tmp, (long)h * g.nw, g.nw, 1, dsp, (long)g.nh * g.nw, g.nh, h, g.nw, 1.0}, n,
+                   s);
+    *plane = dsp;
+  }
+  CfDev *t = nullptr;
+  if ((r = get_cf(c, g.nw, g.nh, &t))) return r;
+  if (t->NN != g.NN) return fail(c, WNDGPU_E_ARG, "Chebyshev-Fourier order mismatch");
+  *tab = t;
+  *geom = g;
+  return 0;
+}
+
 int wndgpu_ad_forward(wndgpu_ctx *c, int op, const double *in, int n, int h, int w, double *out, void *work,
                       void *stream) {
   if (!c) return WNDGPU_E_ARG;
@@ -1976,6 +2070,15 @@ int wndgpu_ad_forward(wndgpu_ctx *c, int op, const double *in, int n, int h, int
     launch_ad_fourier_fwd(in, out, (double2 *)work, n, h, w, pw, ph, s);
   } else if (op == WNDGPU_AD_ZERNIKE) {
     launch_ad_zernike_fwd(in, out, (double *)work, n, h, w, s);
+  } else if (op == WNDGPU_AD_CHEB_FOURIER) {
+    const double *plane;
+    const CfDev *t;
+    CfAdGeom g;
+    if ((r = ad_cf_prepare(c, in, n, h, w, (double *)work, s, &plane, &t, &g))) return r;
+    const CfAdWork L = cf_ad_work(g, n, h);
+    launch_ad_cf_fwd(plane, n, (long)g.nh * g.nw, (const int *)t->kk.p, (const double *)t->tn.p,
+                     (const double *)t->fc.p, (const double *)t->fs.p, t->np, t->NN, g.nchunk,
+                     (double *)work + L.part, out, s);
   } else {
     const int N = oh;
     const double *bx, *by;
@@ -2004,6 +2107,28 @@ int wndgpu_ad_backward(wndgpu_ctx *c, int op, const double *in, const double *gr
     launch_ad_fourier_bwd(in, grad_out, grad_in, (double2 *)work, n, h, w, pw, ph, s);
   } else if (op == WNDGPU_AD_ZERNIKE) {
     launch_ad_zernike_bwd(in, grad_out, grad_in, (double *)work, n, h, w, s);
+  } else if (op == WNDGPU_AD_CHEB_FOURIER) {
+    const double *plane;
+    const CfDev *t;
+    CfAdGeom g;
+    if ((r = ad_cf_prepare(c, in, n, h, w, (double *)work, s, &plane, &t, &g))) return r;
+    const CfAdWork L = cf_ad_work(g, n, h);
+    double *wk = (double *)work;
+    int *slot = (int *)(wk + L.slot);
+    launch_ad_cf_slots((const int *)t->kk.p, t->np, (long)g.nh * g.nw, slot, s);
+    double *gplane = g.ds ? wk + L.gds : grad_in;
+    launch_ad_cf_bwd(plane, grad_out, n, (long)g.nh * g.nw, (const int *)t->kk.p, slot, (const double *)t->tn.p,
+                     (const double *)t->fc.p, (const double *)t->fs.p, t->np, t->NN, g.nchunk, wk + L.part,
+                     wk + L.coef, gplane, s);
+    if (g.ds) {
+      // U[y][k] = sum_j Dy[j][y] G[j][k];  grad[y][x] = sum_k U[y][k] Dx[k][x]
+      const double *Dx, *Dy;
+      if ((r = get_dsdense(c, w, g.dx, g.nw, &Dx)) || (r = get_dsdense(c, h, g.dx, g.nh, &Dy))) return r;
+      double *u = wk + L.tmp;
+      launch_ad_gemm(AdGemm{Dy, 0, 1, h, gplane, (long)g.nh * g.nw, g.nw, 1, u, (long)h * g.nw, h, g.nh, g.nw, 1.0},
+                     n, s);
+      launch_ad_gemm(AdGemm{u, (long)h * g.nw, g.nw, 1, Dx, 0, w, 1, grad_in, (long)h * w, h, g.nw, w, 1.0}, n, s);
+    }
   } else {
     const int N = oh;
     const double *bx, *by;

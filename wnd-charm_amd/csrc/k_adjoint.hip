@@ -460,6 +460,160 @@ size_t zernike_ad_work_doubles(int n, int h, int w) {
   return 4 * (size_t)n + (size_t)n * zernike_ad_chunks(h, w) * ZAD_PAIRS * 6 + (size_t)n * (2 * ZAD_PAIRS + 3);
 }
 
+// ============================================================ Chebyshev-Fourier
+// ChebyshevFourier2D (src/transforms/ChebyshevFourier.cpp:59-160) over the
+// (downsampled) plane v: with the in-disk pixels p and the host-built tables
+// tn[p][a] = T_a(2r-1) (T_0 = 1/2), fc[p][i] = Ftrm cos(mf phi),
+// fs[p][i] = Ftrm sin(-mf phi),
+//   C[i][a] = sum_p v_p (fc[p][i] + i fs[p][i]) tn[p][a],  out = |C| (NN x NN),
+// and the adjoint  dL/dv_p = sum_i fc[p][i] (sum_a Wr[i][a] tn[p][a])
+//                          + fs[p][i] (sum_a Wi[i][a] tn[p][a]),  W = G C / |C|.
+constexpr int CF_NN = 23;
+constexpr int CF_STAGE = 32;  // pixels staged in LDS per step
+
+// part[((b * nchunk + chunk) * 2 + {re, im}) * NN*NN + i*NN + a]
+__global__ void __launch_bounds__(256) k_ad_cf_accum(const double *v, long plane, const int *kk, const double *tn,
+                                                     const double *fc, const double *fs, int np, int NN, int nchunk,
+                                                     double *part) {
+  __shared__ double sv[CF_STAGE], stn[CF_STAGE][CF_NN], sfc[CF_STAGE][CF_NN], sfs[CF_STAGE][CF_NN];
+  const int b = blockIdx.y, chunk = blockIdx.x;
+  const double *img = v + (long)b * plane;
+  const long csz = ((long)np + nchunk - 1) / nchunk;
+  const long p0 = chunk * csz, p1 = p0 + csz < np ? p0 + csz : np;
+  const int nout = NN * NN;
+  double ar[3] = {0, 0, 0}, ai[3] = {0, 0, 0};
+  int oi[3], oa[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int o = threadIdx.x + 256 * k;
+    oi[k] = o < nout ? o / NN : 0;
+    oa[k] = o < nout ? o - (o / NN) * NN : 0;
+  }
+  for (long q0 = p0; q0 < p1; q0 += CF_STAGE) {
+    const int cnt = p1 - q0 < CF_STAGE ? (int)(p1 - q0) : CF_STAGE;
+    __syncthreads();
+    for (int e = threadIdx.x; e < cnt * NN; e += blockDim.x) {
+      const int j = e / NN, a = e - j * NN;
+      const long q = q0 + j;
+      stn[j][a] = tn[q * NN + a];
+      sfc[j][a] = fc[q * NN + a];
+      sfs[j][a] = fs[q * NN + a];
+    }
+    for (int j = threadIdx.x; j < cnt; j += blockDim.x) sv[j] = img[kk[q0 + j]];
+    __syncthreads();
+    for (int j = 0; j < cnt; j++) {
+      const double x = sv[j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        ar[k] += (sfc[j][oi[k]] * stn[j][oa[k]]) * x;
+        ai[k] += (sfs[j][oi[k]] * stn[j][oa[k]]) * x;
+      }
+    }
+  }
+  double *dst = part + ((long)b * nchunk + chunk) * 2 * nout;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int o = threadIdx.x + 256 * k;
+    if (o < nout) {
+      dst[o] = ar[k];
+      dst[nout + o] = ai[k];
+    }
+  }
+}
+
+// per image: sums of the chunk partials in chunk order; forward writes |C|,
+// backward (gout != null) writes W = G C / |C| (re, im) to coef[b][2][NN*NN]
+__global__ void __launch_bounds__(256) k_ad_cf_final(const double *part, int nchunk, int NN, double *out,
+                                                     const double *gout, double *coef) {
+  const int b = blockIdx.x, nout = NN * NN;
+  for (int o = threadIdx.x; o < nout; o += blockDim.x) {
+    double cr = 0, ci = 0;
+    for (int c = 0; c < nchunk; c++) {
+      const double *pp = part + ((long)b * nchunk + c) * 2 * nout;
+      cr += pp[o];
+      ci += pp[nout + o];
+    }
+    const double mag = sqrt(cr * cr + ci * ci);
+    if (!gout) {
+      out[(long)b * nout + o] = mag;
+    } else {
+      const double s = mag > 0 ? gout[(long)b * nout + o] / mag : 0.0;
+      coef[(long)b * 2 * nout + o] = s * cr;
+      coef[(long)b * 2 * nout + nout + o] = s * ci;
+    }
+  }
+}
+
+// grad plane of the (downsampled) image: in-disk pixels get the adjoint,
+// the rest 0 (one thread per plane pixel; `slot` maps a pixel to its
+// in-disk index or -1)
+__global__ void __launch_bounds__(256) k_ad_cf_pixels(const double *coef, const int *slot, const double *tn,
+                                                      const double *fc, const double *fs, long plane, int NN,
+                                                      double *grad) {
+  __shared__ double W[2 * CF_NN * CF_NN];
+  const int b = blockIdx.y, nout = NN * NN;
+  for (int o = threadIdx.x; o < 2 * nout; o += blockDim.x) W[o] = coef[(long)b * 2 * nout + o];
+  __syncthreads();
+  for (long p = blockIdx.x * (long)blockDim.x + threadIdx.x; p < plane; p += (long)gridDim.x * blockDim.x) {
+    const int q = slot[p];
+    double acc = 0;
+    if (q >= 0) {
+      const double *t = tn + (long)q * NN, *c = fc + (long)q * NN, *sn = fs + (long)q * NN;
+      double tv[CF_NN];
+#pragma unroll
+      for (int a = 0; a < CF_NN; a++) tv[a] = a < NN ? t[a] : 0.0;
+      for (int i = 0; i < NN; i++) {
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int a = 0; a < CF_NN; a++) {
+          if (a < NN) {
+            sr += W[i * NN + a] * tv[a];
+            si += W[nout + i * NN + a] * tv[a];
+          }
+        }
+        acc += c[i] * sr + sn[i] * si;
+      }
+    }
+    grad[(long)b * plane + p] = acc;
+  }
+}
+
+__global__ void k_ad_fill_int(int *p, long n, int v) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_ad_cf_scatter(const int *kk, int np, int *slot) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < np) slot[kk[q]] = q;
+}
+
+// slot[pixel] = in-disk index or -1 (kk holds distinct indices below plane)
+void launch_ad_cf_slots(const int *kk, int np, long plane, int *slot, hipStream_t s) {
+  k_ad_fill_int<<<(unsigned)((plane + 255) / 256), 256, 0, s>>>(slot, plane, -1);
+  k_ad_cf_scatter<<<(np + 255) / 256, 256, 0, s>>>(kk, np, slot);
+}
+
+int cf_ad_chunks(int np) {
+  int c = (np + 2047) / 2048;
+  return c < 1 ? 1 : c;
+}
+
+void launch_ad_cf_fwd(const double *v, int n, long plane, const int *kk, const double *tn, const double *fc,
+                      const double *fs, int np, int NN, int nchunk, double *part, double *out, hipStream_t s) {
+  k_ad_cf_accum<<<dim3(nchunk, n), 256, 0, s>>>(v, plane, kk, tn, fc, fs, np, NN, nchunk, part);
+  k_ad_cf_final<<<n, 256, 0, s>>>(part, nchunk, NN, out, nullptr, nullptr);
+}
+
+void launch_ad_cf_bwd(const double *v, const double *gout, int n, long plane, const int *kk, const int *slot,
+                      const double *tn, const double *fc, const double *fs, int np, int NN, int nchunk,
+                      double *part, double *coef, double *grad, hipStream_t s) {
+  k_ad_cf_accum<<<dim3(nchunk, n), 256, 0, s>>>(v, plane, kk, tn, fc, fs, np, NN, nchunk, part);
+  k_ad_cf_final<<<n, 256, 0, s>>>(part, nchunk, NN, nullptr, gout, coef);
+  long gx = (plane + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  k_ad_cf_pixels<<<dim3((unsigned)gx, n), 256, 0, s>>>(coef, slot, tn, fc, fs, plane, NN, grad);
+}
+
 // ============================================================ WND similarity
 // classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586) after
 // signatures::normalize (signatures.cpp:196-215):

@@ -113,6 +113,28 @@ std::vector<double> ds_positions(unsigned len, double dx) {
   return xs;
 }
 
+// One pass of Downsample as a dense [nout][len] matrix: row k holds the
+// weights ds_line gives input a for output k (fractional ends, sequential
+// positions), divided by dx.  The differentiable form applies it as a GEMM
+// and its transpose.
+std::vector<double> ds_dense(unsigned len, double dx, unsigned nout) {
+  const std::vector<double> xs = ds_positions(len, dx);
+  std::vector<double> D((size_t)nout * len, 0.0);
+  for (unsigned k = 0; k < nout && k < xs.size(); k++) {
+    const double x = xs[k];
+    double *row = &D[(size_t)k * len];
+    unsigned a = (unsigned)(floor(x));
+    double frac = ceil(x) - x;
+    if (frac > 0 && a < len) row[a] += frac;
+    for (a = (unsigned)(ceil(x)); a < floor(x + dx); a++)
+      if (a < len) row[a] += 1;
+    frac = x + dx - floor(x + dx);
+    if (frac > 0 && a < len) row[a] += frac;
+    for (unsigned i = 0; i < len; i++) row[i] /= dx;
+  }
+  return D;
+}
+
 // Gabor filters (gabor.cpp:103-162), theta = 3.14159265/2, gamma 0.5,
 // sig2lam 0.56, n = 38; index 0: f0 = 0.1, 1..7: f0 = 1..7
 std::vector<double> gabor_kernels() {

@@ -15,6 +15,7 @@ struct CfTable {
 CfTable cf_table(int w, int h);
 
 std::vector<double> ds_positions(unsigned len, double dx);
+std::vector<double> ds_dense(unsigned len, double dx, unsigned nout);
 std::vector<double> gabor_kernels();
 std::vector<double> zernike_h();
 std::vector<double> radon_trig();

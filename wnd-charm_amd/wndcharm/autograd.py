@@ -9,9 +9,11 @@ kernels behind the C ABI (``wndgpu_ad_*`` in include/wndgpu.h):
   "Chebyshev" (ImageMatrix::ChebyshevTransform, src/cmatrix.cpp:1203-1222);
 * features    "Zernike Coefficients" (mb_zernike2D,
   src/textures/zernike/zernike.cpp:207-341, 72 values, identical to the
-  feature path's) and "Chebyshev Coefficients (raw)", the 20 x 20
+  feature path's), "Chebyshev Coefficients (raw)", the 20 x 20
   coefficients that "Chebyshev Coefficients" histograms
-  (src/FeatureAlgorithms.cpp:73-93);
+  (src/FeatureAlgorithms.cpp:73-93), and "Chebyshev-Fourier Coefficients
+  (raw)", the 23 x 23 magnitudes that "Chebyshev-Fourier Coefficients"
+  histograms, after its Downsample above 300 x 300 (src/FeatureAlgorithms.cpp:54-69);
 * WND5        normalisation against the training set
   (signatures::normalize, src/wndchrm_src/signatures.cpp:196-215) followed by
   TrainingSet::classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586).
@@ -41,12 +43,15 @@ import torch
 from . import (WndGpuError, _Image, _check, _context, _load, fisher_weights, normalize_train,
                FeatureComputationPlan)
 
-__all__ = ['FOURIER', 'CHEBYSHEV', 'ZERNIKE', 'CHEBYSHEV_COEFFS', 'op_forward', 'op_backward', 'fourier',
-           'chebyshev', 'zernike', 'chebyshev_coeffs', 'DifferentiablePlan', 'WNDModel', 'parse_group']
+__all__ = ['FOURIER', 'CHEBYSHEV', 'ZERNIKE', 'CHEBYSHEV_COEFFS', 'CHEB_FOURIER', 'op_forward', 'op_backward',
+           'fourier', 'chebyshev', 'zernike', 'chebyshev_coeffs', 'cheb_fourier', 'DifferentiablePlan', 'WNDModel',
+           'parse_group']
 
-FOURIER, CHEBYSHEV, ZERNIKE, CHEBYSHEV_COEFFS = 0, 1, 2, 3
+FOURIER, CHEBYSHEV, ZERNIKE, CHEBYSHEV_COEFFS, CHEB_FOURIER = 0, 1, 2, 3, 4
 TRANSFORMS = {'Fourier': FOURIER, 'Chebyshev': CHEBYSHEV}
-ALGORITHMS = {'Zernike Coefficients': ZERNIKE, 'Chebyshev Coefficients (raw)': CHEBYSHEV_COEFFS}
+ALGORITHMS = {'Zernike Coefficients': ZERNIKE, 'Chebyshev Coefficients (raw)': CHEBYSHEV_COEFFS,
+              'Chebyshev-Fourier Coefficients (raw)': CHEB_FOURIER}
+RAW_NAMES = ('Chebyshev Coefficients (raw)', 'Chebyshev-Fourier Coefficients (raw)')
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -158,6 +163,12 @@ def chebyshev_coeffs(x):
     return _Op.apply(_planes(x, 'chebyshev_coeffs'), CHEBYSHEV_COEFFS)
 
 
+def cheb_fourier(x):
+    """The NN x NN Chebyshev-Fourier coefficient magnitudes (after the
+    Downsample of planes above 300 x 300), [n, NN*NN], differentiable."""
+    return _Op.apply(_planes(x, 'cheb_fourier'), CHEB_FOURIER).reshape(x.shape[0], -1)
+
+
 class _Node(torch.autograd.Function):
     """Features of the algorithms `algs` on the node chain(x); keeps only x."""
 
@@ -207,10 +218,10 @@ def parse_group(name):
 
 def _split_raw(name):
     # "Chebyshev Coefficients (raw) (Fourier ())": the algorithm name carries "(raw)"
-    if name.strip().startswith('Chebyshev Coefficients (raw)'):
-        rest = name.strip()[len('Chebyshev Coefficients (raw)'):]
-        alg, chain = parse_group('X' + rest)
-        return 'Chebyshev Coefficients (raw)', chain
+    for raw in RAW_NAMES:
+        if name.strip().startswith(raw):
+            alg, chain = parse_group('X' + name.strip()[len(raw):])
+            return raw, chain
     return parse_group(name)
 
 
@@ -236,7 +247,7 @@ class DifferentiablePlan(object):
                     raise WndGpuError('transform "%s" in "%s" has no adjoint' % (t, g))
             if alg in ALGORITHMS:
                 op = ALGORITHMS[alg]
-                k = 72 if op == ZERNIKE else 400
+                k = {ZERNIKE: 72, CHEBYSHEV_COEFFS: 400, CHEB_FOURIER: 529}[op]
                 self._nodes.setdefault(tuple(TRANSFORMS[t] for t in chain), []).append((op, off, k))
             else:
                 if not masked:
@@ -261,11 +272,14 @@ class DifferentiablePlan(object):
     def __call__(self, x):
         """x: [n, h, w] float64 CUDA planes -> features [n, n_features]."""
         x = _planes(x, 'DifferentiablePlan')
-        if any(op == CHEBYSHEV_COEFFS for ops in self._nodes.values() for op, _, _ in ops) and min(x.shape[1:]) < 20:
-            raise WndGpuError('"Chebyshev Coefficients (raw)" needs planes of at least 20 x 20')
         pieces = []
         for chain, ops in self._nodes.items():
             f = _Node.apply(x, chain, tuple(o for o, _, _ in ops))
+            if f.shape[1] != sum(k for _, _, k in ops):
+                # the raw coefficient groups have fixed sizes: 20 x 20 Chebyshev coefficients need a
+                # node plane of at least 20 x 20, 23 x 23 Chebyshev-Fourier ones at least 23 x 23
+                raise WndGpuError('node planes of these %dx%d images are too small for the raw coefficient groups'
+                                  % (x.shape[2], x.shape[1]))
             c = 0
             for _, off, k in ops:
                 pieces.append((off, f[:, c:c + k]))
