@@ -347,6 +347,24 @@ def grad_main(args):
         forward_only()
     torch.cuda.synchronize()
     dtf = (time.perf_counter() - t0) / args.steps
+    # roofline: the FP64 GEMM pair of the 512 x 512 Chebyshev transform over the 64 planes (the
+    # largest k_ad_gemm launches; exact flops 2 M N K per plane per GEMM), timed with HIP events on
+    # torch's current stream, the stream the library launches on
+    h, w = xd.shape[1:]
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ad.op_forward(ad.CHEBYSHEV, xd)
+    torch.cuda.synchronize()
+    reps = 10
+    ev0.record()
+    for _ in range(reps):
+        ad.op_forward(ad.CHEBYSHEV, xd)
+    ev1.record()
+    torch.cuda.synchronize()
+    gemm_ms = ev0.elapsed_time(ev1) / reps / 2  # mean per GEMM launch
+    nmin = min(h, w)
+    # T1 = X Bx (h x w x N), then Bx^T T1 By (N x h x N): exact FP64 flops, mean per launch
+    gemm_flops = (2.0 * h * w * nmin + 2.0 * nmin * h * nmin) * ntest / 2
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
     line = {
         'metric': 'pixel-gradient images/s (WND5 probability of the predicted class, adjoint kernels)',
         'value': round(ntest / dt, 3), 'unit': 'images/s', 'n_gpus': 1, 'steps': args.steps,
@@ -361,8 +379,36 @@ def grad_main(args):
         'training_features_images_per_s': round(ntrain / t_train, 2),
         'generator_s': round(t_gen, 1),
         'grad_finite': bool(torch.isfinite(g).all().item()),
+        'roofline': {'kernel': 'k_ad_gemm (512 x 512 x 512 Chebyshev GEMM, 64 planes per launch)', 'bound': 'mfma',
+                     'achieved': round(achieved, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(achieved / FP64_PEAK_TFLOPS, 4), 'traffic': None,
+                     'launch_ms': round(gemm_ms, 4), 'flops_per_launch': gemm_flops},
     }
+    if not args.no_cpu:
+        line['cpu_baseline'] = grad_cpu_baseline(imgs[ntrain][0], model, plan)
     print(json.dumps(line))
+
+
+def grad_cpu_baseline(img, model, plan, seconds=20.0):
+    """The FP64 torch reference of the differentiable path (tests/ad_reference.py,
+    pinned to the oracle) on the host cores: forward + autograd gradient of the
+    same WND5 probability, one 512 x 512 image at a time, for about `seconds`."""
+    import torch
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import ad_reference as R
+    from wndcharm import autograd as ad
+    groups = [ad._split_raw(g) for g in plan.groups]
+    t = lambda a: a.detach().cpu()
+    train, cls, wts, mins, maxs = t(model.train), t(model.classes), t(model.weights), t(model.mins), t(model.maxs)
+    done, t0 = 0, time.time()
+    while time.time() - t0 < seconds or done == 0:
+        x = torch.from_numpy(img.astype(np.float64)).requires_grad_(True)
+        p = R.wnd5(R.features(x, groups)[None], train, cls, wts, mins, maxs, model.n_classes)[0]
+        p[int(torch.argmax(p.detach()))].backward()
+        done += 1
+    el = time.time() - t0
+    return {'value': round(done / el, 4), 'unit': 'images/s', 'cores': torch.get_num_threads(), 'kind': 'port',
+            'sample': '%d pixel gradients of 512x512 images in %.1f s (torch FP64 reference, autograd)' % (done, el)}
 
 
 def main():
