@@ -125,21 +125,30 @@ void launch_ad_fourier_bwd(const double *in, const double *g, double *grad, doub
 // C[b][r][c] = alpha * sum_a A[b](r, a) * B[b](a, c), C row-major M x N per
 // batch entry.  The Chebyshev transform (src/cmatrix.cpp:1203-1222,
 // chebyshev.cpp:118-165) is two such products against the basis tables and
-// its adjoint is the two transposed ones.  64 x 64 tile per block, 4 x 4
-// outputs per thread, K staged through LDS 16 at a time.
+// its adjoint is the two transposed ones; the Chebyshev-Fourier Downsample
+// is two more.  FP64 on the matrix cores: a 64 x 64 tile per block, K staged
+// through LDS 16 at a time, each wave a 32 x 32 sub-tile of 2 x 2
+// v_mfma_f64_16x16x4 accumulators (operands: lane l holds A[l%16][k0+l/16]
+// and B[k0+l/16][l%16]; result: register q of lane l holds row l/16 + 4q of
+// column l%16).  Per 16-deep stage a wave reads 16 doubles per lane from LDS for
+// 16 MFMAs (16K FMAs), against 8 reads per 16 FMAs for a register tile.
+typedef double ad_d4 __attribute__((ext_vector_type(4)));
+
 __global__ void __launch_bounds__(256) k_ad_gemm(AdGemm d) {
   const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
   const double *A = d.A + (long)blockIdx.z * d.sab;
   const double *B = d.B + (long)blockIdx.z * d.sbb;
   double *C = d.C + (long)blockIdx.z * d.scb;
-  __shared__ double As[16][65];
-  __shared__ double Bs[16][65];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  double acc[4][4];
+  __shared__ double As[16][66];
+  __shared__ double Bs[16][66];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  ad_d4 acc[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int j = 0; j < 2; j++) acc[i][j] = ad_d4{0.0, 0.0, 0.0, 0.0};
   for (int k0 = 0; k0 < d.K; k0 += 16) {
     for (int e = threadIdx.x; e < 16 * 64; e += 256) {
       const int kk = e >> 6, rr = e & 63;
@@ -148,27 +157,26 @@ __global__ void __launch_bounds__(256) k_ad_gemm(AdGemm d) {
       Bs[kk][rr] = (k < d.K && c < d.N) ? B[(long)k * d.sbk + (long)c * d.sbc] : 0.0;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < 16; kk++) {
-      double av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    for (int kk = 0; kk < 16; kk += 4) {
+      const double a0 = As[kk + lk][wr + li], a1 = As[kk + lk][wr + 16 + li];
+      const double b0 = Bs[kk + lk][wc + li], b1 = Bs[kk + lk][wc + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < 2; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
-      if (r < d.M && c < d.N) C[(long)r * d.N + c] = d.alpha * acc[i][j];
-    }
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int r = r0 + wr + 16 * i + lk + 4 * q, c = c0 + wc + 16 * j + li;
+        if (r < d.M && c < d.N) C[(long)r * d.N + c] = d.alpha * acc[i][j][q];
+      }
 }
 
 void launch_ad_gemm(const AdGemm &d, int nbatch, hipStream_t s) {
