@@ -275,7 +275,8 @@ def mixed_main(args):
 GRAD_GROUPS = ['Zernike Coefficients ()', 'Zernike Coefficients (Fourier ())', 'Zernike Coefficients (Chebyshev ())',
                'Zernike Coefficients (Fourier (Chebyshev ()))', 'Zernike Coefficients (Chebyshev (Fourier ()))',
                'Chebyshev Coefficients (raw) ()', 'Chebyshev Coefficients (raw) (Fourier ())',
-               'Chebyshev-Fourier Coefficients (raw) ()', 'Chebyshev-Fourier Coefficients (raw) (Fourier ())']
+               'Chebyshev-Fourier Coefficients (raw) ()', 'Chebyshev-Fourier Coefficients (raw) (Fourier ())',
+               'Comb Moments (raw) ()', 'Comb Moments (raw) (Fourier ())']
 
 
 def _grad_image(k):
@@ -372,7 +373,8 @@ def grad_main(args):
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic (seeded grating/blob/Poisson generator, 512x512, scaled to [0, 1] float32)',
         'config': {'workload': 'differentiation: %d test images 512x512, forward + adjoint through %d features '
-                               '(Zernike on 5 nodes, raw Chebyshev-20 and Chebyshev-Fourier on 2) and WND5 against %d training '
+                               '(Zernike on 5 nodes; raw Chebyshev-20, Chebyshev-Fourier and comb moments on 2) and WND5 '
+                               'against %d training '
                                'samples (8 classes)' % (ntest, plan.n_features, ntrain),
                    'test_images': ntest, 'train_samples': ntrain, 'features': plan.n_features},
         'forward_only_images_per_s': round(ntest / dtf, 3),

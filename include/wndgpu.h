@@ -198,6 +198,9 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
  *                              histograms, after its Downsample above 300 x 300 pixels
  *                              (src/FeatureAlgorithms.cpp:54-69, src/cmatrix.cpp:579-720,1227-1240,
  *                              src/transforms/ChebyshevFourier.cpp:59-160): -> NN*NN values (NN <= 23)
+ *   WNDGPU_AD_COMB             the line moments that CombFirstFourMoments histograms
+ *                              (src/FeatureAlgorithms.cpp:184-200, src/statistics/CombFirst4Moments.cpp:78-217):
+ *                              -> 320 values, [comb][mean, std, skew, kurt][20 positions]
  * All pointers are device memory of the context's GPU: in [n][h][w] FP64
  * planes of one size, out / grad_out [n][out_h][out_w], grad_in [n][h][w]
  * (overwritten).  work: caller-owned device scratch of
@@ -206,7 +209,7 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
  * it; calls return without waiting.  The backward pass recomputes what it
  * needs from `in`. */
 enum { WNDGPU_AD_FOURIER = 0, WNDGPU_AD_CHEBYSHEV = 1, WNDGPU_AD_ZERNIKE = 2, WNDGPU_AD_CHEBYSHEV_COEFFS = 3,
-       WNDGPU_AD_CHEB_FOURIER = 4 };
+       WNDGPU_AD_CHEB_FOURIER = 4, WNDGPU_AD_COMB = 5 };
 int wndgpu_ad_output_shape(int op, int h, int w, int *out_h, int *out_w);
 size_t wndgpu_ad_work_bytes(int op, int n, int h, int w);
 int wndgpu_ad_forward(wndgpu_ctx *ctx, int op, const double *in, int n, int h, int w, double *out, void *work,

@@ -94,6 +94,8 @@ int orc_features_sampled(int set, const void *img, int dtype, int h, int w, cons
 /* the NN x NN magnitudes ChebyshevFourierCoefficients histograms (after its
  * Downsample above 300 x 300); out: 529 doubles, *nn = NN */
 int orc_cf_magnitudes(const double *in, int h, int w, double *out, int *nn);
+/* the line moments CombFirstFourMoments histograms: [comb][mean, std, skew, kurt][20 positions] */
+int orc_comb_raw(const double *in, int h, int w, double *out320);
 double orc_otsu(const double *in, int h, int w);                         /* Otsu(dynamic) */
 int orc_radon_proj(const double *in, int h, int w, double *out4xr, int *rsize);
 int orc_haralick_grays(const double *in, int h, int w, unsigned char *out);

@@ -16,6 +16,9 @@ Definitions follow the reference sources:
 * cf        ChebyshevFourierCoefficients before its histogram (Downsample above
             300 x 300, src/cmatrix.cpp:579-720 and 1227-1240; ChebyshevFourier2D,
             src/transforms/ChebyshevFourier.cpp:59-160): the NN x NN magnitudes.
+* comb      CombFirstFourMoments before its histograms
+            (src/statistics/CombFirst4Moments.cpp:78-217): the mean, std, skew and
+            kurtosis of each of 4 x 20 pixel lines.
 * wnd5      signatures::normalize (src/wndchrm_src/signatures.cpp:196-215) and
             TrainingSet::classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586).
 """
@@ -185,6 +188,67 @@ def cf_magnitudes(img):
     return torch.sqrt(cr * cr + ci * ci).reshape(-1)
 
 
+def comb_lines(h, w):
+    """pixel indices of the 4 x 20 lines of CombFirst4Moments2D
+    (src/statistics/CombFirst4Moments.cpp:78-217): positions ii = round(mp),
+    mp = 1 - m, 1 - m + m/10, ... while mp <= m (m = height, step >= 1);
+    comb 0: |I + ii - J| < 1, comb 1: |I + ii - (n - x)| < 1, comb 2:
+    |J + ii - n/2| < 1, comb 3: |I + ii - m/2| < 1 (I = y+1, J = x+1, integer halves)"""
+    m, n = h, w
+    n2, m2 = n // 2, m // 2
+    step = max(m / 10.0, 1.0)
+    yy, xx = np.mgrid[0:m, 0:n]
+    I, J, J1 = yy + 1.0, xx + 1.0, n - xx
+    lines = []
+    for comb in range(4):
+        mp, per = 1.0 - m, []
+        while mp <= m:
+            ii = float(math.floor(abs(mp) + 0.5) * (1 if mp >= 0 else -1))  # C round: half away from zero
+            if comb == 0:
+                sel = np.abs(I + ii - J) < 1
+            elif comb == 1:
+                sel = np.abs(I + ii - J1) < 1
+            elif comb == 2:
+                sel = np.abs(J + ii - n2) < 1
+            else:
+                sel = np.abs(I + ii - m2) < 1
+            if len(per) < 20:
+                per.append(torch.from_numpy(np.flatnonzero(sel.reshape(-1))))
+            mp += step
+        while len(per) < 20:
+            per.append(torch.zeros(0, dtype=torch.long))
+        lines.append(per)
+    return lines
+
+
+def moments4(v):
+    """mean, std, skew, kurt of Moments4 (src/statistics/Moments.h), two-pass"""
+    n = v.numel()
+    if n == 0:
+        z = v.sum() * 0
+        return [z, z, z, z]
+    mu = v.mean()
+    d = v - mu
+    m2, m3, m4 = (d * d).sum(), (d ** 3).sum(), (d ** 4).sum()
+    z = m2 * 0
+    std = torch.sqrt(m2 / (n - 1)) if n > 2 else z
+    skew = math.sqrt(n) * m3 / m2 ** 1.5 if n > 3 else z
+    kurt = n * m4 / (m2 * m2) if n > 4 else z
+    return [mu, std, skew, kurt]
+
+
+def comb_raw(img):
+    """the 4 x 4 x 20 line moments that CombFirstFourMoments histograms,
+    [comb][mean, std, skew, kurt][position]"""
+    flat = img.reshape(-1)
+    out = []
+    for per in comb_lines(*img.shape):
+        mom = [moments4(flat[ix]) for ix in per]
+        for a in range(4):
+            out += [mom[b][a] for b in range(20)]
+    return torch.stack(out)
+
+
 TRANSFORMS = {'Fourier': fourier, 'Chebyshev': chebyshev}
 
 
@@ -205,6 +269,8 @@ def features(img, groups):
             out.append(chebyshev(p, min(20, *p.shape)).reshape(-1))
         elif alg == 'Chebyshev-Fourier Coefficients (raw)':
             out.append(cf_magnitudes(p))
+        elif alg == 'Comb Moments (raw)':
+            out.append(comb_raw(p))
         else:
             raise ValueError(alg)
     return torch.cat(out)

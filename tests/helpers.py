@@ -35,6 +35,7 @@ def oracle():
         lib.orc_feature_name.restype = ctypes.c_char_p
         lib.orc_otsu.argtypes = [P, I, I]
         lib.orc_cf_magnitudes.argtypes = [P, I, I, P, P]
+        lib.orc_comb_raw.argtypes = [P, I, I, P]
         lib.orc_otsu.restype = D
         lib.orc_haralick_grays.argtypes = [P, I, I, P]
         lib.orc_radon_proj.argtypes = [P, I, I, P, P]

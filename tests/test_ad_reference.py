@@ -36,6 +36,13 @@ def orc_cf(img):
     return out[:nn[0] * nn[0]]
 
 
+def orc_comb(img):
+    img = np.ascontiguousarray(img, np.float64)
+    out = np.zeros(320)
+    assert oracle().orc_comb_raw(ptr(img), img.shape[0], img.shape[1], ptr(out)) == 0
+    return out
+
+
 def orc_node(img, chain):
     for t in chain:
         img = orc_transform({'Fourier': T_FOURIER, 'Chebyshev': T_CHEBYSHEV}[t], img)
@@ -50,6 +57,8 @@ def orc_features(img, groups):
             out.append(orc_zernike(p))
         elif alg == 'Chebyshev-Fourier Coefficients (raw)':
             out.append(orc_cf(p))
+        elif alg == 'Comb Moments (raw)':
+            out.append(orc_comb(p))
         else:  # raw order-20 Chebyshev coefficients = leading block of the full transform
             c = orc_transform(T_CHEBYSHEV, p)
             n = min(20, *p.shape)
@@ -101,6 +110,15 @@ def test_cf_reference_matches_oracle(shape):
     assert rel(got, want) < 1e-12
 
 
+@pytest.mark.parametrize('shape', [(40, 48), (33, 29), (64, 20), (20, 64)])
+def test_comb_reference_matches_oracle(shape):
+    """raw comb line moments (two-pass here, Welford in the oracle)"""
+    img = plane(17, *shape)
+    want = orc_comb(img)
+    got = R.comb_raw(torch.from_numpy(img)).numpy()
+    assert np.all(np.abs(got - want) <= 1e-9 * np.abs(want) + 1e-12)
+
+
 def _training_set(n_classes, per_class, groups, shape, seed):
     lib = oracle()
     feats, cls = [], []
@@ -146,7 +164,8 @@ def test_wnd_reference_matches_oracle():
 
 @pytest.mark.parametrize('group', GROUPS + [('Zernike Coefficients', ['Chebyshev', 'Fourier']),
                                    ('Chebyshev-Fourier Coefficients (raw)', []),
-                                   ('Chebyshev-Fourier Coefficients (raw)', ['Fourier'])],
+                                   ('Chebyshev-Fourier Coefficients (raw)', ['Fourier']),
+                                   ('Comb Moments (raw)', []), ('Comb Moments (raw)', ['Chebyshev'])],
                          ids=lambda g: g[0].split()[0] + '(' + ','.join(g[1]) + ')')
 def test_reference_gradient_matches_oracle_differences(group):
     """d(c . features)/dI from torch autograd = central differences of the oracle"""

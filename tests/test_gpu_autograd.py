@@ -8,7 +8,8 @@ import pytest
 import torch
 
 import ad_reference as R
-from test_ad_reference import GROUPS, orc_cf, orc_features, orc_probabilities, orc_transform, orc_zernike, plane
+from test_ad_reference import (GROUPS, orc_cf, orc_comb, orc_features, orc_probabilities, orc_transform,
+                               orc_zernike, plane)
 from helpers import T_CHEBYSHEV, T_FOURIER
 
 pytestmark = pytest.mark.gpu
@@ -61,6 +62,16 @@ def test_cheb_fourier_forward_matches_oracle(shape):
         assert rel(got[i], want) < 1e-11
 
 
+@pytest.mark.parametrize('shape', [(64, 64), (40, 48), (33, 29), (512, 512)])
+def test_comb_moments_forward_matches_oracle(shape):
+    A = _ad()
+    x = batch([18, 19], *shape)
+    got = A.comb_moments(x).cpu().numpy()
+    for i in range(2):
+        want = orc_comb(x[i].cpu().numpy())
+        assert np.all(np.abs(got[i] - want) <= 1e-9 * np.abs(want) + 1e-12)
+
+
 def test_zernike_matches_feature_path():
     import wndcharm
     A = _ad()
@@ -72,14 +83,15 @@ def test_zernike_matches_feature_path():
         assert np.all(np.abs(z[i] - want) <= 1e-9 * np.abs(want) + 1e-14)
 
 
-@pytest.mark.parametrize('op', ['fourier', 'chebyshev', 'chebyshev_coeffs', 'zernike', 'cheb_fourier'])
+@pytest.mark.parametrize('op', ['fourier', 'chebyshev', 'chebyshev_coeffs', 'zernike', 'cheb_fourier',
+                                'comb_moments'])
 @pytest.mark.parametrize('shape', [(40, 48), (64, 64), (310, 320)])
 def test_adjoint_matches_reference_gradient(op, shape):
     A = _ad()
     x = batch([7, 8], *shape)
     fn = getattr(A, op)
     ref = {'fourier': R.fourier, 'chebyshev': R.chebyshev, 'chebyshev_coeffs': lambda p: R.chebyshev(p, 20),
-           'zernike': R.zernike, 'cheb_fourier': R.cf_magnitudes}[op]
+           'zernike': R.zernike, 'cheb_fourier': R.cf_magnitudes, 'comb_moments': R.comb_raw}[op]
     xg = x.clone().requires_grad_(True)
     y = fn(xg)
     rng = np.random.default_rng(3)
@@ -99,13 +111,13 @@ NAMES = ['Zernike Coefficients ()', 'Zernike Coefficients (Fourier ())', 'Zernik
 def test_plan_features_and_gradient_match_reference():
     A = _ad()
     groups = NAMES + ['Zernike Coefficients (Fourier (Chebyshev ()))', 'Chebyshev-Fourier Coefficients (raw) ()',
-                      'Chebyshev-Fourier Coefficients (raw) (Fourier ())']
+                      'Chebyshev-Fourier Coefficients (raw) (Fourier ())', 'Comb Moments (raw) (Chebyshev ())']
     plan = A.DifferentiablePlan(groups)
     ref_groups = [A._split_raw(g) for g in groups]
     x = batch([9, 10], 48, 56)
     xg = x.clone().requires_grad_(True)
     f = plan(xg)
-    assert f.shape == (2, plan.n_features) and plan.n_features == 4 * 72 + 400 + 2 * 529
+    assert f.shape == (2, plan.n_features) and plan.n_features == 4 * 72 + 400 + 2 * 529 + 320
     rng = np.random.default_rng(4)
     c = torch.from_numpy(rng.standard_normal(plan.n_features)).to(DEV)
     (f * c).sum().backward()
@@ -225,7 +237,7 @@ def _fd_case(groups, seed_model, seed_img, pix_seed):
 
 
 SMOOTH = ['Zernike Coefficients ()', 'Zernike Coefficients (Chebyshev ())', 'Chebyshev Coefficients (raw) ()',
-          'Chebyshev-Fourier Coefficients (raw) ()']
+          'Chebyshev-Fourier Coefficients (raw) ()', 'Comb Moments (raw) ()']
 
 
 def test_gradient_matches_finite_differences_512():

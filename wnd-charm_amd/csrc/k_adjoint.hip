@@ -622,6 +622,165 @@ void launch_ad_cf_bwd(const double *v, const double *gout, int n, long plane, co
   k_ad_cf_pixels<<<dim3((unsigned)gx, n), 256, 0, s>>>(coef, slot, tn, fc, fs, plane, NN, grad);
 }
 
+// ============================================================ Comb moments
+// CombFirst4Moments2D (src/statistics/CombFirst4Moments.cpp:78-217): 4 combs
+// x 20 positions ii = round(mp), mp = 1-m, 1-m+step, ... (step = m/10, at
+// least 1, accumulated as the reference does); each line's Moments4 (mean,
+// std for n > 2, skew = sqrt(n) M3 / M2^1.5 for n > 3, kurt = n M4 / M2^2
+// for n > 4; src/statistics/Moments.h).  With I = y+1, J = x+1 the line
+// conditions are integer equations:
+//   comb 0: x - y = ii;  comb 1: x + y = n-1-ii;  comb 2: x = n/2-1-ii;  comb 3: y = m/2-1-ii
+// (integer halves), so the lines of one comb are disjoint and the adjoint
+// of a pixel sums one cubic in (v - mean) per comb:
+//   dL/dv = c0 + c1 d + c2 d^2 + c3 d^3,  c1 = 2 G2, c2 = 3 G3, c3 = 4 G4,
+//   c0 = g_mean / n - 3 G3 M2 / n - 4 G4 M3 / n,
+// G2, G3, G4 the upstream gradient carried to M2, M3, M4.
+constexpr int COMB_POS = 20;
+
+__device__ int comb_positions(int m, int *ii) {
+  double step = (double)m / 10.0;
+  if (step < 1.0) step = 1.0;
+  int c = 0;
+  for (double mp = 1.0 - m; mp <= m; mp += step) {
+    if (c < COMB_POS) ii[c] = (int)round(mp);
+    c++;
+  }
+  return c < COMB_POS ? c : COMB_POS;
+}
+
+// pixel count and t-th pixel (row-major index) of line (comb, ii)
+__device__ __forceinline__ int comb_line_len(int comb, int ii, int m, int n, int *first) {
+  int y0, y1;
+  switch (comb) {
+    case 0: y0 = ii < 0 ? -ii : 0; y1 = (m - 1 < n - 1 - ii) ? m - 1 : n - 1 - ii; *first = y0; return y1 >= y0 ? y1 - y0 + 1 : 0;
+    case 1: {
+      const int sm = n - 1 - ii;
+      y0 = sm - (n - 1) > 0 ? sm - (n - 1) : 0; y1 = (m - 1 < sm) ? m - 1 : sm; *first = y0;
+      return y1 >= y0 ? y1 - y0 + 1 : 0;
+    }
+    case 2: { const int x = n / 2 - 1 - ii; *first = 0; return (x >= 0 && x < n) ? m : 0; }
+    default: { const int y = m / 2 - 1 - ii; *first = 0; return (y >= 0 && y < m) ? n : 0; }
+  }
+}
+__device__ __forceinline__ long comb_line_pixel(int comb, int ii, int m, int n, int first, int t) {
+  switch (comb) {
+    case 0: { const int y = first + t; return (long)y * n + (y + ii); }
+    case 1: { const int y = first + t; return (long)y * n + (n - 1 - ii - y); }
+    case 2: return (long)t * n + (n / 2 - 1 - ii);
+    default: return (long)(m / 2 - 1 - ii) * n + t;
+  }
+}
+
+// one block per (line, image): out[b][(comb * 4 + moment) * 20 + pos];
+// backward (gout != null): coef[b][comb][pos] = {c0, c1, c2, c3, mean}
+__global__ void __launch_bounds__(256) k_ad_comb_lines(const double *in, int h, int w, double *out,
+                                                       const double *gout, double *coef) {
+  __shared__ double shd[16];
+  __shared__ int pos[COMB_POS];
+  __shared__ int npos;
+  const int b = blockIdx.y, comb = blockIdx.x / COMB_POS, p = blockIdx.x - comb * COMB_POS;
+  if (threadIdx.x == 0) npos = comb_positions(h, pos);
+  __syncthreads();
+  const double *img = in + (long)b * h * w;
+  int first = 0, L = 0;
+  const int ii = p < npos ? pos[p] : 0;
+  if (p < npos) L = comb_line_len(comb, ii, h, w, &first);
+  double s = 0;
+  for (int t = threadIdx.x; t < L; t += blockDim.x) s += img[comb_line_pixel(comb, ii, h, w, first, t)];
+  s = block_reduce(s, SumOp(), shd);
+  const double mu = L > 0 ? s / L : 0.0;
+  double s2 = 0, s3 = 0, s4 = 0;
+  for (int t = threadIdx.x; t < L; t += blockDim.x) {
+    const double d = img[comb_line_pixel(comb, ii, h, w, first, t)] - mu;
+    const double d2 = d * d;
+    s2 += d2;
+    s3 += d2 * d;
+    s4 += d2 * d2;
+  }
+  const double M2 = block_reduce(s2, SumOp(), shd);
+  const double M3 = block_reduce(s3, SumOp(), shd);
+  const double M4 = block_reduce(s4, SumOp(), shd);
+  if (threadIdx.x != 0) return;
+  const double std = L > 2 ? sqrt(M2 / (L - 1)) : 0.0;
+  const double skew = L > 3 ? (sqrt((double)L) * M3) / pow(M2, 1.5) : 0.0;
+  const double kurt = L > 4 ? (L * M4) / (M2 * M2) : 0.0;
+  if (!gout) {
+    double *o = out + (long)b * 320 + comb * 80 + p;
+    o[0] = mu;
+    o[20] = std;
+    o[40] = skew;
+    o[60] = kurt;
+    return;
+  }
+  const double *g = gout + (long)b * 320 + comb * 80 + p;
+  double G2 = 0, G3 = 0, G4 = 0;
+  if (M2 > 0) {
+    if (L > 2) G2 += g[20] / (2 * std * (L - 1));
+    if (L > 3) {
+      G2 += g[40] * (-1.5 * sqrt((double)L) * M3 / pow(M2, 2.5));
+      G3 = g[40] * sqrt((double)L) / pow(M2, 1.5);
+    }
+    if (L > 4) {
+      G2 += g[60] * (-2.0 * L * M4 / (M2 * M2 * M2));
+      G4 = g[60] * L / (M2 * M2);
+    }
+  }
+  double *c = coef + ((long)b * 4 * COMB_POS + comb * COMB_POS + p) * 5;
+  if (L > 0) {
+    c[0] = g[0] / L - 3 * G3 * M2 / L - 4 * G4 * M3 / L;
+    c[1] = 2 * G2;
+    c[2] = 3 * G3;
+    c[3] = 4 * G4;
+  } else {
+    c[0] = c[1] = c[2] = c[3] = 0;
+  }
+  c[4] = mu;
+}
+
+// grad[b][p] = sum over the four combs of the cubic of the pixel's line
+__global__ void __launch_bounds__(256) k_ad_comb_pixels(const double *in, int h, int w, const double *coef,
+                                                        double *grad) {
+  __shared__ double cf[4 * COMB_POS * 5];
+  __shared__ int pos[COMB_POS];
+  __shared__ int npos;
+  const int b = blockIdx.y;
+  if (threadIdx.x == 0) npos = comb_positions(h, pos);
+  for (int i = threadIdx.x; i < 4 * COMB_POS * 5; i += blockDim.x) cf[i] = coef[(long)b * 4 * COMB_POS * 5 + i];
+  __syncthreads();
+  const long np = (long)h * w;
+  for (long q = blockIdx.x * (long)blockDim.x + threadIdx.x; q < np; q += (long)gridDim.x * blockDim.x) {
+    const int y = q / w, x = q - (long)y * w;
+    const double v = in[(long)b * np + q];
+    double acc = 0;
+#pragma unroll
+    for (int comb = 0; comb < 4; comb++) {
+      const int ii = comb == 0 ? x - y : comb == 1 ? w - 1 - x - y : comb == 2 ? w / 2 - 1 - x : h / 2 - 1 - y;
+      int k = -1;
+      for (int j = 0; j < npos; j++)
+        if (pos[j] == ii) k = j;
+      if (k >= 0) {
+        const double *c = cf + (comb * COMB_POS + k) * 5;
+        const double d = v - c[4];
+        acc += c[0] + d * (c[1] + d * (c[2] + d * c[3]));
+      }
+    }
+    grad[(long)b * np + q] = acc;
+  }
+}
+
+void launch_ad_comb_fwd(const double *in, double *out, int n, int h, int w, hipStream_t s) {
+  k_ad_comb_lines<<<dim3(4 * COMB_POS, n), 256, 0, s>>>(in, h, w, out, nullptr, nullptr);
+}
+
+void launch_ad_comb_bwd(const double *in, const double *gout, double *grad, double *coef, int n, int h, int w,
+                        hipStream_t s) {
+  k_ad_comb_lines<<<dim3(4 * COMB_POS, n), 256, 0, s>>>(in, h, w, nullptr, gout, coef);
+  const long np = (long)h * w;
+  long gx = (np + 255) / 256;
+  if (gx > 1024) gx = 1024;
+  k_ad_comb_pixels<<<dim3((unsigned)gx, n), 256, 0, s>>>(in, h, w, coef, grad);
+}
+
 // ============================================================ WND similarity
 // classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586) after
 // signatures::normalize (signatures.cpp:196-215):

@@ -13,7 +13,9 @@ kernels behind the C ABI (``wndgpu_ad_*`` in include/wndgpu.h):
   coefficients that "Chebyshev Coefficients" histograms
   (src/FeatureAlgorithms.cpp:73-93), and "Chebyshev-Fourier Coefficients
   (raw)", the 23 x 23 magnitudes that "Chebyshev-Fourier Coefficients"
-  histograms, after its Downsample above 300 x 300 (src/FeatureAlgorithms.cpp:54-69);
+  histograms, after its Downsample above 300 x 300 (src/FeatureAlgorithms.cpp:54-69),
+  and "Comb Moments (raw)", the mean / std / skew / kurtosis of the 4 x 20
+  pixel lines that "Comb Moments" histograms (src/FeatureAlgorithms.cpp:184-200);
 * WND5        normalisation against the training set
   (signatures::normalize, src/wndchrm_src/signatures.cpp:196-215) followed by
   TrainingSet::classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586).
@@ -43,15 +45,15 @@ import torch
 from . import (WndGpuError, _Image, _check, _context, _load, fisher_weights, normalize_train,
                FeatureComputationPlan)
 
-__all__ = ['FOURIER', 'CHEBYSHEV', 'ZERNIKE', 'CHEBYSHEV_COEFFS', 'CHEB_FOURIER', 'op_forward', 'op_backward',
-           'fourier', 'chebyshev', 'zernike', 'chebyshev_coeffs', 'cheb_fourier', 'DifferentiablePlan', 'WNDModel',
-           'parse_group']
+__all__ = ['FOURIER', 'CHEBYSHEV', 'ZERNIKE', 'CHEBYSHEV_COEFFS', 'CHEB_FOURIER', 'COMB', 'op_forward',
+           'op_backward', 'fourier', 'chebyshev', 'zernike', 'chebyshev_coeffs', 'cheb_fourier', 'comb_moments',
+           'DifferentiablePlan', 'WNDModel', 'parse_group']
 
-FOURIER, CHEBYSHEV, ZERNIKE, CHEBYSHEV_COEFFS, CHEB_FOURIER = 0, 1, 2, 3, 4
+FOURIER, CHEBYSHEV, ZERNIKE, CHEBYSHEV_COEFFS, CHEB_FOURIER, COMB = 0, 1, 2, 3, 4, 5
 TRANSFORMS = {'Fourier': FOURIER, 'Chebyshev': CHEBYSHEV}
 ALGORITHMS = {'Zernike Coefficients': ZERNIKE, 'Chebyshev Coefficients (raw)': CHEBYSHEV_COEFFS,
-              'Chebyshev-Fourier Coefficients (raw)': CHEB_FOURIER}
-RAW_NAMES = ('Chebyshev Coefficients (raw)', 'Chebyshev-Fourier Coefficients (raw)')
+              'Chebyshev-Fourier Coefficients (raw)': CHEB_FOURIER, 'Comb Moments (raw)': COMB}
+RAW_NAMES = ('Chebyshev Coefficients (raw)', 'Chebyshev-Fourier Coefficients (raw)', 'Comb Moments (raw)')
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -169,6 +171,12 @@ def cheb_fourier(x):
     return _Op.apply(_planes(x, 'cheb_fourier'), CHEB_FOURIER).reshape(x.shape[0], -1)
 
 
+def comb_moments(x):
+    """The 4 combs x (mean, std, skew, kurtosis) x 20 line moments, [n, 320],
+    differentiable."""
+    return _Op.apply(_planes(x, 'comb_moments'), COMB).reshape(x.shape[0], -1)
+
+
 class _Node(torch.autograd.Function):
     """Features of the algorithms `algs` on the node chain(x); keeps only x."""
 
@@ -247,7 +255,7 @@ class DifferentiablePlan(object):
                     raise WndGpuError('transform "%s" in "%s" has no adjoint' % (t, g))
             if alg in ALGORITHMS:
                 op = ALGORITHMS[alg]
-                k = {ZERNIKE: 72, CHEBYSHEV_COEFFS: 400, CHEB_FOURIER: 529}[op]
+                k = {ZERNIKE: 72, CHEBYSHEV_COEFFS: 400, CHEB_FOURIER: 529, COMB: 320}[op]
                 self._nodes.setdefault(tuple(TRANSFORMS[t] for t in chain), []).append((op, off, k))
             else:
                 if not masked:
