@@ -276,7 +276,7 @@ GRAD_GROUPS = ['Zernike Coefficients ()', 'Zernike Coefficients (Fourier ())', '
                'Zernike Coefficients (Fourier (Chebyshev ()))', 'Zernike Coefficients (Chebyshev (Fourier ()))',
                'Chebyshev Coefficients (raw) ()', 'Chebyshev Coefficients (raw) (Fourier ())',
                'Chebyshev-Fourier Coefficients (raw) ()', 'Chebyshev-Fourier Coefficients (raw) (Fourier ())',
-               'Comb Moments (raw) ()', 'Comb Moments (raw) (Fourier ())']
+               'Comb Moments (raw) ()', 'Comb Moments (raw) (Fourier ())', 'Gabor Textures (raw) ()']
 
 
 def _grad_image(k):
@@ -366,6 +366,18 @@ def grad_main(args):
     # T1 = X Bx (h x w x N), then Bx^T T1 By (N x h x N): exact FP64 flops, mean per launch
     gemm_flops = (2.0 * h * w * nmin + 2.0 * nmin * h * nmin) * ntest / 2
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
+    # the dominant kernel: the Gabor energy correlation (k_ad_gabor_fwd, 16 of its launches per step),
+    # 38 x 38 complex taps on a real plane = 4 * 1444 flops per output, one launch per filter
+    ad.op_forward(ad.GABOR, xd)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(reps):
+        ad.op_forward(ad.GABOR, xd)
+    ev1.record()
+    torch.cuda.synchronize()
+    gabor_ms = ev0.elapsed_time(ev1) / reps / 8  # per filter launch
+    gabor_flops = 4.0 * 38 * 38 * h * w * ntest
+    gabor_tf = gabor_flops / (gabor_ms * 1e-3) / 1e12
     line = {
         'metric': 'pixel-gradient images/s (WND5 probability of the predicted class, adjoint kernels)',
         'value': round(ntest / dt, 3), 'unit': 'images/s', 'n_gpus': 1, 'steps': args.steps,
@@ -373,7 +385,8 @@ def grad_main(args):
         'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f64',
         'data': 'synthetic (seeded grating/blob/Poisson generator, 512x512, scaled to [0, 1] float32)',
         'config': {'workload': 'differentiation: %d test images 512x512, forward + adjoint through %d features '
-                               '(Zernike on 5 nodes; raw Chebyshev-20, Chebyshev-Fourier and comb moments on 2) and WND5 '
+                               '(Zernike on 5 nodes; raw Chebyshev-20, Chebyshev-Fourier and comb moments on 2; Gabor '
+                               'energy means on the root) and WND5 '
                                'against %d training '
                                'samples (8 classes)' % (ntest, plan.n_features, ntrain),
                    'test_images': ntest, 'train_samples': ntrain, 'features': plan.n_features},
@@ -381,10 +394,15 @@ def grad_main(args):
         'training_features_images_per_s': round(ntrain / t_train, 2),
         'generator_s': round(t_gen, 1),
         'grad_finite': bool(torch.isfinite(g).all().item()),
-        'roofline': {'kernel': 'k_ad_gemm (512 x 512 x 512 Chebyshev GEMM, 64 planes per launch)', 'bound': 'mfma',
-                     'achieved': round(achieved, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': round(achieved / FP64_PEAK_TFLOPS, 4), 'traffic': None,
-                     'launch_ms': round(gemm_ms, 4), 'flops_per_launch': gemm_flops},
+        'roofline': {'kernel': 'k_ad_gabor_fwd (one 38 x 38 complex Gabor filter over 64 planes 512x512)',
+                     'bound': 'mfma', 'achieved': round(gabor_tf, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'frac': round(gabor_tf / FP64_PEAK_TFLOPS, 4), 'traffic': None,
+                     'launch_ms': round(gabor_ms, 4), 'flops_per_launch': gabor_flops,
+                     'note': 'FP64 vector FMA; the FP64 matrix and vector peaks of MI355X are equal'},
+        'roofline_gemm': {'kernel': 'k_ad_gemm (512 x 512 x 512 Chebyshev GEMM, 64 planes per launch)',
+                          'achieved': round(achieved, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                          'frac': round(achieved / FP64_PEAK_TFLOPS, 4), 'launch_ms': round(gemm_ms, 4),
+                          'flops_per_launch': gemm_flops},
     }
     if not args.no_cpu:
         line['cpu_baseline'] = grad_cpu_baseline(imgs[ntrain][0], model, plan)

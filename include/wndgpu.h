@@ -201,6 +201,8 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
  *   WNDGPU_AD_COMB             the line moments that CombFirstFourMoments histograms
  *                              (src/FeatureAlgorithms.cpp:184-200, src/statistics/CombFirst4Moments.cpp:78-217):
  *                              -> 320 values, [comb][mean, std, skew, kurt][20 positions]
+ *   WNDGPU_AD_GABOR            the 8 energy maps GaborTextures thresholds (GaborEnergy,
+ *                              src/textures/gabor.cpp:166-231; f0 = 0.1, then 1..7): -> (8 h) x w
  * All pointers are device memory of the context's GPU: in [n][h][w] FP64
  * planes of one size, out / grad_out [n][out_h][out_w], grad_in [n][h][w]
  * (overwritten).  work: caller-owned device scratch of
@@ -209,7 +211,7 @@ int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_
  * it; calls return without waiting.  The backward pass recomputes what it
  * needs from `in`. */
 enum { WNDGPU_AD_FOURIER = 0, WNDGPU_AD_CHEBYSHEV = 1, WNDGPU_AD_ZERNIKE = 2, WNDGPU_AD_CHEBYSHEV_COEFFS = 3,
-       WNDGPU_AD_CHEB_FOURIER = 4, WNDGPU_AD_COMB = 5 };
+       WNDGPU_AD_CHEB_FOURIER = 4, WNDGPU_AD_COMB = 5, WNDGPU_AD_GABOR = 6 };
 int wndgpu_ad_output_shape(int op, int h, int w, int *out_h, int *out_w);
 size_t wndgpu_ad_work_bytes(int op, int n, int h, int w);
 int wndgpu_ad_forward(wndgpu_ctx *ctx, int op, const double *in, int n, int h, int w, double *out, void *work,

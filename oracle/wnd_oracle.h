@@ -96,6 +96,8 @@ int orc_features_sampled(int set, const void *img, int dtype, int h, int w, cons
 int orc_cf_magnitudes(const double *in, int h, int w, double *out, int *nn);
 /* the line moments CombFirstFourMoments histograms: [comb][mean, std, skew, kurt][20 positions] */
 int orc_comb_raw(const double *in, int h, int w, double *out320);
+/* the 8 Gabor energy maps of GaborTextureFilters2D (f0 = 0.1, 1..7): out [8][h][w] */
+int orc_gabor_energy(const double *in, int h, int w, double *out);
 double orc_otsu(const double *in, int h, int w);                         /* Otsu(dynamic) */
 int orc_radon_proj(const double *in, int h, int w, double *out4xr, int *rsize);
 int orc_haralick_grays(const double *in, int h, int w, unsigned char *out);

@@ -19,6 +19,8 @@ Definitions follow the reference sources:
 * comb      CombFirstFourMoments before its histograms
             (src/statistics/CombFirst4Moments.cpp:78-217): the mean, std, skew and
             kurtosis of each of 4 x 20 pixel lines.
+* gabor     the 8 energy maps GaborTextureFilters2D thresholds (GaborEnergy,
+            src/textures/gabor.cpp:166-231); "Gabor Textures (raw)" is their means.
 * wnd5      signatures::normalize (src/wndchrm_src/signatures.cpp:196-215) and
             TrainingSet::classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586).
 """
@@ -249,6 +251,38 @@ def comb_raw(img):
     return torch.stack(out)
 
 
+def gabor_kernels():
+    """the 8 complex 38 x 38 kernels of GaborTextureFilters2D (src/textures/gabor.cpp:103-162,
+    201-231): gamma 0.5, sig2lam 0.56, theta = 3.14159265/2, f0 = 0.1 then 1..7, each
+    divided by the sum of its magnitudes"""
+    n, gamma, sig2lam, theta = 38, 0.5, 0.56, 3.14159265 / 2
+    t = np.arange(n + 1, dtype=np.float64)[:n] - (n // 2)
+    ty, tx = np.meshgrid(t, t, indexing='ij')
+    out = []
+    for f0 in (0.1, 1, 2, 3, 4, 5, 6, 7):
+        lam = 2 * math.pi / f0
+        sig = sig2lam * lam
+        xte = tx * math.cos(theta) + ty * math.sin(theta)
+        yte = ty * math.cos(theta) - tx * math.sin(theta)
+        ge = np.exp(-1 * (xte * xte + gamma * gamma * yte * yte) / (2 * sig * sig))
+        g = ge * np.cos(xte * f0) + 1j * ge * np.sin(xte * f0)
+        out.append(g / np.abs(g).sum())
+    return np.stack(out)
+
+
+def gabor_energy(img):
+    """the 8 energy maps of GaborEnergy (src/textures/gabor.cpp:166-195): the
+    full convolution with each kernel, cropped at ceil(38/2) = 19 -> [8, h, w]"""
+    h, w = img.shape
+    G = gabor_kernels()
+    # E(b, a) = |sum_{j,i} G[j][i] I[b + 19 - j][a + 19 - i]|: a correlation with the flipped kernel
+    Gf = torch.from_numpy(np.ascontiguousarray(G[:, ::-1, ::-1]))
+    x = torch.nn.functional.pad(img[None, None], (18, 19, 18, 19))
+    re = torch.nn.functional.conv2d(x, Gf.real[:, None].contiguous())[0]
+    im = torch.nn.functional.conv2d(x, Gf.imag[:, None].contiguous())[0]
+    return torch.sqrt(re * re + im * im)
+
+
 TRANSFORMS = {'Fourier': fourier, 'Chebyshev': chebyshev}
 
 
@@ -271,6 +305,8 @@ def features(img, groups):
             out.append(cf_magnitudes(p))
         elif alg == 'Comb Moments (raw)':
             out.append(comb_raw(p))
+        elif alg == 'Gabor Textures (raw)':
+            out.append(gabor_energy(p).mean((1, 2)))
         else:
             raise ValueError(alg)
     return torch.cat(out)

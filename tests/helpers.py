@@ -36,6 +36,7 @@ def oracle():
         lib.orc_otsu.argtypes = [P, I, I]
         lib.orc_cf_magnitudes.argtypes = [P, I, I, P, P]
         lib.orc_comb_raw.argtypes = [P, I, I, P]
+        lib.orc_gabor_energy.argtypes = [P, I, I, P]
         lib.orc_otsu.restype = D
         lib.orc_haralick_grays.argtypes = [P, I, I, P]
         lib.orc_radon_proj.argtypes = [P, I, I, P, P]

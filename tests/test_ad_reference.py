@@ -43,6 +43,13 @@ def orc_comb(img):
     return out
 
 
+def orc_gabor(img):
+    img = np.ascontiguousarray(img, np.float64)
+    out = np.zeros((8,) + img.shape)
+    assert oracle().orc_gabor_energy(ptr(img), img.shape[0], img.shape[1], ptr(out)) == 0
+    return out
+
+
 def orc_node(img, chain):
     for t in chain:
         img = orc_transform({'Fourier': T_FOURIER, 'Chebyshev': T_CHEBYSHEV}[t], img)
@@ -59,6 +66,8 @@ def orc_features(img, groups):
             out.append(orc_cf(p))
         elif alg == 'Comb Moments (raw)':
             out.append(orc_comb(p))
+        elif alg == 'Gabor Textures (raw)':
+            out.append(orc_gabor(p).mean((1, 2)))
         else:  # raw order-20 Chebyshev coefficients = leading block of the full transform
             c = orc_transform(T_CHEBYSHEV, p)
             n = min(20, *p.shape)
@@ -119,6 +128,15 @@ def test_comb_reference_matches_oracle(shape):
     assert np.all(np.abs(got - want) <= 1e-9 * np.abs(want) + 1e-12)
 
 
+@pytest.mark.parametrize('shape', [(40, 48), (20, 64)])
+def test_gabor_reference_matches_oracle(shape):
+    """energy maps: the oracle's conv2comp order against torch's correlation"""
+    img = plane(19, *shape)
+    want = orc_gabor(img)
+    got = R.gabor_energy(torch.from_numpy(img)).numpy()
+    assert rel(got, want) < 1e-12
+
+
 def _training_set(n_classes, per_class, groups, shape, seed):
     lib = oracle()
     feats, cls = [], []
@@ -165,7 +183,8 @@ def test_wnd_reference_matches_oracle():
 @pytest.mark.parametrize('group', GROUPS + [('Zernike Coefficients', ['Chebyshev', 'Fourier']),
                                    ('Chebyshev-Fourier Coefficients (raw)', []),
                                    ('Chebyshev-Fourier Coefficients (raw)', ['Fourier']),
-                                   ('Comb Moments (raw)', []), ('Comb Moments (raw)', ['Chebyshev'])],
+                                   ('Comb Moments (raw)', []), ('Comb Moments (raw)', ['Chebyshev']),
+                                   ('Gabor Textures (raw)', [])],
                          ids=lambda g: g[0].split()[0] + '(' + ','.join(g[1]) + ')')
 def test_reference_gradient_matches_oracle_differences(group):
     """d(c . features)/dI from torch autograd = central differences of the oracle"""

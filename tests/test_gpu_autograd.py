@@ -8,8 +8,8 @@ import pytest
 import torch
 
 import ad_reference as R
-from test_ad_reference import (GROUPS, orc_cf, orc_comb, orc_features, orc_probabilities, orc_transform,
-                               orc_zernike, plane)
+from test_ad_reference import (GROUPS, orc_cf, orc_comb, orc_features, orc_gabor, orc_probabilities,
+                               orc_transform, orc_zernike, plane)
 from helpers import T_CHEBYSHEV, T_FOURIER
 
 pytestmark = pytest.mark.gpu
@@ -72,6 +72,15 @@ def test_comb_moments_forward_matches_oracle(shape):
         assert np.all(np.abs(got[i] - want) <= 1e-9 * np.abs(want) + 1e-12)
 
 
+@pytest.mark.parametrize('shape', [(64, 64), (40, 100), (33, 29)])
+def test_gabor_energy_forward_matches_oracle(shape):
+    A = _ad()
+    x = batch([20, 21], *shape)
+    got = A.gabor_energy(x).cpu().numpy()
+    for i in range(2):
+        assert rel(got[i], orc_gabor(x[i].cpu().numpy())) < 1e-12
+
+
 def test_zernike_matches_feature_path():
     import wndcharm
     A = _ad()
@@ -84,14 +93,15 @@ def test_zernike_matches_feature_path():
 
 
 @pytest.mark.parametrize('op', ['fourier', 'chebyshev', 'chebyshev_coeffs', 'zernike', 'cheb_fourier',
-                                'comb_moments'])
+                                'comb_moments', 'gabor_energy'])
 @pytest.mark.parametrize('shape', [(40, 48), (64, 64), (310, 320)])
 def test_adjoint_matches_reference_gradient(op, shape):
     A = _ad()
     x = batch([7, 8], *shape)
     fn = getattr(A, op)
     ref = {'fourier': R.fourier, 'chebyshev': R.chebyshev, 'chebyshev_coeffs': lambda p: R.chebyshev(p, 20),
-           'zernike': R.zernike, 'cheb_fourier': R.cf_magnitudes, 'comb_moments': R.comb_raw}[op]
+           'zernike': R.zernike, 'cheb_fourier': R.cf_magnitudes, 'comb_moments': R.comb_raw,
+           'gabor_energy': R.gabor_energy}[op]
     xg = x.clone().requires_grad_(True)
     y = fn(xg)
     rng = np.random.default_rng(3)
@@ -111,13 +121,14 @@ NAMES = ['Zernike Coefficients ()', 'Zernike Coefficients (Fourier ())', 'Zernik
 def test_plan_features_and_gradient_match_reference():
     A = _ad()
     groups = NAMES + ['Zernike Coefficients (Fourier (Chebyshev ()))', 'Chebyshev-Fourier Coefficients (raw) ()',
-                      'Chebyshev-Fourier Coefficients (raw) (Fourier ())', 'Comb Moments (raw) (Chebyshev ())']
+                      'Chebyshev-Fourier Coefficients (raw) (Fourier ())', 'Comb Moments (raw) (Chebyshev ())',
+                      'Gabor Textures (raw) ()']
     plan = A.DifferentiablePlan(groups)
     ref_groups = [A._split_raw(g) for g in groups]
     x = batch([9, 10], 48, 56)
     xg = x.clone().requires_grad_(True)
     f = plan(xg)
-    assert f.shape == (2, plan.n_features) and plan.n_features == 4 * 72 + 400 + 2 * 529 + 320
+    assert f.shape == (2, plan.n_features) and plan.n_features == 4 * 72 + 400 + 2 * 529 + 320 + 8
     rng = np.random.default_rng(4)
     c = torch.from_numpy(rng.standard_normal(plan.n_features)).to(DEV)
     (f * c).sum().backward()
@@ -237,7 +248,7 @@ def _fd_case(groups, seed_model, seed_img, pix_seed):
 
 
 SMOOTH = ['Zernike Coefficients ()', 'Zernike Coefficients (Chebyshev ())', 'Chebyshev Coefficients (raw) ()',
-          'Chebyshev-Fourier Coefficients (raw) ()', 'Comb Moments (raw) ()']
+          'Chebyshev-Fourier Coefficients (raw) ()', 'Comb Moments (raw) ()', 'Gabor Textures (raw) ()']
 
 
 def test_gradient_matches_finite_differences_512():

@@ -1991,6 +1991,7 @@ static int ad_dims(int op, int h, int w, int *oh, int *ow) {
     case WNDGPU_AD_CHEBYSHEV_COEFFS: *oh = N < 20 ? N : 20; *ow = *oh; return 0;
     case WNDGPU_AD_ZERNIKE: *oh = 1; *ow = 72; return 0;
     case WNDGPU_AD_COMB: *oh = 1; *ow = 320; return 0;
+    case WNDGPU_AD_GABOR: *oh = 8 * h; *ow = w; return 0;
     default: return -1;
   }
 }
@@ -2008,6 +2009,7 @@ size_t wndgpu_ad_work_bytes(int op, int n, int h, int w) {
     case WNDGPU_AD_FOURIER: return (size_t)n * h * w * sizeof(double2);
     case WNDGPU_AD_CHEB_FOURIER: return cf_ad_work(cf_ad_geom(h, w), n, h).total_bytes;
     case WNDGPU_AD_COMB: return (size_t)n * 80 * 5 * sizeof(double);
+    case WNDGPU_AD_GABOR: return (size_t)n * h * w * 2 * sizeof(double);
     case WNDGPU_AD_CHEBYSHEV:
     case WNDGPU_AD_CHEBYSHEV_COEFFS: return (size_t)n * h * oh * sizeof(double);
     default: return zernike_ad_work_doubles(n, h, w) * sizeof(double);
@@ -2074,6 +2076,9 @@ int wndgpu_ad_forward(wndgpu_ctx *c, int op, const double *in, int n, int h, int
     launch_ad_zernike_fwd(in, out, (double *)work, n, h, w, s);
   } else if (op == WNDGPU_AD_COMB) {
     launch_ad_comb_fwd(in, out, n, h, w, s);
+  } else if (op == WNDGPU_AD_GABOR) {
+    if ((r = ensure_tables(c))) return r;
+    launch_ad_gabor_fwd(in, (const double *)c->gabor.p, out, n, h, w, s);
   } else if (op == WNDGPU_AD_CHEB_FOURIER) {
     const double *plane;
     const CfDev *t;
@@ -2113,6 +2118,9 @@ int wndgpu_ad_backward(wndgpu_ctx *c, int op, const double *in, const double *gr
     launch_ad_zernike_bwd(in, grad_out, grad_in, (double *)work, n, h, w, s);
   } else if (op == WNDGPU_AD_COMB) {
     launch_ad_comb_bwd(in, grad_out, grad_in, (double *)work, n, h, w, s);
+  } else if (op == WNDGPU_AD_GABOR) {
+    if ((r = ensure_tables(c))) return r;
+    launch_ad_gabor_bwd(in, (const double *)c->gabor.p, grad_out, grad_in, (double *)work, n, h, w, s);
   } else if (op == WNDGPU_AD_CHEB_FOURIER) {
     const double *plane;
     const CfDev *t;

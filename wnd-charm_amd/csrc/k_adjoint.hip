@@ -781,6 +781,126 @@ void launch_ad_comb_bwd(const double *in, const double *gout, double *grad, doub
   k_ad_comb_pixels<<<dim3((unsigned)gx, n), 256, 0, s>>>(in, h, w, coef, grad);
 }
 
+// ============================================================ Gabor energy
+// GaborEnergy (src/textures/gabor.cpp:166-195): the full complex convolution
+// of the plane with the 38 x 38 kernel G_f (conv2comp, 19-52) cropped at
+// ceil(38/2) = 19, i.e. a correlation with the flipped kernel:
+//   re + i im = sum_{ju, iv} G_f[37-ju][37-iv] I[b - 18 + ju][a - 18 + iv]  (I = 0 outside),
+//   E_f(b, a) = |re + i im|,
+// for f0 = 0.1 (low pass) then 1..7 (GaborTextureFilters2D, 201-231).  With
+// upstream G_E and Z = G_E (re, im) / E the adjoint is a correlation with
+// the kernel itself:
+//   dL/dI(l, k) = sum_f sum_{u, v} Re Z_f(l - 19 + u, k - 19 + v) G_re[u][v] + Im Z_f(..) G_im[u][v].
+// One block per 16-row output tile of one filter; the input halo in LDS,
+// the taps read as uniform (broadcast) loads, a sliding register window
+// along each tap row.
+constexpr int GB_N = 38, GB_TH = 16;
+
+// MODE 0: out = E (plane of the [n][8][h][w] maps); MODE 1: out = Z (complex [n][h][w])
+template <int MODE>
+__global__ void __launch_bounds__(256) k_ad_gabor_fwd(const double *in, const double *taps, int h, int w,
+                                                      const double *gE, long estride, double *out) {
+  constexpr int TW = 48, Q = 3;  // 16 x 48 outputs, three per thread along a row
+  __shared__ double S[GB_TH + GB_N - 1][TW + GB_N - 1];
+  const int b = blockIdx.z, y0 = blockIdx.y * GB_TH, x0 = blockIdx.x * TW;
+  const double *img = in + (long)b * h * w;
+  for (int e = threadIdx.x; e < (GB_TH + GB_N - 1) * (TW + GB_N - 1); e += blockDim.x) {
+    const int r = e / (TW + GB_N - 1), c = e - r * (TW + GB_N - 1);
+    const int y = y0 - 18 + r, x = x0 - 18 + c;
+    S[r][c] = (y >= 0 && y < h && x >= 0 && x < w) ? img[(long)y * w + x] : 0.0;
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 4, c = (threadIdx.x & 15) * Q;
+  double re[Q] = {0, 0, 0}, im[Q] = {0, 0, 0};
+  for (int ju = 0; ju < GB_N; ju++) {
+    const double *trow = taps + (long)(GB_N - 1 - ju) * GB_N * 2;
+    const double *srow = &S[r + ju][c];
+    double w0 = srow[0], w1 = srow[1];
+    for (int iv = 0; iv < GB_N; iv++) {
+      const double w2 = srow[iv + 2];
+      const double gr = trow[(GB_N - 1 - iv) * 2], gi = trow[(GB_N - 1 - iv) * 2 + 1];
+      re[0] = fma(gr, w0, re[0]); im[0] = fma(gi, w0, im[0]);
+      re[1] = fma(gr, w1, re[1]); im[1] = fma(gi, w1, im[1]);
+      re[2] = fma(gr, w2, re[2]); im[2] = fma(gi, w2, im[2]);
+      w0 = w1;
+      w1 = w2;
+    }
+  }
+  const int y = y0 + r;
+  if (y >= h) return;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int x = x0 + c + q;
+    if (x >= w) continue;
+    const double e = sqrt(re[q] * re[q] + im[q] * im[q]);
+    if (MODE == 0) {
+      out[(long)b * estride + (long)y * w + x] = e;
+    } else {
+      const double g = gE[(long)b * estride + (long)y * w + x];
+      const double sc = e > 0 ? g / e : 0.0;
+      out[((long)b * h * w + (long)y * w + x) * 2] = sc * re[q];
+      out[((long)b * h * w + (long)y * w + x) * 2 + 1] = sc * im[q];
+    }
+  }
+}
+
+// grad (+)= sum_{u,v} Re Z(l - 19 + u, k - 19 + v) G_re[u][v] + Im Z(..) G_im[u][v]
+__global__ void __launch_bounds__(256) k_ad_gabor_adj(const double *Z, const double *taps, int h, int w, int acc,
+                                                      double *grad) {
+  constexpr int TW = 32, Q = 2;  // 16 x 32 outputs, two per thread
+  __shared__ double2 S[GB_TH + GB_N - 1][TW + GB_N - 1];
+  const int b = blockIdx.z, y0 = blockIdx.y * GB_TH, x0 = blockIdx.x * TW;
+  const double2 *zp = (const double2 *)Z + (long)b * h * w;
+  for (int e = threadIdx.x; e < (GB_TH + GB_N - 1) * (TW + GB_N - 1); e += blockDim.x) {
+    const int r = e / (TW + GB_N - 1), c = e - r * (TW + GB_N - 1);
+    const int y = y0 - 19 + r, x = x0 - 19 + c;
+    S[r][c] = (y >= 0 && y < h && x >= 0 && x < w) ? zp[(long)y * w + x] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  const int r = threadIdx.x >> 4, c = (threadIdx.x & 15) * Q;
+  double g[Q] = {0, 0};
+  for (int u = 0; u < GB_N; u++) {
+    const double *trow = taps + (long)u * GB_N * 2;
+    const double2 *srow = &S[r + u][c];
+    double2 w0 = srow[0];
+    for (int v = 0; v < GB_N; v++) {
+      const double2 w1 = srow[v + 1];
+      const double gr = trow[v * 2], gi = trow[v * 2 + 1];
+      g[0] = fma(gr, w0.x, fma(gi, w0.y, g[0]));
+      g[1] = fma(gr, w1.x, fma(gi, w1.y, g[1]));
+      w0 = w1;
+    }
+  }
+  const int y = y0 + r;
+  if (y >= h) return;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int x = x0 + c + q;
+    if (x >= w) continue;
+    double *o = grad + (long)b * h * w + (long)y * w + x;
+    *o = acc ? *o + g[q] : g[q];
+  }
+}
+
+// forward: the 8 maps into out [n][8][h][w]
+void launch_ad_gabor_fwd(const double *in, const double *taps8, double *out, int n, int h, int w, hipStream_t s) {
+  const dim3 grid((w + 47) / 48, (h + GB_TH - 1) / GB_TH, n);
+  for (int f = 0; f < 8; f++)
+    k_ad_gabor_fwd<0><<<grid, 256, 0, s>>>(in, taps8 + (long)f * GB_N * GB_N * 2, h, w, nullptr, 8L * h * w,
+                                          out + (long)f * h * w);
+}
+
+// backward: per filter, Z from a recomputed convolution, then its correlation added into grad
+void launch_ad_gabor_bwd(const double *in, const double *taps8, const double *gE, double *grad, double *Z, int n,
+                         int h, int w, hipStream_t s) {
+  const dim3 gf((w + 47) / 48, (h + GB_TH - 1) / GB_TH, n), ga((w + 31) / 32, (h + GB_TH - 1) / GB_TH, n);
+  for (int f = 0; f < 8; f++) {
+    const double *t = taps8 + (long)f * GB_N * GB_N * 2;
+    k_ad_gabor_fwd<1><<<gf, 256, 0, s>>>(in, t, h, w, gE + (long)f * h * w, 8L * h * w, Z);
+    k_ad_gabor_adj<<<ga, 256, 0, s>>>(Z, t, h, w, f > 0, grad);
+  }
+}
+
 // ============================================================ WND similarity
 // classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586) after
 // signatures::normalize (signatures.cpp:196-215):

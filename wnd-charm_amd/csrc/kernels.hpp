@@ -196,6 +196,9 @@ void launch_ad_cf_bwd(const double *v, const double *gout, int n, long plane, co
 void launch_ad_comb_fwd(const double *in, double *out, int n, int h, int w, hipStream_t s);
 void launch_ad_comb_bwd(const double *in, const double *gout, double *grad, double *coef, int n, int h, int w,
                         hipStream_t s);
+void launch_ad_gabor_fwd(const double *in, const double *taps8, double *out, int n, int h, int w, hipStream_t s);
+void launch_ad_gabor_bwd(const double *in, const double *taps8, const double *gE, double *grad, double *Z, int n,
+                         int h, int w, hipStream_t s);
 void launch_ad_wnd_bwd(const double *train, const int *cls, const double *wts, const double *x, const double *v,
                        const double *mins, const double *maxs, const double *dist, const double *prob,
                        const double *norm, const double *gp, double *gsn, double *gd, int n, int f, int C, int ntest,

@@ -14,8 +14,11 @@ kernels behind the C ABI (``wndgpu_ad_*`` in include/wndgpu.h):
   (src/FeatureAlgorithms.cpp:73-93), and "Chebyshev-Fourier Coefficients
   (raw)", the 23 x 23 magnitudes that "Chebyshev-Fourier Coefficients"
   histograms, after its Downsample above 300 x 300 (src/FeatureAlgorithms.cpp:54-69),
-  and "Comb Moments (raw)", the mean / std / skew / kurtosis of the 4 x 20
-  pixel lines that "Comb Moments" histograms (src/FeatureAlgorithms.cpp:184-200);
+  "Comb Moments (raw)", the mean / std / skew / kurtosis of the 4 x 20
+  pixel lines that "Comb Moments" histograms (src/FeatureAlgorithms.cpp:184-200),
+  and "Gabor Textures (raw)", the means of the 8 energy maps that "Gabor
+  Textures" thresholds (src/FeatureAlgorithms.cpp:422-434; the maps
+  themselves from gabor_energy());
 * WND5        normalisation against the training set
   (signatures::normalize, src/wndchrm_src/signatures.cpp:196-215) followed by
   TrainingSet::classify2 (src/wndchrm_src/TrainingSet.cpp:2429-2586).
@@ -45,15 +48,18 @@ import torch
 from . import (WndGpuError, _Image, _check, _context, _load, fisher_weights, normalize_train,
                FeatureComputationPlan)
 
-__all__ = ['FOURIER', 'CHEBYSHEV', 'ZERNIKE', 'CHEBYSHEV_COEFFS', 'CHEB_FOURIER', 'COMB', 'op_forward',
+__all__ = ['FOURIER', 'CHEBYSHEV', 'ZERNIKE', 'CHEBYSHEV_COEFFS', 'CHEB_FOURIER', 'COMB', 'GABOR', 'op_forward',
            'op_backward', 'fourier', 'chebyshev', 'zernike', 'chebyshev_coeffs', 'cheb_fourier', 'comb_moments',
-           'DifferentiablePlan', 'WNDModel', 'parse_group']
+           'gabor_energy', 'DifferentiablePlan', 'WNDModel', 'parse_group']
 
-FOURIER, CHEBYSHEV, ZERNIKE, CHEBYSHEV_COEFFS, CHEB_FOURIER, COMB = 0, 1, 2, 3, 4, 5
+FOURIER, CHEBYSHEV, ZERNIKE, CHEBYSHEV_COEFFS, CHEB_FOURIER, COMB, GABOR = 0, 1, 2, 3, 4, 5, 6
+GABOR_MEANS = 100  # plan-level: the means of the 8 Gabor energy maps (op GABOR, averaged here)
 TRANSFORMS = {'Fourier': FOURIER, 'Chebyshev': CHEBYSHEV}
 ALGORITHMS = {'Zernike Coefficients': ZERNIKE, 'Chebyshev Coefficients (raw)': CHEBYSHEV_COEFFS,
-              'Chebyshev-Fourier Coefficients (raw)': CHEB_FOURIER, 'Comb Moments (raw)': COMB}
-RAW_NAMES = ('Chebyshev Coefficients (raw)', 'Chebyshev-Fourier Coefficients (raw)', 'Comb Moments (raw)')
+              'Chebyshev-Fourier Coefficients (raw)': CHEB_FOURIER, 'Comb Moments (raw)': COMB,
+              'Gabor Textures (raw)': GABOR_MEANS}
+RAW_NAMES = ('Chebyshev Coefficients (raw)', 'Chebyshev-Fourier Coefficients (raw)', 'Comb Moments (raw)',
+             'Gabor Textures (raw)')
 
 _P = ctypes.c_void_p
 _I = ctypes.c_int
@@ -177,6 +183,26 @@ def comb_moments(x):
     return _Op.apply(_planes(x, 'comb_moments'), COMB).reshape(x.shape[0], -1)
 
 
+def gabor_energy(x):
+    """The 8 Gabor energy maps (f0 = 0.1, then 1..7), [n, 8, h, w], differentiable."""
+    n, h, w = x.shape
+    return _Op.apply(_planes(x, 'gabor_energy'), GABOR).reshape(n, 8, h, w)
+
+
+def _alg_forward(a, p):
+    if a == GABOR_MEANS:
+        return op_forward(GABOR, p).reshape(p.shape[0], 8, -1).mean(2)
+    return op_forward(a, p).reshape(p.shape[0], -1)
+
+
+def _alg_backward(a, p, g):
+    if a == GABOR_MEANS:
+        n, h, w = p.shape
+        gE = (g.to(torch.float64) / (h * w)).reshape(n, 8, 1).expand(n, 8, h * w)
+        return op_backward(GABOR, p, gE.reshape(n, 8 * h, w))
+    return op_backward(a, p, g)
+
+
 class _Node(torch.autograd.Function):
     """Features of the algorithms `algs` on the node chain(x); keeps only x."""
 
@@ -185,7 +211,7 @@ class _Node(torch.autograd.Function):
         p = x
         for t in chain:
             p = op_forward(t, p)
-        outs = [op_forward(a, p).reshape(x.shape[0], -1) for a in algs]
+        outs = [_alg_forward(a, p) for a in algs]
         ctx.save_for_backward(x)
         ctx.chain, ctx.algs = chain, algs
         ctx.sizes = [o.shape[1] for o in outs]
@@ -200,7 +226,7 @@ class _Node(torch.autograd.Function):
         top = planes[-1]
         gt, off = None, 0
         for a, k in zip(ctx.algs, ctx.sizes):
-            ga = op_backward(a, top, g[:, off:off + k])
+            ga = _alg_backward(a, top, g[:, off:off + k])
             gt = ga if gt is None else gt + ga
             off += k
         for t, p in zip(reversed(ctx.chain), reversed(planes[:-1])):
@@ -255,7 +281,7 @@ class DifferentiablePlan(object):
                     raise WndGpuError('transform "%s" in "%s" has no adjoint' % (t, g))
             if alg in ALGORITHMS:
                 op = ALGORITHMS[alg]
-                k = {ZERNIKE: 72, CHEBYSHEV_COEFFS: 400, CHEB_FOURIER: 529, COMB: 320}[op]
+                k = {ZERNIKE: 72, CHEBYSHEV_COEFFS: 400, CHEB_FOURIER: 529, COMB: 320, GABOR_MEANS: 8}[op]
                 self._nodes.setdefault(tuple(TRANSFORMS[t] for t in chain), []).append((op, off, k))
             else:
                 if not masked:
