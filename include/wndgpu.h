@@ -235,6 +235,22 @@ int wndgpu_ad_wnd_backward(wndgpu_ctx *ctx, const double *train, const int *clas
                            const double *x, int n_test, const double *grad_prob, double *grad_x, void *work,
                            void *stream);
 
+/* Batch graphs (no counterpart in the reference, whose executor runs one
+ * image at a time).  mode 0 (default): every batch issues its launches.
+ * mode 1: a batch whose signature (plan, image geometry, sampling, and the
+ * context's arena and table state) was seen before is captured into a
+ * hipGraph once, then replayed: one graph launch per call, with the new
+ * image and output addresses flowing through the staged descriptors that
+ * the graph's first node copies.  mode 2: as 1, and the two launch
+ * schedules (the Haralick chain on a side stream, or every launch on one
+ * stream) are both captured, timed on their first replays and the faster
+ * kept.  Graphs are dropped when the context's arena grows or a table is
+ * freed.  WNDGPU_GRAPH=1 / 2 sets the mode at context creation.
+ * wndgpu_graph_stats: captures and replays so far, graphs cached, and the
+ * schedule picked for the most recently used graph (-1 while undecided). */
+int wndgpu_set_graph_mode(wndgpu_ctx *ctx, int mode);
+int wndgpu_graph_stats(const wndgpu_ctx *ctx, long *captures, long *replays, int *cached, int *schedule_of_last);
+
 /* Per-launch timings of the last wndgpu_compute / wndgpu_compute_device made
  * with WNDGPU_TIMING set in the environment (ms from a HIP event pair around
  * each launch, in execution order; launches are named by feature group, and

@@ -21,6 +21,7 @@
 #include <atomic>
 #include <mutex>
 #include <set>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -83,7 +84,9 @@ struct Group {
   std::string name;
 };
 
+static std::atomic<long> g_plan_uid{0};
 struct wndgpu_plan {
+  long uid = ++g_plan_uid;  // identity of the plan in batch-graph signatures
   std::vector<Group> groups;
   int n_features = 0;
   bool color = false;  // some group reads the colour plane (RGB images only)
@@ -254,6 +257,22 @@ struct wndgpu_ctx {
   long batch_id = 0;
   size_t table_budget = (size_t)4 << 30;
   DevTable gabor_f32, gabor_bhi, gabor_blo;  // Gabor kernels for the reduced-precision forms
+  // Batch graphs (wndgpu_set_graph_mode): the launch sequence of a batch
+  // whose signature (plan, geometry, arena and table state) was seen before
+  // is captured into a hipGraph and replayed; mode 2 also times the two
+  // launch schedules (side stream / one stream) and keeps the faster.
+  struct BatchGraph {
+    hipGraphExec_t exec[2] = {nullptr, nullptr};  // [0] side-stream schedule, [1] one stream
+    float ms[2] = {-1.f, -1.f};
+    int pick = -1;      // schedule in use once both are timed (mode 2)
+    int pending = -1;   // schedule whose events of the previous call are still to be read
+    long used = 0;
+  };
+  int graph_mode = 0;
+  std::map<std::string, BatchGraph> graphs;
+  std::set<std::string> graph_seen, graph_failed;
+  long graph_clock = 0, graph_captures = 0, graph_replays = 0;
+  hipEvent_t gev0 = nullptr, gev1 = nullptr;
   std::vector<std::pair<std::string, double>> times;
   // per-launch timing (WNDGPU_TIMING): an event pair around every launch,
   // read after the compute call has synchronised
@@ -397,6 +416,16 @@ static int get_cf(wndgpu_ctx *c, int w, int h, CfDev **out) {
   return 0;
 }
 
+// Cached batch graphs hold device addresses of the arena and the tables in
+// their kernel arguments: they are dropped whenever either is freed.
+static void drop_graphs(wndgpu_ctx *c) {
+  for (auto &kv : c->graphs)
+    for (hipGraphExec_t &e : kv.second.exec)
+      if (e) (void)hipGraphExecDestroy(e);
+  c->graphs.clear();
+  c->graph_seen.clear();
+}
+
 // The size-dependent tables a batch's sizing pass listed: missing ones are
 // computed on up to 16 host threads (glibc libm, as the reference), then
 // uploaded; afterwards tables no batch has read recently are freed while the
@@ -460,6 +489,7 @@ static int prepare_tables(wndgpu_ctx *c) {
     for (auto &kv : c->cf)
       if (kv.second.used < oldest) { oldest = kv.second.used; which = 1; key = kv.first; }
     if (which < 0) break;  // everything left is read by this batch
+    drop_graphs(c);
     if (which == 0) {
       DevTable &t = c->cheb[key];
       total -= t.bytes;
@@ -929,7 +959,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         hipStream_t s2 = c->serial ? c->stream : c->side;
         push_side(gn + " [co-occurrence, f1-f13, Q]", [=]() {
           launch_gather_stats(sts, nst, ne, s2);
-          hipMemsetAsync(lut_all, 0, 256 * sizeof(int) * (size_t)ne, s2);
+          launch_fill_int(lut_all, 256 * (long)ne, 0, s2);
           launch_haralick_prep(ni, nst, hdd, ne, (int)maxp, s2);
         });
         push_side(gn + " [k_har_hess]", [=]() { launch_haralick_hess(hdd, ne, s2); });
@@ -1228,6 +1258,11 @@ static int ensure_color_lut(wndgpu_ctx *c) {
   return 0;
 }
 
+static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, size_t first_feature, size_t hbytes,
+                       bool timing);
+static int run_batch_graph(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, const Arena &A,
+                           std::vector<LaunchItem> &launches, size_t first_feature);
+
 static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timing) {
   int r;
   if ((r = ensure_tables(c))) return r;
@@ -1239,6 +1274,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   Arena A1{c, true};
   if ((r = build_batch(c, plan, B, A1, launches, &first_feature))) return r;
   if ((r = prepare_tables(c))) return r;
+  if (A1.d + 4096 > c->dev_cap || A1.h + 4096 > c->host_cap) drop_graphs(c);
   if (A1.d + 4096 > c->dev_cap) {
     if (c->dev) hipFree(c->dev);
     c->dev = nullptr;
@@ -1257,18 +1293,25 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   // pass 2: real addresses
   Arena A2{c, false};
   if ((r = build_batch(c, plan, B, A2, launches, &first_feature))) return r;
-  HIPCHK(c, hipMemcpyAsync(c->hdesc, c->host, A2.h, hipMemcpyHostToDevice, c->stream));
   c->times.clear();
   c->tnames.clear();
+  if (c->graph_mode && !timing) return run_batch_graph(c, plan, B, A2, launches, first_feature);
+  return issue_batch(c, launches, first_feature, A2.h, timing);
+}
+
+// The launch sequence of a batch: the descriptor copy, the transform nodes
+// on `stream`, the side-stream launches forked after the nodes, the other
+// feature launches, and the join that makes `stream` wait for the side
+// stream.  Only stream operations: the sequence can be captured.
+static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, size_t first_feature, size_t hbytes,
+                       bool timing) {
+  HIPCHK(c, hipMemcpyAsync(c->hdesc, c->host, hbytes, hipMemcpyHostToDevice, c->stream));
   if (timing)
     while (c->evpool.size() < 2 * launches.size()) {
       hipEvent_t e;
       HIPCHK(c, hipEventCreate(&e));
       c->evpool.push_back(e);
     }
-  // transform nodes on `stream`; then the side-stream launches, forked
-  // after the nodes; then the other feature launches; the join makes
-  // `stream` wait for the side stream
   size_t slot = 0;
   auto issue = [&](const LaunchItem &L) {
     hipStream_t st = (L.side && !c->serial) ? c->side : c->stream;
@@ -1301,6 +1344,118 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   return 0;
 }
 
+// Everything a batch's kernel arguments depend on besides the staged
+// descriptors (which the captured copy refreshes on every replay): the
+// plan, its precision and schedule, the arena and staging sizes and
+// addresses, and the geometry of every image, pre-processing step and
+// sample.  Tables are not in it: freeing one drops every cached graph.
+static std::string batch_signature(const wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, const Arena &A,
+                                   size_t nlaunch) {
+  std::ostringstream o;
+  o << plan->uid << '|' << plan->precision << '|' << c->serial << '|' << (const void *)c->dev << '|'
+    << (const void *)c->hdesc << '|' << (const void *)c->host << '|' << A.d << '|' << A.h << '|' << nlaunch << '|'
+    << B.n << '|' << B.nf << '|' << B.sampled;
+  for (const LoadDesc &L : B.load) o << '|' << L.h << 'x' << L.w << ',' << L.stride << ',' << L.dtype;
+  for (const PreImage &P : B.pre)
+    o << '|' << P.full.h << 'x' << P.full.w << ',' << P.full.stride << ',' << P.full.dtype << ';' << P.roi.h << 'x'
+      << P.roi.w << ';' << P.ch << ',' << P.cw << ',' << P.dh << ',' << P.dw << ',' << P.ds << ',' << P.dx << ','
+      << P.norm << ',' << P.mean << ',' << P.stdev << ',' << P.max_range;
+  for (const SampleRef &q : B.smp) o << '|' << q.img << ',' << q.rot << ',' << q.x0 << ',' << q.y0 << ',' << q.h << ',' << q.w;
+  return o.str();
+}
+
+// capture the launch sequence into an executable graph; on any capture
+// error the stream leaves capture mode and *exec stays null
+static void capture_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, size_t first_feature, size_t hbytes,
+                          hipGraphExec_t *exec) {
+  *exec = nullptr;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) return;
+  const int r = issue_batch(c, launches, first_feature, hbytes, false);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (r == 0 && e == hipSuccess && g) {
+    if (hipGraphInstantiate(exec, g, nullptr, nullptr, 0) != hipSuccess) *exec = nullptr;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipGetLastError();
+  if (*exec) c->graph_captures++;
+}
+
+static int run_batch_graph(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, const Arena &A,
+                           std::vector<LaunchItem> &launches, size_t first_feature) {
+  const std::string key = batch_signature(c, plan, B, A, launches.size());
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    // the first batch of a signature runs eagerly (tables, attributes and
+    // allocations settle); the second is captured
+    if (c->graph_failed.count(key) || c->graph_seen.insert(key).second)
+      return issue_batch(c, launches, first_feature, A.h, false);
+    wndgpu_ctx::BatchGraph G;
+    const bool serial0 = c->serial;
+    const int sched0 = serial0 ? 1 : 0;
+    capture_batch(c, launches, first_feature, A.h, &G.exec[sched0]);
+    if (c->graph_mode == 2 && G.exec[sched0]) {
+      // the other schedule: the same batch built with the other stream choice
+      c->serial = !serial0;
+      std::vector<LaunchItem> other;
+      size_t ff = 0;
+      Arena A3{c, false};
+      int r = build_batch(c, plan, B, A3, other, &ff);
+      if (r == 0 && A3.d == A.d && A3.h == A.h) capture_batch(c, other, ff, A3.h, &G.exec[1 - sched0]);
+      c->serial = serial0;
+      if (r) return r;
+      // restore the staging contents of the batch being run
+      Arena A4{c, false};
+      std::vector<LaunchItem> again;
+      if ((r = build_batch(c, plan, B, A4, again, &ff))) return r;
+    }
+    if (!G.exec[sched0]) {
+      for (hipGraphExec_t e : G.exec)
+        if (e) (void)hipGraphExecDestroy(e);
+      c->graph_failed.insert(key);
+      return issue_batch(c, launches, first_feature, A.h, false);
+    }
+    if (c->graphs.size() >= 16) {  // keep the 16 most recently used
+      auto old = c->graphs.begin();
+      for (auto g = c->graphs.begin(); g != c->graphs.end(); ++g)
+        if (g->second.used < old->second.used) old = g;
+      for (hipGraphExec_t e : old->second.exec)
+        if (e) (void)hipGraphExecDestroy(e);
+      c->graphs.erase(old);
+    }
+    it = c->graphs.emplace(key, G).first;
+  }
+  wndgpu_ctx::BatchGraph &G = it->second;
+  G.used = ++c->graph_clock;
+  if (G.pending >= 0) {  // the previous call has completed (every compute call waits for its streams)
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, c->gev0, c->gev1) == hipSuccess) G.ms[G.pending] = ms;
+    G.pending = -1;
+  }
+  int sched = c->serial ? 1 : 0;
+  if (c->graph_mode == 2 && G.exec[0] && G.exec[1]) {
+    if (G.ms[0] < 0) sched = 0;
+    else if (G.ms[1] < 0) sched = 1;
+    else sched = G.pick = G.ms[1] < G.ms[0] ? 1 : 0;
+    if (G.pick < 0) {
+      if (!c->gev0) {
+        HIPCHK(c, hipEventCreate(&c->gev0));
+        HIPCHK(c, hipEventCreate(&c->gev1));
+      }
+      HIPCHK(c, hipEventRecord(c->gev0, c->stream));
+      HIPCHK(c, hipGraphLaunch(G.exec[sched], c->stream));
+      HIPCHK(c, hipEventRecord(c->gev1, c->stream));
+      G.pending = sched;
+      c->graph_replays++;
+      return 0;
+    }
+  }
+  if (!G.exec[sched]) sched = G.exec[0] ? 0 : 1;
+  HIPCHK(c, hipGraphLaunch(G.exec[sched], c->stream));
+  c->graph_replays++;
+  return 0;
+}
+
 // ============================================================ C ABI
 extern "C" {
 
@@ -1328,6 +1483,7 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     return WNDGPU_E_DEVICE;
   }
   c->serial = getenv("WNDGPU_SERIAL") != nullptr;
+  if (const char *gm = getenv("WNDGPU_GRAPH")) c->graph_mode = std::min(2, std::max(0, atoi(gm)));
   if (const char *mb = getenv("WNDGPU_TABLE_CACHE_MB")) c->table_budget = (size_t)std::max(0L, atol(mb)) << 20;
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS
   *out = c;
@@ -1339,6 +1495,9 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->side);
   hipStreamSynchronize(c->stream);
+  drop_graphs(c);
+  if (c->gev0) hipEventDestroy(c->gev0);
+  if (c->gev1) hipEventDestroy(c->gev1);
   auto fr = [](DevTable &t) { if (t.p) hipFree(t.p); t.p = nullptr; };
   for (auto &kv : c->cheb) fr(kv.second);
   for (auto &kv : c->fft) fr(kv.second.first);
@@ -1912,6 +2071,27 @@ int wndgpu_last_node_times(wndgpu_ctx *c, double *ms, const char **names, int ca
     k++;
   }
   return k;
+}
+
+int wndgpu_set_graph_mode(wndgpu_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 2) return fail(c, WNDGPU_E_ARG, "graph mode must be 0, 1 or 2");
+  if (mode == 0) drop_graphs(c);
+  c->graph_mode = mode;
+  return WNDGPU_OK;
+}
+
+int wndgpu_graph_stats(const wndgpu_ctx *c, long *captures, long *replays, int *cached, int *schedule_of_last) {
+  if (!c) return WNDGPU_E_ARG;
+  if (captures) *captures = c->graph_captures;
+  if (replays) *replays = c->graph_replays;
+  if (cached) *cached = (int)c->graphs.size();
+  if (schedule_of_last) {
+    *schedule_of_last = -1;
+    long best = -1;
+    for (const auto &kv : c->graphs)
+      if (kv.second.used > best) { best = kv.second.used; *schedule_of_last = kv.second.pick; }
+  }
+  return WNDGPU_OK;
 }
 
 // ------------------------------------------------------- differentiable form

@@ -595,6 +595,11 @@ __global__ void k_ad_cf_scatter(const int *kk, int np, int *slot) {
   if (q < np) slot[kk[q]] = q;
 }
 
+// device fill (used where a memset would otherwise sit inside a capturable launch sequence)
+void launch_fill_int(int *p, long n, int v, hipStream_t s) {
+  if (n > 0) k_ad_fill_int<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, n, v);
+}
+
 // slot[pixel] = in-disk index or -1 (kk holds distinct indices below plane)
 void launch_ad_cf_slots(const int *kk, int np, long plane, int *slot, hipStream_t s) {
   k_ad_fill_int<<<(unsigned)((plane + 255) / 256), 256, 0, s>>>(slot, plane, -1);

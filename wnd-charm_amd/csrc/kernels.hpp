@@ -186,6 +186,7 @@ size_t zernike_ad_work_doubles(int n, int h, int w);
 void launch_ad_zernike_fwd(const double *in, double *out, double *work, int n, int h, int w, hipStream_t s);
 void launch_ad_zernike_bwd(const double *in, const double *gout, double *grad, double *work, int n, int h, int w,
                            hipStream_t s);
+void launch_fill_int(int *p, long n, int v, hipStream_t s);
 int cf_ad_chunks(int np);
 void launch_ad_cf_slots(const int *kk, int np, long plane, int *slot, hipStream_t s);
 void launch_ad_cf_fwd(const double *v, int n, long plane, const int *kk, const double *tn, const double *fc,

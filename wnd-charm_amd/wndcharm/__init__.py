@@ -21,7 +21,7 @@ __all__ = ['ImageMatrix', 'PyImageMatrix', 'DoubleVector', 'FeatureComputationPl
            'FeatureComputationPlanExecutor', 'ImageTransform', 'FeatureAlgorithm', 'compute_features',
            'normalize_train', 'normalize_test', 'fisher_weights', 'wnd5', 'WndGpuError', 'library_path',
            'read_tiff', 'write_sig', 'read_sig', 'write_fit', 'read_fit', 'write_weights', 'read_weights',
-           'FeatureTable', 'Sampling', 'FP64', 'FP32', 'BF16X3']
+           'FeatureTable', 'Sampling', 'FP64', 'FP32', 'BF16X3', 'set_graph_mode', 'graph_stats']
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 library_path = os.environ.get('WNDGPU_LIB', os.path.join(os.path.dirname(_HERE), 'lib', 'libwndgpu.so'))
@@ -95,6 +95,8 @@ def _load():
         'wndgpu_table_destroy': (None, [_P]),
         'wndgpu_weights_write': (_I, [ctypes.c_char_p, _P, _P, _I]),
         'wndgpu_weights_read': (_I, [ctypes.c_char_p, _P, _I, _P]),
+        'wndgpu_set_graph_mode': (_I, [_P, _I]),
+        'wndgpu_graph_stats': (_I, [_P, _P, _P, _P, _P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -510,6 +512,22 @@ def wnd5(train, classes, n_classes, weights, test):
     _check(_load().wndgpu_wnd5(_context(), _ptr(tr), _ptr(c), tr.shape[0], tr.shape[1], int(n_classes), _ptr(w),
                                _ptr(te), nt, _ptr(prob), _ptr(norm), _ptr(pred)), 'wnd5')
     return prob, norm, pred
+
+
+def set_graph_mode(mode):
+    """0: every batch issues its launches; 1: repeated batch signatures are
+    captured once into a hipGraph and replayed; 2: also pick the faster of
+    the two launch schedules by timing both (wndgpu_set_graph_mode)."""
+    _check(_load().wndgpu_set_graph_mode(_context(), int(mode)), 'set_graph_mode')
+
+
+def graph_stats():
+    """(captures, replays, graphs cached, schedule of the last graph or -1)"""
+    cap, rep = ctypes.c_long(), ctypes.c_long()
+    cached, sched = _I(), _I()
+    _check(_load().wndgpu_graph_stats(_context(), ctypes.byref(cap), ctypes.byref(rep), ctypes.byref(cached),
+                                      ctypes.byref(sched)), 'graph_stats')
+    return cap.value, rep.value, cached.value, sched.value
 
 
 def last_node_times():
