@@ -21,8 +21,10 @@ Also reported:
 
 forms         the metric's comparison: the same batch with the basis
                 contractions in BF16x3 (batched-mixed; with how many
-                columns moved against FP64), and the plain form, one call
-                per image, on the first 40 images.
+                columns moved against FP64), the batched FP64 form replayed
+                as a hipGraph with its launch schedule autotuned (and
+                whether its features equal the eager ones), and the plain
+                form, one call per image, on the first 40 images.
 
 --tiles N: the uneven-tiles configuration instead -- the first N tiles of
 the 8-class tiles configuration (seed 4004, sides independently uniform in
@@ -563,6 +565,22 @@ def main():
     diff = (out_m != out).cpu().numpy()
     forms['bf16x3_columns_changed'] = int(diff.any(0).sum())
     forms['bf16x3_exact_columns_changed'] = int((diff.any(0) & exact).sum())
+    # the batched FP64 form replayed as a hipGraph (graph mode 2): the first call of the batch runs
+    # eagerly, the second captures both launch schedules, the next two time them, then 3 timed replays
+    out_eager = out.clone()
+    wndcharm.set_graph_mode(2)
+    for _ in range(4):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        step()
+    torch.cuda.synchronize()
+    forms['batched_fp64_graph'] = round(3 * len(imgs) / (time.perf_counter() - t0), 3)
+    _, _, _, sched = wndcharm.graph_stats()
+    forms['graph_schedule'] = {0: 'Haralick chain on a side stream', 1: 'one stream'}.get(sched, 'undecided')
+    forms['graph_equals_eager'] = bool(np.array_equal(out.cpu().numpy(), out_eager.cpu().numpy(), equal_nan=True))
+    wndcharm.set_graph_mode(0)
     npi = min(40, len(imgs))
     st1 = np.zeros(1, np.int32)
 
