@@ -26,11 +26,13 @@ forms         the metric's comparison: the same batch with the basis
                 whether its features equal the eager ones), and the plain
                 form, one call per image, on the first 40 images.
 
---tiles N: the uneven-tiles configuration instead -- the first N tiles of
-the 8-class tiles configuration (seed 4004, sides independently uniform in
-[64, 1024]) through the long set in batches of --batch tiles, inputs
-resident in HBM; reported next to the short-set plain per-image form
-(128x128 images, one call per image) on the same GPU.
+--tiles N: the uneven-tiles configuration instead -- N tiles (N/8 from each
+class) of the 8-class tiles configuration (seed 4004, sides independently
+uniform in [64, 1024]) through the long set in batches of --batch tiles,
+inputs resident in HBM, then WND5-classified against --train-per-class
+training tiles per class (default 500, features computed before the timed
+region); reported next to the short-set plain per-image form (128x128
+images, one call per image) on the same GPU.
 
 --mixed: the mixed-precision configuration instead -- the 20 short-set
 128x128 images (2 classes, periods 8 / 20 px, seed 1001) through the
@@ -148,42 +150,67 @@ def _tile(k):
 
 
 def tiles_main(args):
+    """The uneven-tiles configuration: --tiles test tiles (the same number from
+    each of the 8 classes) through the long set in batches of --batch, then
+    WND5-classified against a training space of --train-per-class tiles per
+    class (Fisher weights, top 15%).  Tile k of the configuration belongs to
+    class k // 1250; training tiles are the first of each class's 1250, test
+    tiles follow from index 500 of each class."""
     from concurrent.futures import ThreadPoolExecutor
-    n = args.tiles
-    # the generator is numpy on the host (it releases the GIL): the tiles are
-    # made on 16 threads, no child processes; the class of tile k is k // 1250
-    # as in the configuration's global order
+    per_test = max(1, args.tiles // 8)
+    n = 8 * per_test
+    per_train = min(args.train_per_class, 500)
+    test_k = [1250 * c + 500 + j for c in range(8) for j in range(per_test)]
+    train_k = [1250 * c + j for c in range(8) for j in range(per_train)]
+    # the generator is numpy on the host (it releases the GIL): 16 threads, no child processes
+    t0 = time.time()
     with ThreadPoolExecutor(16) as ex:
-        tiles = list(ex.map(_tile, range(n)))
+        tiles = list(ex.map(_tile, test_k + train_k))
+    t_gen = time.time() - t0
+    print('tiles: %d generated in %.1f s' % (len(tiles), t_gen), file=sys.stderr, flush=True)
     import torch
     import wndcharm
     from wndcharm import synthetic
-    pix = sum(t.size for t in tiles)
+    from wndcharm import autograd as ad
+    test, train = tiles[:n], tiles[n:]
+    pix = sum(t.size for t in test)
     plan = wndcharm.StdFeatureComputationPlans.getFeatureSetLong()
     nf = plan.n_features
     lib = wndcharm._load()
     ctx = wndcharm._context()
-    dev = [torch.from_numpy(t.view(np.int16)).cuda() for t in tiles]
-    out = torch.zeros((args.batch, nf), dtype=torch.float64, device='cuda')
     status = np.zeros(args.batch, np.int32)
 
-    def run():
-        for b0 in range(0, n, args.batch):
+    def features(group, feats):
+        dev = [torch.from_numpy(t.view(np.int16)).cuda() for t in group]
+        for b0 in range(0, len(dev), args.batch):
             chunk = dev[b0:b0 + args.batch]
             st = (wndcharm._Image * len(chunk))()
             for i, t in enumerate(chunk):
                 st[i] = wndcharm._Image(ctypes.c_void_p(t.data_ptr()), t.shape[0], t.shape[1], t.shape[1], 0)
-            rc = lib.wndgpu_compute_device(ctx, plan.handle, st, len(chunk), ctypes.c_void_p(out.data_ptr()),
-                                           ctypes.c_void_p(status.ctypes.data))
+            rc = lib.wndgpu_compute_device(ctx, plan.handle, st, len(chunk),
+                                           ctypes.c_void_p(feats[b0:].data_ptr()), ctypes.c_void_p(status.ctypes.data))
             if rc != 0:
                 raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
             print('tiles %d..%d done' % (b0, b0 + len(chunk) - 1), file=sys.stderr, flush=True)
 
+    # training space (setup, not timed)
+    t0 = time.time()
+    ftrain = torch.zeros((len(train), nf), dtype=torch.float64, device='cuda')
+    features(train, ftrain)
+    torch.cuda.synchronize()
+    t_train = time.time() - t0
+    classes = np.repeat(np.arange(1, 9), per_train).astype(np.int32)
+    model = ad.WNDModel(ftrain.cpu().numpy(), classes, 8, used_fraction=0.15)
+    truth = torch.from_numpy(np.repeat(np.arange(1, 9), per_test)).cuda()
+    ftest = torch.zeros((n, nf), dtype=torch.float64, device='cuda')
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run()
+    features(test, ftest)
+    with torch.no_grad():
+        pred = torch.argmax(model.probabilities(ftest), 1) + 1
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    accuracy = float((pred == truth).double().mean().item())
     # the short-set plain per-image form on the same GPU, for comparison
     imgs, _ = synthetic.config_images('short_128')
     short = wndcharm.StdFeatureComputationPlans.getFeatureSet()
@@ -193,13 +220,18 @@ def tiles_main(args):
         wndcharm.compute_features([im], short)
     plain = len(imgs) / (time.perf_counter() - t1)
     print(json.dumps({
-        'metric': 'feature vectors/sec (2919-feat), uneven tiles batched', 'value': round(n / el, 3),
+        'metric': 'feature vectors/sec (2919-feat), uneven tiles batched + WND5', 'value': round(n / el, 3),
         'unit': 'feature vectors/s', 'n_gpus': 1, 'higher_is_better': True, 'dtype': 'f64',
         'data': 'synthetic (seeded grating/blob/Poisson generator, uint16 tiles, sides uniform in [64, 1024])',
-        'config': {'workload': 'long set over %d uneven tiles, %d per call' % (n, args.batch), 'tiles': n,
-                   'mean_pixels': round(pix / n), 'seconds': round(el, 3)},
+        'config': {'workload': 'long set over %d uneven test tiles (%d per class), %d per call, then WND5 against '
+                               '%d training tiles (8 x %d)' % (n, per_test, args.batch, len(train), per_train),
+                   'tiles': n, 'mean_pixels': round(pix / n), 'seconds': round(el, 3)},
         'megapixels_per_s': round(pix / el / 1e6, 3),
-        'short_set_plain_per_image_128': round(plain, 3)}))
+        'wnd5_accuracy': round(accuracy, 4),
+        'training_features_tiles_per_s': round(len(train) / t_train, 2),
+        'generator_s': round(t_gen, 1),
+        'short_set_plain_per_image_128': round(plain, 3),
+        'vs_short_set_plain_per_image': round(n / el / plain, 2)}))
 
 
 def mixed_main(args):
