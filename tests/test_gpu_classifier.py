@@ -32,3 +32,43 @@ def test_normalize_fisher_wnd5():
     for i in range(n):
         assert float('%.3g' % nf[i]) == d['kat_norm'][i]
         assert [float('%.3f' % p) for p in prob[i]] == list(d['kat_p'][i])
+
+
+def test_short_set_five_fold_predictions_match_oracle():
+    """The plain-method short-set configuration: 2 classes x 10 seeded 128x128
+    images, short-set features on the GPU against the oracle (integer-valued
+    columns exact, the rest to 1e-6), then a 5-fold train/test split (two
+    images per class held out per fold) with normalisation, Fisher weights
+    (top 15%) and WND5, whose predicted classes must equal the oracle's run
+    on the oracle's features."""
+    from wndcharm import synthetic
+    from helpers import exact_mask_for_names, orc_features, orc_names
+    imgs, cls = synthetic.config_images('short_128')
+    plan = wndcharm.StdFeatureComputationPlans.getFeatureSet()
+    got = wndcharm.compute_features(imgs, plan)
+    want = np.stack([orc_features(im.astype(np.float64), False) for im in imgs])
+    exact = exact_mask_for_names(orc_names(False))
+    assert np.array_equal(got[:, exact], want[:, exact], equal_nan=True)
+    close = (got == want) | (np.abs(got - want) <= 1e-6 * np.abs(want)) | (np.abs(got - want) < 1e-12)
+    assert close[:, ~exact].all()
+    lib = oracle()
+    for k in range(5):
+        test = np.r_[2 * k:2 * k + 2, 10 + 2 * k:10 + 2 * k + 2]
+        train = np.setdiff1d(np.arange(20), test)
+        tr, mins, maxs = wndcharm.normalize_train(got[train])
+        w = wndcharm.fisher_weights(tr, cls[train], 2, 0.15)
+        te = np.stack([wndcharm.normalize_test(got[t], mins, maxs) for t in test])
+        pred = wndcharm.wnd5(tr, cls[train], 2, w, te)[2]
+        otr = np.ascontiguousarray(want[train])
+        n, f = otr.shape
+        omin, omax = np.zeros(f), np.zeros(f)
+        lib.orc_normalize_train(ptr(otr), n, f, ptr(omin), ptr(omax))
+        ocls = np.ascontiguousarray(cls[train], np.int32)
+        ow = np.zeros(f)
+        lib.orc_fisher(ptr(otr), ptr(ocls), n, f, 2, 0.15, ptr(ow))
+        for j, t in enumerate(test):
+            v = np.ascontiguousarray(want[t].copy())
+            lib.orc_normalize_test(ptr(v), f, ptr(omin), ptr(omax))
+            prob, norm = np.zeros(2), np.zeros(1)
+            opred = lib.orc_wnd5(ptr(otr), ptr(ocls), n, f, 2, ptr(ow), ptr(v), ptr(prob), ptr(norm))
+            assert pred[j] == opred, (k, t)

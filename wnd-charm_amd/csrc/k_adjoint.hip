@@ -205,33 +205,33 @@ constexpr int ZAD_L = 15;
 constexpr int ZAD_PAIRS = 72;
 constexpr int ZAD_NCO = 8;  // polynomial coefficients per pair (degree (n-m)/2 <= 7 in t)
 
-__device__ __forceinline__ int zad_off(int n) { return (n / 2) * (n / 2 + 1) + ((n & 1) ? n / 2 + 1 : 0); }
+__host__ __device__ constexpr int zad_off(int n) { return (n / 2) * (n / 2 + 1) + ((n & 1) ? n / 2 + 1 : 0); }
 
 // P_nm coefficients, highest power of t first: c_s = (-1)^s (n-s)! /
-// (s! ((n+m)/2-s)! ((n-m)/2-s)!), s = 0..(n-m)/2.  Fills co[72][8] and
-// nm[72] = n | m << 8.
-__device__ void zad_tables(double *co, int *nm) {
-  for (int k = threadIdx.x; k < ZAD_PAIRS; k += blockDim.x) {
-    int n = 0;
-    while (n < ZAD_L && zad_off(n + 1) <= k) n++;
-    const int m = 2 * (k - zad_off(n)) + (n & 1);
-    auto fact = [](int a) {
-      double r = 1;
-      for (int i = 2; i <= a; i++) r *= i;
-      return r;
-    };
-    const int K = (n - m) / 2;
-    for (int s = 0; s < ZAD_NCO; s++) {
-      double v = 0;
-      if (s <= K) {
-        v = fact(n - s) / (fact(s) * fact((n + m) / 2 - s) * fact(K - s));
-        if (s & 1) v = -v;
-      }
-      co[k * ZAD_NCO + s] = v;
-    }
-    nm[k] = n | (m << 8);
-  }
+// (s! ((n+m)/2-s)! ((n-m)/2-s)!), s = 0..(n-m)/2 (0 beyond), built at
+// compile time into constant memory: every wave of a block reads the same
+// pair at the same time, so the reads are scalar loads.
+struct ZadTable {
+  double co[ZAD_PAIRS * ZAD_NCO];
+};
+constexpr double zad_fact(int a) {
+  double r = 1;
+  for (int i = 2; i <= a; i++) r *= i;
+  return r;
 }
+constexpr ZadTable zad_make() {
+  ZadTable t{};
+  for (int n = 0; n <= ZAD_L; n++)
+    for (int m = n & 1; m <= n; m += 2) {
+      const int k = zad_off(n) + m / 2, K = (n - m) / 2;
+      for (int s = 0; s <= K; s++) {
+        const double v = zad_fact(n - s) / (zad_fact(s) * zad_fact((n + m) / 2 - s) * zad_fact(K - s));
+        t.co[k * ZAD_NCO + s] = (s & 1) ? -v : v;
+      }
+    }
+  return t;
+}
+__constant__ ZadTable ZAD_TAB = zad_make();
 
 // per image: hdr[b] = {S, xc, yc, rad}
 __global__ void __launch_bounds__(1024) k_ad_zern_moments(const double *in, int h, int w, double *hdr) {
@@ -265,11 +265,8 @@ __global__ void __launch_bounds__(1024) k_ad_zern_moments(const double *in, int 
 template <bool DERIV>
 __global__ void __launch_bounds__(256) k_ad_zern_accum(const double *in, int h, int w, const double *hdr,
                                                        double *part, int nchunk) {
-  __shared__ double co[ZAD_PAIRS * ZAD_NCO];
-  __shared__ int nm[ZAD_PAIRS];
   __shared__ double red[4][9 * 6];
-  zad_tables(co, nm);
-  __syncthreads();
+  const double *co = ZAD_TAB.co;
   const int b = blockIdx.z, g = blockIdx.y, chunk = blockIdx.x;
   const double *p = in + (long)b * h * w;
   const double S = hdr[4 * b], xc = hdr[4 * b + 1], yc = hdr[4 * b + 2], rad = hdr[4 * b + 3];
@@ -400,10 +397,8 @@ __global__ void __launch_bounds__(128) k_ad_zern_final(const double *hdr, const 
 // grad[b][p] = (sum_k Re(B_k Z_k(u_p)) - K0 - x_p Kx - y_p Ky) / S
 __global__ void __launch_bounds__(256) k_ad_zern_pixels(int h, int w, const double *hdr, const double *coef,
                                                         double *grad) {
-  __shared__ double co[ZAD_PAIRS * ZAD_NCO];
-  __shared__ int nm[ZAD_PAIRS];
   __shared__ double B[2 * ZAD_PAIRS + 3];
-  zad_tables(co, nm);
+  const double *co = ZAD_TAB.co;
   const int b = blockIdx.y;
   for (int i = threadIdx.x; i < 2 * ZAD_PAIRS + 3; i += blockDim.x) B[i] = coef[(long)b * (2 * ZAD_PAIRS + 3) + i];
   __syncthreads();
