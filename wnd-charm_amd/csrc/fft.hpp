@@ -8,14 +8,16 @@
 
 namespace wnd {
 
-__device__ inline void fft_lds(double2 *a, double2 *b, const FftPlan &pl, double2 **res) {
+// One line per group of `tpl` threads (lane = index in the group); every
+// group of the block runs the same plan, so the barriers are uniform.
+__device__ inline void fft_lds_group(double2 *a, double2 *b, const FftPlan &pl, int lane, int tpl, double2 **res) {
   const int n = pl.n;
   int Ns = 1;
   double2 *src = a, *dst = b;
   for (int s = 0; s < pl.nst; s++) {
     const int R = pl.radix[s];
     const int nr = n / R, step = n / (Ns * R);
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    for (int t = lane; t < n; t += tpl) {
       const int j = t % nr, q = t / nr;
       const int k = j % Ns;
       const int base = k * step + q * nr;
@@ -37,5 +39,12 @@ __device__ inline void fft_lds(double2 *a, double2 *b, const FftPlan &pl, double
   }
   *res = src;
 }
+
+__device__ inline void fft_lds(double2 *a, double2 *b, const FftPlan &pl, double2 **res) {
+  fft_lds_group(a, b, pl, threadIdx.x, blockDim.x, res);
+}
+
+// lines per 256-thread block for a line length n (lines * n <= 1024 up to n = 1024)
+__host__ __device__ inline int fft_lines(int n) { return n <= 256 ? 4 : (n <= 512 ? 2 : 1); }
 
 }  // namespace wnd

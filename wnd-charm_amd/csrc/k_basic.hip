@@ -226,38 +226,53 @@ __global__ void __launch_bounds__(256) k_cheb_gemm(const GemmDesc *ds) {
 // |F(H-ky, (W-kx) mod W)| as the reference does.  Mixed-radix Stockham FFT
 // in LDS (fft.hpp), one work-group per line.
 
-// rows: F1[y][x] = DFT_x(img[y][:]), complex, row-major H x W
+// rows: F1[y][x] = DFT_x(img[y][:]), complex, row-major H x W; fft_lines(W)
+// rows per block, each on its own group of threads and its own LDS pair
 __global__ void __launch_bounds__(256) k_fft_rows(const Img *in, double2 *const *f1, const FftPlan *plw) {
   extern __shared__ double2 sh[];
   const Img a = in[blockIdx.y];
-  if ((int)blockIdx.x >= a.h) return;
   const FftPlan pl = plw[blockIdx.y];
-  double2 *b0 = sh, *b1 = sh + pl.n;
-  for (int x = threadIdx.x; x < a.w; x += blockDim.x) b0[x] = make_double2(a.p[(long)blockIdx.x * a.w + x], 0.0);
+  const int L = fft_lines(pl.n), tpl = blockDim.x / L, line = threadIdx.x / tpl, lane = threadIdx.x - line * tpl;
+  const int row = blockIdx.x * L + line;
+  if ((int)blockIdx.x * L >= a.h) return;
+  double2 *b0 = sh + (long)line * 2 * pl.n, *b1 = b0 + pl.n;
+  for (int x = lane; x < a.w; x += tpl) b0[x] = make_double2(row < a.h ? a.p[(long)row * a.w + x] : 0.0, 0.0);
   __syncthreads();
   double2 *r;
-  fft_lds(b0, b1, pl, &r);
-  double2 *o = f1[blockIdx.y] + (long)blockIdx.x * a.w;
-  for (int x = threadIdx.x; x < a.w; x += blockDim.x) o[x] = r[x];
+  fft_lds_group(b0, b1, pl, lane, tpl, &r);
+  if (row < a.h) {
+    double2 *o = f1[blockIdx.y] + (long)row * a.w;
+    for (int x = lane; x < a.w; x += tpl) o[x] = r[x];
+  }
 }
 
-// columns: out(ky, x) = |DFT_y(F1[:][x])| for ky < H/2+1
+// columns: out(ky, x) = |DFT_y(F1[:][x])| for ky < H/2+1; fft_lines(H)
+// adjacent columns per block, loaded and stored row by row (coalesced)
 __global__ void __launch_bounds__(256) k_fft_cols(const Img *in, double2 *const *f1, const FftPlan *plh,
                                                   const Img *out) {
   extern __shared__ double2 sh[];
   const Img a = in[blockIdx.y], o = out[blockIdx.y];
-  if ((int)blockIdx.x >= a.w) return;
   const FftPlan pl = plh[blockIdx.y];
-  double2 *b0 = sh, *b1 = sh + pl.n;
+  const int n = pl.n, L = fft_lines(n), tpl = blockDim.x / L, line = threadIdx.x / tpl, lane = threadIdx.x - line * tpl;
+  const int x0 = blockIdx.x * L;
+  if (x0 >= a.w) return;
   const double2 *src = f1[blockIdx.y];
-  for (int y = threadIdx.x; y < a.h; y += blockDim.x) b0[y] = src[(long)y * a.w + blockIdx.x];
+  for (int e = threadIdx.x; e < a.h * L; e += blockDim.x) {
+    const int y = e / L, l = e - y * L, x = x0 + l;
+    sh[(long)l * 2 * n + y] = x < a.w ? src[(long)y * a.w + x] : make_double2(0.0, 0.0);
+  }
   __syncthreads();
+  double2 *b0 = sh + (long)line * 2 * n, *b1 = b0 + n;
   double2 *r;
-  fft_lds(b0, b1, pl, &r);
+  fft_lds_group(b0, b1, pl, lane, tpl, &r);
+  const long roff = (r == b1) ? n : 0;  // the same buffer of the pair for every line
   const int hh = a.h / 2 + 1;
-  for (int y = threadIdx.x; y < hh; y += blockDim.x) {
-    const double2 c = r[y];
-    o.p[(long)y * a.w + blockIdx.x] = sqrt(c.x * c.x + c.y * c.y);
+  for (int e = threadIdx.x; e < hh * L; e += blockDim.x) {
+    const int y = e / L, l = e - y * L, x = x0 + l;
+    if (x < a.w) {
+      const double2 c = sh[(long)l * 2 * n + roff + y];
+      o.p[(long)y * a.w + x] = sqrt(c.x * c.x + c.y * c.y);
+    }
   }
 }
 
@@ -459,9 +474,17 @@ void launch_cheb_gemm(const GemmDesc *d, int nimg, int maxM, int maxN, hipStream
 void launch_fft(const Img *in, double2 *const *f1, const FftPlan *plw, const FftPlan *plh, const Img *out, int nimg,
                 int maxh, int maxw, hipStream_t s) {
   const int nmax = maxh > maxw ? maxh : maxw;
-  const size_t lds = 2 * (size_t)nmax * sizeof(double2);
-  k_fft_rows<<<dim3(maxh, nimg), 256, lds, s>>>(in, f1, plw);
-  k_fft_cols<<<dim3(maxw, nimg), 256, lds, s>>>(in, f1, plh, out);
+  const size_t lds = 2 * (size_t)(nmax > 1024 ? nmax : 1024) * sizeof(double2);
+  static const bool attr = [] {
+    (void)hipFuncSetAttribute((const void *)k_fft_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4096 * 16);
+    (void)hipFuncSetAttribute((const void *)k_fft_cols, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4096 * 16);
+    return true;
+  }();
+  (void)attr;
+  // every image's row (column) count fits in its blocks: fft_lines is smallest for the longest line
+  const int lr = fft_lines(maxw), lc = fft_lines(maxh);
+  k_fft_rows<<<dim3((maxh + lr - 1) / lr, nimg), 256, lds, s>>>(in, f1, plw);
+  k_fft_cols<<<dim3((maxw + lc - 1) / lc, nimg), 256, lds, s>>>(in, f1, plh, out);
   const long nf = (long)(maxh - (maxh / 2 + 1) + 1) * maxw;
   k_fft_fill<<<dim3((nf + 255) / 256, nimg), 256, 0, s>>>(out);
 }

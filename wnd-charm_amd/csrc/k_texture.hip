@@ -1272,13 +1272,29 @@ void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s)
 // tamura.cpp:10-299.  scratch per image (5P doubles): [0, P) normalised
 // plane, [P, 2P) summed-area table (laufendeSumme), [2P, 3P) phi,
 // [3P, 4P) per-pixel sum_r terms, [4P, 4.5P) best coarseness scale (int).
-__global__ void k_tam_norm(const Img *imgs, const Stats *st, double *const *scr) {
+// (x - min) / max per pixel; planes are 256-byte aligned (arena), so each
+// thread streams two double pairs through the global address space
+typedef double tam_d2 __attribute__((ext_vector_type(2)));
+__global__ void __launch_bounds__(256) k_tam_norm(const Img *imgs, const Stats *st, double *const *scr) {
   const Img im = imgs[blockIdx.y];
   const Stats S = st[blockIdx.y];
-  const long n = (long)im.h * im.w;
-  double *nr = scr[blockIdx.y];
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    nr[i] = (im.p[i] - S.mn) / S.mx;
+  const long n = (long)im.h * im.w, n4 = n >> 2;
+  const gptr<const tam_d2> src = (gptr<const tam_d2>)to_global((const double *)im.p);
+  const gptr<tam_d2> dst = (gptr<tam_d2>)to_global(scr[blockIdx.y]);
+  const double mn = S.mn, mx = S.mx;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+    tam_d2 a = src[2 * i], b = src[2 * i + 1];
+    a.x = (a.x - mn) / mx;
+    a.y = (a.y - mn) / mx;
+    b.x = (b.x - mn) / mx;
+    b.y = (b.y - mn) / mx;
+    dst[2 * i] = a;
+    dst[2 * i + 1] = b;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+    const long i = (n4 << 2) + threadIdx.x;
+    scr[blockIdx.y][i] = (im.p[i] - mn) / mx;
+  }
 }
 
 // summed-area table with the reference's exact recurrence
@@ -1502,7 +1518,8 @@ void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, Out
                    hipStream_t s) {
   const long maxn = (long)maxh * maxw;
   int blocks = (int)((maxn + 255) / 256);
-  k_tam_norm<<<dim3(blocks > 2048 ? 2048 : blocks, nimg), 256, 0, s>>>(imgs, st, scratch);
+  const int nblocks = (int)((maxn / 4 + 255) / 256);
+  k_tam_norm<<<dim3(nblocks < 1 ? 1 : (nblocks > 1024 ? 1024 : nblocks), nimg), 256, 0, s>>>(imgs, st, scratch);
   k_tam_sat<<<nimg, 64, maxw * sizeof(double), s>>>(imgs, scratch);
   k_tam_pix<<<xcd_grid(blocks, nimg), 256, 0, s>>>(imgs, scratch, nimg, blocks);
   k_tam_final<<<nimg, 1024, 0, s>>>(imgs, scratch, o);
