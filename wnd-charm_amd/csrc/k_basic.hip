@@ -23,8 +23,28 @@ __global__ void k_load(const LoadDesc *d, Img *dst) {
     lo = L.range[0];
     scale = L.range[1] > lo ? (255.0 / (L.range[1] - lo)) : 0.0;
   }
+  if (L.dtype == 0 && L.stride == L.w && ((uintptr_t)L.src & 7) == 0) {
+    // contiguous uint16 (the common case): four pixels per thread, one 8-byte
+    // load and two 16-byte stores (planes are 256-byte aligned in the arena)
+    typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const gptr<const u16x4> src = (gptr<const u16x4>)to_global((const uint16_t *)L.src);
+    const gptr<f64x2> dp = (gptr<f64x2>)to_global(o.p);
+    const long n4 = n >> 2;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n4; i += (long)gridDim.x * blockDim.x) {
+      const u16x4 v = src[i];
+      dp[2 * i] = f64x2{(double)v.x, (double)v.y};
+      dp[2 * i + 1] = f64x2{(double)v.z, (double)v.w};
+    }
+    if (blockIdx.x == 0 && threadIdx.x < (n & 3)) {
+      const long i = (n4 << 2) + threadIdx.x;
+      o.p[i] = (double)((const uint16_t *)L.src)[i];
+    }
+    return;
+  }
+  // planes hold fewer than 2^31 pixels: 32-bit index arithmetic
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const int y = i / L.w, x = i - (long)y * L.w;
+    const int y = (int)i / L.w, x = (int)i - y * L.w;
     const long e = (long)y * L.stride + x;
     double v;
     if (L.dtype == 0) v = (double)((const uint16_t *)L.src)[e];
@@ -97,15 +117,56 @@ __global__ void __launch_bounds__(1024) k_stats(const Img *imgs, Stats *st) {
 // Edge transform (ImageMatrix::EdgeTransform, src/cmatrix.cpp:1294-1311)
 __global__ void k_edge(const Img *in, const Img *out) {
   const Img a = in[blockIdx.y], o = out[blockIdx.y];
-  const long n = (long)a.h * a.w;
-  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int y = i / a.w, x = i - (long)y * a.w;
-  const double c = a.p[i];
-  double my = 0, mx = 0;
-  if (y > 0 && y < a.h - 1) my = fmax(fabs(c - a.p[i - a.w]), fabs(c - a.p[i + a.w]));
-  if (x > 0 && x < a.w - 1) mx = fmax(fabs(c - a.p[i - 1]), fabs(c - a.p[i + 1]));
-  o.p[i] = fmax(mx, my);
+  // planes hold fewer than 2^31 pixels: 32-bit index arithmetic, global address space
+  const gptr<const double> p = to_global((const double *)a.p);
+  const gptr<double> q = to_global(o.p);
+  const int stride = gridDim.x * blockDim.x;
+  if ((a.w & 3) == 0 && (((uintptr_t)a.p | (uintptr_t)o.p) & 15) == 0) {
+    // rows of a multiple of 4 pixels: four adjacent pixels per thread, centre
+    // and vertical neighbours as 16-byte pairs, the two outer horizontal
+    // neighbours as single loads
+    typedef double f64x2 __attribute__((ext_vector_type(2)));
+    const int qw = a.w >> 2, nq = a.h * qw;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += stride) {
+      const int y = t / qw, x0 = (t - y * qw) << 2, i = y * a.w + x0;
+      const gptr<const f64x2> pc = (gptr<const f64x2>)(p + i);
+      const f64x2 c01 = pc[0], c23 = pc[1];
+      const double c[4] = {c01.x, c01.y, c23.x, c23.y};
+      double my[4] = {0, 0, 0, 0};
+      if (y > 0 && y < a.h - 1) {
+        const gptr<const f64x2> pu = (gptr<const f64x2>)(p + i - a.w), pd = (gptr<const f64x2>)(p + i + a.w);
+        const f64x2 u01 = pu[0], u23 = pu[1], d01 = pd[0], d23 = pd[1];
+        const double u[4] = {u01.x, u01.y, u23.x, u23.y}, d[4] = {d01.x, d01.y, d23.x, d23.y};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) my[k] = fmax(fabs(c[k] - u[k]), fabs(c[k] - d[k]));
+      }
+      const double l = x0 > 0 ? p[i - 1] : 0.0, r = x0 + 4 < a.w ? p[i + 4] : 0.0;
+      double e[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int x = x0 + k;
+        double mx = 0;
+        if (x > 0 && x < a.w - 1) {
+          const double left = k > 0 ? c[k - 1] : l, right = k < 3 ? c[k + 1] : r;
+          mx = fmax(fabs(c[k] - left), fabs(c[k] - right));
+        }
+        e[k] = fmax(mx, my[k]);
+      }
+      const gptr<f64x2> po = (gptr<f64x2>)(q + i);
+      po[0] = f64x2{e[0], e[1]};
+      po[1] = f64x2{e[2], e[3]};
+    }
+    return;
+  }
+  const int n = a.h * a.w;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const int y = i / a.w, x = i - y * a.w;
+    const double c = p[i];
+    double my = 0, mx = 0;
+    if (y > 0 && y < a.h - 1) my = fmax(fabs(c - p[i - a.w]), fabs(c - p[i + a.w]));
+    if (x > 0 && x < a.w - 1) mx = fmax(fabs(c - p[i - 1]), fabs(c - p[i + 1]));
+    q[i] = fmax(mx, my);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -451,7 +512,8 @@ void launch_gather_stats(const Stats *const *src, Stats *dst, int n, hipStream_t
   k_gather_stats<<<(n + 255) / 256, 256, 0, s>>>(src, dst, n);
 }
 void launch_load(const LoadDesc *d, Img *dst, int nimg, long maxn, hipStream_t s) {
-  int blocks = (int)((maxn + 255) / 256);
+  // four pixels per thread on the contiguous uint16 path; the other paths stride over the plane
+  int blocks = (int)((maxn + 1023) / 1024);
   if (blocks > 4096) blocks = 4096;
   k_load<<<dim3(blocks, nimg), 256, 0, s>>>(d, dst);
 }
@@ -459,7 +521,10 @@ void launch_stats(const Img *imgs, Stats *st, int nimg, hipStream_t s) {
   k_stats<<<nimg, 1024, 0, s>>>(imgs, st);
 }
 void launch_edge(const Img *in, const Img *out, int nimg, long maxn, hipStream_t s) {
-  k_edge<<<dim3((maxn + 255) / 256, nimg), 256, 0, s>>>(in, out);
+  // four pixels per thread where rows allow it; otherwise each thread strides over the plane
+  int blocks = (int)((maxn + 1023) / 1024);
+  if (blocks > 4096) blocks = 4096;
+  k_edge<<<dim3(blocks, nimg), 256, 0, s>>>(in, out);
 }
 void launch_wavelet(const Img *in, double *const *g, const int *gstride, const Img *out, int nimg, int maxh,
                     int maxw, hipStream_t s) {
