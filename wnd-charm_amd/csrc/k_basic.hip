@@ -23,9 +23,9 @@ __global__ void k_load(const LoadDesc *d, Img *dst) {
     lo = L.range[0];
     scale = L.range[1] > lo ? (255.0 / (L.range[1] - lo)) : 0.0;
   }
-  if (L.dtype == 0 && L.stride == L.w && ((uintptr_t)L.src & 7) == 0) {
+  if (L.dtype == 0 && L.stride == L.w && ((uintptr_t)L.src & 7) == 0 && ((uintptr_t)o.p & 15) == 0) {
     // contiguous uint16 (the common case): four pixels per thread, one 8-byte
-    // load and two 16-byte stores (planes are 256-byte aligned in the arena)
+    // load and two 16-byte stores
     typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
     typedef double f64x2 __attribute__((ext_vector_type(2)));
     const gptr<const u16x4> src = (gptr<const u16x4>)to_global((const uint16_t *)L.src);
