@@ -97,18 +97,19 @@ __global__ void __launch_bounds__(1024) k_har_cooc(const Img *imgs, const HarDes
   if (threadIdx.x == 0) total = 0;
   __syncthreads();
   const int rows = im.h, cols = im.w;
-  const long n = (long)rows * cols;
+  const int n = rows * cols;  // planes hold fewer than 2^31 pixels: 32-bit index arithmetic
+  const gptr<const uint8_t> gray = to_global((const uint8_t *)H.gray);
   int local = 0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) {
-    const int row = i / cols, col = i - (long)row * cols;
-    const int g = H.gray[i];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int row = i / cols, col = i - row * cols;
+    const int g = gray[i];
     if (!g) continue;
     int r2 = row, c2 = col;
     if (angle == 0) { c2 = col + 1; if (!(c2 < cols)) continue; }
     else if (angle == 1) { r2 = row + 1; c2 = col - 1; if (!(r2 < rows && c2 >= 0)) continue; }
     else if (angle == 2) { r2 = row + 1; if (!(r2 < rows)) continue; }
     else { r2 = row + 1; c2 = col + 1; if (!(r2 < rows && c2 < cols)) continue; }
-    const int g2 = H.gray[(long)r2 * cols + c2];
+    const int g2 = gray[r2 * cols + c2];
     if (!g2) continue;
     int x = lut[g], y = lut[g2];
     if (x == y) atomicAdd(&cnt[tri(x, x)], 2u);
