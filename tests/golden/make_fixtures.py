@@ -83,9 +83,14 @@ def main():
                         kat_p=np.array([[float(r[2]), float(r[3])] for r in rows]),
                         kat_pred=np.array([cn.index(r[5]) + 1 for r in rows], np.int32),
                         kat_interp=np.array([float(r[6]) for r in rows]))
+    write_raw(w, p)
+
+
+def write_raw(w=os.path.join(REF, 'wndchrm_tests'), p=os.path.join(REF, 'pywndcharm_tests')):
     # data files held verbatim (gzip-compressed) for the file-format tests:
     # .sig / .fit / weights text as the reference wrote it, and TIFF inputs
-    # (uncompressed big-endian 16-bit, LZW 16-bit, 8-bit strips)
+    # (uncompressed big-endian 16-bit, LZW 16-bit, 8-bit strips; the eight
+    # images of unbalanced.fit for the end-to-end .fit test)
     raw = os.path.join(OUT, 'raw')
     os.makedirs(raw, exist_ok=True)
     for src in [os.path.join(w, 't1_s01_c05_ij-l_precalculated.sig'),
@@ -93,6 +98,13 @@ def main():
                 os.path.join(w, 'unbalanced.fit'), os.path.join(w, 'test-l.fit'),
                 os.path.join(w, 'test_fit-l.weights'), os.path.join(w, 't1_s01_c05_ij.tif'),
                 os.path.join(w, 'images', 'binuc', 'binuc1.tiff'),
+                os.path.join(w, 'images', 'binuc', 'binuc2.tiff'),
+                os.path.join(w, 'images', 'binuc', 'binuc3.tiff'),
+                os.path.join(w, 'images', 'binuc', 'binuc4.tiff'),
+                os.path.join(w, 'images', 'binuc', 'binuc5.tiff'),
+                os.path.join(w, 'images', 'norm', 'norm1.tiff'),
+                os.path.join(w, 'images', 'norm', 'norm2.tiff'),
+                os.path.join(w, 'images', 'norm', 'norm3.tiff'),
                 os.path.join(p, 'lymphoma_eosin_channel_MCL_test_img_sj-05-3362-R2_001_E-t6x5_5_4-l.tiff'),
                 os.path.join(p, 'test-0032-0008-0008.tif')]:
         data = open(src, 'rb').read()

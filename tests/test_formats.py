@@ -147,12 +147,13 @@ def test_predictor_two(tmp_path):
     assert np.array_equal(wndcharm.read_tiff(p), a)
 
 
-def write_rgb16_be(path, a):
+def write_rgb16_be(path, a, rows_per_strip=None):
     """a minimal big-endian uncompressed RGB 16-bit TIFF (one strip)"""
     h, w, _ = a.shape
     pix = a.astype('>u2').tobytes()
+    rps = h if rows_per_strip is None else rows_per_strip
     entries = [(256, 4, 1, w), (257, 4, 1, h), (258, 3, 3, None), (259, 3, 1, 1), (262, 3, 1, 2),
-               (273, 4, 1, None), (277, 3, 1, 3), (278, 4, 1, h), (279, 4, 1, len(pix))]
+               (273, 4, 1, None), (277, 3, 1, 3), (278, 4, 1, rps), (279, 4, 1, len(pix))]
     ifd_off = 8
     ifd_len = 2 + 12 * len(entries) + 4
     bps_off = ifd_off + ifd_len
@@ -181,6 +182,17 @@ def test_rgb16_big_endian(tmp_path):
     write_rgb16_be(p, a)
     got = wndcharm.read_tiff(p)
     assert got.dtype == np.uint16 and np.array_equal(got, a)
+
+
+def test_rows_per_strip_zero(tmp_path):
+    """RowsPerStrip = 0 reads as one strip of the whole image (the field's
+    default), not as an empty strip"""
+    import wndcharm
+    a = np.random.default_rng(5).integers(0, 65536, (9, 14, 3)).astype(np.uint16)
+    p = str(tmp_path / 'z.tif')
+    write_rgb16_be(p, a, rows_per_strip=0)
+    got = wndcharm.read_tiff(p)
+    assert np.array_equal(got, a)
 
 
 def test_refused_tiffs(tmp_path):

@@ -300,6 +300,12 @@ static int fail(wndgpu_ctx *c, int code, const char *fmt, ...) {
     if (e_ != hipSuccess) return fail(ctx, WNDGPU_E_DEVICE, "%s: %s", #x, hipGetErrorString(e_));   \
   } while (0)
 
+// Release calls on paths that cannot return an error (teardown, destructors):
+// a failure is reported on stderr rather than dropped.
+static void release_check(hipError_t e, const char *what) {
+  if (e != hipSuccess) fprintf(stderr, "wndgpu: %s failed: %s\n", what, hipGetErrorString(e));
+}
+
 static int upload(wndgpu_ctx *c, DevTable &t, const void *src, size_t bytes) {
   HIPCHK(c, hipMalloc(&t.p, bytes ? bytes : 8));
   t.bytes = bytes;
@@ -421,7 +427,7 @@ static int get_cf(wndgpu_ctx *c, int w, int h, CfDev **out) {
 static void drop_graphs(wndgpu_ctx *c) {
   for (auto &kv : c->graphs)
     for (hipGraphExec_t &e : kv.second.exec)
-      if (e) (void)hipGraphExecDestroy(e);
+      if (e) release_check(hipGraphExecDestroy(e), "hipGraphExecDestroy");
   c->graphs.clear();
   c->graph_seen.clear();
 }
@@ -493,12 +499,12 @@ static int prepare_tables(wndgpu_ctx *c) {
     if (which == 0) {
       DevTable &t = c->cheb[key];
       total -= t.bytes;
-      hipFree(t.p);
+      HIPCHK(c, hipFree(t.p));
       c->cheb.erase(key);
     } else {
       CfDev &d = c->cf[key];
       total -= d.bytes();
-      for (DevTable *t : {&d.kk, &d.tn, &d.fc, &d.fs}) hipFree(t->p);
+      for (DevTable *t : {&d.kk, &d.tn, &d.fc, &d.fs}) HIPCHK(c, hipFree(t->p));
       c->cf.erase(key);
     }
   }
@@ -1276,15 +1282,18 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   if ((r = prepare_tables(c))) return r;
   if (A1.d + 4096 > c->dev_cap || A1.h + 4096 > c->host_cap) drop_graphs(c);
   if (A1.d + 4096 > c->dev_cap) {
-    if (c->dev) hipFree(c->dev);
+    if (c->dev) HIPCHK(c, hipFree(c->dev));
     c->dev = nullptr;
     size_t cap = (A1.d + 4096) + (A1.d + 4096) / 4;
     HIPCHK(c, hipMalloc(&c->dev, cap));
     c->dev_cap = cap;
   }
   if (A1.h + 4096 > c->host_cap) {
-    if (c->host) hipHostFree(c->host);
-    if (c->hdesc) hipFree(c->hdesc);
+    if (c->host) HIPCHK(c, hipHostFree(c->host));
+    if (c->hdesc) HIPCHK(c, hipFree(c->hdesc));
+    c->host = nullptr;
+    c->hdesc = nullptr;
+    c->host_cap = 0;
     size_t cap = (A1.h + 4096) * 2;
     HIPCHK(c, hipHostMalloc((void **)&c->host, cap, hipHostMallocDefault));
     HIPCHK(c, hipMalloc(&c->hdesc, cap));
@@ -1313,12 +1322,13 @@ static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, s
       c->evpool.push_back(e);
     }
   size_t slot = 0;
+  hipError_t ev_err = hipSuccess;
   auto issue = [&](const LaunchItem &L) {
     hipStream_t st = (L.side && !c->serial) ? c->side : c->stream;
-    if (timing) hipEventRecord(c->evpool[2 * slot], st);
+    if (timing && ev_err == hipSuccess) ev_err = hipEventRecord(c->evpool[2 * slot], st);
     L.f();
     if (timing) {
-      hipEventRecord(c->evpool[2 * slot + 1], st);
+      if (ev_err == hipSuccess) ev_err = hipEventRecord(c->evpool[2 * slot + 1], st);
       c->tnames.push_back(L.name);
     }
     slot++;
@@ -1340,6 +1350,7 @@ static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, s
     HIPCHK(c, hipEventRecord(c->join_ev, c->side));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
   }
+  HIPCHK(c, ev_err);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -1376,7 +1387,7 @@ static void capture_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches
   if (r == 0 && e == hipSuccess && g) {
     if (hipGraphInstantiate(exec, g, nullptr, nullptr, 0) != hipSuccess) *exec = nullptr;
   }
-  if (g) (void)hipGraphDestroy(g);
+  if (g) release_check(hipGraphDestroy(g), "hipGraphDestroy");
   (void)hipGetLastError();
   if (*exec) c->graph_captures++;
 }
@@ -1411,7 +1422,7 @@ static int run_batch_graph(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, con
     }
     if (!G.exec[sched0]) {
       for (hipGraphExec_t e : G.exec)
-        if (e) (void)hipGraphExecDestroy(e);
+        if (e) release_check(hipGraphExecDestroy(e), "hipGraphExecDestroy");
       c->graph_failed.insert(key);
       return issue_batch(c, launches, first_feature, A.h, false);
     }
@@ -1420,7 +1431,7 @@ static int run_batch_graph(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, con
       for (auto g = c->graphs.begin(); g != c->graphs.end(); ++g)
         if (g->second.used < old->second.used) old = g;
       for (hipGraphExec_t e : old->second.exec)
-        if (e) (void)hipGraphExecDestroy(e);
+        if (e) release_check(hipGraphExecDestroy(e), "hipGraphExecDestroy");
       c->graphs.erase(old);
     }
     it = c->graphs.emplace(key, G).first;
@@ -1475,10 +1486,10 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
-    if (c->join_ev) hipEventDestroy(c->join_ev);
-    if (c->fork_ev) hipEventDestroy(c->fork_ev);
-    if (c->side) hipStreamDestroy(c->side);
-    hipStreamDestroy(c->stream);
+    if (c->join_ev) release_check(hipEventDestroy(c->join_ev), "hipEventDestroy");
+    if (c->fork_ev) release_check(hipEventDestroy(c->fork_ev), "hipEventDestroy");
+    if (c->side) release_check(hipStreamDestroy(c->side), "hipStreamDestroy");
+    release_check(hipStreamDestroy(c->stream), "hipStreamDestroy");
     delete c;
     return WNDGPU_E_DEVICE;
   }
@@ -1492,13 +1503,16 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
 
 void wndgpu_destroy(wndgpu_ctx *c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  hipStreamSynchronize(c->side);
-  hipStreamSynchronize(c->stream);
+  release_check(hipSetDevice(c->device), "hipSetDevice");
+  release_check(hipStreamSynchronize(c->side), "hipStreamSynchronize");
+  release_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   drop_graphs(c);
-  if (c->gev0) hipEventDestroy(c->gev0);
-  if (c->gev1) hipEventDestroy(c->gev1);
-  auto fr = [](DevTable &t) { if (t.p) hipFree(t.p); t.p = nullptr; };
+  if (c->gev0) release_check(hipEventDestroy(c->gev0), "hipEventDestroy");
+  if (c->gev1) release_check(hipEventDestroy(c->gev1), "hipEventDestroy");
+  auto fr = [](DevTable &t) {
+    if (t.p) release_check(hipFree(t.p), "hipFree");
+    t.p = nullptr;
+  };
   for (auto &kv : c->cheb) fr(kv.second);
   for (auto &kv : c->fft) fr(kv.second.first);
   for (auto &kv : c->cf) { fr(kv.second.kk); fr(kv.second.tn); fr(kv.second.fc); fr(kv.second.fs); }
@@ -1506,14 +1520,14 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   for (auto &kv : c->comb) fr(kv.second);
   fr(c->gabor); fr(c->zern); fr(c->rtrig); fr(c->color_lut);
   fr(c->gabor_f32); fr(c->gabor_bhi); fr(c->gabor_blo);
-  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
-  if (c->dev) hipFree(c->dev);
-  if (c->hdesc) hipFree(c->hdesc);
-  if (c->host) hipHostFree(c->host);
-  hipEventDestroy(c->fork_ev);
-  hipEventDestroy(c->join_ev);
-  hipStreamDestroy(c->side);
-  hipStreamDestroy(c->stream);
+  for (hipEvent_t e : c->evpool) release_check(hipEventDestroy(e), "hipEventDestroy");
+  if (c->dev) release_check(hipFree(c->dev), "hipFree");
+  if (c->hdesc) release_check(hipFree(c->hdesc), "hipFree");
+  if (c->host) release_check(hipHostFree(c->host), "hipHostFree");
+  release_check(hipEventDestroy(c->fork_ev), "hipEventDestroy");
+  release_check(hipEventDestroy(c->join_ev), "hipEventDestroy");
+  release_check(hipStreamDestroy(c->side), "hipStreamDestroy");
+  release_check(hipStreamDestroy(c->stream), "hipStreamDestroy");
   delete c;
 }
 
@@ -1761,7 +1775,7 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (size_t k = 0; k < c->tnames.size(); k++) {
     float ms = 0;
-    hipEventElapsedTime(&ms, c->evpool[2 * k], c->evpool[2 * k + 1]);
+    HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[2 * k], c->evpool[2 * k + 1]));
     c->times.emplace_back(c->tnames[k], ms);
   }
   c->tnames.clear();
@@ -1812,11 +1826,15 @@ int wndgpu_algorithm(wndgpu_ctx *c, const char *name, const double *in, int h, i
 
 struct DBuf {
   void *p = nullptr;
-  ~DBuf() { if (p) hipFree(p); }
+  ~DBuf() {
+    if (p) release_check(hipFree(p), "hipFree");
+  }
 };
 struct FreeList {
   std::vector<void *> v;
-  ~FreeList() { for (void *p : v) hipFree(p); }
+  ~FreeList() {
+    for (void *p : v) release_check(hipFree(p), "hipFree");
+  }
 };
 
 // One gray transform of a device plane (h x w) into dout (out_size of t):

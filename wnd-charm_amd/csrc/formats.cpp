@@ -244,11 +244,13 @@ int wndgpu_tiff_read(const char *path, void **pixels, wndgpu_image *image) {
   if (comp != 1 && comp != 5 && comp != 8 && comp != 32946 && comp != 32773)
     return io_fail(WNDGPU_E_FORMAT, "'%s': compression %u is not read", path, comp);
   if (w > 1u << 20 || h > 1u << 20) return io_fail(WNDGPU_E_FORMAT, "'%s': image too large", path);
-  const uint32_t rps = std::min<uint32_t>(get(278, h), h);
+  // RowsPerStrip absent or 0: the whole image is one strip (the TIFF default 2^32-1)
+  uint32_t rps = std::min<uint32_t>(get(278, h), h);
+  if (!rps) rps = h;
   std::vector<uint32_t> offs, counts;
   if (!get_all(273, offs) || offs.empty()) return io_fail(WNDGPU_E_FORMAT, "'%s': no strip offsets", path);
   const bool have_counts = get_all(279, counts);
-  const size_t nstrips = ((size_t)h + (rps ? rps : h) - 1) / (rps ? rps : h);
+  const size_t nstrips = ((size_t)h + rps - 1) / rps;
   if (offs.size() < nstrips) return io_fail(WNDGPU_E_FORMAT, "'%s': %zu strips for %u rows", path, offs.size(), h);
   const size_t bytes_ps = bps / 8;
   const size_t line = (size_t)w * spp * bytes_ps;
