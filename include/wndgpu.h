@@ -235,6 +235,23 @@ int wndgpu_ad_wnd_backward(wndgpu_ctx *ctx, const double *train, const int *clas
                            const double *x, int n_test, const double *grad_prob, double *grad_x, void *work,
                            void *stream);
 
+/* wndgpu_ad_wnd_forward without the work argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * work lives in a file-scope device array of WNDGPU_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds from the library's load until the process ends; its
+ * address is looked up once per device by wndgpu_create.  The call needs
+ * 8 * n_test * (n_features + n_samples + n_classes + 2) bytes of it and
+ * returns WNDGPU_E_ARG above the maximum (split the test vectors).  It calls
+ * no allocating routine.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other host threads, other
+ * contexts) race, so order them.  Inputs, output and stream as for
+ * wndgpu_ad_wnd_forward. */
+#define WNDGPU_NOALLOC_SCRATCH_BYTES (32u << 20)
+int wndgpu_ad_wnd_forward_noalloc(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples,
+                                  int n_features, int n_classes, const double *weights, const double *mins,
+                                  const double *maxs, const double *x, int n_test, double *prob, void *stream);
+
 /* Batch graphs (no counterpart in the reference, whose executor runs one
  * image at a time).  mode 0 (default): every batch issues its launches.
  * mode 1: a batch whose signature (plan, image geometry, sampling, and the

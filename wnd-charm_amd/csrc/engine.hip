@@ -231,6 +231,7 @@ struct wndgpu_ctx {
   // default priority.
   hipStream_t side = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  void *scratch = nullptr;  // this device's copy of `scratch` (wndgpu_ad_wnd_forward_noalloc)
   bool serial = false;  // WNDGPU_SERIAL: every launch on `stream` (per-kernel timing without overlap)
   std::string err;
   // arena
@@ -1472,14 +1473,43 @@ extern "C" {
 
 const char *wndgpu_version(void) { return "wndgpu 0.1 (gfx950)"; }
 
+// Scratch of wndgpu_ad_wnd_forward_noalloc: a file-scope device array, so
+// that entry point needs no allocation.  Every GPU the library runs on holds
+// its own copy (WNDGPU_NOALLOC_SCRATCH_BYTES) until the process ends.  The
+// array's name is a host-side stand-in: host code uses only the device
+// address hipGetSymbolAddress returns, looked up once per device at context
+// creation with that device current.
+static __device__ __align__(256) unsigned char scratch[WNDGPU_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static hipError_t scratch_address(int device, void **p) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *q = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = q;
+  }
+  *p = g_scratch_dev[device];
+  return hipSuccess;
+}
+
 int wndgpu_create(int device, wndgpu_ctx **out) {
   if (!out) return WNDGPU_E_ARG;
   *out = nullptr;
   int n = 0;
-  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return WNDGPU_E_DEVICE;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n || device >= kMaxDevices)
+    return WNDGPU_E_DEVICE;
   wndgpu_ctx *c = new wndgpu_ctx;
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return WNDGPU_E_DEVICE;
+  }
+  if (scratch_address(device, &c->scratch) != hipSuccess) {
+    release_check(hipStreamDestroy(c->stream), "hipStreamDestroy");
     delete c;
     return WNDGPU_E_DEVICE;
   }
@@ -2384,6 +2414,31 @@ int wndgpu_ad_wnd_forward(wndgpu_ctx *c, const double *train, const int *classes
   hipStream_t s = (hipStream_t)stream;
   int r = ad_wnd_common(c, train, classes, n_samples, n_features, n_classes, weights, mins, maxs, x, n_test, work, s,
                         prob);
+  if (r) return r;
+  HIPCHK(c, hipGetLastError());
+  return WNDGPU_OK;
+}
+
+int wndgpu_ad_wnd_forward_noalloc(wndgpu_ctx *c, const double *train, const int *classes, int n_samples,
+                                  int n_features, int n_classes, const double *weights, const double *mins,
+                                  const double *maxs, const double *x, int n_test, double *prob, void *stream) {
+  if (!c) return WNDGPU_E_ARG;
+  if (!prob) return fail(c, WNDGPU_E_ARG, "null argument");
+  if (n_samples <= 0 || n_features <= 0 || n_classes <= 0 || n_test <= 0)
+    return fail(c, WNDGPU_E_ARG, "bad argument");
+  // the forward pass uses the normalised copy, distances, probabilities,
+  // norm factors and predictions of the work layout (ad_wnd_common); every
+  // element is written before it is read, so nothing is cleared
+  const double need = 8.0 * (double)n_test * ((double)n_features + (double)n_samples + (double)n_classes + 2.0);
+  if (need > (double)WNDGPU_NOALLOC_SCRATCH_BYTES)
+    return fail(c, WNDGPU_E_ARG,
+                "wndgpu_ad_wnd_forward_noalloc: %d test vectors against %d samples of %d features need %.0f bytes "
+                "of scratch, above the %u-byte maximum; split the test vectors into smaller calls",
+                n_test, n_samples, n_features, need, (unsigned)WNDGPU_NOALLOC_SCRATCH_BYTES);
+  if (!c->scratch) return fail(c, WNDGPU_E_DEVICE, "no scratch address for device %d", c->device);
+  hipStream_t s = (hipStream_t)stream;
+  int r = ad_wnd_common(c, train, classes, n_samples, n_features, n_classes, weights, mins, maxs, x, n_test,
+                        c->scratch, s, prob);
   if (r) return r;
   HIPCHK(c, hipGetLastError());
   return WNDGPU_OK;
