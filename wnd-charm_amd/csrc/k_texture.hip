@@ -443,7 +443,11 @@ __device__ __forceinline__ double hess_reflector(gptr<double> A, int n, int s, i
 
 // Column pass after step i (columns c = i+2..n-1, one thread per column;
 // threads left over split the rows above the dot region):
-//   upd  A = A H of step i on rows i+1..n-1 (the rows above are deferred):
+//   upd  A = H A and A = A H of step i on rows i+1..n-1 (the rows above are
+//        deferred).  The tile pass computed H A only in LDS, so the left
+//        update is applied here to the stored values first, with the same
+//        operations: A(r,c) -= fac(r) * twold[c] (fac = tau for row i+1,
+//        tau * ess[r-i-2] below; twold = the left dot of step i), then
 //        A(r,c) -= tu[r] * ess[c-i-2], with
 //        tu[r] = tau * (row dot), Eigen's (tau * tmp) * essential^T
 //   dot  the left dot of step i+1 on the updated column, in Eigen's ColMajor
@@ -454,7 +458,8 @@ __device__ __forceinline__ double hess_reflector(gptr<double> A, int n, int s, i
 //        leftover terms one by one; a skipped leading term last; then
 //        + A(i+2, c) (applyHouseholderOnTheLeft's tmp += row(0)).  -> tw[c]
 __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool upd, const double *ess,
-                                             const double *tu, bool dot, const double *nx, double *tw) {
+                                             const double *tu, double tau, const double *twold, bool dot,
+                                             const double *nx, double *tw) {
   const int t = threadIdx.x;
   const int mB = n - i - 2;
   if (mB <= 0) return;
@@ -463,6 +468,9 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   if (sub >= k) return;
   const int c = i + 2 + cc;
   const double f = upd ? ess[cc] : 0.0;
+  const double wl = upd ? twold[c] : 0.0;  // left dot of step i for this column
+  // left factor of row r (r >= i+1) of step i: tau * (1, ess)
+  auto lfac = [&](int r) { return r == i + 1 ? tau : tau * ess[r - i - 2]; };
   const int d0 = dot ? i + 2 : n;  // first row of the dot region
   if (upd && !(dot && k > 1 && sub == 0)) {
     const int parts = (dot && k > 1) ? k - 1 : k, ps = (dot && k > 1) ? sub - 1 : sub;
@@ -475,9 +483,9 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
 #pragma unroll
       for (int q = 0; q < HR; q++) v[q] = A[o + q * n];
 #pragma unroll
-      for (int q = 0; q < HR; q++) A[o + q * n] = v[q] - tu[r + q] * f;
+      for (int q = 0; q < HR; q++) A[o + q * n] = (v[q] - lfac(r + q) * wl) - tu[r + q] * f;
     }
-    for (; r < rb; r++, o += n) A[o] = A[o] - tu[r] * f;
+    for (; r < rb; r++, o += n) A[o] = (A[o] - lfac(r) * wl) - tu[r] * f;
   }
   if (!dot || sub != 0) return;
   const int s = i + 1;       // the step whose left dot this is
@@ -489,14 +497,14 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   const int bound = ((ms - skip) / 4) * 4 + skip;
   const unsigned ot = (unsigned)((s + 1) * n + c);
   double top = A[ot];
-  if (upd) { top = top - tu[s + 1] * f; A[ot] = top; }
+  if (upd) { top = (top - lfac(s + 1) * wl) - tu[s + 1] * f; A[ot] = top; }
   // rows s+2+j are addressed as A[o0 + j*n]: a uniform base and a 32-bit
   // offset per load (one register per address)
   const unsigned o0 = ot + n;  // row s+2: term 0
   double x0 = 0.0;
   if (skip) {
     double v = A[o0];
-    if (upd) { v = v - tu[s + 2] * f; A[o0] = v; }
+    if (upd) { v = (v - lfac(s + 2) * wl) - tu[s + 2] * f; A[o0] = v; }
     x0 = v * nx[0];
   }
   double acc = 0.0;
@@ -508,7 +516,7 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
     if (upd) {
 #pragma unroll
       for (int q = 0; q < HR; q++) {
-        v[q] = v[q] - tu[s + 2 + g + q] * f;
+        v[q] = (v[q] - lfac(s + 2 + g + q) * wl) - tu[s + 2 + g + q] * f;
         A[o0 + (g + q) * n] = v[q];
       }
     }
@@ -528,7 +536,7 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
     if (upd) {
 #pragma unroll
       for (int q = 0; q < 4; q++) {
-        v[q] = v[q] - tu[s + 2 + g + q] * f;
+        v[q] = (v[q] - lfac(s + 2 + g + q) * wl) - tu[s + 2 + g + q] * f;
         A[o0 + (g + q) * n] = v[q];
       }
     }
@@ -539,7 +547,7 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
   }
   for (int j = bound; j < ms; j++) {
     double v = A[o0 + j * n];
-    if (upd) { v = v - tu[s + 2 + j] * f; A[o0 + j * n] = v; }
+    if (upd) { v = (v - lfac(s + 2 + j) * wl) - tu[s + 2 + j] * f; A[o0 + j * n] = v; }
     acc = acc + v * nx[j];
   }
   if (skip) acc = acc + x0;
@@ -556,7 +564,9 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
 //   tile pass    columns i+1..n-1 in HT-column LDS tiles of the active rows.
 //                The loading thread applies A = H A of step i (row i+1 -=
 //                tau*tw, bottom -= (tau*ess)*tw; tw = the left dot from the
-//                previous column pass) and writes it back; the row's thread
+//                previous column pass) in LDS only (the column pass applies
+//                it again to the stored values, so the block is written once
+//                per step); the row's thread
 //                then accumulates the row dot in Eigen's RowMajor GEMV order
 //                (general_matrix_vector_product<RowMajor>,
 //                GeneralMatrixVector.h:289-552): even and odd terms in two
@@ -573,7 +583,7 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_har_hess(const HarDesc *hd) {
   __shared__ double tile[256 * HTP];
   __shared__ double essb[2][256];  // reflector of step i and of step i+1
-  __shared__ double tw[256];       // left dot of the current step, by column
+  __shared__ double twb[2][256];   // left dots by column: current step, next step
   __shared__ double tu[256];       // tau * right dot of the current step, by row
   __shared__ double xcol[256];
   __shared__ double sc[4];
@@ -587,7 +597,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   for (int k = t; k < n - 1; k += blockDim.x) xcol[k] = A[(long)(1 + k) * n];
   __syncthreads();
   double tau = hess_reflector(A, n, 0, n - 1, xcol, essb[0], sc, Rf);
-  if (n - 1 >= 2) hess_colpass(A, n, -1, false, nullptr, nullptr, true, essb[0], tw);
+  if (n - 1 >= 2) hess_colpass(A, n, -1, false, nullptr, nullptr, 0.0, nullptr, true, essb[0], twb[0]);
   __syncthreads();
   int pp = 0;
   const int lc = t % HT, lr = t / HT;
@@ -595,6 +605,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     const int rem = n - i - 1, m = rem - 1, asz = m & ~1;
     const int rb = i + 1;  // first active row; tile row k holds row rb + k
     const double *ess = essb[pp];
+    const double *tw = twb[pp];
     double rv[HT];
     auto load = [&](int ct) {
       if (ct + lc < rem) {
@@ -615,7 +626,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
         const double w = tw[c];
         int r0 = lr;
         asm volatile("" : "+v"(r0));
-        const unsigned o = (unsigned)((rb + r0) * n + c);
         double *tl0 = tile + (r0 * HTP + lc);
 #pragma unroll
         for (int q = 0; q < HT; q++) {
@@ -624,7 +634,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
           if (r < n) {
             const double fac = (r == i + 1) ? tau : tau * ess[r - i - 2];
             v = v - fac * w;
-            A[o + HS * q * n] = v;
           }
           tl0[HS * q * HTP] = v;  // rows >= n are never read
         }
@@ -662,7 +671,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     __syncthreads();
     const double tau_n = hess_reflector(A, n, i + 1, m, xcol, essb[pp ^ 1], sc, Rf);
-    hess_colpass(A, n, i, true, ess, tu, m >= 2, essb[pp ^ 1], tw);
+    hess_colpass(A, n, i, true, ess, tu, tau, tw, m >= 2, essb[pp ^ 1], twb[pp ^ 1]);
     __syncthreads();
     pp ^= 1;
     tau = tau_n;
