@@ -114,6 +114,15 @@ long orc_wnd5(const double *train, const int *cls, int n, int f, int n_classes,
               const double *weights, const double *test_normalized,
               double *probabilities, double *norm_factor);
 
+/* Classifier as the wndcharm Python package computes it
+ * (wndcharm/utils.py, FeatureWeights.py, FeatureSpacePrediction.py). */
+double orc_np_sum(const double *a, long n); /* numpy's 1-D float64 sum */
+int orc_py_normalize(double *data, int n, int f, int flags, double *mins, double *maxs, double *means,
+                     double *stdevs);
+void orc_py_fisher(const double *data, const int *cls, int n, int f, int n_classes, double *weights);
+long orc_py_wnd5(const double *train, const int *cls, int n, int f, int n_classes, const double *weights,
+                 const double *test, double *probabilities, double *norm_factor);
+
 #ifdef __cplusplus
 }
 #endif

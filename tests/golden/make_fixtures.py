@@ -96,7 +96,8 @@ def write_raw(w=os.path.join(REF, 'wndchrm_tests'), p=os.path.join(REF, 'pywndch
     for src in [os.path.join(w, 't1_s01_c05_ij-l_precalculated.sig'),
                 os.path.join(w, '010067_301x300-l_precalculated.sig'),
                 os.path.join(w, 'unbalanced.fit'), os.path.join(w, 'test-l.fit'),
-                os.path.join(w, 'test_fit-l.weights'), os.path.join(w, 't1_s01_c05_ij.tif'),
+                os.path.join(w, 'test_fit-l.weights'), os.path.join(w, 'test_fit-l-normalized.fit'),
+                os.path.join(w, 't1_s01_c05_ij.tif'),
                 os.path.join(w, 'images', 'binuc', 'binuc1.tiff'),
                 os.path.join(w, 'images', 'binuc', 'binuc2.tiff'),
                 os.path.join(w, 'images', 'binuc', 'binuc3.tiff'),

@@ -30,7 +30,8 @@ def synth(h, w, seed):
 def grays_of(img):
     h, w = img.shape
     g = np.zeros(h * w, np.uint8)
-    oracle().orc_haralick_grays(ptr(np.ascontiguousarray(img)), h, w, ptr(g))
+    src = np.ascontiguousarray(img, np.float64)  # held by name across the call
+    oracle().orc_haralick_grays(ptr(src), h, w, ptr(g))
     return g
 
 
