@@ -26,6 +26,11 @@
  *   wndgpu_normalize_test       signatures::normalize (src/wndchrm_src/signatures.cpp:196-215)
  *   wndgpu_fisher               TrainingSet::SetFisherScores (TrainingSet.cpp:2183-2324)
  *   wndgpu_wnd5                 TrainingSet::classify2 (TrainingSet.cpp:2429-2586)
+ *   wndgpu_py_normalize / wndgpu_py_fisher / wndgpu_py_wnd5
+ *                               the wndcharm Python package's normalize_by_columns
+ *                               (wndcharm/utils.py:124-240), FisherFeatureWeights.
+ *                               NewFromFeatureSpace (FeatureWeights.py:165-236) and
+ *                               NewWND5 (FeatureSpacePrediction.py:641-799)
  *   wndgpu_compute_sampled      TrainingSet::AddImageFile's sampling (TrainingSet.cpp:1425-1480):
  *                               OpenImage pre-processing, rotations, tiles
  *   wndgpu_tiff_read            ImageMatrix::LoadTIFF (src/cmatrix.cpp:85-203), the file part
@@ -181,6 +186,36 @@ int wndgpu_fisher(wndgpu_ctx *ctx, const double *data, const int *classes, int n
 int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
                 int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
                 double *norm_factor, long *predicted);
+
+/* The same classifier as the wndcharm Python package computes it (its maths
+ * differs from TrainingSet's; host arrays, FP64, row-major [n][F]):
+ *   wndgpu_py_normalize  normalize_by_columns (wndcharm/utils.py:172-240),
+ *                        with ReplaceNonReal (utils.py:124-169): flags
+ *                        WNDGPU_PY_ZSCORE (else min-max to 0..100),
+ *                        WNDGPU_PY_NON_REAL_CHECK, WNDGPU_PY_GIVEN_* (those
+ *                        arrays are reference values, read; the others of
+ *                        the chosen form are written), WNDGPU_PY_REPLACE_ONLY
+ *                        (ReplaceNonReal alone).  *replaced (may be NULL):
+ *                        whether the matrix held a NaN or +/-inf.
+ *   wndgpu_py_fisher     FisherFeatureWeights.NewFromFeatureSpace
+ *                        (wndcharm/FeatureWeights.py:165-236): ddof=1 class
+ *                        variances, no floor, NaN / inf weights -> 0; rows of
+ *                        class 0 count in the population mean only.
+ *   wndgpu_py_wnd5       FeatureSpaceClassification.NewWND5
+ *                        (wndcharm/FeatureSpacePrediction.py:641-799): weighted
+ *                        features, squared euclidean distances, distances
+ *                        <= DBL_EPSILON masked, per-class mean of d^-5; a class
+ *                        with no unmasked sample gives no prediction (NaN
+ *                        probabilities and norm factor, predicted 0). */
+enum { WNDGPU_PY_ZSCORE = 1, WNDGPU_PY_NON_REAL_CHECK = 2, WNDGPU_PY_GIVEN_MINS = 4, WNDGPU_PY_GIVEN_MAXS = 8,
+       WNDGPU_PY_GIVEN_MEANS = 16, WNDGPU_PY_GIVEN_STDEVS = 32, WNDGPU_PY_REPLACE_ONLY = 64 };
+int wndgpu_py_normalize(wndgpu_ctx *ctx, double *data, int n_samples, int n_features, int flags, double *mins,
+                        double *maxs, double *means, double *stdevs, int *replaced);
+int wndgpu_py_fisher(wndgpu_ctx *ctx, const double *data, const int *classes, int n_samples, int n_features,
+                     int n_classes, double *weights);
+int wndgpu_py_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
+                   int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
+                   double *norm_factor, long *predicted);
 
 /* ---- Differentiable form: forward and adjoint (reverse-mode) passes of the
  * smooth transforms and features, and of the WND similarity, for gradients

@@ -34,6 +34,12 @@ namespace wnd {
 void launch_norm_train(double *data, int n, int f, double *mins, double *maxs, hipStream_t s);
 void launch_norm_test(double *vec, int nvec, int f, const double *mins, const double *maxs, hipStream_t s);
 void launch_fisher(const double *data, const int *cls, int n, int f, int C, double used, double *w, hipStream_t s);
+void launch_py_normalize(double *M, int n, int f, int flags, double *mins, double *maxs, double *means,
+                         double *stdevs, int *flag, hipStream_t s);
+void launch_py_fisher(const double *data, const int *cls, int n, int f, int C, double *w, hipStream_t s);
+void launch_py_wnd5(const double *train, const double *w, const double *test, const int *rows, const int *start,
+                    int n, int f, int C, int ntest, double *dist, double *sims, double *cmean, double *prob,
+                    double *norm, long *pred, hipStream_t s);
 void launch_wnd5(const double *train, const int *cls, const double *w, const double *test, int n, int f, int C,
                  int ntest, double *dist, double *prob, double *norm, long *pred, hipStream_t s);
 }  // namespace wnd
@@ -2093,6 +2099,106 @@ int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f
   HIPCHK(c, hipMemcpy(te.p, test, (size_t)ntest * f * 8, hipMemcpyHostToDevice));
   launch_wnd5((const double *)tr.p, (const int *)k.p, (const double *)wd.p, (const double *)te.p, n, f, C, ntest,
               (double *)dist.p, (double *)pr.p, (double *)nm.p, (long *)pd.p, c->stream);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(prob, pr.p, (size_t)ntest * C * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(norm, nm.p, (size_t)ntest * 8, hipMemcpyDeviceToHost));
+  HIPCHK(c, hipMemcpy(pred, pd.p, (size_t)ntest * sizeof(long), hipMemcpyDeviceToHost));
+  return WNDGPU_OK;
+}
+
+// ---- the wndcharm Python package's classifier maths
+
+int wndgpu_py_normalize(wndgpu_ctx *c, double *data, int n, int f, int flags, double *mins, double *maxs,
+                        double *means, double *stdevs, int *replaced) {
+  if (!c) return WNDGPU_E_ARG;
+  if (!data || n <= 0 || f <= 0 || (flags & ~127)) return fail(c, WNDGPU_E_ARG, "bad argument");
+  const bool zs = flags & WNDGPU_PY_ZSCORE, ronly = flags & WNDGPU_PY_REPLACE_ONLY;
+  // arrays read (given) or written (computed) by the chosen form
+  if (((flags & WNDGPU_PY_GIVEN_MINS) || (!zs && !ronly)) && !mins) return fail(c, WNDGPU_E_ARG, "mins is NULL");
+  if (((flags & WNDGPU_PY_GIVEN_MAXS) || (!zs && !ronly)) && !maxs) return fail(c, WNDGPU_E_ARG, "maxs is NULL");
+  if (zs && !ronly && (!means || !stdevs)) return fail(c, WNDGPU_E_ARG, "means / stdevs are NULL");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t fb = (size_t)f * 8;
+  DBuf d, a0, a1, a2, a3, fl;
+  DALLOC(c, d, (size_t)n * fb);
+  DALLOC(c, a0, fb);
+  DALLOC(c, a1, fb);
+  DALLOC(c, a2, fb);
+  DALLOC(c, a3, fb);
+  DALLOC(c, fl, sizeof(int));
+  HIPCHK(c, hipMemcpy(d.p, data, (size_t)n * fb, hipMemcpyHostToDevice));
+  double *host[4] = {mins, maxs, means, stdevs};
+  void *dev[4] = {a0.p, a1.p, a2.p, a3.p};
+  for (int k = 0; k < 4; k++)
+    if (host[k]) HIPCHK(c, hipMemcpy(dev[k], host[k], fb, hipMemcpyHostToDevice));
+  launch_py_normalize((double *)d.p, n, f, flags, (double *)a0.p, (double *)a1.p, (double *)a2.p, (double *)a3.p,
+                      (int *)fl.p, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(data, d.p, (size_t)n * fb, hipMemcpyDeviceToHost));
+  for (int k = 0; k < 4; k++)
+    if (host[k]) HIPCHK(c, hipMemcpy(host[k], dev[k], fb, hipMemcpyDeviceToHost));
+  int any = 0;
+  HIPCHK(c, hipMemcpy(&any, fl.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (replaced) *replaced = any;
+  return WNDGPU_OK;
+}
+
+int wndgpu_py_fisher(wndgpu_ctx *c, const double *data, const int *cls, int n, int f, int C, double *w) {
+  if (!c) return WNDGPU_E_ARG;
+  if (!data || !cls || !w || n <= 0 || f <= 0 || C <= 0) return fail(c, WNDGPU_E_ARG, "bad argument");
+  if (int r = check_labels(c, cls, n, C)) return r;
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf d, k, wd;
+  DALLOC(c, d, (size_t)n * f * 8);
+  DALLOC(c, k, (size_t)n * 4);
+  DALLOC(c, wd, (size_t)f * 8);
+  HIPCHK(c, hipMemcpy(d.p, data, (size_t)n * f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(k.p, cls, (size_t)n * 4, hipMemcpyHostToDevice));
+  launch_py_fisher((const double *)d.p, (const int *)k.p, n, f, C, (double *)wd.p, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(w, wd.p, (size_t)f * 8, hipMemcpyDeviceToHost));
+  return WNDGPU_OK;
+}
+
+int wndgpu_py_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f, int C, const double *w,
+                   const double *test, int ntest, double *prob, double *norm, long *pred) {
+  if (!c) return WNDGPU_E_ARG;
+  if (!train || !cls || !w || !test || !prob || !norm || !pred || n <= 0 || f <= 0 || C <= 0 || ntest <= 0)
+    return fail(c, WNDGPU_E_ARG, "bad argument");
+  if (int r = check_labels(c, cls, n, C)) return r;
+  // the class views of the training set: its rows grouped by class 1..C, in order
+  std::vector<int> rows, start(C + 1, 0);
+  for (int cl = 1; cl <= C; cl++) {
+    start[cl - 1] = (int)rows.size();
+    for (int i = 0; i < n; i++)
+      if (cls[i] == cl) rows.push_back(i);
+  }
+  start[C] = (int)rows.size();
+  if (rows.empty()) return fail(c, WNDGPU_E_ARG, "no training sample has a class in 1..%d", C);
+  HIPCHK(c, hipSetDevice(c->device));
+  DBuf tr, wd, te, rw, st, dist, sims, cm, pr, nm, pd;
+  DALLOC(c, tr, (size_t)n * f * 8);
+  DALLOC(c, wd, (size_t)f * 8);
+  DALLOC(c, te, (size_t)ntest * f * 8);
+  DALLOC(c, rw, rows.size() * 4);
+  DALLOC(c, st, (size_t)(C + 1) * 4);
+  DALLOC(c, dist, (size_t)ntest * n * 8);
+  DALLOC(c, sims, (size_t)ntest * n * 8);
+  DALLOC(c, cm, (size_t)ntest * C * 8);
+  DALLOC(c, pr, (size_t)ntest * C * 8);
+  DALLOC(c, nm, (size_t)ntest * 8);
+  DALLOC(c, pd, (size_t)ntest * sizeof(long));
+  HIPCHK(c, hipMemcpy(tr.p, train, (size_t)n * f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(wd.p, w, (size_t)f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(te.p, test, (size_t)ntest * f * 8, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(rw.p, rows.data(), rows.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(c, hipMemcpy(st.p, start.data(), (size_t)(C + 1) * 4, hipMemcpyHostToDevice));
+  launch_py_wnd5((const double *)tr.p, (const double *)wd.p, (const double *)te.p, (const int *)rw.p,
+                 (const int *)st.p, n, f, C, ntest, (double *)dist.p, (double *)sims.p, (double *)cm.p,
+                 (double *)pr.p, (double *)nm.p, (long *)pd.p, c->stream);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(prob, pr.p, (size_t)ntest * C * 8, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(norm, nm.p, (size_t)ntest * 8, hipMemcpyDeviceToHost));

@@ -74,6 +74,9 @@ def _load():
         'wndgpu_normalize_test': (_I, [_P, _P, _I, _P, _P]),
         'wndgpu_fisher': (_I, [_P, _P, _P, _I, _I, _I, _D, _P]),
         'wndgpu_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
+        'wndgpu_py_normalize': (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P, _P]),
+        'wndgpu_py_fisher': (_I, [_P, _P, _P, _I, _I, _I, _P]),
+        'wndgpu_py_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
         'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
         'wndgpu_last_haralick_work': (_I, [_P, _P, _P, _I]),
         'wndgpu_sampling_n_samples': (_I, [_P]),
