@@ -7,17 +7,21 @@ step = one pass of the DAG over one batch of images already resident in
 HBM (wndgpu_compute_device); value = images/s over the timed steps.
 
 Also reported:
-  roofline      the dominant kernel (largest HIP-event time per step,
-                events recorded around every launch on the stream that
-                launch runs on) against MI355X's FP64 peak, per launch
-                (k_har_hess runs as two launches per step, one per half of
-                the Haralick matrices): k_har_hess does (exactly)
+  roofline      the dominant kernel -- the largest per-step time, among the
+                kernels with a work model (KERNEL_MODELS), in a serial pass
+                (every launch on one stream) after the timed steps -- with its
+                launch time measured live in the timed steps (HIP events
+                recorded around every launch on the stream that launch runs
+                on), against MI355X's FP64 peak: k_har_hess does (exactly)
                 hess_flops(Ng) per Q matrix, k_har_qr reports the flops its
-                Francis sweeps issued; traffic (HBM bytes per launch) is read
-                from the rocprofv3 --pmc summary committed under profiles/
-                when present.
+                Francis sweeps issued, k_gabor_conv 8 x 1444 complex taps per
+                pixel; traffic (HBM bytes per launch) is read from the
+                rocprofv3 --pmc summary committed under profiles/ when present.
+  stage_ms_serial  per-launch times of the serial pass (the kernels' own
+                times, comparable with rocprofv3 under WNDGPU_SERIAL=1).
   cpu_baseline  the FP64 C++ oracle, one image per thread on the host
-                cores, on a bounded sample of the same images.
+                cores (threads of this process, at most 16: one GPU's share
+                of the host), on a bounded sample of the same images.
 
 forms         the metric's comparison: the same batch with the basis
                 contractions in BF16x3 (batched-mixed; with how many
@@ -96,11 +100,23 @@ def exact_columns(plan):
     return m
 
 
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
 def cpu_baseline(imgs, seconds=12.0):
-    """Oracle (CPU restatement) throughput on host cores; a bounded sample."""
+    """Oracle (CPU restatement) throughput on host cores; a bounded sample.
+    Threads of this process (the oracle releases the GIL in ctypes calls),
+    as many as the host has up to 16, the CPU share of one GPU."""
     lib = ctypes.CDLL(os.path.join(ROOT, 'oracle', 'liboracle.so'))
     lib.orc_features.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
-    cores = min(os.cpu_count() or 1, 16)
+    cores = min(len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else (os.cpu_count() or 1), 16)
     done = [0]
     lock = threading.Lock()
     stop = time.time() + seconds
@@ -124,7 +140,11 @@ def cpu_baseline(imgs, seconds=12.0):
         t.join()
     el = time.time() - t0
     return {'value': round(done[0] / el, 3), 'unit': 'feature vectors/s', 'cores': cores, 'kind': 'port',
-            'sample': '%d long-set vectors of 256x256 images in %.1f s, one image per thread' % (done[0], el)}
+            'cpu_model': cpu_model(), 'host_cpus': os.cpu_count(),
+            'sample': '%d long-set vectors of 256x256 images in %.1f s, one image per thread, %d threads, '
+                      'g++ -O2 -ffp-contract=off' % (done[0], el, cores),
+            'note': 'the reference build itself needs FFTW3 (absent here), so no restatement/reference '
+                    'time ratio is measured'}
 
 
 def profiled_traffic(kernel):
@@ -135,6 +155,21 @@ def profiled_traffic(kernel):
         return None
     m = re.search(r'traffic_bytes_per_launch:\s*([0-9.eE+]+)', open(path).read())
     return float(m.group(1)) if m else None
+
+
+def _gabor_flops(sizes):
+    """k_gabor_conv: 8 filters x 38 x 38 complex taps x 4 flops per root pixel"""
+    return float(8 * 1444 * 4 * sum(h * w for h, w in sizes))
+
+
+# launch name (as wndgpu_last_node_times reports it) -> (kernel, roof, work per step)
+# work(ctx) gets {'tones': Ng per Haralick entry, 'schur': Schur flops per matrix, 'root_sizes': [(h, w)]}
+KERNEL_MODELS = {
+    'Haralick Textures [k_har_qr]': ('k_har_qr', 'fp64-valu', lambda w: float(w['schur'].sum())),
+    'Haralick Textures [k_har_hess]': ('k_har_hess', 'fp64-valu',
+                                       lambda w: float(4 * sum(hess_flops(int(n)) for n in w['tones']))),
+    'Gabor Textures': ('k_gabor_conv', 'fp64-valu', lambda w: _gabor_flops(w['root_sizes'])),
+}
 
 
 MIXED_GROUPS = [('Chebyshev Coefficients ()', 32), ('Zernike Coefficients ()', 72),
@@ -556,19 +591,49 @@ def main():
     value = len(imgs) * args.steps / el
     mean = {k: v / args.steps for k, v in acc.items()}
 
-    # dominant kernel (largest time per step) and its FP64 roofline, per launch
-    kern = {k: v for k, v in mean.items() if '[k_' in k}
-    dom = max(kern, key=kern.get)
-    kname = re.search(r'\[(k_\w+)\]', dom).group(1)
-    per_step = cnt[dom] // args.steps  # launches of that kernel per step
     tones, schur = wndcharm.last_haralick_work()
-    flops = float(4 * sum(hess_flops(int(n)) for n in tones)) if kname == 'k_har_hess' else float(schur.sum())
-    launch_ms = acc[dom] / cnt[dom]
+    work = {'tones': tones, 'schur': schur, 'root_sizes': [im.shape for im in imgs]}
+    # serial pass: a second context with every launch on one stream, per-launch
+    # event times = the kernels' own times (no overlap with the side stream)
+    os.environ['WNDGPU_SERIAL'] = '1'
+    ctx_serial = ctypes.c_void_p()
+    rc = lib.wndgpu_create(0, ctypes.byref(ctx_serial))
+    os.environ.pop('WNDGPU_SERIAL')
+    if rc != 0:
+        raise RuntimeError('wndgpu_create (serial) failed')
+    serial = {}
+    for _ in range(2):  # the second pass is the one kept
+        rc = lib.wndgpu_compute_device(ctx_serial, plan.handle, structs, len(imgs), ctypes.c_void_p(out.data_ptr()),
+                                       ctypes.c_void_p(status.ctypes.data))
+        if rc != 0:
+            raise RuntimeError(lib.wndgpu_last_error(ctx_serial).decode())
+        ms_arr = (ctypes.c_double * 4096)()
+        nm_arr = (ctypes.c_char_p * 4096)()
+        nt = lib.wndgpu_last_node_times(ctx_serial, ms_arr, nm_arr, 4096)
+        serial = {}
+        for k in range(nt):
+            nm = nm_arr[k].decode()
+            serial[nm] = serial.get(nm, 0.0) + ms_arr[k]
+    lib.wndgpu_destroy(ctx_serial)
+
+    # dominant kernel: largest serial time per step among the modelled launches
+    dom = max((k for k in serial if k in KERNEL_MODELS), key=lambda k: serial[k])
+    kname, bound, model = KERNEL_MODELS[dom]
+    per_step = cnt[dom] // args.steps  # launches of that kernel per step
+    flops = model(work)
+    launch_ms = acc[dom] / cnt[dom]  # timed steps (two-stream schedule)
     achieved = flops / per_step / (launch_ms * 1e-3) / 1e12
-    roof = {'kernel': kname, 'bound': 'mfma', 'achieved': round(achieved, 4), 'peak': FP64_PEAK_TFLOPS,
+    roof = {'kernel': kname, 'bound': bound, 'achieved': round(achieved, 4), 'peak': FP64_PEAK_TFLOPS,
             'unit': 'TFLOP/s', 'frac': round(achieved / FP64_PEAK_TFLOPS, 5), 'traffic': profiled_traffic(kname),
-            'launch_ms': round(launch_ms, 3), 'launches_per_step': per_step, 'flops_per_launch': flops / per_step,
-            'matrices_per_launch': 4 * len(tones) / per_step}
+            'launch_ms': round(launch_ms, 3), 'launch_ms_serial': round(serial[dom] / per_step, 3),
+            'launches_per_step': per_step, 'flops_per_launch': flops / per_step,
+            'note': 'FP64 vector peak (equal to the FP64 matrix peak on MI355X); launch_ms from HIP events in '
+                    'the timed steps, launch_ms_serial from the serial pass'}
+    if kname in ('k_har_qr', 'k_har_hess'):
+        roof['matrices_per_launch'] = 4 * len(tones) / per_step
+    biggest = max(serial, key=serial.get)
+    if biggest != dom:
+        roof['largest_unmodelled_launch'] = {'name': biggest, 'serial_ms': round(serial[biggest], 3)}
     # the metric's comparison: the batched form with BF16x3 basis contractions,
     # and the plain per-image form (one call per image) on the first 40 images
     forms = {'batched_fp64': round(value, 3)}
@@ -629,11 +694,7 @@ def main():
     torch.cuda.synchronize()
     forms['plain_per_image_fp64'] = round(npi / (time.perf_counter() - t0), 3)
     forms['batched_bf16x3_vs_plain_per_image'] = round(forms['batched_bf16x3'] / forms['plain_per_image_fp64'], 2)
-    stage = {}
-    for k, v in mean.items():
-        key = k if '[' in k else k.split(' (')[0]
-        stage[key] = stage.get(key, 0.0) + v
-    top = sorted(stage.items(), key=lambda kv: -kv[1])
+    top = sorted(serial.items(), key=lambda kv: -kv[1])
     line = {
         'metric': 'feature vectors/sec (2919-feat)', 'value': round(value, 3), 'unit': 'feature vectors/s',
         'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': round(ms_step, 3),
@@ -643,7 +704,8 @@ def main():
                    'images_per_step': len(imgs), 'set': 'long', 'precision': 'fp64'},
         'roofline': roof,
         'forms': forms,
-        'stage_ms': {k: round(v, 3) for k, v in top[:14]},
+        'stage_ms_serial': {k: round(v, 3) for k, v in top[:14]},
+        'serial_pass_ms': round(sum(serial.values()), 3),
     }
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(imgs[:64])
