@@ -103,17 +103,19 @@ int wndgpu_plan_from_groups(const char *const *group_names, int n_groups, wndgpu
 void wndgpu_plan_destroy(wndgpu_plan *plan);
 int wndgpu_plan_n_features(const wndgpu_plan *plan);
 const char *wndgpu_plan_feature_name(const wndgpu_plan *plan, int column);
-/* Arithmetic of the basis contractions of "Chebyshev Coefficients",
- * "Zernike Coefficients", "Chebyshev-Fourier Coefficients" and "Gabor
- * Textures" (the mixed-precision form; no counterpart in the reference,
- * which computes in FP64): WNDGPU_FP64 (default, the plain form),
- * WNDGPU_FP32, or WNDGPU_BF16X3.  BF16X3 splits each operand of the
- * GEMM-shaped contractions (Chebyshev, Gabor) into bf16 hi + lo and takes
- * hi*hi + hi*lo + lo*hi on the bf16 matrix cores with FP32 accumulation;
- * Zernike and Chebyshev-Fourier accumulate in FP32 in both reduced forms
- * (Zernike's radial polynomials stay FP64).  Histogram and Otsu
- * post-processing stays FP64; its counts can move where a value lands on a
- * bin edge.  Set the precision on a plan of your own
+/* Arithmetic of the basis contractions (the mixed-precision form; no
+ * counterpart in the reference, which computes in FP64): WNDGPU_FP64
+ * (default, the plain form), WNDGPU_FP32 or WNDGPU_BF16X3.  The reduced
+ * forms run the Gabor filter bank ("Gabor Textures") in FP32 fused
+ * multiply-adds over groups of 8 taps, or in bf16 split products on the
+ * matrix cores (pixels split exactly into two bf16 parts, weights into
+ * three, five products per tap, FP32 chunks of 80 products), the chunks
+ * summed in FP64.  Each energy carries an a-priori error bound, and every
+ * count that its interval leaves open (maximum, minimum, Otsu bin,
+ * threshold, base count) is taken on the energy recomputed in FP64 in the
+ * plain form's order: integer-valued features equal the plain form's.  The
+ * Chebyshev and Chebyshev-Fourier histograms and the Zernike sums stay FP64
+ * in every precision.  Set the precision on a plan of your own
  * (wndgpu_plan_from_groups): the standard plans are meant to stay plain. */
 enum { WNDGPU_FP64 = 0, WNDGPU_FP32 = 1, WNDGPU_BF16X3 = 2 };
 int wndgpu_plan_set_precision(wndgpu_plan *plan, int precision);

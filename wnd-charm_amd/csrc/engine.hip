@@ -263,7 +263,9 @@ struct wndgpu_ctx {
   std::set<std::pair<int, int>> need_cheb, need_cf;
   long batch_id = 0;
   size_t table_budget = (size_t)4 << 30;
-  DevTable gabor_f32, gabor_bhi, gabor_blo;  // Gabor kernels for the reduced-precision forms
+  // Gabor kernels for the reduced-precision forms: FP32, bf16 hi / mid / lo
+  // planes, and per filter the sum of |re| + |im| of the taps (error bounds)
+  DevTable gabor_f32, gabor_bhi, gabor_bmd, gabor_blo, gabor_wsum;
   // Batch graphs (wndgpu_set_graph_mode): the launch sequence of a batch
   // whose signature (plan, geometry, arena and table state) was seen before
   // is captured into a hipGraph and replayed; mode 2 also times the two
@@ -1011,15 +1013,12 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         }
         GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
         Img *vd = A.stage(vals);
-        const int prec = plan->precision;
+        // FP64 in every precision: the order-20 contraction streams the plane
+        // once at ~5 flops per byte (HBM-bound), so reduced precision buys no
+        // time, and its 32-bin histogram stays exact
         push(gn, [=]() {
-          if (prec) {
-            launch_gemm_mixed(prec, d1, ne, maxH, maxN, s);
-            launch_gemm_mixed(prec, d2, ne, maxN, maxN, s);
-          } else {
-            launch_cheb_gemm(d1, ne, maxH, maxN, s);
-            launch_cheb_gemm(d2, ne, maxN, maxN, s);
-          }
+          launch_cheb_gemm(d1, ne, maxH, maxN, s);
+          launch_cheb_gemm(d2, ne, maxN, maxN, s);
           launch_hist_values(vd, 32, o, ne, s);
         });
       } break;
@@ -1103,10 +1102,12 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         CfDesc *cdd = A.stage(cd);
         DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
         const int nds = (int)ds.size();
-        const int prec = plan->precision;
+        // FP64 in every precision: an a-priori bound on FP32 sums over the
+        // disk leaves most of the 529 magnitudes' histogram bins open, so an
+        // exact histogram would recompute nearly every entry in FP64
         push(gn, [=]() {
           if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
-          launch_cf(grpd, ngrp, cdd, ne, o, s, prec);
+          launch_cf(grpd, ngrp, cdd, ne, o, s, 0);
         });
       } break;
       case A_EDGE: {
@@ -1156,14 +1157,22 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         push(gn, [=]() { gather(); launch_objects(ni, nst, inv, odd, o, ne, (int)maxp, s); });
       } break;
       case A_GABOR: {
-        std::vector<double *> en(ne);
-        for (int k = 0; k < ne; k++) en[k] = A.dalloc_n<double>(8 * (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i]);
-        double **end_ = A.stage(en);
-        const double *gk = (const double *)c->gabor.p;
         const int prec = plan->precision;
-        const void *k0 = prec == 1 ? c->gabor_f32.p : c->gabor_bhi.p, *k1 = c->gabor_blo.p;
+        std::vector<double *> en(ne);
+        std::vector<float *> bd(prec ? ne : 0);
+        for (int k = 0; k < ne; k++) {
+          const size_t px = (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i];
+          en[k] = A.dalloc_n<double>(8 * px);
+          if (prec) bd[k] = A.dalloc_n<float>(8 * px);  // error bound of each reduced-precision energy
+        }
+        double **end_ = A.stage(en);
+        float **bdd = prec ? A.stage(bd) : nullptr;
+        double *gprm = prec ? A.dalloc_n<double>(32 * (size_t)ne) : nullptr;  // per entry: max, min, base count
+        const double *gk = (const double *)c->gabor.p;
+        const void *k0 = prec == 1 ? c->gabor_f32.p : c->gabor_bhi.p, *k1 = c->gabor_bmd.p, *k2 = c->gabor_blo.p;
+        const double *wsum = (const double *)c->gabor_wsum.p;
         push(gn, [=]() {
-          if (prec) launch_gabor_mixed(prec, ni, k0, k1, end_, o, ne, mh, mw, s);
+          if (prec) launch_gabor_mixed(prec, ni, k0, k1, k2, wsum, gk, end_, bdd, gprm, o, ne, mh, mw, s);
           else launch_gabor(ni, gk, end_, o, ne, mh, mw, s);
         });
       } break;
@@ -1232,31 +1241,48 @@ static unsigned short host_bf16(float f) {
 // Gabor kernels for the reduced-precision forms: FP32 (re, im) as the FP64
 // table; BF16x3 as hi and lo planes [j][i-block of 16][n = 2 f + part][16],
 // taps 38..47 zero
+static float bf16_value(unsigned short h) {
+  const unsigned b = (unsigned)h << 16;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
 static int ensure_mixed_tables(wndgpu_ctx *c) {
   if (c->gabor_f32.p) return 0;
   const std::vector<double> g = gabor_kernels();
   const int N = 38;
   std::vector<float> f32(g.begin(), g.end());
-  std::vector<unsigned short> hi((size_t)N * 3 * 16 * 16, 0), lo(hi.size(), 0);
+  // w = hi + mid + lo to 2^-24 |w|, each part the bf16 nearest to the rest
+  std::vector<unsigned short> hi((size_t)N * 3 * 16 * 16, 0), md(hi.size(), 0), lo(hi.size(), 0);
   for (int j = 0; j < N; j++)
     for (int ib = 0; ib < 3; ib++)
       for (int n = 0; n < 16; n++)
         for (int k = 0; k < 16; k++) {
           const int i = 16 * ib + k;
           if (i >= N) continue;
-          const float w = (float)g[(size_t)(n >> 1) * N * N * 2 + 2 * ((size_t)j * N + i) + (n & 1)];
-          const unsigned short h = host_bf16(w);
-          const unsigned hb = (unsigned)h << 16;
-          float hf;
-          memcpy(&hf, &hb, 4);
+          const double w = g[(size_t)(n >> 1) * N * N * 2 + 2 * ((size_t)j * N + i) + (n & 1)];
+          const unsigned short h = host_bf16((float)w);
+          const double r1 = w - (double)bf16_value(h);
+          const unsigned short m = host_bf16((float)r1);
+          const double r2 = r1 - (double)bf16_value(m);
           const size_t o = (((size_t)(j * 3 + ib) * 16 + n) * 16) + k;
           hi[o] = h;
-          lo[o] = host_bf16(w - hf);
+          md[o] = m;
+          lo[o] = host_bf16((float)r2);
         }
+  std::vector<double> wsum(8, 0.0);
+  for (int f = 0; f < 8; f++) {
+    double sm = 0.0;
+    for (int e = 0; e < 2 * N * N; e++) sm += fabs(g[(size_t)f * N * N * 2 + e]);
+    wsum[f] = sm * (1.0 + 1e-12);
+  }
   int r;
   if ((r = upload(c, c->gabor_f32, f32.data(), f32.size() * sizeof(float)))) return r;
   if ((r = upload(c, c->gabor_bhi, hi.data(), hi.size() * 2))) return r;
+  if ((r = upload(c, c->gabor_bmd, md.data(), md.size() * 2))) return r;
   if ((r = upload(c, c->gabor_blo, lo.data(), lo.size() * 2))) return r;
+  if ((r = upload(c, c->gabor_wsum, wsum.data(), wsum.size() * sizeof(double)))) return r;
   return 0;
 }
 
@@ -1555,7 +1581,7 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   for (auto &kv : c->dspos) fr(kv.second);
   for (auto &kv : c->comb) fr(kv.second);
   fr(c->gabor); fr(c->zern); fr(c->rtrig); fr(c->color_lut);
-  fr(c->gabor_f32); fr(c->gabor_bhi); fr(c->gabor_blo);
+  fr(c->gabor_f32); fr(c->gabor_bhi); fr(c->gabor_bmd); fr(c->gabor_blo); fr(c->gabor_wsum);
   for (hipEvent_t e : c->evpool) release_check(hipEventDestroy(e), "hipEventDestroy");
   if (c->dev) release_check(hipFree(c->dev), "hipFree");
   if (c->hdesc) release_check(hipFree(c->hdesc), "hipFree");

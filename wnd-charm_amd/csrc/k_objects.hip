@@ -11,15 +11,9 @@ namespace wnd {
 // Otsu threshold of a plane (ImageMatrix::Otsu(dynamic), cmatrix.cpp:1520-1570):
 // 1024-bin histogram over [mn, mx], then the reference's sequential
 // omega / myu / sigma recurrence in one thread.  hist must hold 1024 ints.
-template <typename ValF>
-__device__ double block_otsu(long n, double mn, double mx, ValF val, int *hist) {
-  for (int b = threadIdx.x; b < 1024; b += blockDim.x) hist[b] = 0;
-  __syncthreads();
-  const double sc = hist_scale(mn, mx, 1024);
-  for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[hist_bin(val(i), mn, sc, 1024)], 1);
-  __syncthreads();
-  __shared__ double thr_s;
-  if (threadIdx.x == 0) {
+// The threshold of a filled 1024-bin histogram (one thread).
+__device__ double otsu_threshold(const int *hist, long n, double mn, double mx) {
+  {
     const double wh = (double)(unsigned)n;
     // myu[L-1] first (the recurrence is replayed below with identical operations)
     double my_last = 0.0;
@@ -39,8 +33,19 @@ __device__ double block_otsu(long n, double mn, double mx, ValF val, int *hist) 
       } else sg = 0.0;
       if (sg > maxs) { maxs = sg; thr = i; }
     }
-    thr_s = ((((double)thr / (double)(1023)) * (mx - mn)) + mn);
+    return ((((double)thr / (double)(1023)) * (mx - mn)) + mn);
   }
+}
+
+template <typename ValF>
+__device__ double block_otsu(long n, double mn, double mx, ValF val, int *hist) {
+  for (int b = threadIdx.x; b < 1024; b += blockDim.x) hist[b] = 0;
+  __syncthreads();
+  const double sc = hist_scale(mn, mx, 1024);
+  for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[hist_bin(val(i), mn, sc, 1024)], 1);
+  __syncthreads();
+  __shared__ double thr_s;
+  if (threadIdx.x == 0) thr_s = otsu_threshold(hist, n, mn, mx);
   __syncthreads();
   return thr_s;
 }
@@ -440,6 +445,212 @@ __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *co
     for (long i = threadIdx.x; i < n; i += 1024) c += (E[i] / mx) > thr;
     c = block_reduce(c, SumOp(), shl);
     if (threadIdx.x == 0) out[f - 1] = (double)c / (double)orig;
+  }
+}
+
+// The Gabor ratios from reduced-precision energies E with bounds D
+// (|E - E64| <= D, E64 = k_gabor_conv's value), equal to k_gabor_post on
+// the FP64 energies.  Every decision whose interval [E - D, E + D] leaves
+// it open is taken on E64: the pixels concerned are marked (D < 0), and
+// k_gabor_repair recomputes them from an LDS tile in k_gabor_conv's tap
+// order (E = E64, D = 0).  Five per-image phases of k_gabor_post_exact,
+// each followed by a repair except the last:
+//   1  mark the candidates for the maximum (upper end >= largest lower end)
+//   2  max M; filter 0: mark the pixels open at 0.4 M; other filters: mark
+//      the candidates for the minimum of e / M
+//   3  filter 0: the base count; others: min a and max b of e / M, mark the
+//      pixels whose Otsu bin is open
+//   4  Otsu histogram and threshold; mark the pixels open at the threshold
+//   5  the counts above the threshold, the ratios
+// Per image and filter, params holds M, a, b, the threshold (and the base
+// count in filter 0's slot).
+constexpr int GP_STRIDE = 4;
+
+// the marked pixels (D < 0) of one 32 x 64 tile of one filter's map: the
+// input window staged in LDS as k_gabor_conv does, each marked output then
+// summed in k_gabor_conv's tap order by one lane
+__global__ void __launch_bounds__(256) k_gabor_repair(const Img *imgs, const double *kernels, double *const *energy,
+                                                      float *const *bound) {
+  __shared__ double tile[GB_TR + GB_N - 1][GB_LD];
+  __shared__ double2 wk[GB_N * GB_N];
+  __shared__ int list[GB_TR * GB_TC];
+  __shared__ int nlist;
+  const Img im = imgs[blockIdx.z];
+  const int f = blockIdx.y;
+  const int tilesx = (im.w + GB_TC - 1) / GB_TC, tilesy = (im.h + GB_TR - 1) / GB_TR;
+  if ((int)blockIdx.x >= tilesx * tilesy) return;
+  const int b0 = (blockIdx.x / tilesx) * GB_TR, a0 = (blockIdx.x % tilesx) * GB_TC;
+  const int t = threadIdx.x;
+  const long plane = (long)im.h * im.w;
+  double *E = energy[blockIdx.z] + f * plane;
+  float *D = bound[blockIdx.z] + f * plane;
+  if (t == 0) nlist = 0;
+  __syncthreads();
+  for (int e = t; e < GB_TR * GB_TC; e += 256) {
+    const int r = e / GB_TC, c = e - r * GB_TC;
+    const int y = b0 + r, x = a0 + c;
+    if (y < im.h && x < im.w && D[(long)y * im.w + x] < 0.0f) list[atomicAdd(&nlist, 1)] = e;
+  }
+  __syncthreads();
+  const int nl = nlist;
+  if (nl == 0) return;
+  const double *K = kernels + (long)f * GB_N * GB_N * 2;
+  for (int e = t; e < GB_N * GB_N; e += 256) wk[e] = make_double2(K[2 * e], K[2 * e + 1]);
+  for (int r = t / 64; r < GB_TR + GB_N - 1; r += 4) {
+    const int y = b0 - 18 + r;
+    for (int c = t % 64; c < GB_TC + GB_N - 1; c += 64) {
+      const int x = a0 - 18 + c;
+      tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+    }
+  }
+  __syncthreads();
+  for (int k = t; k < nl; k += 256) {
+    const int e = list[k];
+    const int r = e / GB_TC, c = e - r * GB_TC;
+    // output (b0 + r, a0 + c) reads tile row r + 37 - j, column c + 37 - i
+    double re = 0.0, imv = 0.0;
+    for (int j = 0; j < GB_N; j++) {
+      const double *row = &tile[r + 37 - j][c + 37];
+      const double2 *wr = &wk[j * GB_N];
+#pragma unroll
+      for (int i = 0; i < GB_N; i++) {
+        const double v = row[-i];
+        const double2 w = wr[i];
+        re += v * w.x;
+        imv += v * w.y;
+      }
+    }
+    const long o = (long)(b0 + r) * im.w + a0 + c;
+    E[o] = sqrt(re * re + imv * imv);
+    D[o] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img *imgs, double *const *energy,
+                                                           float *const *bound, double *params, OutRef o) {
+  __shared__ int hist[1024];
+  __shared__ double shd[16];
+  __shared__ long long shl[16];
+  __shared__ double thr_s;
+  const Img im = imgs[blockIdx.x];
+  const long n = (long)im.h * im.w;
+  double *const E0 = energy[blockIdx.x];
+  float *const D0 = bound[blockIdx.x];
+  double *prm = params + (long)blockIdx.x * 8 * GP_STRIDE;
+  for (int f = (phase >= 4 ? 1 : 0); f < 8; f++) {
+    const double *E = E0 + (long)f * n;
+    float *D = D0 + (long)f * n;
+    double *P = prm + f * GP_STRIDE;
+    auto mark = [&](long i) { D[i] = -D[i]; };
+    if (phase == 1) {
+      double lo = -1e308;
+      for (long i = threadIdx.x; i < n; i += 1024) lo = fmax(lo, E[i] - (double)D[i]);
+      lo = block_reduce(lo, MaxOp(), shd);
+      for (long i = threadIdx.x; i < n; i += 1024)
+        if (D[i] != 0.0f && E[i] + (double)D[i] >= lo) mark(i);
+      continue;
+    }
+    if (phase == 2) {
+      double mx = -1e308;  // the candidates are exact now; nothing else can reach them
+      for (long i = threadIdx.x; i < n; i += 1024)
+        if (D[i] == 0.0f) mx = fmax(mx, E[i]);
+      mx = block_reduce(mx, MaxOp(), shd);
+      if (threadIdx.x == 0) P[0] = mx;
+      if (!(mx > 0)) {  // no energy: every value exact, then the FP64 rules as written
+        for (long i = threadIdx.x; i < n; i += 1024)
+          if (D[i] != 0.0f) mark(i);
+        continue;
+      }
+      if (f == 0) {
+        const double lvl = mx * 0.4;
+        for (long i = threadIdx.x; i < n; i += 1024) {
+          const double d = D[i];
+          if (d != 0.0 && E[i] - d <= lvl && E[i] + d > lvl) mark(i);
+        }
+        continue;
+      }
+      double hi_min = 1e308;
+      for (long i = threadIdx.x; i < n; i += 1024) {
+        const double v = (E[i] + (double)D[i]) / mx;
+        if (moments_ok(v)) hi_min = fmin(hi_min, v);
+      }
+      hi_min = block_reduce(hi_min, MinOp(), shd);
+      for (long i = threadIdx.x; i < n; i += 1024)
+        if (D[i] != 0.0f && (E[i] - (double)D[i]) / mx <= hi_min) mark(i);
+      continue;
+    }
+    const double mx = P[0];
+    if (phase == 3) {
+      if (f == 0) {
+        const double lvl = mx * 0.4;
+        long long c = 0;
+        for (long i = threadIdx.x; i < n; i += 1024) {
+          const double d = D[i];
+          c += d == 0.0 ? E[i] > lvl : E[i] - d > lvl;  // open pixels were recomputed
+        }
+        c = block_reduce(c, SumOp(), shl);
+        if (threadIdx.x == 0) P[3] = (double)c;
+        continue;
+      }
+      double a = 1e308, b = -1e308;
+      for (long i = threadIdx.x; i < n; i += 1024) {
+        const double v = E[i] / mx;
+        if (D[i] == 0.0f && moments_ok(v)) { a = fmin(a, v); b = fmax(b, v); }
+      }
+      a = block_reduce(a, MinOp(), shd);
+      b = block_reduce(b, MaxOp(), shd);
+      if (threadIdx.x == 0) { P[1] = a; P[2] = b; }
+      const double sc = hist_scale(a, b, 1024);
+      for (long i = threadIdx.x; i < n; i += 1024) {
+        const double d = D[i];
+        if (d == 0.0) continue;
+        // the true value lies in [a, b]: clamp the interval's ends there
+        const double vl = fmax((E[i] - d) / mx, a), vh = fmin((E[i] + d) / mx, b);
+        if (hist_bin(vl, a, sc, 1024) != hist_bin(vh, a, sc, 1024)) mark(i);
+      }
+      continue;
+    }
+    const double a = P[1], b = P[2];
+    if (phase == 4) {
+      for (int k = threadIdx.x; k < 1024; k += 1024) hist[k] = 0;
+      __syncthreads();
+      const double sc = hist_scale(a, b, 1024);
+      for (long i = threadIdx.x; i < n; i += 1024) {
+        const double e = E[i], d = D[i];
+        // d > 0: the bin was decided by the interval (both ends give it)
+        const double v = d == 0.0 ? e / mx : fmax((e - d) / mx, a);
+        atomicAdd(&hist[hist_bin(v, a, sc, 1024)], 1);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) thr_s = otsu_threshold(hist, n, a, b);
+      __syncthreads();
+      const double thr = thr_s;
+      if (threadIdx.x == 0) P[3] = thr;
+      for (long i = threadIdx.x; i < n; i += 1024) {
+        const double d = D[i];
+        if (d != 0.0 && (E[i] - d) / mx <= thr && (E[i] + d) / mx > thr) mark(i);
+      }
+      __syncthreads();  // hist is reused by the next filter
+      continue;
+    }
+    // phase 5
+    const double thr = P[3];
+    long long c = 0;
+    for (long i = threadIdx.x; i < n; i += 1024) {
+      const double d = D[i];
+      c += d == 0.0 ? (E[i] / mx) > thr : (E[i] - d) / mx > thr;
+    }
+    c = block_reduce(c, SumOp(), shl);
+    if (threadIdx.x == 0) o.rows[blockIdx.x][f - 1] = (double)c / prm[3];
+  }
+}
+
+void launch_gabor_post_exact(const Img *imgs, double *const *energy, float *const *bound, const double *taps,
+                             double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s) {
+  const int tiles = ((maxw + GB_TC - 1) / GB_TC) * ((maxh + GB_TR - 1) / GB_TR);
+  for (int phase = 1; phase <= 5; phase++) {
+    k_gabor_post_exact<<<nimg, 1024, 0, s>>>(phase, imgs, energy, bound, params, o);
+    if (phase < 5) k_gabor_repair<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, taps, energy, bound);
   }
 }
 

@@ -157,8 +157,11 @@ __global__ void k_zern_final(const Img *imgs, const double *partial, int nchunk,
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
                     hipStream_t s, int precision) {
   k_zern_moments<<<nimg, 1024, 0, s>>>(imgs, partial, nchunk);
-  if (precision) k_zern_accum<float><<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
-  else k_zern_accum<double><<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
+  // FP64 accumulation in every precision: the radial polynomials are FP64
+  // anyway, and on MI355X the FP32 and FP64 vector rates are equal, so FP32
+  // sums would add error (above 1e-5 on transformed planes) and save nothing
+  (void)precision;
+  k_zern_accum<double><<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
   k_zern_final<<<nimg, 128, 0, s>>>(imgs, partial, nchunk, o);
 }
 

@@ -162,9 +162,14 @@ void launch_edge_final(const Img *mag, const Img *dir, const Stats *st2, const d
                        hipStream_t s);
 void launch_gabor_post(const Img *imgs, double *const *energy, OutRef o, int nimg, hipStream_t s);
 // reduced-precision contractions (k_mixed.hip): precision 1 = FP32, 2 = BF16x3 on MFMA.
-// Gabor: FP32 takes float kernels (k0); BF16x3 takes the hi / lo bf16 planes (k0, k1)
-void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const void *k1, double *const *energy,
-                        OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
+// Gabor: FP32 takes float kernels (k0); BF16x3 takes the hi / mid / lo bf16
+// planes (k0, k1, k2); both write per-pixel error bounds and finish with the
+// exact post-processing (FP64 recomputation of the undecided pixels, taps64)
+void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const void *k1, const void *k2,
+                        const double *wsum, const double *taps64, double *const *energy, float *const *bound,
+                        double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
+void launch_gabor_post_exact(const Img *imgs, double *const *energy, float *const *bound, const double *taps,
+                             double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
 void launch_gemm_mixed(int precision, const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s);
 void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
                   int maxw, hipStream_t s);
