@@ -319,6 +319,15 @@ int wndgpu_last_node_times(wndgpu_ctx *ctx, double *ms, const char **names, int 
  * counterpart in the reference (Ng is local to CVIPtexture.cpp:400-414). */
 int wndgpu_last_haralick_work(wndgpu_ctx *ctx, int *tones, double *schur_flops, int cap);
 
+/* Chebyshev-Fourier entries of the last compute (one (image, node) pair
+ * each, in the order of the plan's entries for that group): open[k] = 1
+ * where the histogram of the matrix-core sums could not be certified equal
+ * to the reference's and the entry was recomputed in the reference's order
+ * (ChebyshevFourier.cpp:112-135).  Returns the number of entries (<= cap
+ * are written).  WNDGPU_CF_SLACK (a factor >= 1, read at context creation)
+ * widens the certification bound.  No counterpart in the reference. */
+int wndgpu_last_cf_open(wndgpu_ctx *ctx, int *open, int cap);
+
 /* ---- files (host only: no context, no GPU).  Errors: WNDGPU_E_IO /
  * WNDGPU_E_FORMAT / WNDGPU_E_ARG with a message in wndgpu_io_error() of the
  * calling thread. */

@@ -291,6 +291,13 @@ struct wndgpu_ctx {
   int *har_ng = nullptr;
   double *har_work = nullptr;
   int har_n = 0;
+  // Chebyshev-Fourier entries of the last compute: open[k] = 1 where the
+  // matrix-core histogram was left uncertified and recomputed in the
+  // reference's order.  cf_slack (WNDGPU_CF_SLACK at creation, >= 1)
+  // scales the certification bound.
+  int *cf_open = nullptr;
+  int cf_n = 0;
+  double cf_slack = 1;
   bool sized = false;
 };
 
@@ -697,7 +704,10 @@ static int build_sampled_root(wndgpu_ctx *c, const wndgpu_plan *plan, const Batc
 // transform nodes).
 static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, Arena &A,
                        std::vector<LaunchItem> &launches, size_t *first_feature) {
-  if (!A.sizing) c->har_n = 0;
+  if (!A.sizing) {
+    c->har_n = 0;
+    c->cf_n = 0;
+  }
   hipStream_t s = c->stream;
   const int n = B.n;
   std::map<std::vector<int>, NodeRun> nodes;
@@ -1077,37 +1087,71 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           C.sum = A.dalloc_n<double>(2 * 23 * 23);
           cd[k] = C;
         }
-        // groups of up to CF_GROUP entries sharing one table set (same plane size)
-        std::vector<CfGroup> grp(ne);  // sized for the worst case (one entry per group)
-        int ngrp = 0;
-        {
-          std::map<const int *, std::vector<int>> by_table;
-          for (int k = 0; k < ne; k++) by_table[cd[k].kk].push_back(k);
-          for (auto &kv : by_table)
-            for (size_t q = 0; q < kv.second.size(); q += CF_GROUP) {
-              CfGroup &G = grp[ngrp++];
-              const CfDesc &C0 = cd[kv.second[q]];
-              G.kk = C0.kk; G.tn = C0.tn; G.fc = C0.fc; G.fs = C0.fs;
-              G.np = C0.np; G.NN = C0.NN;
-              G.ne = (int)std::min<size_t>(CF_GROUP, kv.second.size() - q);
-              for (int g = 0; g < CF_GROUP; g++) {
-                const CfDesc &Cg = cd[kv.second[q + std::min(g, G.ne - 1)]];
-                G.img[g] = Cg.img;
-                G.sum[g] = Cg.sum;
-              }
-            }
-          if (A.sizing) ngrp = ne;
+        // Entries sharing one table set (same plane size) form groups of
+        // CFM_GROUP for the matrix-core sums and of CF_GROUP for the
+        // reference-order recomputation.  The grouping is by size, so the
+        // sizing pass forms the same groups, and its pixel counts (w*h)
+        // bound the real ones: nsplit and the partial buffers it sizes are
+        // upper bounds.
+        std::map<std::pair<int, int>, std::vector<int>> by_size;
+        for (int k = 0; k < ne; k++) by_size[{cd[k].w, cd[k].h}].push_back(k);
+        int nmg = 0, maxchunk = 1;
+        for (auto &kv : by_size) {
+          nmg += (int)((kv.second.size() + CFM_GROUP - 1) / CFM_GROUP);
+          maxchunk = std::max(maxchunk, (cd[kv.second[0]].np + 63) / 64);
         }
+        // about 2048 blocks for the 256 CUs, at least 8 chunks of 64 pixels per split
+        const int nsplit = std::max(1, std::min(std::min(64, maxchunk / 8), (2048 + nmg - 1) / nmg));
+        std::vector<CfMGroup> mgrp;
+        std::vector<CfGroup> grp;
+        for (auto &kv : by_size) {
+          const std::vector<int> &v = kv.second;
+          const CfDesc &C0 = cd[v[0]];
+          for (size_t q = 0; q < v.size(); q += CFM_GROUP) {
+            CfMGroup G;
+            G.kk = C0.kk; G.tn = C0.tn; G.fc = C0.fc; G.fs = C0.fs;
+            G.np = C0.np; G.NN = C0.NN;
+            G.ne = (int)std::min<size_t>(CFM_GROUP, v.size() - q);
+            for (int g = 0; g < CFM_GROUP; g++) {
+              const int k = v[q + std::min(g, G.ne - 1)];
+              if (g < G.ne) cd[k].part = A.dalloc_n<double>((size_t)nsplit * CFM_PART);
+              G.img[g] = cd[k].img;
+              G.part[g] = cd[k].part;
+            }
+            mgrp.push_back(G);
+          }
+          for (size_t q = 0; q < v.size(); q += CF_GROUP) {
+            CfGroup G;
+            G.kk = C0.kk; G.tn = C0.tn; G.fc = C0.fc; G.fs = C0.fs;
+            G.np = C0.np; G.NN = C0.NN;
+            G.ne = (int)std::min<size_t>(CF_GROUP, v.size() - q);
+            for (int g = 0; g < CF_GROUP; g++) {
+              const int k = v[q + std::min(g, G.ne - 1)];
+              G.img[g] = cd[k].img;
+              G.sum[g] = cd[k].sum;
+              G.ent[g] = k;
+            }
+            grp.push_back(G);
+          }
+        }
+        for (int k = 0; k < ne; k++) cd[k].nsplit = nsplit;
+        int *open = A.dalloc_n<int>(ne);
+        if (!A.sizing) {
+          c->cf_open = open;
+          c->cf_n = ne;
+        }
+        const int ngrp = (int)grp.size();
+        CfMGroup *mgrpd = A.stage(mgrp);
         CfGroup *grpd = A.stage(grp);
         CfDesc *cdd = A.stage(cd);
         DsDesc *dsd = ds.empty() ? nullptr : A.stage(ds);
         const int nds = (int)ds.size();
-        // FP64 in every precision: an a-priori bound on FP32 sums over the
-        // disk leaves most of the 529 magnitudes' histogram bins open, so an
-        // exact histogram would recompute nearly every entry in FP64
+        const double slack = c->cf_slack;
+        // FP64 in every precision: the certified histogram needs FP64 sums
+        // (an FP32 bound over the disk would leave most bins open)
         push(gn, [=]() {
           if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
-          launch_cf(grpd, ngrp, cdd, ne, o, s, 0);
+          launch_cf(mgrpd, nmg, nsplit, grpd, ngrp, cdd, ne, open, slack, o, s);
         });
       } break;
       case A_EDGE: {
@@ -1556,6 +1600,7 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     return WNDGPU_E_DEVICE;
   }
   c->serial = getenv("WNDGPU_SERIAL") != nullptr;
+  if (const char *cs = getenv("WNDGPU_CF_SLACK")) c->cf_slack = std::max(1.0, atof(cs));
   if (const char *gm = getenv("WNDGPU_GRAPH")) c->graph_mode = std::min(2, std::max(0, atoi(gm)));
   if (const char *mb = getenv("WNDGPU_TABLE_CACHE_MB")) c->table_budget = (size_t)std::max(0L, atol(mb)) << 20;
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS
@@ -2239,6 +2284,13 @@ int wndgpu_last_haralick_work(wndgpu_ctx *c, int *tones, double *schur_flops, in
   if (k > 0 && schur_flops)
     HIPCHK(c, hipMemcpy(schur_flops, c->har_work, 4 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost));
   return c->har_n;
+}
+
+int wndgpu_last_cf_open(wndgpu_ctx *c, int *open, int cap) {
+  if (!c || c->cf_n == 0) return 0;
+  const int k = cap < c->cf_n ? cap : c->cf_n;
+  if (k > 0 && open) HIPCHK(c, hipMemcpy(open, c->cf_open, (size_t)k * sizeof(int), hipMemcpyDeviceToHost));
+  return c->cf_n;
 }
 
 int wndgpu_last_node_times(wndgpu_ctx *c, double *ms, const char **names, int cap) {

@@ -169,26 +169,242 @@ void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *
 // ChebyshevFourier2D (src/transforms/ChebyshevFourier.cpp:59-160), N = 11,
 // NN = 23: sum_r[im][a] += (Ftrm*cos(mf*phi) * T_a(2r-1)) * I  over in-disk
 // pixels; the geometry tables are host-computed with the reference's libm
-// calls.
+// calls.  The 529 magnitudes only feed a 32-bin histogram, so the sums are
+// taken on the FP64 matrix cores and every bin is certified against an
+// a-priori bound on the difference from the reference's sequential sums;
+// an entry with a bin left open is recomputed in the reference's order.
 constexpr int CF_CHUNK = 64;
 constexpr int CF_NN = 23;
 
-// acc + a * b: unfused in FP64 (the reference's rounding), fused in FP32
-__device__ __forceinline__ double mac(double acc, double a, double b) { return acc + a * b; }
-__device__ __forceinline__ float mac(float acc, float a, float b) { return fmaf(a, b, acc); }
+typedef double cf_d4 __attribute__((ext_vector_type(4)));
 
+// One K-step (4 pixels) of NW entries.  The sums of an entry are the GEMM
+//   S[r][a] = sum_p A[r][p] B[p][a],  A[r][p] = fc[p][r] (r < NN), fs[p][r-NN] (NN <= r < 2NN),
+//   B[p][a] = tn[p][a] I[p],
+// 3 row tiles x 2 column tiles of v_mfma_f64_16x16x4 (lane l holds
+// A[l%16][k0+l/16] and B[k0+l/16][l%16]); rows >= 2NN and columns >= NN
+// are zero in LDS.
+template <int NW>
+__device__ __forceinline__ void cfm_step(cf_d4 (&acc)[4][6], const double (*Acat)[48], const double (*Tcat)[32],
+                                         const double (*iv)[CF_CHUNK], int e0, int s, int lane) {
+  const int p = 4 * s + (lane >> 4), c = lane & 15;
+  const double a0 = Acat[p][c], a1 = Acat[p][16 + c], a2 = Acat[p][32 + c];
+  const double t0 = Tcat[p][c], t1 = Tcat[p][16 + c];
+#pragma unroll
+  for (int e = 0; e < NW; e++) {
+    const double v = iv[e0 + e][p];
+    const double b0 = t0 * v, b1 = t1 * v;
+    acc[e][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[e][0], 0, 0, 0);
+    acc[e][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[e][1], 0, 0, 0);
+    acc[e][2] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[e][2], 0, 0, 0);
+    acc[e][3] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[e][3], 0, 0, 0);
+    acc[e][4] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b0, acc[e][4], 0, 0, 0);
+    acc[e][5] = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b1, acc[e][5], 0, 0, 0);
+  }
+}
+
+template <int NW>
+__device__ __forceinline__ void cfm_chunk(cf_d4 (&acc)[4][6], const double (*Acat)[48], const double (*Tcat)[32],
+                                          const double (*iv)[CF_CHUNK], int e0, int lane) {
+#pragma unroll 4
+  for (int s = 0; s < CF_CHUNK / 4; s++) cfm_step<NW>(acc, Acat, Tcat, iv, e0, s, lane);
+}
+
+// Block (split x, group y) = CFM_GROUP entries sharing one table set; wave w
+// owns entries 4w..4w+3 over the chunk range of split x.  The tables of a
+// chunk are staged in LDS once for all 16 entries; the next chunk's table
+// entries and pixels are fetched into registers while the current one is
+// consumed.  Output per (entry, split): the 6 accumulator tiles as held in
+// the MFMA registers ([tile][q][lane]) and sum |I| over the split's pixels.
+__global__ void __launch_bounds__(256) k_cf_mfma(const CfMGroup *groups, int nsplit) {
+  __shared__ double Acat[CF_CHUNK][48];
+  __shared__ double Tcat[CF_CHUNK][32];
+  __shared__ double iv[CFM_GROUP][CF_CHUNK];
+  const CfMGroup &G = groups[blockIdx.y];
+  const int NN = G.NN, ne = G.ne, np = G.np;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  for (int k = t; k < CF_CHUNK * 48; k += 256) (&Acat[0][0])[k] = 0.0;
+  for (int k = t; k < CF_CHUNK * 32; k += 256) (&Tcat[0][0])[k] = 0.0;
+  const int nchunk = (np + CF_CHUNK - 1) / CF_CHUNK;
+  const int c0 = (int)((long)blockIdx.x * nchunk / nsplit), c1 = (int)((long)(blockIdx.x + 1) * nchunk / nsplit);
+  constexpr int CF_PER = (CF_CHUNK * CF_NN + 255) / 256;
+  const gptr<const double> Gtn = to_global(G.tn), Gfc = to_global(G.fc), Gfs = to_global(G.fs);
+  double rt[CF_PER], rc[CF_PER], rs[CF_PER], rv[4];
+  // staging roles: table element t + 256k; pixel t % 64 of entries t/64 + 4j
+  const int q = t & 63;
+  auto fetch = [&](int ch) {
+    const int p0 = ch * CF_CHUNK;
+    const int npc = min(np - p0, CF_CHUNK);
+    const long base = (long)p0 * NN;
+#pragma unroll
+    for (int k = 0; k < CF_PER; k++) {
+      const int e = min(t + 256 * k, npc * NN - 1);  // past the chunk: re-read its last entry (unused)
+      rt[k] = Gtn[base + e];
+      rc[k] = Gfc[base + e];
+      rs[k] = Gfs[base + e];
+    }
+    const int kidx = to_global(G.kk)[p0 + min(q, npc - 1)];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int e = wv + 4 * j;
+      rv[j] = (e < ne && q < npc) ? to_global(G.img[e])[kidx] : 0.0;
+    }
+  };
+  cf_d4 acc[4][6];
+#pragma unroll
+  for (int e = 0; e < 4; e++)
+#pragma unroll
+    for (int k = 0; k < 6; k++) acc[e][k] = cf_d4{0.0, 0.0, 0.0, 0.0};
+  double sabs[4] = {0.0, 0.0, 0.0, 0.0};
+  const int nw = max(0, min(4, ne - 4 * wv));
+  if (c0 < c1) fetch(c0);
+  for (int ch = c0; ch < c1; ch++) {
+    const int npc = min(np - ch * CF_CHUNK, CF_CHUNK);
+    __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+    for (int k = 0; k < CF_PER; k++) {
+      const int e = t + 256 * k;
+      if (e < npc * NN) {
+        const int p = e / NN, a = e - p * NN;
+        Tcat[p][a] = rt[k];
+        Acat[p][a] = rc[k];
+        Acat[p][NN + a] = rs[k];
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      iv[wv + 4 * j][q] = rv[j];  // zero past the chunk's pixels and for absent entries
+      sabs[j] += fabs(rv[j]);
+    }
+    __syncthreads();
+    if (ch + 1 < c1) fetch(ch + 1);
+    switch (nw) {
+      case 4: cfm_chunk<4>(acc, Acat, Tcat, iv, 4 * wv, lane); break;
+      case 3: cfm_chunk<3>(acc, Acat, Tcat, iv, 4 * wv, lane); break;
+      case 2: cfm_chunk<2>(acc, Acat, Tcat, iv, 4 * wv, lane); break;
+      case 1: cfm_chunk<1>(acc, Acat, Tcat, iv, 4 * wv, lane); break;
+      default: break;
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    if (e < nw) {
+      const gptr<double> P = to_global(G.part[4 * wv + e] + (long)blockIdx.x * CFM_PART);
+#pragma unroll
+      for (int k = 0; k < 6; k++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) P[k * 256 + r * 64 + lane] = acc[e][k][r];
+    }
+  }
+  // sum |I| of entries wv + 4j over this split's pixels (a bound input only)
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double v = sabs[j];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if (lane == 0 && wv + 4 * j < ne) to_global(G.part[wv + 4 * j] + (long)blockIdx.x * CFM_PART)[6 * 256] = v;
+  }
+}
+
+// element (r, a) of the 48x32 padded sum matrix in the [tile][q][lane] layout
+__device__ __forceinline__ int cfm_index(int r, int a) {
+  const int tile = (r >> 4) * 2 + (a >> 4), rr = r & 15;
+  return tile * 256 + (rr >> 2) * 64 + (rr & 3) * 16 + (a & 15);
+}
+
+// Certified histogram of one entry.  With |Ftrm cos|, |Ftrm sin| <= 1 and
+// |T_a| <= 1, every term of a coefficient sum is at most |I_p|, so both the
+// reference's sequential sum (K - 1 additions after two products per term)
+// and the matrix-core sum (one product for B, the MFMA product and at most
+// K + nsplit + 8 additions on any term's path) are within
+// gamma_{K+nsplit+8} * sum|I| of the exact sum of the terms (Higham,
+// Accuracy and Stability of Numerical Algorithms, 2nd ed., eq. 4.4), FP64
+// subnormals adding at most 2^-1074 per operation.  Each magnitude then lies
+// within e = 1.5 E + 8u c of the reference's (the 2-norm is 1-Lipschitz;
+// pow, the sum and sqrt round within 4u on either side), and the
+// reference's min, max and every (c - min) * 32 / (max - min) lie in
+// intervals from those; a bin is certified when both ends of its interval,
+// widened by 1e-12 relative for the rounding of the reference's and this
+// kernel's own arithmetic, floor to the same bin.  open[e] = 1 marks an
+// entry whose bins are not all certified (its row is then left unwritten).
+__global__ void __launch_bounds__(256) k_cf_certify(const CfDesc *cds, int *open, double slack, OutRef o) {
+  __shared__ double shd[16];
+  __shared__ double cc[CF_NN * CF_NN], ce[CF_NN * CF_NN];
+  __shared__ int bins[32];
+  __shared__ int bad;
+  const CfDesc &C = cds[blockIdx.x];
+  const int NN = C.NN, NN2 = NN * NN, nsp = C.nsplit;
+  const gptr<const double> P = to_global((const double *)C.part);
+  if (threadIdx.x == 0) bad = 0;
+  if (threadIdx.x < 32) bins[threadIdx.x] = 0;
+  double S = 0;
+  for (int s = 0; s < nsp; s++) S += P[(long)s * CFM_PART + 6 * 256];
+  const double u = 0x1p-53;
+  const double n = (double)C.np + nsp + 8;
+  const double gam = n * u / (1 - n * u);
+  const double E = slack * (gam * S * (1 + 1e-6) + (n + 16) * 0x1p-1070);
+  double mnL = kInf, mnU = kInf, mxL = -kInf, mxU = -kInf;
+  bool fin = isfinite(E) && n * u < 0.01;
+  __syncthreads();
+  for (int k = threadIdx.x; k < NN2; k += blockDim.x) {
+    const int im = k / NN, a = k - im * NN;
+    const int ir = cfm_index(im, a), ii = cfm_index(NN + im, a);
+    double sr = 0, si = 0;
+    for (int s = 0; s < nsp; s++) {
+      sr += P[(long)s * CFM_PART + ir];
+      si += P[(long)s * CFM_PART + ii];
+    }
+    const double c = sqrt(sr * sr + si * si);
+    const double e = 2 * E + 16 * u * c + 0x1p-1000;
+    fin = fin && isfinite(c) && isfinite(e);
+    cc[k] = c;
+    ce[k] = e;
+    mnL = fmin(mnL, c - e);
+    mnU = fmin(mnU, c + e);
+    mxL = fmax(mxL, c - e);
+    mxU = fmax(mxU, c + e);
+  }
+  if (!fin) bad = 1;
+  mnL = block_reduce(mnL, MinOp(), shd);
+  mnU = block_reduce(mnU, MinOp(), shd);
+  mxL = block_reduce(mxL, MaxOp(), shd);
+  mxU = block_reduce(mxU, MaxOp(), shd);
+  const double den_lo = mxL - mnU, den_hi = mxU - mnL;
+  if (!(den_lo > 0) || bad) {
+    if (threadIdx.x == 0) open[blockIdx.x] = 1;
+    return;
+  }
+  for (int k = threadIdx.x; k < NN2; k += blockDim.x) {
+    const double c = cc[k], e = ce[k];
+    const double num_lo = fmax(0.0, (c - e) - mnU), num_hi = (c + e) - mnL;
+    const double tl = 32 * num_lo / den_hi * (1 - 1e-12);
+    const double tu = 32 * num_hi / den_lo * (1 + 1e-12) + 1e-300;
+    const int bl = tl >= 31 ? 31 : (int)tl, bu = tu >= 31 ? 31 : (int)tu;
+    if (bl != bu) bad = 1;
+    atomicAdd(&bins[bl], 1);
+  }
+  __syncthreads();
+  if (bad) {
+    if (threadIdx.x == 0) open[blockIdx.x] = 1;
+    return;
+  }
+  if (threadIdx.x == 0) open[blockIdx.x] = 0;
+  if (threadIdx.x < 32) o.rows[blockIdx.x][threadIdx.x] = bins[threadIdx.x];
+}
+
+// Reference-order recomputation of the groups holding an open entry.
 // Block = one group of entries sharing the basis tables.  Each thread owns
 // up to five of the 2*NN*NN coefficients and keeps one running sum per
 // coefficient and entry across all pixel chunks, so every sum is taken in
-// the reference's pixel order (ChebyshevFourier.cpp:112-135); the tables
-// of a chunk are staged in LDS once for all entries of the group.
-// T: the arithmetic (double, the plain form; float, the reduced forms).
-template <class T>
-__global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
-  __shared__ T tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN];
-  __shared__ T iv[CF_GROUP][CF_CHUNK];
+// the reference's pixel order (ChebyshevFourier.cpp:112-135); the
+// tables of a chunk are staged in LDS once for all entries of the group.
+__global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups, const int *open) {
+  __shared__ double tn[CF_CHUNK][CF_NN], fc[CF_CHUNK][CF_NN], fs[CF_CHUNK][CF_NN];
+  __shared__ double iv[CF_GROUP][CF_CHUNK];
   const CfGroup G = groups[blockIdx.x];
   const int NN = G.NN, NN2 = NN * NN, ne = G.ne;
+  bool any = false;
+  for (int g = 0; g < ne; g++) any = any || open[G.ent[g]];
+  if (!any) return;
   const int t = threadIdx.x;
   // this thread's coefficients: o = t + 256 q, q < 5 (2*23*23 = 1058 <= 1280)
   int oim[5], oa[5];
@@ -202,13 +418,11 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
     oim[q] = on[q] ? k / NN : 0;
     oa[q] = on[q] ? k - (k / NN) * NN : 0;
   }
-  T acc[CF_GROUP][5];
+  double acc[CF_GROUP][5];
 #pragma unroll
   for (int g = 0; g < CF_GROUP; g++)
 #pragma unroll
-    for (int q = 0; q < 5; q++) acc[g][q] = (T)0;
-  // the next chunk's table entries and pixels are fetched into registers
-  // while the current chunk is consumed from LDS
+    for (int q = 0; q < 5; q++) acc[g][q] = 0.0;
   constexpr int CF_PER = (CF_CHUNK * CF_NN + 255) / 256;
   const gptr<const double> Gtn = to_global(G.tn), Gfc = to_global(G.fc), Gfs = to_global(G.fs);
   double rt[CF_PER], rc[CF_PER], rs[CF_PER], rv[CF_GROUP];
@@ -235,14 +449,14 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
       const int e = t + 256 * k;
       if (e < np * NN) {
         const int p = e / NN, a = e - p * NN;
-        tn[p][a] = (T)rt[k];
-        fc[p][a] = (T)rc[k];
-        fs[p][a] = (T)rs[k];
+        tn[p][a] = rt[k];
+        fc[p][a] = rc[k];
+        fs[p][a] = rs[k];
       }
     }
     if (t < np) {
 #pragma unroll
-      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = (T)rv[g];
+      for (int g = 0; g < CF_GROUP; g++) iv[g][t] = rv[g];
     }
     __syncthreads();
     if (p0 + CF_CHUNK < G.np) fetch(p0 + CF_CHUNK);
@@ -250,10 +464,10 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
 #pragma unroll
       for (int q = 0; q < 5; q++) {
         if (!on[q]) continue;
-        const T f = ore[q] ? fc[p][oim[q]] : fs[p][oim[q]];
-        const T ft = f * tn[p][oa[q]];
+        const double f = ore[q] ? fc[p][oim[q]] : fs[p][oim[q]];
+        const double ft = f * tn[p][oa[q]];
 #pragma unroll
-        for (int g = 0; g < CF_GROUP; g++) acc[g][q] = mac(acc[g][q], ft, iv[g][p]);
+        for (int g = 0; g < CF_GROUP; g++) acc[g][q] = acc[g][q] + ft * iv[g][p];
       }
     }
   }
@@ -262,13 +476,16 @@ __global__ void __launch_bounds__(256) k_cf_accum(const CfGroup *groups) {
     if (g < ne)
 #pragma unroll
       for (int q = 0; q < 5; q++)
-        if (on[q]) G.sum[g][t + 256 * q] = (double)acc[g][q];
+        if (on[q]) G.sum[g][t + 256 * q] = acc[g][q];
 }
 
-__global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) {
+// Histogram of the reference-order sums of the open entries
+// (ChebyshevFourier.cpp:137-153).
+__global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, const int *open, OutRef o) {
   __shared__ double shd[16];
   __shared__ double coeff[CF_NN * CF_NN];
   __shared__ int bins[32];
+  if (!open[blockIdx.x]) return;
   const CfDesc C = cds[blockIdx.x];
   const int NN = C.NN, NN2 = NN * NN;
   double mn = kInf, mx = -kInf;
@@ -291,11 +508,12 @@ __global__ void __launch_bounds__(1024) k_cf_final(const CfDesc *cds, OutRef o) 
   if (threadIdx.x < 32) o.rows[blockIdx.x][threadIdx.x] = bins[threadIdx.x];
 }
 
-void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s,
-               int precision) {
-  if (precision) k_cf_accum<float><<<ngroups, 256, 0, s>>>(groups);
-  else k_cf_accum<double><<<ngroups, 256, 0, s>>>(groups);
-  k_cf_final<<<nimg, 1024, 0, s>>>(cd, o);
+void launch_cf(const CfMGroup *mgroups, int nmgroups, int nsplit, const CfGroup *groups, int ngroups,
+               const CfDesc *cd, int nimg, int *open, double slack, OutRef o, hipStream_t s) {
+  k_cf_mfma<<<dim3(nsplit, nmgroups), 256, 0, s>>>(mgroups, nsplit);
+  k_cf_certify<<<nimg, 256, 0, s>>>(cd, open, slack, o);
+  k_cf_accum<<<ngroups, 256, 0, s>>>(groups, open);
+  k_cf_final<<<nimg, 1024, 0, s>>>(cd, open, o);
 }
 
 // ================================================================== Downsample

@@ -51,10 +51,13 @@ struct CfDesc {
   const double *fs;     // [np][NN] Ftrm*sin(-mf*phi)
   int np, NN;
   int w, h;             // plane size (row-major [h][w])
-  double *sum;          // [2][NN*NN] real and imaginary coefficient sums
+  double *sum;          // [2][NN*NN] real and imaginary coefficient sums (reference order)
+  double *part;         // [nsplit][CFM_PART] matrix-core partial sums
+  int nsplit;
 };
 
-// up to CF_GROUP entries whose planes share one set of basis tables
+// up to CF_GROUP entries whose planes share one set of basis tables (the
+// reference-order recomputation of entries whose histogram stays open)
 constexpr int CF_GROUP = 4;
 struct CfGroup {
   const int *kk;
@@ -62,6 +65,19 @@ struct CfGroup {
   int np, NN, ne;
   const double *img[CF_GROUP];
   double *sum[CF_GROUP];
+  int ent[CF_GROUP];  // entry (output row) of each member
+};
+
+// up to CFM_GROUP entries of one table set for the matrix-core contraction;
+// a launch splits each group's pixels into nsplit contiguous chunk ranges
+constexpr int CFM_GROUP = 16;
+constexpr int CFM_PART = 6 * 256 + 8;  // 3x2 16x16 FP64 tiles as MFMA registers, then sum |I|
+struct CfMGroup {
+  const int *kk;
+  const double *tn, *fc, *fs;
+  int np, NN, ne;
+  const double *img[CFM_GROUP];
+  double *part[CFM_GROUP];
 };
 
 struct HsvDsDesc {  // Downsample of a colour plane (3 bytes per pixel), positions as DsDesc
@@ -175,7 +191,11 @@ void launch_gabor(const Img *imgs, const double *kernels, double *const *energy,
                   int maxw, hipStream_t s);
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
                     hipStream_t s, int precision = 0);
-void launch_cf(const CfGroup *groups, int ngroups, const CfDesc *cd, int nimg, OutRef o, hipStream_t s, int precision = 0);
+// Chebyshev-Fourier histograms: matrix-core sums certified against an
+// a-priori bound (slack >= 1 scales it), entries left open (open[k] = 1)
+// recomputed in the reference's order
+void launch_cf(const CfMGroup *mgroups, int nmgroups, int nsplit, const CfGroup *groups, int ngroups,
+               const CfDesc *cd, int nimg, int *open, double slack, OutRef o, hipStream_t s);
 void launch_downsample(const DsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
 void launch_downsample_hsv(const HsvDsDesc *dd, int nimg, int maxoh, int maxow, int maxnh, int maxnw, hipStream_t s);
 void launch_normalize(NormDesc *nd, int nimg, long maxn, hipStream_t s);

@@ -79,6 +79,7 @@ def _load():
         'wndgpu_py_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
         'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
         'wndgpu_last_haralick_work': (_I, [_P, _P, _P, _I]),
+        'wndgpu_last_cf_open': (_I, [_P, _P, _I]),
         'wndgpu_sampling_n_samples': (_I, [_P]),
         'wndgpu_compute_sampled': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P, _P]),
         'wndgpu_io_error': (ctypes.c_char_p, []),
@@ -550,6 +551,17 @@ def last_haralick_work():
     flops = np.zeros((max(n, 1), 4))
     lib.wndgpu_last_haralick_work(_context(), _ptr(tones), _ptr(flops), n)
     return tones[:n], flops[:n]
+
+
+def last_cf_open():
+    """[k] int32 of the Chebyshev-Fourier entries of the last compute: 1 where
+    the matrix-core histogram was left uncertified and the entry was
+    recomputed in the reference's order."""
+    lib = _load()
+    n = lib.wndgpu_last_cf_open(_context(), None, 0)
+    out = np.zeros(max(n, 1), np.int32)
+    lib.wndgpu_last_cf_open(_context(), _ptr(out), n)
+    return out[:n]
 
 
 # ---------------------------------------------------------------- files
