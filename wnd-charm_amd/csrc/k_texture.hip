@@ -1632,23 +1632,36 @@ __device__ __forceinline__ double radon_ridx(int n, int m, int M, int N, double 
 // in the sub-row m, so the sub-columns touching bin r form an interval
 // (binary search) and, per sub-column, the sub-rows with floor(rIdx) in
 // {r-1, r} form a range that moves slowly with n (tracked by stepping).
-// Each thread walks its own (n, m) sequence as one flat loop, so the lanes
-// of a wave stay busy whatever the angle.
+// Each bin's terms are added in the reference's order: sub-columns
+// ascending, sub-rows ascending within one.
+//   At angle 0 a bin is a pixel column and the lanes of a wave read
+// adjacent pixels of one row, so pixels are read from global memory
+// directly.  At the other angles a lane follows its own slanted line, so a
+// direct read touches one cache line per lane; there the block streams the
+// plane through LDS in chunks of whole pixel columns (rows limited to the
+// band its bins can reach, with a direct read as the fallback outside it),
+// and every lane takes its sub-columns of a chunk in order.
+// doubles of the LDS chunk buffer: 16 KB keeps 8 blocks per CU; the
+// kernel is bound by each bin's sequential sum, and larger chunks (lower
+// occupancy) measured slower
+constexpr int RADON_TILE = 2048;
 __device__ __forceinline__ double radon_xc(int nn, int xO, double cs, int rFirst) {
   return ((-xO - 0.25) + 0.5 * nn) * cs - rFirst;
 }
 
 __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const double *trig, double *const *proj,
-                                                    int nbg, int nimg) {
-  extern __shared__ double yS[];
+                                                    int nbg, int nimg, int ysz, int a0, int na) {
+  extern __shared__ double lds[];
+  double *yS = lds, *tile = lds + ysz;
+  __shared__ int ired[8];
   int ent, sub;
-  if (!xcd_item(4 * nbg, nimg, ent, sub)) return;
+  if (!xcd_item(na * nbg, nimg, ent, sub)) return;
   const Img im = imgs[ent];
   const int M = im.h, N = im.w;
   const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
                                    (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
   const int rFirst = -rLast, rSize = rLast - rFirst + 1;
-  const int a = sub / nbg, r0 = (sub - a * nbg) * 256;
+  const int a = a0 + sub / nbg, r0 = (sub % nbg) * 256;
   if (r0 >= rSize) return;
   const double cs = trig[2 * a], sn = trig[2 * a + 1];
   const int xO = (N - 1) / 2, yO = (M - 1) / 2;
@@ -1656,13 +1669,15 @@ __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const doubl
   for (int p = threadIdx.x; p < M2; p += blockDim.x) yS[p] = ((-yO - 0.25) + 0.5 * (M2 - 1 - p)) * sn;
   __syncthreads();
   const int r = r0 + threadIdx.x;
-  if (r >= rSize) return;
+  const bool act = r < rSize;
   const double yS0 = yS[0], ySL = yS[M2 - 1];
   // column nn is skipped when floor(xc + yS0) < r-1 (A) or floor(xc + ySL) > r (B)
   auto predA = [&](int nn) { return (int)(radon_xc(nn, xO, cs, rFirst) + yS0) < r - 1; };
   auto predB = [&](int nn) { return (int)(radon_xc(nn, xO, cs, rFirst) + ySL) > r; };
   int lo = 0, hi = N2;
-  if (cs > 0) {  // A: true then false; B: false then true
+  if (!act) {
+    hi = 0;
+  } else if (cs > 0) {  // A: true then false; B: false then true
     int l = 0, h = N2;
     while (l < h) { const int mid = (l + h) >> 1; if (!predA(mid)) h = mid; else l = mid + 1; }
     lo = l;
@@ -1679,14 +1694,12 @@ __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const doubl
   } else if (predA(0) || predB(0)) {
     hi = 0;
   }
-  // columns nn in [lo, hi) in order, and within a column the sub-rows
-  // [m0, m1) whose bin is r or r-1 in order: the reference's per-bin
-  // accumulation order.  [m0, m1) moves monotonically with nn.
   const gptr<const double> P = to_global((const double *)im.p);
   double acc = 0.0;
   int m0 = 0, m1 = 0;
-  for (int nn = lo; nn < hi; nn++) {
-    const double xc = radon_xc(nn, xO, cs, rFirst);
+  // sub-rows [m0, m1) of column nn whose bin is r or r-1; [m0, m1) moves
+  // monotonically with nn
+  auto range = [&](int nn, double xc) {
     if (nn == lo) {
       int l = 0, h = M2;
       while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS[mid]) <= r) h = mid; else l = mid + 1; }
@@ -1701,21 +1714,67 @@ __global__ void __launch_bounds__(256) k_radon_proj(const Img *imgs, const doubl
       while (m1 > m0 && (int)(xc + yS[m1 - 1]) < r - 1) m1--;
       while (m1 < M2 && (int)(xc + yS[m1]) >= r - 1) m1++;
     }
-    const gptr<const double> col = P + (nn >> 1);
+  };
+  auto term = [&](double pixel, double xc, int mm) {
+    if (pixel) {
+      pixel *= 0.25;
+      const double rIdx = (xc + yS[mm]);
+      const int rLow = (int)rIdx;
+      const double pixelLow = pixel * (1 - rIdx + rLow);
+      if (rLow == r) acc += pixelLow;
+      else acc += pixel - pixelLow;
+    }
+  };
+  if (fabs(sn) < 0.5) {  // angle 0: lanes read adjacent pixels of a row
+    for (int nn = lo; nn < hi; nn++) {
+      const double xc = radon_xc(nn, xO, cs, rFirst);
+      range(nn, xc);
+      const gptr<const double> col = P + (nn >> 1);
 #pragma unroll 4
-    for (int mm = m0; mm < m1; mm++) {
-      double pixel = col[(unsigned)((mm >> 1) * N)];
-      if (pixel) {
-        pixel *= 0.25;
-        const double rIdx = (xc + yS[mm]);
-        const int rLow = (int)rIdx;
-        const double pixelLow = pixel * (1 - rIdx + rLow);
-        if (rLow == r) acc += pixelLow;
-        else acc += pixel - pixelLow;
+      for (int mm = m0; mm < m1; mm++) term(col[(unsigned)((mm >> 1) * N)], xc, mm);
+    }
+    if (act) proj[ent][(long)a * rSize + r] = acc;
+    return;
+  }
+  // pixel columns any bin of the block touches
+  const int xlo = block_reduce(lo < hi ? lo >> 1 : N, MinOp(), ired);
+  const int xhi = block_reduce(lo < hi ? (hi + 1) >> 1 : 0, MaxOp(), ired);
+  const int Wc = max(1, min(32, RADON_TILE / (M + 1)));  // wc * ld <= RADON_TILE (ld <= M + 1)
+  const int r1 = min(r0 + 256, rSize);
+  int nn = lo;
+  for (int x0 = xlo; x0 < xhi; x0 += Wc) {
+    const int x1 = min(x0 + Wc, N), wc = x1 - x0;
+    // rows the block's bins [r0, r1) can reach in sub-columns [2 x0, 2 x1):
+    // floor(rIdx) in [r0 - 1, r1 - 1] with rIdx = xc + yS(mm) and
+    // yS(mm) = (-yO - 0.25 + 0.5 (M2 - 1 - mm)) sn, sn > 0; two rows of margin
+    const double xa = radon_xc(2 * x0, xO, cs, rFirst), xb = radon_xc(2 * x1 - 1, xO, cs, rFirst);
+    const double xcmin = fmin(xa, xb), xcmax = fmax(xa, xb);
+    const double c0 = -yO - 0.25;
+    const double mlo = (M2 - 1) - 2 * ((r1 - xcmin) / sn - c0);
+    const double mhi = (M2 - 1) - 2 * ((r0 - 1 - xcmax) / sn - c0);
+    const int rb0 = max(0, (int)floor(fmax(mlo, -4.0) * 0.5) - 2);
+    const int rb1 = min(M, (int)floor(fmin(mhi, (double)M2 + 4) * 0.5) + 3);
+    const int band = max(0, rb1 - rb0);
+    const int ld = band | 1;  // odd stride: column reads of adjacent rows fall in different banks
+    __syncthreads();  // the previous chunk is consumed
+    for (int e = threadIdx.x; e < wc * band; e += blockDim.x) {
+      const int row = e / wc, c = e - row * wc;
+      tile[c * ld + row] = P[(long)(rb0 + row) * N + x0 + c];
+    }
+    __syncthreads();
+    const int nend = min(hi, 2 * x1);
+    for (; nn < nend; nn++) {
+      const double xc = radon_xc(nn, xO, cs, rFirst);
+      range(nn, xc);
+      const int c = (nn >> 1) - x0;
+      for (int mm = m0; mm < m1; mm++) {
+        const int row = mm >> 1;
+        const double pixel = (row >= rb0 && row < rb1) ? tile[c * ld + row - rb0] : P[(long)row * N + (nn >> 1)];
+        term(pixel, xc, mm);
       }
     }
   }
-  proj[ent][(long)a * rSize + r] = acc;
+  if (act) proj[ent][(long)a * rSize + r] = acc;
 }
 
 // yTable / xTable entries are exact multiples of 0.25, so the closed forms
@@ -1746,11 +1805,17 @@ __global__ void k_radon_hist(const Img *imgs, double *const *proj, OutRef o) {
 void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
                   int maxh, int maxw, hipStream_t s) {
   (void)maxw;
-  const size_t radon_lds = 2 * (size_t)maxh * sizeof(double);  // y*sin table
-  if (radon_lds > 65536)
-    (void)hipFuncSetAttribute((const void *)k_radon_proj, hipFuncAttributeMaxDynamicSharedMemorySize, (int)radon_lds);
+  // LDS: the y*sin table (2 maxh <= 8192 doubles), then, for the slanted
+  // angles, the chunk buffer
+  static const bool lds_ok = hipFuncSetAttribute((const void *)k_radon_proj, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                  (2 * 4096 + RADON_TILE) * (int)sizeof(double)) == hipSuccess;
+  (void)lds_ok;
+  const int ysz = 2 * maxh;
   const int nbg = (maxbins + 255) / 256;
-  k_radon_proj<<<xcd_grid(4 * nbg, nimg), 256, radon_lds, s>>>(imgs, trig, proj, nbg, nimg);
+  // angle 0 (direct reads, small LDS for occupancy), then angles 1..3
+  k_radon_proj<<<xcd_grid(nbg, nimg), 256, (size_t)ysz * sizeof(double), s>>>(imgs, trig, proj, nbg, nimg, ysz, 0, 1);
+  k_radon_proj<<<xcd_grid(3 * nbg, nimg), 256, ((size_t)ysz + RADON_TILE) * sizeof(double), s>>>(imgs, trig, proj, nbg,
+                                                                                                 nimg, ysz, 1, 3);
   k_radon_hist<<<nimg, 64, 0, s>>>(imgs, proj, o);
 }
 
