@@ -1830,8 +1830,14 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
                                hipMemcpyHostToDevice, c->stream));
     }
   }
-  HIPCHK(c, hipMallocAsync(&out_buf.p, (size_t)B.n * B.nf * sizeof(double), c->stream));
-  dout = (double *)out_buf.p;
+  // device outputs with every image accepted: the rows are written in place
+  const bool in_place = device_io && !rejected;
+  if (in_place) {
+    dout = out;
+  } else {
+    HIPCHK(c, hipMallocAsync(&out_buf.p, (size_t)B.n * B.nf * sizeof(double), c->stream));
+    dout = (double *)out_buf.p;
+  }
   for (size_t k = 0; k < idx.size(); k++) {
     const wndgpu_image &im = images[idx[k]];
     LoadDesc L;
@@ -1873,10 +1879,15 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t k = 0; k < idx.size(); k++)
       memcpy(out + (size_t)idx[k] * rowb, h.data() + k * rowb, rowb * sizeof(double));
-  } else {
-    for (size_t k = 0; k < idx.size(); k++)
-      HIPCHK(c, hipMemcpyAsync(out + (size_t)idx[k] * rowb, dout + k * rowb, rowb * sizeof(double),
+  } else if (!in_place) {
+    // one copy per run of consecutive accepted images
+    for (size_t k = 0; k < idx.size();) {
+      size_t e = k + 1;
+      while (e < idx.size() && idx[e] == idx[e - 1] + 1) e++;
+      HIPCHK(c, hipMemcpyAsync(out + (size_t)idx[k] * rowb, dout + k * rowb, (e - k) * rowb * sizeof(double),
                                hipMemcpyDeviceToDevice, c->stream));
+      k = e;
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));

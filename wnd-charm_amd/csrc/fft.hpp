@@ -8,36 +8,52 @@
 
 namespace wnd {
 
+// Exact t / d for t < 2^20, d <= 4096 as a multiply-high: with
+// M = floor((2^32 - 1) / d) + 1 (2^32 / d for powers of two),
+// floor(t M / 2^32) = floor(t / d) because t (M - 2^32/d) < 2^32 / d.
+__device__ __forceinline__ unsigned long long fft_magic(unsigned d) { return (unsigned long long)(0xFFFFFFFFu / d) + 1; }
+__device__ __forceinline__ unsigned fft_div(unsigned t, unsigned long long m) {
+  return (unsigned)(((unsigned long long)t * m) >> 32);
+}
+
 // One line per group of `tpl` threads (lane = index in the group); every
-// group of the block runs the same plan, so the barriers are uniform.
-__device__ inline void fft_lds_group(double2 *a, double2 *b, const FftPlan &pl, int lane, int tpl, double2 **res) {
+// group of the block runs the same plan, so the barriers are uniform.  tw:
+// the plan's twiddles, in global memory or staged in LDS by the caller.
+template <class TW>
+__device__ inline void fft_lds_group_tw(double2 *a, double2 *b, const FftPlan &pl, TW tw, int lane, int tpl,
+                                        double2 **res) {
   const int n = pl.n;
   int Ns = 1;
   double2 *src = a, *dst = b;
   for (int s = 0; s < pl.nst; s++) {
     const int R = pl.radix[s];
     const int nr = n / R, step = n / (Ns * R);
+    const unsigned long long mnr = fft_magic(nr), mns = fft_magic(Ns);
     for (int t = lane; t < n; t += tpl) {
-      const int j = t % nr, q = t / nr;
-      const int k = j % Ns;
+      const int q = fft_div(t, mnr), j = t - q * nr;
+      const int jq = fft_div(j, mns), k = j - jq * Ns;
       const int base = k * step + q * nr;
       double re = 0, im = 0;
-      long idx = 0;
+      int idx = 0;
       for (int r = 0; r < R; r++) {
         const double2 v = src[j + r * nr];
-        const double2 w = pl.tw[idx];
+        const double2 w = tw[idx];
         re += v.x * w.x - v.y * w.y;
         im += v.x * w.y + v.y * w.x;
         idx += base;
         if (idx >= n) idx -= n;
       }
-      dst[(j / Ns) * Ns * R + k + q * Ns] = make_double2(re, im);
+      dst[jq * Ns * R + k + q * Ns] = make_double2(re, im);
     }
     __syncthreads();
     double2 *tt = src; src = dst; dst = tt;
     Ns *= R;
   }
   *res = src;
+}
+
+__device__ inline void fft_lds_group(double2 *a, double2 *b, const FftPlan &pl, int lane, int tpl, double2 **res) {
+  fft_lds_group_tw(a, b, pl, pl.tw, lane, tpl, res);
 }
 
 __device__ inline void fft_lds(double2 *a, double2 *b, const FftPlan &pl, double2 **res) {

@@ -7,6 +7,7 @@
 #include "color.hpp"
 #include "kernels.hpp"
 #include "blocksort.hpp"
+#include <algorithm>
 #include "fft.hpp"
 
 namespace wnd {
@@ -298,9 +299,13 @@ __global__ void __launch_bounds__(256) k_fft_rows(const Img *in, double2 *const 
   if ((int)blockIdx.x * L >= a.h) return;
   double2 *b0 = sh + (long)line * 2 * pl.n, *b1 = b0 + pl.n;
   for (int x = lane; x < a.w; x += tpl) b0[x] = make_double2(row < a.h ? a.p[(long)row * a.w + x] : 0.0, 0.0);
+  double2 *twl = sh + (long)L * 2 * pl.n;  // twiddles staged after the lines (n <= 1024)
+  if (pl.n <= 1024)
+    for (int k = threadIdx.x; k < pl.n; k += blockDim.x) twl[k] = pl.tw[k];
   __syncthreads();
   double2 *r;
-  fft_lds_group(b0, b1, pl, lane, tpl, &r);
+  if (pl.n <= 1024) fft_lds_group_tw(b0, b1, pl, (const double2 *)twl, lane, tpl, &r);
+  else fft_lds_group(b0, b1, pl, lane, tpl, &r);
   if (row < a.h) {
     double2 *o = f1[blockIdx.y] + (long)row * a.w;
     for (int x = lane; x < a.w; x += tpl) o[x] = r[x];
@@ -322,10 +327,14 @@ __global__ void __launch_bounds__(256) k_fft_cols(const Img *in, double2 *const 
     const int y = e / L, l = e - y * L, x = x0 + l;
     sh[(long)l * 2 * n + y] = x < a.w ? src[(long)y * a.w + x] : make_double2(0.0, 0.0);
   }
+  double2 *twl = sh + (long)L * 2 * n;  // twiddles staged after the lines (n <= 1024)
+  if (n <= 1024)
+    for (int k = threadIdx.x; k < n; k += blockDim.x) twl[k] = pl.tw[k];
   __syncthreads();
   double2 *b0 = sh + (long)line * 2 * n, *b1 = b0 + n;
   double2 *r;
-  fft_lds_group(b0, b1, pl, lane, tpl, &r);
+  if (n <= 1024) fft_lds_group_tw(b0, b1, pl, (const double2 *)twl, lane, tpl, &r);
+  else fft_lds_group(b0, b1, pl, lane, tpl, &r);
   const long roff = (r == b1) ? n : 0;  // the same buffer of the pair for every line
   const int hh = a.h / 2 + 1;
   for (int e = threadIdx.x; e < hh * L; e += blockDim.x) {
@@ -539,7 +548,11 @@ void launch_cheb_gemm(const GemmDesc *d, int nimg, int maxM, int maxN, hipStream
 void launch_fft(const Img *in, double2 *const *f1, const FftPlan *plw, const FftPlan *plh, const Img *out, int nimg,
                 int maxh, int maxw, hipStream_t s) {
   const int nmax = maxh > maxw ? maxh : maxw;
-  const size_t lds = 2 * (size_t)(nmax > 1024 ? nmax : 1024) * sizeof(double2);
+  // per block: fft_lines(n) lines of 2n entries, then n twiddles for
+  // n <= 1024; the largest over n <= nmax
+  const int ent = nmax <= 256 ? 4 * 2 * 256 + 256 : nmax <= 512 ? 2 * 2 * 512 + 512 : nmax <= 1024 ? 2 * 1024 + 1024
+                                                                                                      : std::max(2 * nmax, 3072);
+  const size_t lds = (size_t)ent * sizeof(double2);
   static const bool attr = [] {
     (void)hipFuncSetAttribute((const void *)k_fft_rows, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4096 * 16);
     (void)hipFuncSetAttribute((const void *)k_fft_cols, hipFuncAttributeMaxDynamicSharedMemorySize, 2 * 4096 * 16);
