@@ -215,18 +215,45 @@ def tiles_main(args):
     ctx = wndcharm._context()
     status = np.zeros(args.batch, np.int32)
 
+    times = {'host_ms': 0.0, 'table_ms': 0.0, 'device_ms': 0.0}
+
     def features(group, feats):
+        """features of the tiles in calls of --batch; before each call the
+        tables of the next call are queued on the library's host threads, so
+        that host work overlaps the device's"""
         dev = [torch.from_numpy(t.view(np.int16)).cuda() for t in group]
-        for b0 in range(0, len(dev), args.batch):
+
+        def structs(b0):
             chunk = dev[b0:b0 + args.batch]
             st = (wndcharm._Image * len(chunk))()
             for i, t in enumerate(chunk):
                 st[i] = wndcharm._Image(ctypes.c_void_p(t.data_ptr()), t.shape[0], t.shape[1], t.shape[1], 0)
-            rc = lib.wndgpu_compute_device(ctx, plan.handle, st, len(chunk),
+            return st, len(chunk)
+
+        for k in times:
+            times[k] = 0.0
+        starts = list(range(0, len(dev), args.batch))
+        if not args.no_prefetch:
+            st0, n0 = structs(0)
+            if lib.wndgpu_prefetch_tables(ctx, plan.handle, st0, n0) != 0:
+                raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
+        for j, b0 in enumerate(starts):
+            st, nc = structs(b0)
+            if j + 1 < len(starts) and not args.no_prefetch:
+                st1, n1 = structs(starts[j + 1])
+                if lib.wndgpu_prefetch_tables(ctx, plan.handle, st1, n1) != 0:
+                    raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
+            rc = lib.wndgpu_compute_device(ctx, plan.handle, st, nc,
                                            ctypes.c_void_p(feats[b0:].data_ptr()), ctypes.c_void_p(status.ctypes.data))
             if rc != 0:
                 raise RuntimeError(lib.wndgpu_last_error(ctx).decode())
-            print('tiles %d..%d done' % (b0, b0 + len(chunk) - 1), file=sys.stderr, flush=True)
+            h, tb, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+            lib.wndgpu_last_call_times(ctx, ctypes.byref(h), ctypes.byref(tb), ctypes.byref(d))
+            times['host_ms'] += h.value
+            times['table_ms'] += tb.value
+            times['device_ms'] += d.value
+            print('tiles %d..%d done (host %.0f ms, tables %.0f ms, device %.0f ms)'
+                  % (b0, b0 + nc - 1, h.value, tb.value, d.value), file=sys.stderr, flush=True)
 
     # training space (setup, not timed)
     t0 = time.time()
@@ -245,6 +272,7 @@ def tiles_main(args):
         pred = torch.argmax(model.probabilities(ftest), 1) + 1
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
+    test_times = dict(times)
     accuracy = float((pred == truth).double().mean().item())
     # the short-set plain per-image form on the same GPU, for comparison
     imgs, _ = synthetic.config_images('short_128')
@@ -264,6 +292,12 @@ def tiles_main(args):
         'megapixels_per_s': round(pix / el / 1e6, 3),
         'wnd5_accuracy': round(accuracy, 4),
         'training_features_tiles_per_s': round(len(train) / t_train, 2),
+        'per_tile_ms': {'host': round(test_times['host_ms'] / n, 3), 'tables': round(test_times['table_ms'] / n, 3),
+                        'device': round(test_times['device_ms'] / n, 3),
+                        'note': 'host: issue path of each call (sizing, tables, descriptors); tables: its '
+                                'part building or awaiting size-dependent tables; device: waiting for the '
+                                'device after the launches are issued'},
+        'table_prefetch': not args.no_prefetch,
         'generator_s': round(t_gen, 1),
         'short_set_plain_per_image_128': round(plain, 3),
         'vs_short_set_plain_per_image': round(n / el / plain, 2)}))
@@ -516,6 +550,8 @@ def main():
                     help='only time wndgpu_compute with host image and feature buffers (PCIe-inclusive rate)')
     ap.add_argument('--tiles', type=int, default=0,
                     help='the uneven-tiles configuration over this many tiles (batches of --batch)')
+    ap.add_argument('--no-prefetch', action='store_true',
+                    help='tiles: build each call\'s size-dependent tables inside the call, not ahead of it')
     ap.add_argument('--mixed', action='store_true',
                     help='the mixed-precision configuration (FP64 / FP32 / BF16x3 basis contractions)')
     ap.add_argument('--grad', action='store_true',

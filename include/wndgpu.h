@@ -328,6 +328,24 @@ int wndgpu_last_haralick_work(wndgpu_ctx *ctx, int *tones, double *schur_flops, 
  * widens the certification bound.  No counterpart in the reference. */
 int wndgpu_last_cf_open(wndgpu_ctx *ctx, int *open, int cap);
 
+/* Size-dependent tables ahead of time: runs the sizing pass of a batch that
+ * a later wndgpu_compute / wndgpu_compute_device will be given and queues the
+ * Chebyshev bases and Chebyshev-Fourier geometry tables it lacks (glibc libm,
+ * as ChebyshevFourier.cpp:59-103 and chebyshev.cpp compute them) on a pool
+ * of up to 16 host threads, then returns.  The compute call that needs a
+ * queued table waits for that table only; tables no compute takes within two
+ * calls are dropped.  Pixels are not read (images only need their sizes and
+ * types).  Meant for a loop that prefetches batch k+1 before computing batch
+ * k, so the host work overlaps the device's.  No counterpart in the reference
+ * (its tables are per-call locals). */
+int wndgpu_prefetch_tables(wndgpu_ctx *ctx, const wndgpu_plan *plan, const wndgpu_image *images, int n_images);
+
+/* Wall times (ms) of the last compute call: host_ms from entry until every
+ * launch was issued (sizing, tables, descriptors), table_ms the part of it
+ * spent building or awaiting size-dependent tables, device_ms waiting for the
+ * device afterwards. */
+int wndgpu_last_call_times(const wndgpu_ctx *ctx, double *host_ms, double *table_ms, double *device_ms);
+
 /* ---- files (host only: no context, no GPU).  Errors: WNDGPU_E_IO /
  * WNDGPU_E_FORMAT / WNDGPU_E_ARG with a message in wndgpu_io_error() of the
  * calling thread. */

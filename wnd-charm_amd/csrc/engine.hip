@@ -19,9 +19,15 @@
 #include <map>
 #include <memory>
 #include <atomic>
+#include <chrono>
+#include <climits>
+#include <condition_variable>
+#include <deque>
+#include <future>
 #include <mutex>
 #include <set>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -209,6 +215,48 @@ struct DevTable {
   long used = 0;  // last batch that read it
 };
 
+// A fixed pool of host threads for size-dependent tables computed ahead of
+// the batch that needs them (wndgpu_prefetch_tables).  Tasks only read their
+// own arguments and fulfil a promise: they never touch the context.
+struct TablePool {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<std::function<void()>> q;
+  std::vector<std::thread> th;
+  bool stop = false;
+  explicit TablePool(unsigned n) {
+    for (unsigned t = 0; t < n; t++)
+      th.emplace_back([this]() {
+        for (;;) {
+          std::function<void()> f;
+          {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [this]() { return stop || !q.empty(); });
+            if (stop && q.empty()) return;
+            f = std::move(q.front());
+            q.pop_front();
+          }
+          f();
+        }
+      });
+  }
+  void push(std::function<void()> f) {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      q.push_back(std::move(f));
+    }
+    cv.notify_one();
+  }
+  ~TablePool() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto &t : th) t.join();
+  }
+};
+
 struct CfDev {
   DevTable kk, tn, fc, fs;
   int np = 0, NN = 0;
@@ -262,7 +310,20 @@ struct wndgpu_ctx {
   // freed between batches.
   std::set<std::pair<int, int>> need_cheb, need_cf;
   long batch_id = 0;
-  size_t table_budget = (size_t)4 << 30;
+  size_t table_budget = (size_t)16 << 30;
+  // tables queued by wndgpu_prefetch_tables: taken by the batch that needs
+  // them, dropped two batches after they were queued otherwise
+  std::unique_ptr<TablePool> pool;
+  template <class T>
+  struct Pending {
+    std::future<T> f;
+    long queued;
+  };
+  std::map<std::pair<int, int>, Pending<DevTable>> pend_cheb;
+  std::map<std::pair<int, int>, Pending<CfDev>> pend_cf;
+  hipStream_t ustream = nullptr;  // non-blocking stream of the pool's table uploads
+  // wall times of the last compute call (wndgpu_last_call_times)
+  double t_host_ms = 0, t_table_ms = 0, t_device_ms = 0;
   // Gabor kernels for the reduced-precision forms: FP32, bf16 hi / mid / lo
   // planes, and per filter the sum of |re| + |im| of the taps (error bounds)
   DevTable gabor_f32, gabor_bhi, gabor_bmd, gabor_blo, gabor_wsum;
@@ -322,10 +383,14 @@ static void release_check(hipError_t e, const char *what) {
   if (e != hipSuccess) fprintf(stderr, "wndgpu: %s failed: %s\n", what, hipGetErrorString(e));
 }
 
+// Tables come from the device's stream-ordered pool (hipMallocAsync), so
+// evicting one is a hipFreeAsync into the pool, not a synchronising hipFree.
+// Uploads run while `stream` is idle (before a batch's launches are issued).
 static int upload(wndgpu_ctx *c, DevTable &t, const void *src, size_t bytes) {
-  HIPCHK(c, hipMalloc(&t.p, bytes ? bytes : 8));
+  HIPCHK(c, hipMallocAsync(&t.p, bytes ? bytes : 8, c->stream));
   t.bytes = bytes;
-  if (bytes) HIPCHK(c, hipMemcpy(t.p, src, bytes, hipMemcpyHostToDevice));
+  if (bytes) HIPCHK(c, hipMemcpyAsync(t.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return 0;
 }
 
@@ -452,7 +517,77 @@ static void drop_graphs(wndgpu_ctx *c) {
 // computed on up to 16 host threads (glibc libm, as the reference), then
 // uploaded; afterwards tables no batch has read recently are freed while the
 // cache holds more than table_budget bytes.
+// Prefetched tables queued before batch `before` (all for LONG_MAX): their
+// tasks are awaited and their device copies freed.
+static void drop_pending(wndgpu_ctx *c, long before) {
+  auto fr = [c](DevTable &t) {
+    if (t.p) release_check(hipFreeAsync(t.p, c->stream), "hipFreeAsync");
+    t.p = nullptr;
+  };
+  for (auto it = c->pend_cheb.begin(); it != c->pend_cheb.end();) {
+    if (it->second.queued >= before) { ++it; continue; }
+    try { DevTable t = it->second.f.get(); fr(t); } catch (const std::exception &) {}
+    it = c->pend_cheb.erase(it);
+  }
+  for (auto it = c->pend_cf.begin(); it != c->pend_cf.end();) {
+    if (it->second.queued >= before) { ++it; continue; }
+    try {
+      CfDev d = it->second.f.get();
+      fr(d.kk); fr(d.tn); fr(d.fc); fr(d.fs);
+    } catch (const std::exception &) {}
+    it = c->pend_cf.erase(it);
+  }
+}
+
+// A host table copied to the device on the pool's upload stream (runs on a
+// pool thread: no context access; errors are thrown to the awaiting call).
+static DevTable upload_on(int device, hipStream_t st, const void *src, size_t bytes) {
+  auto chk = [](hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+  };
+  DevTable t;
+  chk(hipSetDevice(device), "hipSetDevice");
+  chk(hipMallocAsync(&t.p, bytes ? bytes : 8, st), "hipMallocAsync");
+  t.bytes = bytes;
+  hipError_t e = bytes ? hipMemcpyAsync(t.p, src, bytes, hipMemcpyHostToDevice, st) : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    release_check(hipFreeAsync(t.p, st), "hipFreeAsync");
+    chk(e, "hipMemcpyAsync");
+  }
+  return t;
+}
+
+static CfDev upload_cf_on(int device, hipStream_t st, const CfTable &t) {
+  CfDev d;
+  d.np = (int)t.kk.size();
+  d.NN = t.NN;
+  try {
+    d.kk = upload_on(device, st, t.kk.data(), t.kk.size() * sizeof(int));
+    d.tn = upload_on(device, st, t.tn.data(), t.tn.size() * sizeof(double));
+    d.fc = upload_on(device, st, t.fc.data(), t.fc.size() * sizeof(double));
+    d.fs = upload_on(device, st, t.fs.data(), t.fs.size() * sizeof(double));
+  } catch (...) {
+    for (DevTable *x : {&d.kk, &d.tn, &d.fc, &d.fs})
+      if (x->p) release_check(hipFreeAsync(x->p, st), "hipFreeAsync");
+    throw;
+  }
+  return d;
+}
+
+static double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+static int prepare_tables_impl(wndgpu_ctx *c);
 static int prepare_tables(wndgpu_ctx *c) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int r = prepare_tables_impl(c);
+  c->t_table_ms += ms_since(t0);
+  return r;
+}
+
+static int prepare_tables_impl(wndgpu_ctx *c) {
   c->batch_id++;
   std::vector<std::pair<int, int>> cheb_keys, cf_keys;
   for (const auto &k : c->need_cheb)
@@ -463,9 +598,16 @@ static int prepare_tables(wndgpu_ctx *c) {
   if (nwork) {
     std::vector<std::vector<double>> cheb_v(cheb_keys.size());
     std::vector<CfTable> cf_v(cf_keys.size());
+    std::vector<DevTable> cheb_d(cheb_keys.size());  // prefetched: already on the device
+    std::vector<CfDev> cf_d(cf_keys.size());
+    // keys already queued by a prefetch are awaited, the rest computed here
+    std::vector<char> queued(nwork, 0);
+    for (size_t j = 0; j < cheb_keys.size(); j++) queued[j] = c->pend_cheb.count(cheb_keys[j]) != 0;
+    for (size_t j = 0; j < cf_keys.size(); j++) queued[cheb_keys.size() + j] = c->pend_cf.count(cf_keys[j]) != 0;
     std::atomic<size_t> next{0};
     auto worker = [&]() {
       for (size_t j = next++; j < nwork; j = next++) {
+        if (queued[j]) continue;
         if (j < cheb_keys.size()) cheb_v[j] = cheb_basis(cheb_keys[j].first, cheb_keys[j].second);
         else cf_v[j - cheb_keys.size()] = cf_table(cf_keys[j - cheb_keys.size()].first, cf_keys[j - cheb_keys.size()].second);
       }
@@ -475,13 +617,32 @@ static int prepare_tables(wndgpu_ctx *c) {
     for (size_t t = 1; t < nt; t++) th.emplace_back(worker);
     worker();
     for (auto &t : th) t.join();
+    std::string perr;
+    for (size_t j = 0; j < cheb_keys.size(); j++)
+      if (queued[j]) {
+        try { cheb_d[j] = c->pend_cheb[cheb_keys[j]].f.get(); } catch (const std::exception &e) { perr = e.what(); }
+        c->pend_cheb.erase(cheb_keys[j]);
+      }
+    for (size_t j = 0; j < cf_keys.size(); j++)
+      if (queued[cheb_keys.size() + j]) {
+        try { cf_d[j] = c->pend_cf[cf_keys[j]].f.get(); } catch (const std::exception &e) { perr = e.what(); }
+        c->pend_cf.erase(cf_keys[j]);
+      }
+    // the cache owns every awaited table from here; a failed task left none
+    for (size_t j = 0; j < cheb_keys.size(); j++)
+      if (queued[j] && cheb_d[j].p) c->cheb.emplace(cheb_keys[j], cheb_d[j]);
+    for (size_t j = 0; j < cf_keys.size(); j++)
+      if (queued[cheb_keys.size() + j] && cf_d[j].kk.p) c->cf.emplace(cf_keys[j], cf_d[j]);
+    if (!perr.empty()) return fail(c, WNDGPU_E_NOMEM, "prefetched table: %s", perr.c_str());
     int r;
     for (size_t j = 0; j < cheb_keys.size(); j++) {
+      if (queued[j]) continue;
       DevTable t;
       if ((r = upload(c, t, cheb_v[j].data(), cheb_v[j].size() * sizeof(double)))) return r;
       c->cheb.emplace(cheb_keys[j], t);
     }
     for (size_t j = 0; j < cf_keys.size(); j++) {
+      if (queued[cheb_keys.size() + j]) continue;
       const CfTable &t = cf_v[j];
       CfDev d;
       d.np = (int)t.kk.size();
@@ -493,10 +654,12 @@ static int prepare_tables(wndgpu_ctx *c) {
       c->cf.emplace(cf_keys[j], d);
     }
   }
-  for (const auto &k : c->need_cheb) c->cheb[k].used = c->batch_id;
-  for (const auto &k : c->need_cf) c->cf[k].used = c->batch_id;
+  for (const auto &k : c->need_cheb) c->cheb[k].used = std::max(c->cheb[k].used, c->batch_id);
+  for (const auto &k : c->need_cf) c->cf[k].used = std::max(c->cf[k].used, c->batch_id);
   c->need_cheb.clear();
   c->need_cf.clear();
+  // prefetched tables no batch took: awaited and freed
+  drop_pending(c, c->batch_id - 2);
   // eviction: previous batches have completed (every compute call waits for
   // its streams), so tables this batch does not read can be freed
   size_t total = 0;
@@ -515,12 +678,12 @@ static int prepare_tables(wndgpu_ctx *c) {
     if (which == 0) {
       DevTable &t = c->cheb[key];
       total -= t.bytes;
-      HIPCHK(c, hipFree(t.p));
+      HIPCHK(c, hipFreeAsync(t.p, c->stream));
       c->cheb.erase(key);
     } else {
       CfDev &d = c->cf[key];
       total -= d.bytes();
-      for (DevTable *t : {&d.kk, &d.tn, &d.fc, &d.fs}) HIPCHK(c, hipFree(t->p));
+      for (DevTable *t : {&d.kk, &d.tn, &d.fc, &d.fs}) HIPCHK(c, hipFreeAsync(t->p, c->stream));
       c->cf.erase(key);
     }
   }
@@ -1599,6 +1762,16 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     delete c;
     return WNDGPU_E_DEVICE;
   }
+  // tables are freed into the device's stream-ordered pool: keep that memory
+  // in the pool for the next tables instead of returning it at every
+  // synchronisation (the table budget bounds what is live)
+  {
+    hipMemPool_t mp;
+    if (hipDeviceGetDefaultMemPool(&mp, device) == hipSuccess) {
+      uint64_t keep = UINT64_MAX;
+      release_check(hipMemPoolSetAttribute(mp, hipMemPoolAttrReleaseThreshold, &keep), "hipMemPoolSetAttribute");
+    }
+  }
   c->serial = getenv("WNDGPU_SERIAL") != nullptr;
   if (const char *cs = getenv("WNDGPU_CF_SLACK")) c->cf_slack = std::max(1.0, atof(cs));
   if (const char *gm = getenv("WNDGPU_GRAPH")) c->graph_mode = std::min(2, std::max(0, atoi(gm)));
@@ -1613,20 +1786,26 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   release_check(hipSetDevice(c->device), "hipSetDevice");
   release_check(hipStreamSynchronize(c->side), "hipStreamSynchronize");
   release_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  c->pool.reset();  // runs the queued table tasks to the end and joins the threads
+  drop_pending(c, LONG_MAX);
+  if (c->ustream) release_check(hipStreamDestroy(c->ustream), "hipStreamDestroy");
   drop_graphs(c);
   if (c->gev0) release_check(hipEventDestroy(c->gev0), "hipEventDestroy");
   if (c->gev1) release_check(hipEventDestroy(c->gev1), "hipEventDestroy");
-  auto fr = [](DevTable &t) {
-    if (t.p) release_check(hipFree(t.p), "hipFree");
+  auto fr = [c](DevTable &t) {  // tables from upload(): stream-ordered pool
+    if (t.p) release_check(hipFreeAsync(t.p, c->stream), "hipFreeAsync");
     t.p = nullptr;
   };
   for (auto &kv : c->cheb) fr(kv.second);
   for (auto &kv : c->fft) fr(kv.second.first);
   for (auto &kv : c->cf) { fr(kv.second.kk); fr(kv.second.tn); fr(kv.second.fc); fr(kv.second.fs); }
   for (auto &kv : c->dspos) fr(kv.second);
+  for (auto &kv : c->dsdense) fr(kv.second);
   for (auto &kv : c->comb) fr(kv.second);
-  fr(c->gabor); fr(c->zern); fr(c->rtrig); fr(c->color_lut);
+  fr(c->gabor); fr(c->zern); fr(c->rtrig);
   fr(c->gabor_f32); fr(c->gabor_bhi); fr(c->gabor_bmd); fr(c->gabor_blo); fr(c->gabor_wsum);
+  release_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  if (c->color_lut.p) release_check(hipFree(c->color_lut.p), "hipFree");
   for (hipEvent_t e : c->evpool) release_check(hipEventDestroy(e), "hipEventDestroy");
   if (c->dev) release_check(hipFree(c->dev), "hipFree");
   if (c->hdesc) release_check(hipFree(c->hdesc), "hipFree");
@@ -1864,7 +2043,11 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   B.pre = std::move(pre);
   B.smp = std::move(smp);
   B.out = dout;
+  c->t_table_ms = 0;
+  const auto t_run = std::chrono::steady_clock::now();
   int r = run_batch(c, plan, B, getenv("WNDGPU_TIMING") != nullptr);
+  c->t_host_ms = ms_since(t_run);
+  const auto t_dev = std::chrono::steady_clock::now();
   if (r) {
     // work already issued on either stream may still read the staging buffers
     (void)hipStreamSynchronize(c->side);
@@ -1879,6 +2062,7 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t k = 0; k < idx.size(); k++)
       memcpy(out + (size_t)idx[k] * rowb, h.data() + k * rowb, rowb * sizeof(double));
+    c->t_device_ms = ms_since(t_dev);
   } else if (!in_place) {
     // one copy per run of consecutive accepted images
     for (size_t k = 0; k < idx.size();) {
@@ -1891,6 +2075,7 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (device_io) c->t_device_ms = ms_since(t_dev);
   for (size_t k = 0; k < c->tnames.size(); k++) {
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[2 * k], c->evpool[2 * k + 1]));
@@ -2295,6 +2480,92 @@ int wndgpu_last_haralick_work(wndgpu_ctx *c, int *tones, double *schur_flops, in
   if (k > 0 && schur_flops)
     HIPCHK(c, hipMemcpy(schur_flops, c->har_work, 4 * (size_t)k * sizeof(double), hipMemcpyDeviceToHost));
   return c->har_n;
+}
+
+int wndgpu_prefetch_tables(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_image *images, int n) {
+  if (!c || !plan || !images || n <= 0) return fail(c, WNDGPU_E_ARG, "null argument or empty batch");
+  HIPCHK(c, hipSetDevice(c->device));
+  // the images a compute call would accept, as its sizing pass sees them
+  Batch B;
+  B.nf = plan->n_features;
+  for (int i = 0; i < n; i++) {
+    const wndgpu_image &im = images[i];
+    if (im.dtype < 0 || im.dtype > 5 || !im.data || im.stride < im.width) continue;
+    if (plan->color && im.dtype < 4) continue;
+    if (im.height < WNDGPU_MIN_SIDE || im.width < WNDGPU_MIN_SIDE) continue;
+    if (im.height > WNDGPU_MAX_SIDE || im.width > WNDGPU_MAX_SIDE) continue;
+    B.h0.push_back(im.height);
+    B.w0.push_back(im.width);
+    LoadDesc L;
+    L.src = im.data;
+    L.h = im.height;
+    L.w = im.width;
+    L.stride = im.stride;
+    L.dtype = im.dtype;
+    L.hsv = nullptr;
+    L.range = nullptr;
+    B.load.push_back(L);
+  }
+  B.n = (int)B.load.size();
+  if (!B.n) return WNDGPU_OK;
+  int r;
+  if ((r = ensure_tables(c))) return r;
+  std::vector<LaunchItem> launches;
+  size_t first_feature = 0;
+  Arena A{c, true};
+  c->need_cheb.clear();
+  c->need_cf.clear();
+  r = build_batch(c, plan, B, A, launches, &first_feature);
+  if (!r) {
+    if (!c->ustream) HIPCHK(c, hipStreamCreateWithFlags(&c->ustream, hipStreamNonBlocking));
+    if (!c->pool) c->pool.reset(new TablePool(std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    const int dev = c->device;
+    hipStream_t us = c->ustream;
+    // tables already cached stay until the batch after the next has run
+    for (const auto &k : c->need_cheb)
+      if (c->cheb.count(k)) c->cheb[k].used = std::max(c->cheb[k].used, c->batch_id + 2);
+    for (const auto &k : c->need_cf)
+      if (c->cf.count(k)) c->cf[k].used = std::max(c->cf[k].used, c->batch_id + 2);
+    for (const auto &k : c->need_cheb)
+      if (!c->cheb.count(k) && !c->pend_cheb.count(k)) {
+        auto pr = std::make_shared<std::promise<DevTable>>();
+        c->pend_cheb[k] = {pr->get_future(), c->batch_id};
+        const int a = k.first, b = k.second;
+        c->pool->push([pr, a, b, dev, us]() {
+          try {
+            const std::vector<double> v = cheb_basis(a, b);
+            pr->set_value(upload_on(dev, us, v.data(), v.size() * sizeof(double)));
+          } catch (...) {
+            pr->set_exception(std::current_exception());
+          }
+        });
+      }
+    for (const auto &k : c->need_cf)
+      if (!c->cf.count(k) && !c->pend_cf.count(k)) {
+        auto pr = std::make_shared<std::promise<CfDev>>();
+        c->pend_cf[k] = {pr->get_future(), c->batch_id};
+        const int a = k.first, b = k.second;
+        c->pool->push([pr, a, b, dev, us]() {
+          try {
+            const CfTable t = cf_table(a, b);
+            pr->set_value(upload_cf_on(dev, us, t));
+          } catch (...) {
+            pr->set_exception(std::current_exception());
+          }
+        });
+      }
+  }
+  c->need_cheb.clear();
+  c->need_cf.clear();
+  return r;
+}
+
+int wndgpu_last_call_times(const wndgpu_ctx *c, double *host_ms, double *table_ms, double *device_ms) {
+  if (!c) return WNDGPU_E_ARG;
+  if (host_ms) *host_ms = c->t_host_ms;
+  if (table_ms) *table_ms = c->t_table_ms;
+  if (device_ms) *device_ms = c->t_device_ms;
+  return WNDGPU_OK;
 }
 
 int wndgpu_last_cf_open(wndgpu_ctx *c, int *open, int cap) {

@@ -80,6 +80,8 @@ def _load():
         'wndgpu_last_node_times': (_I, [_P, _P, _P, _I]),
         'wndgpu_last_haralick_work': (_I, [_P, _P, _P, _I]),
         'wndgpu_last_cf_open': (_I, [_P, _P, _I]),
+        'wndgpu_prefetch_tables': (_I, [_P, _P, _P, _I]),
+        'wndgpu_last_call_times': (_I, [_P, _P, _P, _P]),
         'wndgpu_sampling_n_samples': (_I, [_P]),
         'wndgpu_compute_sampled': (_I, [_P, _P, ctypes.POINTER(_Image), _I, _P, _P, _P]),
         'wndgpu_io_error': (ctypes.c_char_p, []),
@@ -551,6 +553,28 @@ def last_haralick_work():
     flops = np.zeros((max(n, 1), 4))
     lib.wndgpu_last_haralick_work(_context(), _ptr(tones), _ptr(flops), n)
     return tones[:n], flops[:n]
+
+
+def prefetch_tables(images, plan=None):
+    """Queues the size-dependent tables a later compute_features(images,
+    plan) will need on the library's host threads and returns at once (the
+    pixels are not read)."""
+    if plan is None:
+        plan = StdFeatureComputationPlans.getFeatureSetLong()
+    keep = []
+    structs = (_Image * len(images))()
+    for i, im in enumerate(images):
+        a, st = _as_image_struct(np.asarray(im))
+        keep.append(a)
+        structs[i] = st
+    _check(_load().wndgpu_prefetch_tables(_context(), plan.handle, structs, len(images)), 'prefetch_tables')
+
+
+def last_call_times():
+    """(host_ms, table_ms, device_ms) of the last compute call."""
+    v = [ctypes.c_double() for _ in range(3)]
+    _check(_load().wndgpu_last_call_times(_context(), *[ctypes.byref(x) for x in v]), 'last_call_times')
+    return tuple(x.value for x in v)
 
 
 def last_cf_open():
