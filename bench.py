@@ -176,27 +176,30 @@ MIXED_GROUPS = [('Chebyshev Coefficients ()', 32), ('Zernike Coefficients ()', 7
                 ('Chebyshev-Fourier Coefficients ()', 32), ('Gabor Textures ()', 7)]
 
 
-def _tile(k):
+def _tile(kc):
+    """tile k of the generator (seed base + k) drawn for class c"""
     from wndcharm import synthetic
+    k, c = kc
     base, periods, per_class, _ = synthetic.CONFIGS['tiles_mixed']
     r = np.random.Generator(np.random.PCG64(base * 1000003 + k))
     shp = (int(r.integers(64, 1025)), int(r.integers(64, 1025)))
-    return synthetic.image(base + k, periods[k // per_class], shp)
+    return synthetic.image(base + k, periods[c], shp)
 
 
 def tiles_main(args):
     """The uneven-tiles configuration: --tiles test tiles (the same number from
     each of the 8 classes) through the long set in batches of --batch, then
     WND5-classified against a training space of --train-per-class tiles per
-    class (Fisher weights, top 15%).  Tile k of the configuration belongs to
-    class k // 1250; training tiles are the first of each class's 1250, test
-    tiles follow from index 500 of each class."""
+    class (Fisher weights, top 15%).  The configuration's 10,000 tiles are
+    k = 0..9999 of the generator, tile k of class k // 1250 (the first
+    --tiles / 8 of each class are taken); training tiles are k = 10000 + 500 c
+    + j of class c, so no training tile is a test tile."""
     from concurrent.futures import ThreadPoolExecutor
-    per_test = max(1, args.tiles // 8)
+    per_test = max(1, min(1250, args.tiles // 8))
     n = 8 * per_test
     per_train = min(args.train_per_class, 500)
-    test_k = [1250 * c + 500 + j for c in range(8) for j in range(per_test)]
-    train_k = [1250 * c + j for c in range(8) for j in range(per_train)]
+    test_k = [(1250 * c + j, c) for c in range(8) for j in range(per_test)]
+    train_k = [(10000 + 500 * c + j, c) for c in range(8) for j in range(per_train)]
     # the generator is numpy on the host (it releases the GIL): 16 threads, no child processes
     t0 = time.time()
     with ThreadPoolExecutor(16) as ex:
