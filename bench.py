@@ -19,6 +19,9 @@ Also reported:
                 rocprofv3 --pmc summary committed under profiles/ when present.
   stage_ms_serial  per-launch times of the serial pass (the kernels' own
                 times, comparable with rocprofv3 under WNDGPU_SERIAL=1).
+  cf_entries_recomputed  Chebyshev-Fourier entries of the last timed step
+                whose matrix-core histogram was left uncertified and was
+                recomputed in the reference's order.
   cpu_baseline  the FP64 C++ oracle, one image per thread on the host
                 cores (threads of this process, at most 16: one GPU's share
                 of the host), on a bounded sample of the same images.
@@ -637,6 +640,9 @@ def main():
 
     tones, schur = wndcharm.last_haralick_work()
     work = {'tones': tones, 'schur': schur, 'root_sizes': [im.shape for im in imgs]}
+    # Chebyshev-Fourier entries of the last step whose matrix-core histogram
+    # was not certified and was recomputed in the reference's order
+    cf_open = wndcharm.last_cf_open()
     # serial pass: a second context with every launch on one stream, per-launch
     # event times = the kernels' own times (no overlap with the side stream)
     os.environ['WNDGPU_SERIAL'] = '1'
@@ -750,6 +756,7 @@ def main():
         'forms': forms,
         'stage_ms_serial': {k: round(v, 3) for k, v in top[:14]},
         'serial_pass_ms': round(sum(serial.values()), 3),
+        'cf_entries_recomputed': {'open': int(cf_open.sum()), 'of': int(cf_open.size)},
     }
     if not args.no_cpu:
         line['cpu_baseline'] = cpu_baseline(imgs[:64])

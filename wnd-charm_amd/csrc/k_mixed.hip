@@ -1,13 +1,13 @@
-// Reduced-precision forms of the basis contractions (the mixed-precision
-// configuration): the Gabor filter bank (GaborEnergy's conv2comp,
-// src/textures/gabor.cpp:19-52, 166-195) and the Chebyshev coefficient
-// contractions (Chebyshev2D, src/transforms/chebyshev.cpp:118-165) in
-//   * FP32: the FP64 kernels' tiling with FP32 fused multiply-adds;
-//   * BF16x3: each operand split into bf16 hi + lo (x = hi + lo to ~16 bits),
-//     the product taken as hi*hi + hi*lo + lo*hi on the bf16 matrix cores
-//     (v_mfma_f32_16x16x16_bf16) with FP32 accumulation.
-// Inputs and outputs stay FP64 planes; the energies / coefficients go on to
-// the same FP64 post-processing as the plain form.
+// Reduced-precision forms of the Gabor filter bank (GaborEnergy's
+// conv2comp, src/textures/gabor.cpp:19-52, 166-195) for the mixed-precision
+// plans:
+//   * FP32: the FP64 kernel's tiling with FP32 fused multiply-adds;
+//   * BF16: pixels split into two bf16 parts and weights into three, five
+//     exact bf16 products per tap on the bf16 matrix cores
+//     (v_mfma_f32_16x16x16_bf16), FP32 chunks added to FP64 sums.
+// Each energy comes with an a-priori bound on its distance from the FP64
+// kernel's value; the FP64 post-processing certifies every count from those
+// bounds and recomputes in FP64 only the energies a count leaves open.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -284,88 +284,6 @@ void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const vo
                                                              wsum, energy, bound);
   }
   launch_gabor_post_exact(imgs, energy, bound, taps64, params, o, nimg, maxh, maxw, s);
-}
-
-// ---------------------------------------------------------------- Chebyshev GEMMs
-// C[r][c] = sum_a A[r*sar + a*sak] B[a*N + c] / 2
-__global__ void __launch_bounds__(256) k_gemm_f32(const GemmDesc *ds) {
-  const GemmDesc d = ds[blockIdx.z];
-  const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
-  if (r0 >= d.M || c0 >= d.N) return;
-  __shared__ float As[16][65];
-  __shared__ float Bs[16][65];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  float acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
-  for (int k0 = 0; k0 < d.K; k0 += 16) {
-    for (int e = threadIdx.x; e < 16 * 64; e += 256) {
-      const int kk = e >> 6, rr = e & 63;
-      const int k = k0 + kk, r = r0 + rr, c = c0 + rr;
-      As[kk][rr] = (k < d.K && r < d.M) ? (float)d.A[(long)r * d.sar + (long)k * d.sak] : 0.f;
-      Bs[kk][rr] = (k < d.K && c < d.N) ? (float)d.B[(long)k * d.N + c] : 0.f;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < 16; kk++) {
-      float av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
-#pragma unroll
-      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = fmaf(av[i], bv[j], acc[i][j]);
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int r = r0 + ty * 4 + i, c = c0 + tx * 4 + j;
-      if (r < d.M && c < d.N) d.C[(long)r * d.N + c] = (double)(acc[i][j] * 0.5f);
-    }
-}
-
-// one wavefront per 16 x 16 tile of C, K in steps of 16 on the bf16 matrix cores
-__global__ void __launch_bounds__(64) k_gemm_bf16x3(const GemmDesc *ds) {
-  const GemmDesc d = ds[blockIdx.z];
-  const int r0 = blockIdx.y * 16, c0 = blockIdx.x * 16;
-  if (r0 >= d.M || c0 >= d.N) return;
-  const int lane = threadIdx.x, m = lane & 15, g = lane >> 4;
-  const int r = r0 + m, c = c0 + m;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < d.K; k0 += 16) {
-    bf16x4 Ah, Al, Bh, Bl;
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int k = k0 + 4 * g + u;
-      const float a = (k < d.K && r < d.M) ? (float)d.A[(long)r * d.sar + (long)k * d.sak] : 0.f;
-      const float b = (k < d.K && c < d.N) ? (float)d.B[(long)k * d.N + c] : 0.f;
-      const unsigned sa = bf16_split(a), sb = bf16_split(b);
-      Ah[u] = (short)(sa & 0xffff);
-      Al[u] = (short)(sa >> 16);
-      Bh[u] = (short)(sb & 0xffff);
-      Bl[u] = (short)(sb >> 16);
-    }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Al, Bh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bl, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(Ah, Bh, acc, 0, 0, 0);
-  }
-#pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int rr = r0 + 4 * g + u;
-    if (rr < d.M && c < d.N) d.C[(long)rr * d.N + c] = (double)(acc[u] * 0.5f);
-  }
-}
-
-void launch_gemm_mixed(int precision, const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s) {
-  if (precision == 1) k_gemm_f32<<<dim3((maxN + 63) / 64, (maxM + 63) / 64, nimg), 256, 0, s>>>(d);
-  else k_gemm_bf16x3<<<dim3((maxN + 15) / 16, (maxM + 15) / 16, nimg), 64, 0, s>>>(d);
 }
 
 }  // namespace wnd

@@ -186,7 +186,6 @@ void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const vo
                         double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
 void launch_gabor_post_exact(const Img *imgs, double *const *energy, float *const *bound, const double *taps,
                              double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
-void launch_gemm_mixed(int precision, const GemmDesc *d, int nimg, int maxM, int maxN, hipStream_t s);
 void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
                   int maxw, hipStream_t s);
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
