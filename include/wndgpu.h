@@ -26,6 +26,7 @@
  *   wndgpu_normalize_test       signatures::normalize (src/wndchrm_src/signatures.cpp:196-215)
  *   wndgpu_fisher               TrainingSet::SetFisherScores (TrainingSet.cpp:2183-2324)
  *   wndgpu_wnd5                 TrainingSet::classify2 (TrainingSet.cpp:2429-2586)
+ *   wndgpu_wnn                  TrainingSet::WNNclassify (TrainingSet.cpp:2375-2417)
  *   wndgpu_py_normalize / wndgpu_py_fisher / wndgpu_py_wnd5
  *                               the wndcharm Python package's normalize_by_columns
  *                               (wndcharm/utils.py:124-240), FisherFeatureWeights.
@@ -188,6 +189,17 @@ int wndgpu_fisher(wndgpu_ctx *ctx, const double *data, const int *classes, int n
 int wndgpu_wnd5(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
                 int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
                 double *norm_factor, long *predicted);
+/* Weighted nearest neighbour (WNNclassify) for n_test already-normalised
+ * test vectors, same arrays: distance sqrt(sum_s w_s (x_s - y_s)^2)
+ * (TrainingSet::distance with power 2), an identical training vector
+ * (distance below 1/INF) skipped, the class of the first closest sample
+ * predicted (0 if none), probabilities[c] = (1/d_c) / sum_c' (1/d_c') with
+ * d_c the closest distance of class c, norm_factor = that sum.  The
+ * reference's skip of a training sample with the test sample's file path
+ * has no counterpart (vectors carry no paths). */
+int wndgpu_wnn(wndgpu_ctx *ctx, const double *train, const int *classes, int n_samples, int n_features,
+               int n_classes, const double *weights, const double *test, int n_test, double *probabilities,
+               double *norm_factor, long *predicted);
 
 /* The same classifier as the wndcharm Python package computes it (its maths
  * differs from TrainingSet's; host arrays, FP64, row-major [n][F]):

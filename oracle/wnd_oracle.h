@@ -113,6 +113,9 @@ void orc_fisher(const double *data, const int *cls, int n, int f, int n_classes,
 long orc_wnd5(const double *train, const int *cls, int n, int f, int n_classes,
               const double *weights, const double *test_normalized,
               double *probabilities, double *norm_factor);
+long orc_wnn(const double *train, const int *cls, int n, int f, int n_classes,
+             const double *weights, const double *test_normalized,
+             double *probabilities, double *norm_factor);
 
 /* Classifier as the wndcharm Python package computes it
  * (wndcharm/utils.py, FeatureWeights.py, FeatureSpacePrediction.py). */

@@ -49,6 +49,8 @@ def oracle():
         lib.orc_fisher.argtypes = [P, P, I, I, I, D, P]
         lib.orc_wnd5.argtypes = [P, P, I, I, I, P, P, P, P]
         lib.orc_wnd5.restype = ctypes.c_long
+        lib.orc_wnn.argtypes = [P, P, I, I, I, P, P, P, P]
+        lib.orc_wnn.restype = ctypes.c_long
         lib.orc_plan_n_features.argtypes = [I]
         lib.orc_features_rgb.argtypes = [I, P, I, I, I, P]
         lib.orc_rgb_planes.argtypes = [P, I, I, I, P, P]

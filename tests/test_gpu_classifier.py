@@ -72,3 +72,28 @@ def test_short_set_five_fold_predictions_match_oracle():
             prob, norm = np.zeros(2), np.zeros(1)
             opred = lib.orc_wnd5(ptr(otr), ptr(ocls), n, f, 2, ptr(ow), ptr(v), ptr(prob), ptr(norm))
             assert pred[j] == opred, (k, t)
+
+
+def test_wnn_matches_oracle():
+    """WNNclassify on the GPU (wndgpu_wnn) against the oracle's: predicted
+    classes equal, probabilities and norm factors within 1e-12 (the GPU takes
+    d*d and sqrt where the reference calls glibc's pow), test vectors that
+    duplicate a training vector included"""
+    rng = np.random.default_rng(77)
+    n, f, C = 240, 300, 4
+    train = rng.uniform(0, 100, (n, f))
+    cls = np.repeat(np.arange(1, C + 1), n // C).astype(np.int32)
+    w = rng.uniform(0, 1, f)
+    w[rng.random(f) < 0.3] = 0.0
+    tests = rng.uniform(0, 100, (40, f))
+    tests[::7] = train[::35][:len(tests[::7])]
+    print('wnn', n, f, C, len(tests), flush=True)
+    prob, norm, pred = wndcharm.wnn(train, cls, C, w, tests)
+    lib = oracle()
+    for k, t in enumerate(tests):
+        op, on = np.zeros(C), np.zeros(1)
+        opred = lib.orc_wnn(ptr(np.ascontiguousarray(train)), ptr(cls), n, f, C, ptr(np.ascontiguousarray(w)),
+                            ptr(np.ascontiguousarray(t)), ptr(op), ptr(on))
+        assert pred[k] == opred, k
+        np.testing.assert_allclose(prob[k], op, rtol=1e-12)
+        np.testing.assert_allclose(norm[k], on[0], rtol=1e-12)

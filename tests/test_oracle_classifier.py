@@ -60,3 +60,53 @@ def test_wnd5_classify_table():
         assert np.array_equal(np.array([float('%.3f' % p) for p in prob]), d['kat_p'][i])
         interp = prob[0] * 2 + prob[1] * 4
         assert float('%.3f' % interp) == d['kat_interp'][i]
+
+
+def _wnn(train, cls, C, w, t):
+    prob, nf = np.zeros(C), np.zeros(1)
+    tr = np.ascontiguousarray(train)
+    c = np.ascontiguousarray(cls, np.int32)
+    pred = oracle().orc_wnn(ptr(tr), ptr(c), tr.shape[0], tr.shape[1], C, ptr(np.ascontiguousarray(w)),
+                            ptr(np.ascontiguousarray(t)), ptr(prob), ptr(nf))
+    return pred, prob, nf[0]
+
+
+def test_wnn_is_weighted_minkowski_nearest_neighbour():
+    """WNNclassify (TrainingSet.cpp:2375-2417): the distance is scipy's
+    weighted Minkowski distance with p = 2 and the weights to the first power
+    (TrainingSet::distance, 2357-2363); the class of the nearest sample is
+    predicted; the marginals weight each class by the inverse of its closest
+    distance, and norm_factor * marginal * distance = 1 as the reference's
+    comment on WNNclassify states"""
+    from scipy.spatial.distance import cdist
+    rng = np.random.default_rng(41)
+    n, f, C = 60, 14, 3
+    train = rng.uniform(0, 100, (n, f))
+    cls = np.repeat(np.arange(1, C + 1), n // C)
+    w = rng.uniform(0, 2, f)
+    w[3] = 0.0
+    tests = rng.uniform(0, 100, (12, f))
+    d = cdist(tests, train, 'minkowski', p=2, w=w)
+    for k, t in enumerate(tests):
+        pred, prob, nf = _wnn(train, cls, C, w, t)
+        assert pred == cls[np.argmin(d[k])]
+        dmin = np.array([d[k][cls == c].min() for c in range(1, C + 1)])
+        np.testing.assert_allclose(prob[:, None] / prob[None, :], dmin[None, :] / dmin[:, None], rtol=1e-12)
+        np.testing.assert_allclose(prob.sum(), 1.0, rtol=1e-14)
+        np.testing.assert_allclose(nf * prob * dmin, 1.0, rtol=1e-12)
+        assert np.argmax(prob) + 1 == pred
+
+
+def test_wnn_identical_sample_and_empty_class():
+    """a training vector equal to the test vector is skipped (distance below
+    1/INF counts as INF), the first of two equally close samples wins, and a
+    class with no sample keeps INF (its marginal 1e-201 / sum)"""
+    train = np.array([[0.0, 0.0], [3.0, 4.0], [1.0, 1.0], [-1.0, -1.0]])
+    cls = np.array([1, 2, 2, 1])
+    w = np.ones(2)
+    pred, prob, nf = _wnn(train, cls, 3, w, np.array([0.0, 0.0]))  # sample 0 is skipped; 2 and 3 tie
+    assert pred == 2
+    assert prob[0] == prob[1] and prob[2] > 0 and prob[2] < 1e-200
+    pred, prob, nf = _wnn(train, cls, 3, w, np.array([3.0, 4.0]))  # nearest other: (1, 1), class 2
+    assert pred == 2
+    np.testing.assert_allclose(prob[1] / prob[0], np.hypot(3, 4) / np.hypot(2, 3), rtol=1e-14)

@@ -48,6 +48,8 @@ void launch_py_wnd5(const double *train, const double *w, const double *test, co
                     double *norm, long *pred, hipStream_t s);
 void launch_wnd5(const double *train, const int *cls, const double *w, const double *test, int n, int f, int C,
                  int ntest, double *dist, double *prob, double *norm, long *pred, hipStream_t s);
+void launch_wnn(const double *train, const int *cls, const double *w, const double *test, int n, int f, int C,
+                int ntest, double *dist, double *prob, double *norm, long *pred, hipStream_t s);
 }  // namespace wnd
 
 using namespace wnd;
@@ -2344,8 +2346,9 @@ int wndgpu_fisher(wndgpu_ctx *c, const double *data, const int *cls, int n, int 
   return WNDGPU_OK;
 }
 
-int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f, int C, const double *w,
-                const double *test, int ntest, double *prob, double *norm, long *pred) {
+// classify2 (wnn = false) or WNNclassify (wnn = true) of n_test normalised vectors
+static int classify_common(wndgpu_ctx *c, bool wnn, const double *train, const int *cls, int n, int f, int C,
+                           const double *w, const double *test, int ntest, double *prob, double *norm, long *pred) {
   if (!c || !train || !cls || !w || !test || !prob || !norm || !pred || n <= 0 || f <= 0 || C <= 0 || C > 63 ||
       ntest <= 0)
     return fail(c, WNDGPU_E_ARG, "bad argument");
@@ -2364,13 +2367,25 @@ int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f
   HIPCHK(c, hipMemcpy(k.p, cls, (size_t)n * 4, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(wd.p, w, (size_t)f * 8, hipMemcpyHostToDevice));
   HIPCHK(c, hipMemcpy(te.p, test, (size_t)ntest * f * 8, hipMemcpyHostToDevice));
-  launch_wnd5((const double *)tr.p, (const int *)k.p, (const double *)wd.p, (const double *)te.p, n, f, C, ntest,
-              (double *)dist.p, (double *)pr.p, (double *)nm.p, (long *)pd.p, c->stream);
+  (wnn ? launch_wnn : launch_wnd5)((const double *)tr.p, (const int *)k.p, (const double *)wd.p,
+                                    (const double *)te.p, n, f, C, ntest, (double *)dist.p, (double *)pr.p,
+                                    (double *)nm.p, (long *)pd.p, c->stream);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipMemcpy(prob, pr.p, (size_t)ntest * C * 8, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(norm, nm.p, (size_t)ntest * 8, hipMemcpyDeviceToHost));
   HIPCHK(c, hipMemcpy(pred, pd.p, (size_t)ntest * sizeof(long), hipMemcpyDeviceToHost));
   return WNDGPU_OK;
+}
+
+int wndgpu_wnd5(wndgpu_ctx *c, const double *train, const int *cls, int n, int f, int C, const double *w,
+                const double *test, int ntest, double *prob, double *norm, long *pred) {
+  return classify_common(c, false, train, cls, n, f, C, w, test, ntest, prob, norm, pred);
+}
+
+int wndgpu_wnn(wndgpu_ctx *c, const double *train, const int *cls, int n, int f, int C, const double *w,
+               const double *test, int ntest, double *prob, double *norm, long *pred) {
+  return classify_common(c, true, train, cls, n, f, C, w, test, ntest, prob, norm, pred);
 }
 
 // ---- the wndcharm Python package's classifier maths

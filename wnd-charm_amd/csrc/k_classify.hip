@@ -157,6 +157,62 @@ __global__ void k_wnd_classes(const double *dist, const int *cls, int n, int C, 
   pred[q] = best;
 }
 
+// WNNclassify (TrainingSet.cpp:2375-2417) with TrainingSet::distance(.., 2)
+// (TrainingSet.cpp:2357-2363): one thread per (test, train) pair,
+// sqrt(sum_s w_s (x_s - y_s)^2) in feature order, an identical sample
+// (distance below 1/INF) counted as INF
+__global__ void k_wnn_dist(const double *train, const double *wts, const double *test, int n, int f, int ntest,
+                           double *dist) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * ntest) return;
+  const int q = e / n, i = e - (long)q * n;
+  const double *tv = test + (long)q * f, *sv = train + (long)i * f;
+  double acc = 0.0;
+  for (int s = 0; s < f; s++) {
+    const double d = tv[s] - sv[s];
+    acc = acc + wts[s] * (d * d);
+  }
+  double d = sqrt(acc);
+  if (d < 1 / kInf) d = kInf;
+  dist[e] = d;
+}
+
+// per test vector, samples in order: the first strictly closest sample's
+// class, each class's closest distance, marginals 1/d over their sum
+__global__ void k_wnn_classes(const double *dist, const int *cls, int n, int C, int ntest, double *prob,
+                              double *norm, long *pred) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ntest) return;
+  double p[64];
+  for (int c = 0; c <= C; c++) p[c] = kInf;
+  long best = 0;
+  double closest = kInf;
+  for (int i = 0; i < n; i++) {
+    const double d = dist[(long)q * n + i];
+    const int ci = cls[i];
+    if (ci < 0 || ci > C) continue;  // labels are validated on the host; never index outside
+    if (d < closest) {
+      closest = d;
+      best = ci;
+    }
+    if (d < p[ci]) p[ci] = d;
+  }
+  double sum = 0;
+  for (int c = 1; c <= C; c++)
+    if (p[c] != 0) sum += 1 / p[c];
+  for (int c = 1; c <= C; c++)
+    prob[(long)q * C + c - 1] = sum == 0 ? 0.0 : (p[c] == 0 ? 1.0 : (1 / p[c]) / sum);
+  norm[q] = sum;
+  pred[q] = best;
+}
+
+void launch_wnn(const double *train, const int *cls, const double *w, const double *test, int n, int f, int C,
+                int ntest, double *dist, double *prob, double *norm, long *pred, hipStream_t s) {
+  const long e = (long)n * ntest;
+  k_wnn_dist<<<(e + 255) / 256, 256, 0, s>>>(train, w, test, n, f, ntest, dist);
+  k_wnn_classes<<<(ntest + 63) / 64, 64, 0, s>>>(dist, cls, n, C, ntest, prob, norm, pred);
+}
+
 void launch_norm_train(double *data, int n, int f, double *mins, double *maxs, hipStream_t s) {
   k_norm_train<<<(f + 127) / 128, 128, 0, s>>>(data, n, f, mins, maxs);
 }

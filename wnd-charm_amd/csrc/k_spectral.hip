@@ -158,8 +158,9 @@ void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *
                     hipStream_t s, int precision) {
   k_zern_moments<<<nimg, 1024, 0, s>>>(imgs, partial, nchunk);
   // FP64 accumulation in every precision: the radial polynomials are FP64
-  // anyway, and on MI355X the FP32 and FP64 vector rates are equal, so FP32
-  // sums would add error (above 1e-5 on transformed planes) and save nothing
+  // anyway, and without packed FP32 instructions (this library is built
+  // with them off) FP32 and FP64 FMAs issue at the same rate, so FP32 sums
+  // would add error (above 1e-5 on transformed planes) and save nothing
   (void)precision;
   k_zern_accum<double><<<dim3(nchunk, nimg), 256, 0, s>>>(imgs, partial, nchunk, htab);
   k_zern_final<<<nimg, 128, 0, s>>>(imgs, partial, nchunk, o);

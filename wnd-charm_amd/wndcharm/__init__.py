@@ -74,6 +74,7 @@ def _load():
         'wndgpu_normalize_test': (_I, [_P, _P, _I, _P, _P]),
         'wndgpu_fisher': (_I, [_P, _P, _P, _I, _I, _I, _D, _P]),
         'wndgpu_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
+        'wndgpu_wnn': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
         'wndgpu_py_normalize': (_I, [_P, _P, _I, _I, _I, _P, _P, _P, _P, _P]),
         'wndgpu_py_fisher': (_I, [_P, _P, _P, _I, _I, _I, _P]),
         'wndgpu_py_wnd5': (_I, [_P, _P, _P, _I, _I, _I, _P, _P, _I, _P, _P, _P]),
@@ -517,6 +518,25 @@ def wnd5(train, classes, n_classes, weights, test):
     pred = np.zeros(nt, np.int64)
     _check(_load().wndgpu_wnd5(_context(), _ptr(tr), _ptr(c), tr.shape[0], tr.shape[1], int(n_classes), _ptr(w),
                                _ptr(te), nt, _ptr(prob), _ptr(norm), _ptr(pred)), 'wnd5')
+    return prob, norm, pred
+
+
+def wnn(train, classes, n_classes, weights, test):
+    """WNNclassify (weighted nearest neighbour) for a batch of normalised
+    test vectors: (probabilities, norm_factor, predicted class 1..n_classes)."""
+    tr = np.ascontiguousarray(train, np.float64)
+    c = np.ascontiguousarray(classes, np.int32)
+    w = np.ascontiguousarray(weights, np.float64)
+    te = np.ascontiguousarray(np.atleast_2d(test), np.float64)
+    if tr.ndim != 2 or c.shape != (tr.shape[0],) or w.shape != (tr.shape[1],) or te.shape[1] != tr.shape[1]:
+        raise WndGpuError('wnn: shapes of train %s, classes %s, weights %s and test %s disagree'
+                          % (tr.shape, c.shape, w.shape, te.shape))
+    nt = te.shape[0]
+    prob = np.zeros((nt, n_classes))
+    norm = np.zeros(nt)
+    pred = np.zeros(nt, np.int64)
+    _check(_load().wndgpu_wnn(_context(), _ptr(tr), _ptr(c), tr.shape[0], tr.shape[1], int(n_classes), _ptr(w),
+                              _ptr(te), nt, _ptr(prob), _ptr(norm), _ptr(pred)), 'wnn')
     return prob, norm, pred
 
 
