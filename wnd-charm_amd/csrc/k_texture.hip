@@ -771,11 +771,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 }
 
 // Francis QR on the Hessenberg matrix left in H.Q by k_har_hess.
-// One wavefront per (angle, image): the Schur iteration is a chain of
-// dependent steps, so many matrices per CU (small LDS window, one wave each)
-// beat many waves per matrix.
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_har_qr(const HarDesc *hd) {
-  __shared__ double Rw[QW * QWP];
+// NT threads per (angle, image).  The Schur iteration is a chain of
+// dependent steps, so for large batches many matrices per CU (NT = 64: a
+// small LDS window and one wave each) beat many waves per matrix.  For a
+// few matrices (a call of one or a few images) NT = 256: the chase still
+// runs on wave 0, while the strip updates after each window, whose columns
+// and rows are independent, take one round over 256 threads instead of
+// several over 64 (and the rows above the window one LDS chunk of up to 256
+// rows instead of chunks of 32).  Every entry receives the same operations
+// in the same order either way.
+template <int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4 : 1)))
+k_har_qr(const HarDesc *hd) {
+  constexpr int RS = NT / 32;             // window rows per pass of the block (32 columns per row)
+  constexpr int CH = NT == 64 ? QW : NT;  // rows above the window staged per chunk
+  __shared__ double Rw[CH * QWP];
   __shared__ double rf[QB][4];
   __shared__ double sc[8];
   __shared__ int si[8];
@@ -1011,17 +1021,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
       const int a = k0 - 1 > 0 ? k0 - 1 : 0;
       const int b = (k1 + 3 < iuq + 1) ? k1 + 3 : iuq + 1;
       const int wd = b - a;
-      {  // 32 x 32 tile, two rows per wave instruction, all loads in flight
+      {  // 32 x 32 tile, RS rows per block pass, all loads in flight
         int rb = t >> 5;
         asm volatile("" : "+v"(rb));  // per-row values are formed here, not hoisted out of the sweep loop
         const int cl = t & 31;
         const int cc = min(cl, wd - 1);  // out-of-window lanes re-read in-window entries (unused)
-        double v[QW / 2];
+        double v[QW / RS];
 #pragma unroll
-        for (int it = 0; it < QW / 2; it++) v[it] = A[(unsigned)((a + min(rb + 2 * it, wd - 1)) * n + a + cc)];
+        for (int it = 0; it < QW / RS; it++) v[it] = A[(unsigned)((a + min(rb + RS * it, wd - 1)) * n + a + cc)];
 #pragma unroll
-        for (int it = 0; it < QW / 2; it++) {
-          const int r = rb + 2 * it;
+        for (int it = 0; it < QW / RS; it++) {
+          const int r = rb + RS * it;
           if (r < wd && cl < wd) Rw[r * QWP + cl] = v[it];
         }
       }
@@ -1083,7 +1093,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         }
       }
       __syncthreads();
-      for (int r = t >> 5; r < wd; r += 2)
+      for (int r = t >> 5; r < wd; r += RS)
         if ((t & 31) < wd) AT(a + r, a + (t & 31)) = Rw[r * QWP + (t & 31)];
 #undef RW
       // strip right of R: rows k0 .. k1+1, columns b .. iuq (left reflectors).
@@ -1127,24 +1137,24 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         ps[n] = u1;
       }
       // strip above R: rows 0 .. a-1, columns k0 .. k1+1 (right reflectors).
-      // Staged through the (now free) window buffer in chunks of QW rows with
+      // Staged through the (now free) window buffer in chunks of CH rows with
       // coalesced loads and stores; one lane streams each row through the
       // reflectors in LDS.
       __syncthreads();
-      for (int r0 = 0; r0 < a; r0 += QW) {  // chunks of QW rows: a chunk fills at most the window buffer
-        const int nr = (a - r0 < QW) ? a - r0 : QW;
+      for (int r0 = 0; r0 < a; r0 += CH) {  // a chunk fills at most the window buffer
+        const int nr = (a - r0 < CH) ? a - r0 : CH;
         const int nc = nstep + 2;
         {
           int rb = t >> 5;
           asm volatile("" : "+v"(rb));
           const int cl = t & 31;
           const int cc = min(cl, nc - 1);
-          double v[QW / 2];
+          double v[CH / RS];
 #pragma unroll
-          for (int it = 0; it < QW / 2; it++) v[it] = A[(unsigned)((r0 + min(rb + 2 * it, nr - 1)) * n + k0 + cc)];
+          for (int it = 0; it < CH / RS; it++) v[it] = A[(unsigned)((r0 + min(rb + RS * it, nr - 1)) * n + k0 + cc)];
 #pragma unroll
-          for (int it = 0; it < QW / 2; it++) {
-            const int rr = rb + 2 * it;
+          for (int it = 0; it < CH / RS; it++) {
+            const int rr = rb + RS * it;
             if (rr < nr && cl < nc) Rw[rr * QWP + cl] = v[it];
           }
         }
@@ -1171,7 +1181,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
           row[nstep + 1] = u1;
         }
         __syncthreads();
-        for (int rr = t >> 5; rr < nr; rr += 2)
+        for (int rr = t >> 5; rr < nr; rr += RS)
           if ((t & 31) < nc) AT(r0 + rr, k0 + (t & 31)) = Rw[rr * QWP + (t & 31)];
         __syncthreads();
       }
@@ -1272,7 +1282,10 @@ void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s) {
   k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
 }
 void launch_haralick_qr(const HarDesc *hd, int nimg, hipStream_t s) {
-  k_har_qr<<<dim3(4, nimg), 64, 0, s>>>(hd);
+  // up to 1024 matrices: one 256-thread block each (all resident at once);
+  // beyond, one wavefront each
+  if (4 * nimg <= 1024) k_har_qr<256><<<dim3(4, nimg), 256, 0, s>>>(hd);
+  else k_har_qr<64><<<dim3(4, nimg), 64, 0, s>>>(hd);
 }
 void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s) {
   k_har_final<<<nimg, 64, 0, s>>>(hd, o);
