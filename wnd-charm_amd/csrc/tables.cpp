@@ -147,10 +147,10 @@ std::vector<double> gabor_kernels() {
     const double lambda = 2 * M_PI / f0;
     const double ct = cos(theta), st = sin(theta);
     const double sig = sig2lam * lambda;
-    std::vector<double> tx(n + 1), ty(n + 1), G((size_t)n * n * 2);
+    std::vector<double> tx(n + 1), G((size_t)n * n * 2);
     tx[0] = -(n / 2);
     for (int x = 1; x <= n; x++) tx[x] = tx[x - 1] + 1;
-    ty = tx;
+    const std::vector<double> ty(tx);
     double sum = 0;
     for (int y = 0; y < n; y++)
       for (int x = 0; x < n; x++) {
