@@ -146,8 +146,14 @@ def cpu_baseline(imgs, seconds=12.0):
             'cpu_model': cpu_model(), 'host_cpus': os.cpu_count(),
             'sample': '%d long-set vectors of 256x256 images in %.1f s, one image per thread, %d threads, '
                       'g++ -O2 -ffp-contract=off' % (done[0], el, cores),
-            'note': 'the reference build itself needs FFTW3 (absent here), so no restatement/reference '
-                    'time ratio is measured'}
+            'calibration': {'oracle_over_reference_time': {'haralick_f14': 1.193, 'chebyshev_transform': 0.993,
+                                                           'radon': 1.048, 'wavelet': 1.15},
+                            'share_of_oracle_long_set_time': {'haralick_f14': 0.342, 'chebyshev_transform': 0.103,
+                                                              'radon': 0.009, 'wavelet': 0.003},
+                            'note': 'oracle/calibrate.py in the build container (one thread, 8 images 256x256): '
+                                    'the units whose reference sources build there; on those 46% of its time '
+                                    'the oracle takes 1.15x the reference time. The whole reference path needs '
+                                    'FFTW3, absent here, so no whole-vector ratio exists'}}
 
 
 def profiled_traffic(kernel):
