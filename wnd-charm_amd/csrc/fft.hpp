@@ -16,6 +16,38 @@ __device__ __forceinline__ unsigned fft_div(unsigned t, unsigned long long m) {
   return (unsigned)(((unsigned long long)t * m) >> 32);
 }
 
+// One radix-R stage for R in {2, 3, 4, 5, 7}: one thread per butterfly j
+// loads its R inputs once and forms the R outputs q; each output is the same
+// sum, in the same order and with the same twiddles, as in the per-output
+// form below.
+template <int R, class TW>
+__device__ __forceinline__ void fft_stage(const double2 *src, double2 *dst, TW tw, int n, int Ns, int lane,
+                                          int tpl) {
+  const int nr = n / R, step = n / (Ns * R);
+  const unsigned long long mns = fft_magic(Ns);
+  for (int j = lane; j < nr; j += tpl) {
+    const int jq = fft_div(j, mns), k = j - jq * Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = src[j + r * nr];
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int base = k * step + q * nr;
+      double re = 0, im = 0;
+      int idx = 0;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const double2 w = tw[idx];
+        re += v[r].x * w.x - v[r].y * w.y;
+        im += v[r].x * w.y + v[r].y * w.x;
+        idx += base;
+        if (idx >= n) idx -= n;
+      }
+      dst[jq * Ns * R + k + q * Ns] = make_double2(re, im);
+    }
+  }
+}
+
 // One line per group of `tpl` threads (lane = index in the group); every
 // group of the block runs the same plan, so the barriers are uniform.  tw:
 // the plan's twiddles, in global memory or staged in LDS by the caller.
@@ -27,23 +59,32 @@ __device__ inline void fft_lds_group_tw(double2 *a, double2 *b, const FftPlan &p
   double2 *src = a, *dst = b;
   for (int s = 0; s < pl.nst; s++) {
     const int R = pl.radix[s];
-    const int nr = n / R, step = n / (Ns * R);
-    const unsigned long long mnr = fft_magic(nr), mns = fft_magic(Ns);
-    for (int t = lane; t < n; t += tpl) {
-      const int q = fft_div(t, mnr), j = t - q * nr;
-      const int jq = fft_div(j, mns), k = j - jq * Ns;
-      const int base = k * step + q * nr;
-      double re = 0, im = 0;
-      int idx = 0;
-      for (int r = 0; r < R; r++) {
-        const double2 v = src[j + r * nr];
-        const double2 w = tw[idx];
-        re += v.x * w.x - v.y * w.y;
-        im += v.x * w.y + v.y * w.x;
-        idx += base;
-        if (idx >= n) idx -= n;
+    switch (R) {
+      case 2: fft_stage<2>(src, dst, tw, n, Ns, lane, tpl); break;
+      case 3: fft_stage<3>(src, dst, tw, n, Ns, lane, tpl); break;
+      case 4: fft_stage<4>(src, dst, tw, n, Ns, lane, tpl); break;
+      case 5: fft_stage<5>(src, dst, tw, n, Ns, lane, tpl); break;
+      case 7: fft_stage<7>(src, dst, tw, n, Ns, lane, tpl); break;
+      default: {  // other primes: one output per thread, as a direct R-point DFT
+        const int nr = n / R, step = n / (Ns * R);
+        const unsigned long long mnr = fft_magic(nr), mns = fft_magic(Ns);
+        for (int t = lane; t < n; t += tpl) {
+          const int q = fft_div(t, mnr), j = t - q * nr;
+          const int jq = fft_div(j, mns), k = j - jq * Ns;
+          const int base = k * step + q * nr;
+          double re = 0, im = 0;
+          int idx = 0;
+          for (int r = 0; r < R; r++) {
+            const double2 v = src[j + r * nr];
+            const double2 w = tw[idx];
+            re += v.x * w.x - v.y * w.y;
+            im += v.x * w.y + v.y * w.x;
+            idx += base;
+            if (idx >= n) idx -= n;
+          }
+          dst[jq * Ns * R + k + q * Ns] = make_double2(re, im);
+        }
       }
-      dst[jq * Ns * R + k + q * Ns] = make_double2(re, im);
     }
     __syncthreads();
     double2 *tt = src; src = dst; dst = tt;

@@ -293,7 +293,7 @@ __global__ void __launch_bounds__(256) k_cheb_gemm(const GemmDesc *ds) {
 __global__ void __launch_bounds__(256) k_fft_rows(const Img *in, double2 *const *f1, const FftPlan *plw) {
   extern __shared__ double2 sh[];
   const Img a = in[blockIdx.y];
-  const FftPlan pl = plw[blockIdx.y];
+  const FftPlan &pl = plw[blockIdx.y];
   const int L = fft_lines(pl.n), tpl = blockDim.x / L, line = threadIdx.x / tpl, lane = threadIdx.x - line * tpl;
   const int row = blockIdx.x * L + line;
   if ((int)blockIdx.x * L >= a.h) return;
@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(256) k_fft_cols(const Img *in, double2 *const 
                                                   const Img *out) {
   extern __shared__ double2 sh[];
   const Img a = in[blockIdx.y], o = out[blockIdx.y];
-  const FftPlan pl = plh[blockIdx.y];
+  const FftPlan &pl = plh[blockIdx.y];
   const int n = pl.n, L = fft_lines(n), tpl = blockDim.x / L, line = threadIdx.x / tpl, lane = threadIdx.x - line * tpl;
   const int x0 = blockIdx.x * L;
   if (x0 >= a.w) return;
