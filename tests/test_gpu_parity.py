@@ -151,3 +151,20 @@ def test_bad_image_status():
     assert list(st) == [0, 1, 0]
     assert np.all(np.isnan(out[1]))
     assert np.array_equal(out[0], out[2])
+
+
+def test_haralick_large_batch_equals_small_batches():
+    """the Schur kernel runs one wavefront per matrix above 1024 matrices
+    (256 images) and one 256-thread block per matrix below; both apply the
+    same operations to every entry, so the 28 Haralick values of 260 images
+    computed in one call equal those computed in two calls of 130 (the
+    small-batch form is the one the oracle tests above run)"""
+    imgs = [synthetic.image(5000 + k, 4 + k % 9, (64, 64 + 2 * (k % 5))) for k in range(260)]
+    plan = wndcharm.FeatureComputationPlan('haralick')
+    for g in ('Haralick Textures ()', 'Haralick Textures (Fourier ())'):
+        plan.add(g)
+    plan.finalize()
+    print('haralick 260 vs 2 x 130', flush=True)
+    big = wndcharm.compute_features(imgs, plan)
+    small = np.concatenate([wndcharm.compute_features(imgs[:130], plan), wndcharm.compute_features(imgs[130:], plan)])
+    assert np.array_equal(big, small, equal_nan=True)

@@ -52,3 +52,38 @@ def test_prefetch_rejects_nothing_and_reads_no_pixels():
     wndcharm.prefetch_tables([tiny, ok], plan)
     got = wndcharm.compute_features([ok], plan)
     assert np.isfinite(got).any()
+
+
+def test_tiny_table_budget_evicts_and_stays_exact():
+    """a context whose size-dependent table cache holds almost nothing
+    (WNDGPU_TABLE_CACHE_MB=1): every batch evicts the previous batches'
+    tables (stream-ordered frees) while prefetched ones are pinned for the
+    batch they were queued for; features equal a default context's"""
+    import os
+    shapes = [[(97, 181), (120, 64)], [(333, 317), (64, 250)], [(97, 181), (301, 300)], [(80, 90)]]
+    batches = [[synthetic.image(3200 + 10 * b + k, 5, shp) for k, shp in enumerate(bs)] for b, bs in enumerate(shapes)]
+    plan = wndcharm.StdFeatureComputationPlans.getFeatureSetLong()
+    want = _fresh_context_features(batches, plan)
+    lib = wndcharm._load()
+    old = wndcharm._context()
+    os.environ['WNDGPU_TABLE_CACHE_MB'] = '1'
+    c = ctypes.c_void_p()
+    try:
+        assert lib.wndgpu_create(0, ctypes.byref(c)) == 0
+    finally:
+        os.environ.pop('WNDGPU_TABLE_CACHE_MB')
+    wndcharm._ctx = c
+    try:
+        print('tiny table budget', [[im.shape for im in b] for b in batches], flush=True)
+        wndcharm.prefetch_tables(batches[0], plan)
+        got = []
+        for k, b in enumerate(batches):
+            if k + 1 < len(batches):
+                wndcharm.prefetch_tables(batches[k + 1], plan)
+            got.append(wndcharm.compute_features(b, plan))
+        wndcharm.prefetch_tables(batches[0], plan)  # queued, never taken: dropped at teardown
+    finally:
+        wndcharm._ctx = old
+        lib.wndgpu_destroy(c)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w, equal_nan=True)
