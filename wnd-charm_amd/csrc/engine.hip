@@ -283,10 +283,12 @@ struct wndgpu_ctx {
   // The Haralick chain (co-occurrence .. Schur) runs on `side`, forked from
   // `stream` once every transform node is issued and joined back before the
   // batch completes: its memory- and latency-bound eigen kernels overlap
-  // the arithmetic-bound feature kernels on `stream`.  Both streams have the
-  // default priority.
+  // the arithmetic-bound feature kernels, which alternate between `stream`
+  // and `fstream` so that the tail of one launch overlaps the next.  All
+  // streams have the default priority.
   hipStream_t side = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  hipStream_t fstream = nullptr;  // every other non-Haralick feature launch
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, fjoin_ev = nullptr;
   void *scratch = nullptr;  // this device's copy of `scratch` (wndgpu_ad_wnd_forward_noalloc)
   bool serial = false;  // WNDGPU_SERIAL: every launch on `stream` (per-kernel timing without overlap)
   std::string err;
@@ -730,7 +732,7 @@ static void out_size(int kind, int h, int w, int *oh, int *ow) {
 // it runs on the side stream.
 struct LaunchItem {
   std::string name;
-  std::function<void()> f;
+  std::function<void(hipStream_t)> f;  // issues the launch's kernels on the given stream
   bool side;
 };
 
@@ -851,7 +853,7 @@ static int build_sampled_root(wndgpu_ctx *c, const wndgpu_plan *plan, const Batc
   Img *ri = R.imgs;
   Stats *rs = R.stats;
   const int nds = (int)ds.size(), nnd = (int)nd.size(), nsd = (int)sd.size();
-  push("root", [=]() {
+  push("root", [=](hipStream_t s) {
     if (wide) launch_rgb_range(fd, range, ni, s);
     launch_load(rd, cd, ni, maxc, s);
     if (nds) launch_downsample(dsd, nds, mx_oh, mx_ow, mx_nh, mx_nw, s);
@@ -877,10 +879,10 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
   const int n = B.n;
   std::map<std::vector<int>, NodeRun> nodes;
   auto maxof = [](const std::vector<int> &v) { return *std::max_element(v.begin(), v.end()); };
-  auto push = [&](const std::string &name, std::function<void()> f) {
+  auto push = [&](const std::string &name, std::function<void(hipStream_t)> f) {
     if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), false});
   };
-  auto push_side = [&](const std::string &name, std::function<void()> f) {
+  auto push_side = [&](const std::string &name, std::function<void(hipStream_t)> f) {
     if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), true});
   };
   // ---- root: load + stats
@@ -918,7 +920,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     LoadDesc *ld = A.stage(ldv);
     Img *ri = R.imgs;
     Stats *rs = R.stats;
-    push("root", [=]() {
+    push("root", [=](hipStream_t s) {
       if (wide) launch_rgb_range(ld, range, n, s);
       launch_load(ld, ri, n, maxp, s);
       launch_stats(ri, rs, n, s);
@@ -959,9 +961,9 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
       const unsigned char *const *hv = P.hsvd;
       const unsigned char *lut = (const unsigned char *)c->color_lut.p;
       const int ck = kind == T_COLOR ? 0 : 1;
-      push(nm, [=]() { launch_color_node(hv, lut, pout, ck, n, maxp, s); launch_stats(pout, nst, n, s); });
+      push(nm, [=](hipStream_t s) { launch_color_node(hv, lut, pout, ck, n, maxp, s); launch_stats(pout, nst, n, s); });
     } else if (kind == T_EDGE) {
-      push(nm, [=]() { launch_edge(pin, pout, n, maxp, s); launch_stats(pout, nst, n, s); });
+      push(nm, [=](hipStream_t s) { launch_edge(pin, pout, n, maxp, s); launch_stats(pout, nst, n, s); });
     } else if (kind == T_WAVELET) {
       std::vector<double *> g(n);
       std::vector<int> gs(n);
@@ -972,7 +974,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
       }
       double **gd = A.stage(g);
       int *gsd = A.stage(gs);
-      push(nm, [=]() { launch_wavelet(pin, gd, gsd, pout, n, mph, mpw, s); launch_stats(pout, nst, n, s); });
+      push(nm, [=](hipStream_t s) { launch_wavelet(pin, gd, gsd, pout, n, mph, mpw, s); launch_stats(pout, nst, n, s); });
     } else if (kind == T_CHEBYSHEV) {
       // Chebyshev2D (chebyshev.cpp:118-165): rows against the x basis, then
       // the transposed intermediate against the y basis
@@ -996,7 +998,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         maxH = std::max(maxH, h);
       }
       GemmDesc *d1 = A.stage(g1), *d2 = A.stage(g2);
-      push(nm, [=]() {
+      push(nm, [=](hipStream_t s) {
         launch_cheb_gemm(d1, n, maxH, maxN, s);
         launch_cheb_gemm(d2, n, maxN, maxN, s);
         launch_stats(pout, nst, n, s);
@@ -1014,7 +1016,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
       }
       double2 **f1d = A.stage(f1);
       FftPlan *pwd = A.stage(pw), *phd = A.stage(ph);
-      push(nm, [=]() { launch_fft(pin, f1d, pwd, phd, pout, n, mph, mpw, s); launch_stats(pout, nst, n, s); });
+      push(nm, [=](hipStream_t s) { launch_fft(pin, f1d, pwd, phd, pout, n, mph, mpw, s); launch_stats(pout, nst, n, s); });
     }
     N.planes.resize(n);
     for (int i = 0; i < n; i++) N.planes[i] = im[i].p;
@@ -1080,16 +1082,16 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     const OutRef o = L.o;
     const int ne = L.ne, mh = L.mh, mw = L.mw;
     const long maxp = L.maxp;
-    auto gather = [=]() { launch_gather_stats(sts, nst, ne, s); };
+    auto gather = [=](hipStream_t gs) { launch_gather_stats(sts, nst, ne, gs); };
     switch (alg) {
-      case A_MSH: push(gn, [=]() { gather(); launch_msh(ni, nst, o, ne, s); }); break;
-      case A_FRACTAL: push(gn, [=]() { launch_fractal(ni, o, ne, s); }); break;
+      case A_MSH: push(gn, [=](hipStream_t s) { gather(s); launch_msh(ni, nst, o, ne, s); }); break;
+      case A_FRACTAL: push(gn, [=](hipStream_t s) { launch_fractal(ni, o, ne, s); }); break;
       case A_COLOR_HIST: {  // root entries only
         std::vector<unsigned char *> hv(ne);
         for (int k = 0; k < ne; k++) hv[k] = E[k].N->hsv[E[k].i];
         unsigned char **hvd = A.stage(hv);
         const unsigned char *lut = (const unsigned char *)c->color_lut.p;
-        push(gn, [=]() { launch_color_hist(hvd, ni, lut, o, ne, s); });
+        push(gn, [=](hipStream_t s) { launch_color_hist(hvd, ni, lut, o, ne, s); });
       } break;
       case A_COMB: {
         std::vector<double> iis;
@@ -1098,7 +1100,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           iis.insert(iis.end(), v.begin(), v.end());
         }
         double *d = A.stage(iis);
-        push(gn, [=]() { launch_comb(ni, d, o, ne, s); });
+        push(gn, [=](hipStream_t s) { launch_comb(ni, d, o, ne, s); });
       } break;
       case A_RADON: {
         std::vector<double *> pr(ne);
@@ -1112,14 +1114,14 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         }
         double **prd = A.stage(pr);
         const double *trig = (const double *)c->rtrig.p;
-        push(gn, [=]() { launch_radon(ni, trig, prd, o, ne, maxbins, mh, mw, s); });
+        push(gn, [=](hipStream_t s) { launch_radon(ni, trig, prd, o, ne, maxbins, mh, mw, s); });
       } break;
       case A_TAMURA: {
         std::vector<double *> sc(ne);
         for (int k = 0; k < ne; k++)
           sc[k] = A.dalloc_n<double>(5 * (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i] + 8);
         double **scd = A.stage(sc);
-        push(gn, [=]() { gather(); launch_tamura(ni, nst, scd, o, ne, mh, mw, s); });
+        push(gn, [=](hipStream_t s) { gather(s); launch_tamura(ni, nst, scd, o, ne, mh, mw, s); });
       } break;
       case A_HARALICK: {
         std::vector<HarDesc> hd(ne);
@@ -1146,22 +1148,21 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           c->har_n = ne;
         }
         HarDesc *hdd = A.stage(hd);
-        hipStream_t s2 = c->serial ? c->stream : c->side;
-        push_side(gn + " [co-occurrence, f1-f13, Q]", [=]() {
+        push_side(gn + " [co-occurrence, f1-f13, Q]", [=](hipStream_t s2) {
           launch_gather_stats(sts, nst, ne, s2);
           launch_fill_int(lut_all, 256 * (long)ne, 0, s2);
           launch_haralick_prep(ni, nst, hdd, ne, (int)maxp, s2);
         });
-        push_side(gn + " [k_har_hess]", [=]() { launch_haralick_hess(hdd, ne, s2); });
-        push_side(gn + " [k_har_qr]", [=]() { launch_haralick_qr(hdd, ne, s2); });
-        push_side(gn + " [mean, range]", [=]() { launch_haralick_final(hdd, o, ne, s2); });
+        push_side(gn + " [k_har_hess]", [=](hipStream_t s2) { launch_haralick_hess(hdd, ne, s2); });
+        push_side(gn + " [k_har_qr]", [=](hipStream_t s2) { launch_haralick_qr(hdd, ne, s2); });
+        push_side(gn + " [mean, range]", [=](hipStream_t s2) { launch_haralick_final(hdd, o, ne, s2); });
       } break;
       case A_ZERNIKE: {
         const int nchunk = (int)std::max<long>(1, std::min<long>(64, (maxp + 4095) / 4096));
         double *part = A.dalloc_n<double>((size_t)ne * (4 + nchunk * 144));
         const double *ht = (const double *)c->zern.p;
         const int prec = plan->precision;
-        push(gn, [=]() { launch_zernike(ni, part, nchunk, ht, o, ne, s, prec); });
+        push(gn, [=](hipStream_t s) { launch_zernike(ni, part, nchunk, ht, o, ne, s, prec); });
       } break;
       case A_CHEBYSHEV: {
         std::vector<GemmDesc> g1(ne), g2(ne);
@@ -1191,7 +1192,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         // FP64 in every precision: the order-20 contraction streams the plane
         // once at ~5 flops per byte (HBM-bound), so reduced precision buys no
         // time, and its 32-bin histogram stays exact
-        push(gn, [=]() {
+        push(gn, [=](hipStream_t s) {
           launch_cheb_gemm(d1, ne, maxH, maxN, s);
           launch_cheb_gemm(d2, ne, maxN, maxN, s);
           launch_hist_values(vd, 32, o, ne, s);
@@ -1314,7 +1315,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         const double slack = c->cf_slack;
         // FP64 in every precision: the certified histogram needs FP64 sums
         // (an FP32 bound over the disk would leave most bins open)
-        push(gn, [=]() {
+        push(gn, [=](hipStream_t s) {
           if (nds) launch_downsample(dsd, nds, moh, mow, mnh, mnw, s);
           launch_cf(mgrpd, nmg, nsplit, grpd, ngrp, cdd, ne, open, slack, o, s);
         });
@@ -1340,7 +1341,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         long *capd = A.stage(cap);
         Stats *st2 = A.dalloc_n<Stats>(2 * ne);
         double *med = A.dalloc_n<double>(2 * ne);
-        push(gn, [=]() {
+        push(gn, [=](hipStream_t s) {
           launch_prewitt(ni, mgd, drd, ne, maxp, s);
           launch_stats(mdd, st2, 2 * ne, s);
           launch_sort(mdd, sbd, capd, 2 * ne, s);
@@ -1363,7 +1364,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         }
         ObjDesc *odd = A.stage(od);
         const int inv = alg == A_INV_OTSU;
-        push(gn, [=]() { gather(); launch_objects(ni, nst, inv, odd, o, ne, (int)maxp, s); });
+        push(gn, [=](hipStream_t s) { gather(s); launch_objects(ni, nst, inv, odd, o, ne, (int)maxp, s); });
       } break;
       case A_GABOR: {
         const int prec = plan->precision;
@@ -1380,7 +1381,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         const double *gk = (const double *)c->gabor.p;
         const void *k0 = prec == 1 ? c->gabor_f32.p : c->gabor_bhi.p, *k1 = c->gabor_bmd.p, *k2 = c->gabor_blo.p;
         const double *wsum = (const double *)c->gabor_wsum.p;
-        push(gn, [=]() {
+        push(gn, [=](hipStream_t s) {
           if (prec) launch_gabor_mixed(prec, ni, k0, k1, k2, wsum, gk, end_, bdd, gprm, o, ne, mh, mw, s);
           else launch_gabor(ni, gk, end_, o, ne, mh, mw, s);
         });
@@ -1419,7 +1420,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
       Stats *nst = L.st;
       const Stats *const *sts = L.stsrc;
       const int ne = L.ne;
-      push("Pixel Intensity Statistics / Gini", [=]() {
+      push("Pixel Intensity Statistics / Gini", [=](hipStream_t s) {
         launch_gather_stats(sts, nst, ne, s);
         launch_sort(ni, sbd, capd, ne, s);
         launch_median_gini(ni, sbd, nst, nullptr, op, og, ne, s);
@@ -1565,32 +1566,36 @@ static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, s
     }
   size_t slot = 0;
   hipError_t ev_err = hipSuccess;
-  auto issue = [&](const LaunchItem &L) {
-    hipStream_t st = (L.side && !c->serial) ? c->side : c->stream;
+  auto issue = [&](const LaunchItem &L, hipStream_t st) {
     if (timing && ev_err == hipSuccess) ev_err = hipEventRecord(c->evpool[2 * slot], st);
-    L.f();
+    L.f(st);
     if (timing) {
       if (ev_err == hipSuccess) ev_err = hipEventRecord(c->evpool[2 * slot + 1], st);
       c->tnames.push_back(L.name);
     }
     slot++;
   };
-  for (size_t k = 0; k < first_feature; k++) issue(launches[k]);
-  bool forked = false;
+  for (size_t k = 0; k < first_feature; k++) issue(launches[k], c->stream);
+  // feature launches read only the node planes and stats and write their own
+  // outputs: the Haralick chain goes to the side stream and the others
+  // alternate between the main and the feature stream, all forked from the
+  // main stream once the nodes are issued and joined back at the end
+  const bool fork = !c->serial && launches.size() > first_feature;
+  if (fork) {
+    HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    HIPCHK(c, hipStreamWaitEvent(c->fstream, c->fork_ev, 0));
+  }
   for (size_t k = first_feature; k < launches.size(); k++)
-    if (launches[k].side) {
-      if (!forked && !c->serial) {
-        HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
-        HIPCHK(c, hipStreamWaitEvent(c->side, c->fork_ev, 0));
-        forked = true;
-      }
-      issue(launches[k]);
-    }
+    if (launches[k].side) issue(launches[k], fork ? c->side : c->stream);
+  size_t nmain = 0;
   for (size_t k = first_feature; k < launches.size(); k++)
-    if (!launches[k].side) issue(launches[k]);
-  if (forked) {
+    if (!launches[k].side) issue(launches[k], (fork && (nmain++ & 1)) ? c->fstream : c->stream);
+  if (fork) {
     HIPCHK(c, hipEventRecord(c->join_ev, c->side));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
+    HIPCHK(c, hipEventRecord(c->fjoin_ev, c->fstream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->fjoin_ev, 0));
   }
   HIPCHK(c, ev_err);
   HIPCHK(c, hipGetLastError());
@@ -1755,10 +1760,14 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
     return WNDGPU_E_DEVICE;
   }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->fstream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fjoin_ev, hipEventDisableTiming) != hipSuccess) {
+    if (c->fjoin_ev) release_check(hipEventDestroy(c->fjoin_ev), "hipEventDestroy");
     if (c->join_ev) release_check(hipEventDestroy(c->join_ev), "hipEventDestroy");
     if (c->fork_ev) release_check(hipEventDestroy(c->fork_ev), "hipEventDestroy");
+    if (c->fstream) release_check(hipStreamDestroy(c->fstream), "hipStreamDestroy");
     if (c->side) release_check(hipStreamDestroy(c->side), "hipStreamDestroy");
     release_check(hipStreamDestroy(c->stream), "hipStreamDestroy");
     delete c;
@@ -1787,6 +1796,7 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   if (!c) return;
   release_check(hipSetDevice(c->device), "hipSetDevice");
   release_check(hipStreamSynchronize(c->side), "hipStreamSynchronize");
+  release_check(hipStreamSynchronize(c->fstream), "hipStreamSynchronize");
   release_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   c->pool.reset();  // runs the queued table tasks to the end and joins the threads
   drop_pending(c, LONG_MAX);
@@ -1814,6 +1824,8 @@ void wndgpu_destroy(wndgpu_ctx *c) {
   if (c->host) release_check(hipHostFree(c->host), "hipHostFree");
   release_check(hipEventDestroy(c->fork_ev), "hipEventDestroy");
   release_check(hipEventDestroy(c->join_ev), "hipEventDestroy");
+  release_check(hipEventDestroy(c->fjoin_ev), "hipEventDestroy");
+  release_check(hipStreamDestroy(c->fstream), "hipStreamDestroy");
   release_check(hipStreamDestroy(c->side), "hipStreamDestroy");
   release_check(hipStreamDestroy(c->stream), "hipStreamDestroy");
   delete c;
@@ -2051,8 +2063,9 @@ static int compute_common(wndgpu_ctx *c, const wndgpu_plan *plan, const wndgpu_i
   c->t_host_ms = ms_since(t_run);
   const auto t_dev = std::chrono::steady_clock::now();
   if (r) {
-    // work already issued on either stream may still read the staging buffers
+    // work already issued on any of the streams may still read the staging buffers
     (void)hipStreamSynchronize(c->side);
+    (void)hipStreamSynchronize(c->fstream);
     (void)hipStreamSynchronize(c->stream);
     return r;
   }
