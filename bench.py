@@ -731,7 +731,7 @@ def main():
     torch.cuda.synchronize()
     forms['batched_fp64_graph'] = round(3 * len(imgs) / (time.perf_counter() - t0), 3)
     _, _, _, sched = wndcharm.graph_stats()
-    forms['graph_schedule'] = {0: 'Haralick chain on a side stream', 1: 'one stream'}.get(sched, 'undecided')
+    forms['graph_schedule'] = {0: 'Haralick chain on a side stream, other features on two streams', 1: 'one stream'}.get(sched, 'undecided')
     forms['graph_equals_eager'] = bool(np.array_equal(out.cpu().numpy(), out_eager.cpu().numpy(), equal_nan=True))
     wndcharm.set_graph_mode(0)
     npi = min(40, len(imgs))
