@@ -233,3 +233,16 @@ def test_unbalanced_fit_end_to_end():
         np.testing.assert_allclose(back.values, got[:, order], rtol=1e-7, atol=0)
     finally:
         shutil.rmtree(tmp)
+
+
+@pytest.mark.parametrize('shape', [(16, 16), (17, 23), (31, 16), (16, 40)])
+def test_long_set_smallest_planes(shape):
+    """the long set on planes at the smallest accepted sides (16): tiny
+    Haralick tone counts and Chebyshev-Fourier orders (N = (min side - 1) / 2),
+    short FFT lines with prime lengths, wavelet nodes below 16 pixels,
+    against the oracle (integer-valued columns exact, the rest to 1e-6)"""
+    img = synthetic.image(7000 + shape[0] * 100 + shape[1], 5, shape)
+    print('long set on', shape, flush=True)
+    got = wndcharm.compute_features([img], wndcharm.StdFeatureComputationPlans.getFeatureSetLong())[0]
+    want = orc_features(img.astype(np.float64), True)
+    compare_rows(got, want, exact_mask_for_names(orc_names(True)), str(shape))
