@@ -1128,6 +1128,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         int *lut_all = A.dalloc_n<int>(256 * (size_t)ne);
         int *ng_all = A.dalloc_n<int>(ne);
         double *work_all = A.dalloc_n<double>(4 * (size_t)ne);
+        int *order = A.dalloc_n<int>(ne);
         for (int k = 0; k < ne; k++) {
           HarDesc &H = hd[k];
           H.img = nullptr;
@@ -1152,9 +1153,10 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           launch_gather_stats(sts, nst, ne, s2);
           launch_fill_int(lut_all, 256 * (long)ne, 0, s2);
           launch_haralick_prep(ni, nst, hdd, ne, (int)maxp, s2);
+          launch_haralick_order(ng_all, ne, order, s2);
         });
-        push_side(gn + " [k_har_hess]", [=](hipStream_t s2) { launch_haralick_hess(hdd, ne, s2); });
-        push_side(gn + " [k_har_qr]", [=](hipStream_t s2) { launch_haralick_qr(hdd, ne, s2); });
+        push_side(gn + " [k_har_hess]", [=](hipStream_t s2) { launch_haralick_hess(hdd, order, ne, s2); });
+        push_side(gn + " [k_har_qr]", [=](hipStream_t s2) { launch_haralick_qr(hdd, order, ne, s2); });
         push_side(gn + " [mean, range]", [=](hipStream_t s2) { launch_haralick_final(hdd, o, ne, s2); });
       } break;
       case A_ZERNIKE: {

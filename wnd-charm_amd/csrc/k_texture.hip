@@ -580,7 +580,8 @@ __device__ __forceinline__ void hess_colpass(gptr<double> A, int n, int i, bool 
 // tau = 0 (a one-element reflector): an exact no-op, not performed.
 // Two blocks per CU (waves_per_eu 2): the ~500 matrices in flight then fit
 // in the MALL, which serves the re-reads of later steps.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_har_hess(const HarDesc *hd) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_har_hess(const HarDesc *hd, const int *order) {
   __shared__ double tile[256 * HTP];
   __shared__ double essb[2][256];  // reflector of step i and of step i+1
   __shared__ double twb[2][256];   // left dots by column: current step, next step
@@ -588,7 +589,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   __shared__ double xcol[256];
   __shared__ double sc[4];
   const int angle = blockIdx.x;
-  const HarDesc H = hd[blockIdx.y];
+  const HarDesc H = hd[order[blockIdx.y]];
   const int n = *H.ng;
   if (n < 2) return;
   gptr<double> A = to_global(H.Q + (long)angle * n * n);
@@ -782,7 +783,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 // in the same order either way.
 template <int NT>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(NT == 64 ? 4 : 1)))
-k_har_qr(const HarDesc *hd) {
+k_har_qr(const HarDesc *hd, const int *order) {
   constexpr int RS = NT / 32;             // window rows per pass of the block (32 columns per row)
   constexpr int CH = NT == 64 ? QW : NT;  // rows above the window staged per chunk
   __shared__ double Rw[CH * QWP];
@@ -793,7 +794,7 @@ k_har_qr(const HarDesc *hd) {
   __shared__ double dred[8];
   __shared__ int s_iu;
   const int angle = blockIdx.x;
-  const HarDesc H = hd[blockIdx.y];
+  const HarDesc H = hd[order[blockIdx.y]];
   const int n = *H.ng;
   gptr<double> A = to_global(H.Q + (long)angle * n * n);
   const int t = threadIdx.x;
@@ -1278,14 +1279,60 @@ void launch_haralick_prep(const Img *imgs, const Stats *st, const HarDesc *hd, i
   k_har_stats<<<dim3(4, nimg), 256, 0, s>>>(hd);
   k_har_q<<<xcd_grid(QRB, 4 * nimg), 256, 0, s>>>(hd, nimg);
 }
-void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s) {
-  k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd);
+// Entries in decreasing order of tones (ng), for the eigen kernels: their
+// cost grows with ng (about ng^2 for the Schur sweeps, ng^3 for the
+// Hessenberg reduction), and ng ranges over 10..256 within a batch, so
+// dispatching the largest matrices first keeps the last wave of blocks
+// short (longest processing time first).  Every matrix is computed on its
+// own, so the order changes no result.  Thread b owns key b = 256 - ng and
+// scans the entries in index order (keys staged in LDS, chunk by chunk), so
+// ties keep index order and the order is the same on every run.
+constexpr int ORDER_CHUNK = 16384;
+__global__ void __launch_bounds__(320) k_har_order(const int *ng, int nimg, int *order) {
+  __shared__ __align__(16) uint16_t key[ORDER_CHUNK];
+  __shared__ int cnt[257];
+  const int t = threadIdx.x;
+  for (int b = t; b < 257; b += blockDim.x) cnt[b] = 0;
+  __syncthreads();
+  for (int e = t; e < nimg; e += blockDim.x) atomicAdd(&cnt[256 - min(max(ng[e], 0), 256)], 1);
+  __syncthreads();
+  if (t == 0) {
+    int sum = 0;
+    for (int b = 0; b < 257; b++) {
+      const int c = cnt[b];
+      cnt[b] = sum;
+      sum += c;
+    }
+  }
+  __syncthreads();
+  int pos = t < 257 ? cnt[t] : 0;
+  for (int base = 0; base < nimg; base += ORDER_CHUNK) {
+    const int m = min(ORDER_CHUNK, nimg - base), m8 = (m + 7) & ~7;
+    __syncthreads();  // the previous chunk's keys are consumed
+    for (int e = t; e < m8; e += blockDim.x)  // pad to eight keys with a key no thread owns
+      key[e] = e < m ? (uint16_t)(256 - min(max(ng[base + e], 0), 256)) : (uint16_t)0xffff;
+    __syncthreads();
+    if (t < 257)
+      for (int e = 0; e < m8; e += 8) {  // eight keys per LDS read
+        const uint4 kk = *(const uint4 *)(key + e);
+        const unsigned w[4] = {kk.x, kk.y, kk.z, kk.w};
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+          if ((int)((w[j >> 1] >> (16 * (j & 1))) & 0xffff) == t) order[pos++] = base + e + j;
+      }
+  }
 }
-void launch_haralick_qr(const HarDesc *hd, int nimg, hipStream_t s) {
+void launch_haralick_order(const int *ng, int nimg, int *order, hipStream_t s) {
+  k_har_order<<<1, 320, 0, s>>>(ng, nimg, order);
+}
+void launch_haralick_hess(const HarDesc *hd, const int *order, int nimg, hipStream_t s) {
+  k_har_hess<<<dim3(4, nimg), 256, 0, s>>>(hd, order);
+}
+void launch_haralick_qr(const HarDesc *hd, const int *order, int nimg, hipStream_t s) {
   // up to 1024 matrices: one 256-thread block each (all resident at once);
   // beyond, one wavefront each
-  if (4 * nimg <= 1024) k_har_qr<256><<<dim3(4, nimg), 256, 0, s>>>(hd);
-  else k_har_qr<64><<<dim3(4, nimg), 64, 0, s>>>(hd);
+  if (4 * nimg <= 1024) k_har_qr<256><<<dim3(4, nimg), 256, 0, s>>>(hd, order);
+  else k_har_qr<64><<<dim3(4, nimg), 64, 0, s>>>(hd, order);
 }
 void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s) {
   k_har_final<<<nimg, 64, 0, s>>>(hd, o);

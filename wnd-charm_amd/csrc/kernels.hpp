@@ -153,8 +153,12 @@ struct HarDesc {
 // Haralick in four stream-ordered steps: quantise / co-occurrence / f1..f13
 // / Q, Hessenberg reduction of Q, Francis QR (f14), mean and range output
 void launch_haralick_prep(const Img *imgs, const Stats *st, const HarDesc *hd, int nimg, int maxn, hipStream_t s);
-void launch_haralick_hess(const HarDesc *hd, int nimg, hipStream_t s);
-void launch_haralick_qr(const HarDesc *hd, int nimg, hipStream_t s);
+// order: the entries by decreasing ng (launch_haralick_order, from the
+// entries' ng values, contiguous), the dispatch order of the Hessenberg and
+// Schur kernels
+void launch_haralick_order(const int *ng, int nimg, int *order, hipStream_t s);
+void launch_haralick_hess(const HarDesc *hd, const int *order, int nimg, hipStream_t s);
+void launch_haralick_qr(const HarDesc *hd, const int *order, int nimg, hipStream_t s);
 void launch_haralick_final(const HarDesc *hd, OutRef o, int nimg, hipStream_t s);
 void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, OutRef o, int nimg, int maxh, int maxw,
                    hipStream_t s);
