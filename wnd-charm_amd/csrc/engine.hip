@@ -728,12 +728,14 @@ static void out_size(int kind, int h, int w, int *oh, int *ow) {
   else { *oh = h; *ow = w; }
 }
 
-// One launch closure of a batch: its name (timing), the kernels, and whether
-// it runs on the side stream.
+// One launch closure of a batch: its name (timing), the kernels, whether it
+// runs on the side stream, and whether it is issued after the other feature
+// launches of its stream (late).
 struct LaunchItem {
   std::string name;
   std::function<void(hipStream_t)> f;  // issues the launch's kernels on the given stream
   bool side;
+  bool late;
 };
 
 // Root of a sampled batch: load each image's crop, Downsample it (pixels and
@@ -880,10 +882,13 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
   std::map<std::vector<int>, NodeRun> nodes;
   auto maxof = [](const std::vector<int> &v) { return *std::max_element(v.begin(), v.end()); };
   auto push = [&](const std::string &name, std::function<void(hipStream_t)> f) {
-    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), false});
+    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), false, false});
   };
   auto push_side = [&](const std::string &name, std::function<void(hipStream_t)> f) {
-    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), true});
+    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), true, false});
+  };
+  auto push_late = [&](const std::string &name, std::function<void(hipStream_t)> f) {
+    if (!A.sizing) launches.push_back(LaunchItem{name, std::move(f), false, true});
   };
   // ---- root: load + stats
   if (B.sampled) {
@@ -1164,7 +1169,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         double *part = A.dalloc_n<double>((size_t)ne * (4 + nchunk * 144));
         const double *ht = (const double *)c->zern.p;
         const int prec = plan->precision;
-        push(gn, [=](hipStream_t s) { launch_zernike(ni, part, nchunk, ht, o, ne, s, prec); });
+        push_late(gn, [=](hipStream_t s) { launch_zernike(ni, part, nchunk, ht, o, ne, s, prec); });
       } break;
       case A_CHEBYSHEV: {
         std::vector<GemmDesc> g1(ne), g2(ne);
@@ -1383,7 +1388,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
         const double *gk = (const double *)c->gabor.p;
         const void *k0 = prec == 1 ? c->gabor_f32.p : c->gabor_bhi.p, *k1 = c->gabor_bmd.p, *k2 = c->gabor_blo.p;
         const double *wsum = (const double *)c->gabor_wsum.p;
-        push(gn, [=](hipStream_t s) {
+        push_late(gn, [=](hipStream_t s) {
           if (prec) launch_gabor_mixed(prec, ni, k0, k1, k2, wsum, gk, end_, bdd, gprm, o, ne, mh, mw, s);
           else launch_gabor(ni, gk, end_, o, ne, mh, mw, s);
         });
@@ -1581,7 +1586,13 @@ static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, s
   // feature launches read only the node planes and stats and write their own
   // outputs: the Haralick chain goes to the side stream and the others
   // alternate between the main and the feature stream, all forked from the
-  // main stream once the nodes are issued and joined back at the end
+  // main stream once the nodes are issued and joined back at the end.  The
+  // chain's Hessenberg and Schur kernels are memory-bound and fill every CU's
+  // LDS while they run; a kernel issued meanwhile gets a CU only as their
+  // blocks leave one, and holds up whatever follows it on its stream.  So
+  // the long compute-bound launches (Zernike, Gabor: marked late) go last on
+  // their streams, where they fill in beside the chain, after the short
+  // memory-bound ones, which then run early next to the chain's first kernels.
   const bool fork = !c->serial && launches.size() > first_feature;
   if (fork) {
     HIPCHK(c, hipEventRecord(c->fork_ev, c->stream));
@@ -1591,8 +1602,10 @@ static int issue_batch(wndgpu_ctx *c, const std::vector<LaunchItem> &launches, s
   for (size_t k = first_feature; k < launches.size(); k++)
     if (launches[k].side) issue(launches[k], fork ? c->side : c->stream);
   size_t nmain = 0;
-  for (size_t k = first_feature; k < launches.size(); k++)
-    if (!launches[k].side) issue(launches[k], (fork && (nmain++ & 1)) ? c->fstream : c->stream);
+  for (const bool late : {false, true})
+    for (size_t k = first_feature; k < launches.size(); k++)
+      if (!launches[k].side && launches[k].late == late)
+        issue(launches[k], (fork && (nmain++ & 1)) ? c->fstream : c->stream);
   if (fork) {
     HIPCHK(c, hipEventRecord(c->join_ev, c->side));
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));

@@ -628,15 +628,17 @@ k_har_hess(const HarDesc *hd, const int *order) {
         int r0 = lr;
         asm volatile("" : "+v"(r0));
         double *tl0 = tile + (r0 * HTP + lc);
+        // every LDS operand is read before the arithmetic (clamped indices,
+        // unused where clamped), so the reads are in flight together rather
+        // than one round trip per row
+        double e[HT];
+#pragma unroll
+        for (int q = 0; q < HT; q++) e[q] = ess[min(max(rb + r0 + HS * q - i - 2, 0), 255)];
 #pragma unroll
         for (int q = 0; q < HT; q++) {
           const int r = rb + r0 + HS * q;
-          double v = rv[q];
-          if (r < n) {
-            const double fac = (r == i + 1) ? tau : tau * ess[r - i - 2];
-            v = v - fac * w;
-          }
-          tl0[HS * q * HTP] = v;  // rows >= n are never read
+          const double fac = (r == i + 1) ? tau : tau * e[q];
+          tl0[HS * q * HTP] = (r < n) ? rv[q] - fac * w : rv[q];  // rows >= n are never read
         }
       }
       __syncthreads();
@@ -644,15 +646,25 @@ k_har_hess(const HarDesc *hd, const int *order) {
       if (t < rem) {
         const double *tr = tile + t * HTP;
 #pragma unroll
-        for (int cc = 0; cc < HT; cc++) {
-          if (cc < ncol) {
-            const int j = ct + cc - 1;  // ct is even: j is even for odd cc
-            const double v = tr[cc];
-            if (j < 0) col0 = v;
-            else if (j < asz) {
-              if (cc & 1) l0 = l0 + v * ess[j];
-              else l1 = l1 + v * ess[j];
-            } else tl = v * ess[j];
+        for (int h = 0; h < HT; h += HT / 2) {  // half a tile row at a time: its LDS reads in flight together
+          double tv[HT / 2], te[HT / 2];
+#pragma unroll
+          for (int u = 0; u < HT / 2; u++) {
+            tv[u] = tr[h + u];
+            te[u] = ess[min(max(ct + h + u - 1, 0), 255)];
+          }
+#pragma unroll
+          for (int u = 0; u < HT / 2; u++) {
+            const int cc = h + u;
+            if (cc < ncol) {
+              const int j = ct + cc - 1;  // ct is even: j is even for odd cc
+              const double v = tv[u];
+              if (j < 0) col0 = v;
+              else if (j < asz) {
+                if (cc & 1) l0 = l0 + v * te[u];
+                else l1 = l1 + v * te[u];
+              } else tl = v * te[u];
+            }
           }
         }
       }
