@@ -1106,6 +1106,10 @@ k_har_qr(const HarDesc *hd, const int *order) {
         }
       }
       __syncthreads();
+      // the window's steps whose reflector is applied (beta != 0), one bit per
+      // step: a uniform mask, so the strip loops branch on it without an LDS
+      // round trip per step
+      const unsigned long long fon = __ballot((t & 63) < QB && rf[min(t & 63, QB - 1)][3] != 0.0);
       for (int r = t >> 5; r < wd; r += RS)
         if ((t & 31) < wd) AT(a + r, a + (t & 31)) = Rw[r * QWP + (t & 31)];
 #undef RW
@@ -1124,13 +1128,17 @@ k_har_qr(const HarDesc *hd, const int *order) {
             pl += n;
           }
         }
+        // every load has arrived before the first store: the stores below then
+        // never stand in the way of a load's wait (the vector-memory counter
+        // counts both)
+        __builtin_amdgcn_s_waitcnt(0);
         gptr<double> ps = p;
         double u0 = w[0], u1 = w[1];
 #pragma unroll
         for (int q = 0; q < QB; q++) {
           if (q < nstep) {
             double u2 = w[q + 2];
-            if (rf[q][3] != 0.0) {
+            if ((fon >> q) & 1) {
               const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
               double tt = e0 * u1;
               tt += e1 * u2;
@@ -1173,22 +1181,31 @@ k_har_qr(const HarDesc *hd, const int *order) {
         }
         __syncthreads();
         if (t < nr) {
+          // the row's entries first (LDS reads in flight together; entries
+          // past the window's columns are read but left as they are)
           double *row = Rw + t * QWP;
-          double u0 = row[0], u1 = row[1];
-          for (int q = 0; q < nstep; q++) {
-            double u2 = row[q + 2];
-            if (rf[q][3] != 0.0) {
-              const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
-              double tt = u1 * e0;
-              tt += u2 * e1;
-              tt += u0;
-              u0 -= tau * tt;
-              u1 -= (tau * tt) * e0;
-              u2 -= (tau * tt) * e1;
+          double x[QB + 2];
+#pragma unroll
+          for (int q = 0; q < QB + 2; q++) x[q] = row[q];
+          double u0 = x[0], u1 = x[1];
+#pragma unroll
+          for (int q = 0; q < QB; q++) {
+            if (q < nstep) {
+              double u2 = x[q + 2];
+              if ((fon >> q) & 1) {
+                const double tau = rf[q][0], e0 = rf[q][1], e1 = rf[q][2];
+                double tt = u1 * e0;
+                tt += u2 * e1;
+                tt += u0;
+                u0 -= tau * tt;
+                u1 -= (tau * tt) * e0;
+                u2 -= (tau * tt) * e1;
+              }
+              row[q] = u0;
+              u0 = u1;
+              u1 = u2;
             }
-            row[q] = u0;
-            u0 = u1;
-            u1 = u2;
+            __asm__ volatile("" ::: "memory");  // the reflector reads step by step
           }
           row[nstep] = u0;
           row[nstep + 1] = u1;
