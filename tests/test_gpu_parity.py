@@ -168,3 +168,27 @@ def test_haralick_large_batch_equals_small_batches():
     big = wndcharm.compute_features(imgs, plan)
     small = np.concatenate([wndcharm.compute_features(imgs[:130], plan), wndcharm.compute_features(imgs[130:], plan)])
     assert np.array_equal(big, small, equal_nan=True)
+
+
+HARALICK_NODES = ['()', '(Chebyshev ())', '(Chebyshev (Fourier ()))', '(Chebyshev (Wavelet ()))', '(Edge ())',
+                  '(Fourier ())', '(Fourier (Chebyshev ()))', '(Fourier (Edge ()))', '(Fourier (Wavelet ()))',
+                  '(Wavelet ())', '(Wavelet (Edge ()))', '(Wavelet (Fourier ()))']
+
+
+def test_haralick_dispatch_order_beyond_one_chunk():
+    """the eigen kernels take their matrices largest first through an order
+    that a one-block counting sort builds from the entries' tone counts, its
+    keys staged in LDS 16,384 entries at a time.  1,400 images x the long
+    set's 12 Haralick nodes are 16,800 entries, two chunks: every entry must
+    still be computed exactly once, so one call equals two calls of 700
+    (8,400 entries, one chunk) bit for bit"""
+    imgs = [synthetic.image(6100 + k, 3 + k % 11, (24 + k % 9, 24 + (k // 9) % 7)) for k in range(1400)]
+    plan = wndcharm.FeatureComputationPlan('haralick, 12 nodes')
+    for node in HARALICK_NODES:
+        plan.add('Haralick Textures ' + node)
+    plan.finalize()
+    print('haralick 1400 x 12 nodes vs 2 x 700', flush=True)
+    big = wndcharm.compute_features(imgs, plan)
+    small = np.concatenate([wndcharm.compute_features(imgs[:700], plan), wndcharm.compute_features(imgs[700:], plan)])
+    assert big.shape == (1400, 12 * 28) and np.isfinite(big).any()
+    assert np.array_equal(big, small, equal_nan=True)
