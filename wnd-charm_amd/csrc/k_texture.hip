@@ -131,6 +131,7 @@ __global__ void __launch_bounds__(1024) k_har_cooc(const Img *imgs, const HarDes
 // f1..f13 (CVIPtexture.cpp:631-958) and Q (_Q_calc_eigen, 961-987) per angle
 __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
   __shared__ double px[256], py[256], pxpy_s[512], pxpy_d[256];
+  __shared__ double lse[512], lde[256], lhx[256], lhy[256];  // the entropy terms, formed in parallel
   __shared__ double shd[4];
   const int angle = blockIdx.x;
   const HarDesc H = hd[blockIdx.y];
@@ -156,10 +157,13 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     pxpy_s[k] = s;
   }
   if (t < ng) {
+    // both loads at clamped (in-range) columns every row, so they are not
+    // behind a branch and can be in flight together; the adds are as before
     double s = 0;
     for (int i = 0; i < ng; i++) {
-      if (i - t >= 0) s += P[i * ng + i - t];
-      if (t > 0 && i + t < ng) s += P[i * ng + i + t];
+      const double lo = P[i * ng + max(i - t, 0)], hi = P[i * ng + min(i + t, ng - 1)];
+      if (i - t >= 0) s += lo;
+      if (t > 0 && i + t < ng) s += hi;
     }
     pxpy_d[t] = s;
   }
@@ -191,6 +195,19 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     var4 += (i - mean4) * (i - mean4) * P[e];
   }
   var4 = block_reduce(var4, SumOp(), shd);
+  // the terms of the sum entropy, difference entropy and HX / HY, each formed
+  // by one thread with the serial loop's operations; thread 0 then adds them
+  // in the loops' order
+  for (int k = t; k <= 2 * ng - 2; k += blockDim.x) {
+    const double v = pxpy_s[k];
+    lse[k] = v * log10(v + EPS) / LOG2;
+  }
+  if (t < ng) {
+    lde[t] = pxpy_d[t] * log10(pxpy_d[t] + EPS) / LOG2;
+    lhx[t] = px[t] * log10(px[t] + EPS) / LOG2;
+    lhy[t] = py[t] * log10(py[t] + EPS) / LOG2;
+  }
+  __syncthreads();
   if (t == 0) {
     double *F = H.feat + angle * 14;
     F[0] = f1;
@@ -207,10 +224,7 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     for (int i = 0; i <= 2 * ng - 2; i++) savg += i * pxpy_s[i];
     F[5] = savg;
     double se = 0;
-    for (int i = 2; i <= 2 * ng; i++) {
-      const double v = (i - 2 <= 2 * ng - 2) ? pxpy_s[i - 2] : 0.0;
-      se -= v * log10(v + EPS) / LOG2;
-    }
+    for (int i = 2; i <= 2 * ng; i++) se -= lse[i - 2];  // i - 2 <= 2 ng - 2 throughout
     F[7] = se;
     double sv = 0;
     for (int i = 0; i <= 2 * ng - 2; i++) sv += (i - se) * (i - se) * pxpy_s[i];
@@ -219,12 +233,12 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
     double s = 0, ss = 0, de = 0;
     for (int i = 0; i < ng; i++) { s += i * pxpy_d[i]; ss += i * i * pxpy_d[i]; }
     F[9] = ss - s * s;
-    for (int i = 0; i < ng; i++) de += pxpy_d[i] * log10(pxpy_d[i] + EPS) / LOG2;
+    for (int i = 0; i < ng; i++) de += lde[i];
     F[10] = -de;
     double hx = 0, hy = 0;
     for (int i = 0; i < ng; i++) {
-      hx -= px[i] * log10(px[i] + EPS) / LOG2;
-      hy -= py[i] * log10(py[i] + EPS) / LOG2;
+      hx -= lhx[i];
+      hy -= lhy[i];
     }
     const double hxy = -ent;
     const double mxh = hx > hy ? hx : hy;
