@@ -143,9 +143,19 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
   // loops).  P is symmetric (k_har_cooc stores the counts as a triangle),
   // so row t equals column t term by term and both sums walk the column,
   // which the lanes read coalesced.
+  // (each sequential sum loads its terms eight at a time, then adds them in
+  // order: the loads are in flight together instead of one latency a term)
   if (t < ng) {
     double b = 0;
-    for (int i = 0; i < ng; i++) b += P[i * ng + t];
+    int i = 0;
+    for (; i + 8 <= ng; i += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = P[(i + q) * ng + t];
+#pragma unroll
+      for (int q = 0; q < 8; q++) b += v[q];
+    }
+    for (; i < ng; i++) b += P[i * ng + t];
     px[t] = b;
     py[t] = b;
   }
@@ -153,14 +163,36 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
   for (int k = t; k < 2 * ng - 1; k += blockDim.x) {
     double s = 0;
     const int i0 = k - (ng - 1) > 0 ? k - (ng - 1) : 0, i1 = k < ng - 1 ? k : ng - 1;
-    for (int i = i0; i <= i1; i++) s += P[i * ng + (k - i)];
+    int i = i0;
+    for (; i + 8 <= i1 + 1; i += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = P[(i + q) * ng + (k - i - q)];
+#pragma unroll
+      for (int q = 0; q < 8; q++) s += v[q];
+    }
+    for (; i <= i1; i++) s += P[i * ng + (k - i)];
     pxpy_s[k] = s;
   }
   if (t < ng) {
     // both loads at clamped (in-range) columns every row, so they are not
     // behind a branch and can be in flight together; the adds are as before
     double s = 0;
-    for (int i = 0; i < ng; i++) {
+    int i = 0;
+    for (; i + 4 <= ng; i += 4) {
+      double lo[4], hi[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        lo[q] = P[(i + q) * ng + max(i + q - t, 0)];
+        hi[q] = P[(i + q) * ng + min(i + q + t, ng - 1)];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        if (i + q - t >= 0) s += lo[q];
+        if (t > 0 && i + q + t < ng) s += hi[q];
+      }
+    }
+    for (; i < ng; i++) {
       const double lo = P[i * ng + max(i - t, 0)], hi = P[i * ng + min(i + t, ng - 1)];
       if (i - t >= 0) s += lo;
       if (t > 0 && i + t < ng) s += hi;
@@ -170,9 +202,18 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
   __syncthreads();
   // raster double sums (parallel)
   double f1 = 0, tmp3 = 0, mean4 = 0, idm = 0, ent = 0, hxy1 = 0, hxy2 = 0;
-  for (int e = t; e < ng * ng; e += blockDim.x) {
+  const int nn = ng * ng, bs = blockDim.x;
+  double pb[4] = {0.0, 0.0, 0.0, 0.0};  // this thread's next four elements, loaded ahead
+  if (nn > 0)
+#pragma unroll
+    for (int q = 0; q < 4; q++) pb[q] = P[min(t + q * bs, nn - 1)];
+  for (int e = t; e < nn; e += bs) {
     const int i = e / ng, j = e - i * ng;
-    const double p = P[e];
+    const double p = pb[0];
+    pb[0] = pb[1];
+    pb[1] = pb[2];
+    pb[2] = pb[3];
+    pb[3] = P[min(e + 4 * bs, nn - 1)];  // four elements ahead (clamped: unused past the end)
     f1 += p * p;
     tmp3 += i * j * p;
     mean4 += i * p;
@@ -190,9 +231,22 @@ __global__ void __launch_bounds__(256) k_har_stats(const HarDesc *hd) {
   hxy1 = block_reduce(hxy1, SumOp(), shd);
   hxy2 = block_reduce(hxy2, SumOp(), shd);
   double var4 = 0;
-  for (int e = t; e < ng * ng; e += blockDim.x) {
-    const int i = e / ng;
-    var4 += (i - mean4) * (i - mean4) * P[e];
+  {
+    int e = t;
+    for (; e + 7 * bs < nn; e += 8 * bs) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) v[q] = P[e + q * bs];
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int i = (e + q * bs) / ng;
+        var4 += (i - mean4) * (i - mean4) * v[q];
+      }
+    }
+    for (; e < nn; e += bs) {
+      const int i = e / ng;
+      var4 += (i - mean4) * (i - mean4) * P[e];
+    }
   }
   var4 = block_reduce(var4, SumOp(), shd);
   // the terms of the sum entropy, difference entropy and HX / HY, each formed
