@@ -42,6 +42,7 @@ __device__ double block_otsu(long n, double mn, double mx, ValF val, int *hist) 
   for (int b = threadIdx.x; b < 1024; b += blockDim.x) hist[b] = 0;
   __syncthreads();
   const double sc = hist_scale(mn, mx, 1024);
+#pragma unroll 4
   for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[hist_bin(val(i), mn, sc, 1024)], 1);
   __syncthreads();
   __shared__ double thr_s;
@@ -422,18 +423,22 @@ __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *co
   long long orig = 0;
   for (int f = 0; f < 8; f++) {
     const double *E = E0 + (long)f * n;
+    // (the per-pixel loops are unrolled so that four loads are in flight)
     double mx = -1e308;
+#pragma unroll 4
     for (long i = threadIdx.x; i < n; i += 1024) mx = fmax(mx, E[i]);
     mx = block_reduce(mx, MaxOp(), shd);
     if (f == 0) {
       long long c = 0;
       const double lvl = mx * 0.4;
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) c += E[i] > lvl;
       orig = block_reduce(c, SumOp(), shl);
       continue;
     }
     // e / max, its min / max, Otsu threshold, count above
     double a = 1e308, b = -1e308;
+#pragma unroll 4
     for (long i = threadIdx.x; i < n; i += 1024) {
       const double v = E[i] / mx;
       if (moments_ok(v)) { a = fmin(a, v); b = fmax(b, v); }
@@ -442,6 +447,7 @@ __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *co
     b = block_reduce(b, MaxOp(), shd);
     const double thr = block_otsu(n, a, b, [&](long i) { return E[i] / mx; }, hist);
     long long c = 0;
+#pragma unroll 4
     for (long i = threadIdx.x; i < n; i += 1024) c += (E[i] / mx) > thr;
     c = block_reduce(c, SumOp(), shl);
     if (threadIdx.x == 0) out[f - 1] = (double)c / (double)orig;
