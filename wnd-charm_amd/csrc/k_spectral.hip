@@ -49,9 +49,13 @@ __global__ void __launch_bounds__(1024) k_zern_moments(const Img *imgs, double *
 // is unrolled at compile time) and are reduced across lanes at the end.
 // T: the accumulators' type (double for the plain form; float for the
 // reduced-precision forms, where the radial polynomials stay FP64).
+// Zernike coefficients are floating-point features (held to 1e-6 of the
+// reference), so this function, unlike the exact kernels, lets the compiler
+// contract multiply-adds into FMAs and divides by r^2 once per pixel.
 template <int N0, int N1, class T>
 __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int lane, double cxm, double cym, double sum,
                                            const double *H1, const double *H2, const double *H3, double *dst) {
+#pragma clang fp contract(fast)
   constexpr int NP = [] {
     int c = 0;
     for (int n = N0; n <= N1; n++) c += n / 2 + 1;
@@ -69,6 +73,7 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
     const double r2 = x * x + y * y;
     const double r = sqrt(r2);
     if (r < kDblEps || r > 1.0) continue;
+    const double ir2 = 1.0 / r2;
     double R[N1 + 1];
     R[0] = 1;
 #pragma unroll
@@ -94,7 +99,7 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
         if (m == n) { Rnm = Rn; Rnmp4 = Rn; }
         else if (m == n - 2) { const double Rnnm2 = n * Rn - (n - 1) * Rnm2; Rnm = Rnnm2; Rnmp2 = Rnnm2; }
         else {
-          Rnm = H1[n * 32 + m] * Rnmp4 + (H2[n * 32 + m] + (H3[n * 32 + m] / r2)) * Rnmp2;
+          Rnm = H1[n * 32 + m] * Rnmp4 + (H2[n * 32 + m] + H3[n * 32 + m] * ir2) * Rnmp2;
           Rnmp4 = Rnmp2;
           Rnmp2 = Rnm;
         }
