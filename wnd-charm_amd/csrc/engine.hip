@@ -363,6 +363,12 @@ struct wndgpu_ctx {
   int *cf_open = nullptr;
   int cf_n = 0;
   double cf_slack = 1;
+  // Gabor energies of the plain form: FP64 fused multiply-adds whose counts
+  // are certified (the default), or the reference's separate multiply and
+  // add throughout (gabor_direct, WNDGPU_GABOR_DIRECT=1 at creation).
+  // gabor_slack (WNDGPU_GABOR_SLACK, >= 1) scales the certification bound.
+  bool gabor_direct = false;
+  double gabor_slack = 1;
   bool sized = false;
 };
 
@@ -1375,21 +1381,25 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
       } break;
       case A_GABOR: {
         const int prec = plan->precision;
+        const bool cert = prec || !c->gabor_direct;  // energies with bounds, counts certified
         std::vector<double *> en(ne);
-        std::vector<float *> bd(prec ? ne : 0);
+        std::vector<float *> bd(cert ? ne : 0);
         for (int k = 0; k < ne; k++) {
           const size_t px = (size_t)E[k].N->h[E[k].i] * E[k].N->w[E[k].i];
           en[k] = A.dalloc_n<double>(8 * px);
-          if (prec) bd[k] = A.dalloc_n<float>(8 * px);  // error bound of each reduced-precision energy
+          if (cert) bd[k] = A.dalloc_n<float>(8 * px);  // error bound of each energy
         }
         double **end_ = A.stage(en);
-        float **bdd = prec ? A.stage(bd) : nullptr;
-        double *gprm = prec ? A.dalloc_n<double>(32 * (size_t)ne) : nullptr;  // per entry: max, min, base count
+        float **bdd = cert ? A.stage(bd) : nullptr;
+        double *gprm = cert ? A.dalloc_n<double>(32 * (size_t)ne) : nullptr;  // per entry: max, min, base count
+        int *gflag = cert ? A.dalloc_n<int>(8 * 64 * (size_t)ne) : nullptr;  // per entry and filter: open energies
         const double *gk = (const double *)c->gabor.p;
         const void *k0 = prec == 1 ? c->gabor_f32.p : c->gabor_bhi.p, *k1 = c->gabor_bmd.p, *k2 = c->gabor_blo.p;
         const double *wsum = (const double *)c->gabor_wsum.p;
+        const double gslack = c->gabor_slack;
         push_late(gn, [=](hipStream_t s) {
-          if (prec) launch_gabor_mixed(prec, ni, k0, k1, k2, wsum, gk, end_, bdd, gprm, o, ne, mh, mw, s);
+          if (prec) launch_gabor_mixed(prec, ni, k0, k1, k2, wsum, gk, end_, bdd, gprm, gflag, o, ne, mh, mw, s);
+          else if (cert) launch_gabor_certified(ni, gk, wsum, gslack, end_, bdd, gprm, gflag, o, ne, mh, mw, s);
           else launch_gabor(ni, gk, end_, o, ne, mh, mw, s);
         });
       } break;
@@ -1523,7 +1533,7 @@ static int run_batch(wndgpu_ctx *c, const wndgpu_plan *plan, Batch &B, bool timi
   int r;
   if ((r = ensure_tables(c))) return r;
   if (plan->color && (r = ensure_color_lut(c))) return r;
-  if (plan->precision && (r = ensure_mixed_tables(c))) return r;
+  if ((r = ensure_mixed_tables(c))) return r;  // (their filter sums bound the plain form's Gabor energies too)
   std::vector<LaunchItem> launches;
   size_t first_feature = 0;
   // pass 1: sizes
@@ -1800,6 +1810,8 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
   }
   c->serial = getenv("WNDGPU_SERIAL") != nullptr;
   if (const char *cs = getenv("WNDGPU_CF_SLACK")) c->cf_slack = std::max(1.0, atof(cs));
+  if (const char *gd = getenv("WNDGPU_GABOR_DIRECT")) c->gabor_direct = atoi(gd) != 0;
+  if (const char *gs = getenv("WNDGPU_GABOR_SLACK")) c->gabor_slack = std::max(1.0, atof(gs));
   if (const char *gm = getenv("WNDGPU_GRAPH")) c->graph_mode = std::min(2, std::max(0, atoi(gm)));
   if (const char *mb = getenv("WNDGPU_TABLE_CACHE_MB")) c->table_budget = (size_t)std::max(0L, atol(mb)) << 20;
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS

@@ -31,7 +31,7 @@ __device__ __forceinline__ unsigned bf16_split(float f) {
 
 // ---------------------------------------------------------------- Gabor
 // Both reduced forms write, beside each energy E, an a-priori bound D on
-// |E - E64|, E64 being the FP64 kernel's value (k_gabor_conv), so that the
+// |E - E64|, E64 being the FP64 kernel's value (k_gabor_conv_t<false>), so that the
 // exact post-processing (k_gabor_post_exact) can certify every count:
 //   re, im = sum_{j,i} x w over the 38 x 38 taps, S = sum |x| (|wr| + |wi|)
 //   <= maxwin(|x|) * wsum_f (maxwin: the largest |x| in the pixel's window,
@@ -273,7 +273,7 @@ __global__ void __launch_bounds__(256) k_gabor_conv_bf16x3(const Img *imgs, cons
 
 void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const void *k1, const void *k2,
                         const double *wsum, const double *taps64, double *const *energy, float *const *bound,
-                        double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s) {
+                        double *params, int *flags, OutRef o, int nimg, int maxh, int maxw, hipStream_t s) {
   if (precision == 1) {
     const int tiles = ((maxw + GM_TC - 1) / GM_TC) * ((maxh + GM_TR - 1) / GM_TR);
     k_gabor_conv_f32<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, (const float *)k0, wsum, energy, bound);
@@ -283,7 +283,7 @@ void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const vo
                                                              (const unsigned short *)k1, (const unsigned short *)k2,
                                                              wsum, energy, bound);
   }
-  launch_gabor_post_exact(imgs, energy, bound, taps64, params, o, nimg, maxh, maxw, s);
+  launch_gabor_post_exact(imgs, energy, bound, taps64, params, flags, false, o, nimg, maxh, maxw, s);
 }
 
 }  // namespace wnd

@@ -354,9 +354,22 @@ void launch_edge_final(const Img *mag, const Img *dir, const Stats *st2, const d
 // a group needs for the thread's 8 outputs are read from LDS once into
 // registers, and every output still adds its taps in the reference's (j, i)
 // order (conv2comp, gabor.cpp:19-52).
-__global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const double *kernels, double *const *energy) {
+//   CERT = false: separate multiply and add, the reference's rounding.
+//   CERT = true: fused multiply-adds (half the instructions), and beside
+// each energy E an a-priori bound D on its distance from the CERT = false
+// value E64, for k_gabor_post_exact to certify every count (k_gabor_repair
+// recomputes in the CERT = false order each energy a count leaves open).
+// Both sums of a component are within gamma_1444 S of the exact sum,
+// S = sum |x| |w| <= (largest |x| of the block's input tile) x wsum_f, and
+// the two magnitudes round by at most 2u each, so |E - E64| <= 2 gamma_1444
+// S + 4u E: D = coef * tmax * wsum_f + 1e-14 E, coef = 4e-13 (2 gamma_1444
+// = 3.21e-13) times the context's slack, rounded up.
+template <bool CERT>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_gabor_conv_t(const Img *imgs, const double *kernels, double *const *energy,
+                                                      const double *wsum, float *const *bound, double coef) {
   __shared__ double tile[GB_TR + GB_N - 1][GB_LD];
   __shared__ double2 wk[GB_N * GB_N];
+  __shared__ double shd[4];
   const Img im = imgs[blockIdx.z];
   const int f = blockIdx.y % 8;
   const int tilesx = (im.w + GB_TC - 1) / GB_TC, tilesy = (im.h + GB_TR - 1) / GB_TR;
@@ -366,20 +379,25 @@ __global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const doubl
   const double *K = kernels + (long)f * GB_N * GB_N * 2;
   for (int e = t; e < GB_N * GB_N; e += 256) wk[e] = make_double2(K[2 * e], K[2 * e + 1]);
   // input rows b0-18 .. b0+GB_TR+18, columns a0-18 .. a0+GB_TC+18
+  double tmax = 0.0;
   for (int r = t / 64; r < GB_TR + GB_N - 1; r += 4) {
     const int y = b0 - 18 + r;
     for (int c = t % 64; c < GB_TC + GB_N - 1; c += 64) {
       const int x = a0 - 18 + c;
-      tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+      const double v = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+      tile[r][c] = v;
+      if (CERT) tmax = fmax(tmax, fabs(v));
     }
   }
-  __syncthreads();
+  if (CERT) tmax = block_reduce(tmax, MaxOp(), shd);  // (synchronises the block)
+  else __syncthreads();
   const int ty = t / 8, tx = t % 8;
   const int b = b0 + ty, a = a0 + tx * 8;
   double re[8], imv[8];
 #pragma unroll
   for (int q = 0; q < 8; q++) re[q] = imv[q] = 0.0;
   const int rb = ty + 37, cb = tx * 8 + 37;
+#pragma unroll 1
   for (int j = 0; j < GB_N; j++) {
     const double *row = &tile[rb - j][0];
     const double2 *wr = &wk[j * GB_N];
@@ -398,8 +416,13 @@ __global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const doubl
 #pragma unroll
           for (int q = 0; q < 8; q++) {
             const double v = u[q - di + 7];
-            re[q] += v * w.x;
-            imv[q] += v * w.y;
+            if (CERT) {
+              re[q] = fma(v, w.x, re[q]);
+              imv[q] = fma(v, w.y, imv[q]);
+            } else {
+              re[q] += v * w.x;
+              imv[q] += v * w.y;
+            }
           }
         }
       }
@@ -409,7 +432,13 @@ __global__ void __launch_bounds__(256) k_gabor_conv(const Img *imgs, const doubl
   if (b < im.h)
 #pragma unroll
     for (int q = 0; q < 8; q++)
-      if (a + q < im.w) E[(long)b * im.w + a + q] = sqrt(re[q] * re[q] + imv[q] * imv[q]);
+      if (a + q < im.w) {
+        const double e = sqrt(re[q] * re[q] + imv[q] * imv[q]);
+        E[(long)b * im.w + a + q] = e;
+        if (CERT)
+          bound[blockIdx.z][(long)f * im.h * im.w + (long)b * im.w + a + q] =
+              (float)((coef * tmax * wsum[f] + 1e-14 * e) * 1.000001);
+      }
 }
 
 __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *const *energy, OutRef o) {
@@ -454,13 +483,14 @@ __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *co
   }
 }
 
-// The Gabor ratios from reduced-precision energies E with bounds D
-// (|E - E64| <= D, E64 = k_gabor_conv's value), equal to k_gabor_post on
-// the FP64 energies.  Every decision whose interval [E - D, E + D] leaves
-// it open is taken on E64: the pixels concerned are marked (D < 0), and
-// k_gabor_repair recomputes them from an LDS tile in k_gabor_conv's tap
-// order (E = E64, D = 0).  Five per-image phases of k_gabor_post_exact,
-// each followed by a repair except the last:
+// The Gabor ratios from energies E with bounds D (|E - E64| <= D, E64 =
+// k_gabor_conv_t<false>'s value: the fused multiply-add, FP32 and BF16
+// forms), equal to k_gabor_post on the FP64 energies.  Every decision whose
+// interval [E - D, E + D] leaves it open is taken on E64: the pixels
+// concerned are marked (D < 0), and the repairs recompute them in
+// k_gabor_conv_t<false>'s tap order (E = E64, D = 0).  Five phases of
+// k_gabor_post_exact, one block per (image, filter), each followed by the
+// repairs except the last:
 //   1  mark the candidates for the maximum (upper end >= largest lower end)
 //   2  max M; filter 0: mark the pixels open at 0.4 M; other filters: mark
 //      the candidates for the minimum of e / M
@@ -473,40 +503,29 @@ __global__ void __launch_bounds__(1024) k_gabor_post(const Img *imgs, double *co
 constexpr int GP_STRIDE = 4;
 
 // the marked pixels (D < 0) of one 32 x 64 tile of one filter's map: the
-// input window staged in LDS as k_gabor_conv does, each marked output then
-// summed in k_gabor_conv's tap order by one lane
-__global__ void __launch_bounds__(256) k_gabor_repair(const Img *imgs, const double *kernels, double *const *energy,
-                                                      float *const *bound) {
-  __shared__ double tile[GB_TR + GB_N - 1][GB_LD];
-  __shared__ double2 wk[GB_N * GB_N];
-  __shared__ int list[GB_TR * GB_TC];
-  __shared__ int nlist;
-  const Img im = imgs[blockIdx.z];
-  const int f = blockIdx.y;
-  const int tilesx = (im.w + GB_TC - 1) / GB_TC, tilesy = (im.h + GB_TR - 1) / GB_TR;
-  if ((int)blockIdx.x >= tilesx * tilesy) return;
-  const int b0 = (blockIdx.x / tilesx) * GB_TR, a0 = (blockIdx.x % tilesx) * GB_TC;
+// input window staged in LDS as k_gabor_conv_t does, each marked output then
+// summed in k_gabor_conv_t<false>'s tap order by one lane
+__device__ void gabor_repair_tile(const Img &im, int tile, const double *K, double *E, float *D,
+                                  double (*tile_px)[GB_LD], double2 *wk, int *list, int *nlist) {
+  const int tilesx = (im.w + GB_TC - 1) / GB_TC;
+  const int b0 = (tile / tilesx) * GB_TR, a0 = (tile % tilesx) * GB_TC;
   const int t = threadIdx.x;
-  const long plane = (long)im.h * im.w;
-  double *E = energy[blockIdx.z] + f * plane;
-  float *D = bound[blockIdx.z] + f * plane;
-  if (t == 0) nlist = 0;
+  if (t == 0) *nlist = 0;
   __syncthreads();
   for (int e = t; e < GB_TR * GB_TC; e += 256) {
     const int r = e / GB_TC, c = e - r * GB_TC;
     const int y = b0 + r, x = a0 + c;
-    if (y < im.h && x < im.w && D[(long)y * im.w + x] < 0.0f) list[atomicAdd(&nlist, 1)] = e;
+    if (y < im.h && x < im.w && D[(long)y * im.w + x] < 0.0f) list[atomicAdd(nlist, 1)] = e;
   }
   __syncthreads();
-  const int nl = nlist;
-  if (nl == 0) return;
-  const double *K = kernels + (long)f * GB_N * GB_N * 2;
+  const int nl = *nlist;
+  if (nl == 0) return;  // (uniform: every thread returns)
   for (int e = t; e < GB_N * GB_N; e += 256) wk[e] = make_double2(K[2 * e], K[2 * e + 1]);
   for (int r = t / 64; r < GB_TR + GB_N - 1; r += 4) {
     const int y = b0 - 18 + r;
     for (int c = t % 64; c < GB_TC + GB_N - 1; c += 64) {
       const int x = a0 - 18 + c;
-      tile[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
+      tile_px[r][c] = (y >= 0 && y < im.h && x >= 0 && x < im.w) ? im.p[(long)y * im.w + x] : 0.0;
     }
   }
   __syncthreads();
@@ -516,7 +535,7 @@ __global__ void __launch_bounds__(256) k_gabor_repair(const Img *imgs, const dou
     // output (b0 + r, a0 + c) reads tile row r + 37 - j, column c + 37 - i
     double re = 0.0, imv = 0.0;
     for (int j = 0; j < GB_N; j++) {
-      const double *row = &tile[r + 37 - j][c + 37];
+      const double *row = &tile_px[r + 37 - j][c + 37];
       const double2 *wr = &wk[j * GB_N];
 #pragma unroll
       for (int i = 0; i < GB_N; i++) {
@@ -530,75 +549,159 @@ __global__ void __launch_bounds__(256) k_gabor_repair(const Img *imgs, const dou
     E[o] = sqrt(re * re + imv * imv);
     D[o] = 0.0f;
   }
+  __syncthreads();  // the buffers are reused by the next tile
+}
+
+// Per (image, filter), flags[(8 image + filter) * GR_STRIDE] holds the
+// number of energies the last phase marked there and, when at most GR_CAP,
+// their pixel indices.  k_gabor_repair_list recomputes listed energies one
+// per thread straight from global memory; k_gabor_repair takes the filters
+// with more, tile by tile.  Grid (tiles, 8, images), one tile per block, or
+// (1, 8, images) with each block taking its filter's tiles in turn
+// (loop_tiles: for bounds that leave few energies open).
+constexpr int GR_CAP = 63;
+constexpr int GR_STRIDE = GR_CAP + 1;
+__global__ void __launch_bounds__(64) k_gabor_repair_list(const Img *imgs, const double *kernels, double *const *energy,
+                                                          float *const *bound, const int *flags) {
+  const int f = blockIdx.y;
+  const int *fl = flags + (long)(blockIdx.z * 8 + f) * GR_STRIDE;
+  const int cnt = fl[0];
+  if (cnt == 0 || cnt > GR_CAP) return;
+  const Img im = imgs[blockIdx.z];
+  const long plane = (long)im.h * im.w;
+  double *E = energy[blockIdx.z] + f * plane;
+  float *D = bound[blockIdx.z] + f * plane;
+  const gptr<const double> K = to_global(kernels + (long)f * GB_N * GB_N * 2);
+  const gptr<const double> px = to_global((const double *)im.p);
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const int i = fl[1 + k];
+    const int y = i / im.w, x = i - y * im.w;
+    // the taps in k_gabor_conv_t<false>'s (and the tile repair's) order: input
+    // (y + 19 - j, x + 19 - i), +0 outside the image
+    double re = 0.0, imv = 0.0;
+    for (int j = 0; j < GB_N; j++) {
+      const int yy = y + 19 - j;
+      const bool rowin = yy >= 0 && yy < im.h;
+#pragma unroll 2
+      for (int c = 0; c < GB_N; c++) {
+        const int xx = x + 19 - c;
+        const double v = (rowin && xx >= 0 && xx < im.w) ? px[(long)yy * im.w + xx] : 0.0;
+        re += v * K[2 * (j * GB_N + c)];
+        imv += v * K[2 * (j * GB_N + c) + 1];
+      }
+    }
+    E[i] = sqrt(re * re + imv * imv);
+    D[i] = 0.0f;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_gabor_repair(const Img *imgs, const double *kernels, double *const *energy,
+                                                      float *const *bound, const int *flags, int loop_tiles) {
+  __shared__ double tile[GB_TR + GB_N - 1][GB_LD];
+  __shared__ double2 wk[GB_N * GB_N];
+  __shared__ int list[GB_TR * GB_TC];
+  __shared__ int nlist;
+  const int f = blockIdx.y;
+  if (flags[(long)(blockIdx.z * 8 + f) * GR_STRIDE] <= GR_CAP) return;  // none, or listed
+  const Img im = imgs[blockIdx.z];
+  const int ntiles = ((im.w + GB_TC - 1) / GB_TC) * ((im.h + GB_TR - 1) / GB_TR);
+  const long plane = (long)im.h * im.w;
+  double *E = energy[blockIdx.z] + f * plane;
+  float *D = bound[blockIdx.z] + f * plane;
+  const double *K = kernels + (long)f * GB_N * GB_N * 2;
+  if (loop_tiles) {
+    for (int tl = 0; tl < ntiles; tl++) gabor_repair_tile(im, tl, K, E, D, tile, wk, list, &nlist);
+  } else if ((int)blockIdx.x < ntiles) {
+    gabor_repair_tile(im, blockIdx.x, K, E, D, tile, wk, list, &nlist);
+  }
 }
 
 __global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img *imgs, double *const *energy,
-                                                           float *const *bound, double *params, OutRef o) {
+                                                           float *const *bound, double *params, OutRef o,
+                                                           int *flags) {
   __shared__ int hist[1024];
   __shared__ double shd[16];
   __shared__ long long shl[16];
   __shared__ double thr_s;
+  __shared__ int marks[8];            // energies marked per filter in this phase
+  __shared__ int mlist[8][GR_CAP];    // the first GR_CAP of them
+  if (threadIdx.x < 8) marks[threadIdx.x] = 0;
+  __syncthreads();
   const Img im = imgs[blockIdx.x];
   const long n = (long)im.h * im.w;
   double *const E0 = energy[blockIdx.x];
   float *const D0 = bound[blockIdx.x];
   double *prm = params + (long)blockIdx.x * 8 * GP_STRIDE;
-  for (int f = (phase >= 4 ? 1 : 0); f < 8; f++) {
+  const int f = blockIdx.y;  // one block per (image, filter)
+  if (!(phase >= 4 && f == 0)) do {
     const double *E = E0 + (long)f * n;
     float *D = D0 + (long)f * n;
     double *P = prm + f * GP_STRIDE;
-    auto mark = [&](long i) { D[i] = -D[i]; };
+    auto mark = [&](long i) {
+      D[i] = -D[i];
+      const int k = atomicAdd(&marks[f], 1);
+      if (k < GR_CAP) mlist[f][k] = (int)i;
+    };
     if (phase == 1) {
       double lo = -1e308;
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) lo = fmax(lo, E[i] - (double)D[i]);
       lo = block_reduce(lo, MaxOp(), shd);
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024)
         if (D[i] != 0.0f && E[i] + (double)D[i] >= lo) mark(i);
-      continue;
+      break;
     }
     if (phase == 2) {
       double mx = -1e308;  // the candidates are exact now; nothing else can reach them
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024)
         if (D[i] == 0.0f) mx = fmax(mx, E[i]);
       mx = block_reduce(mx, MaxOp(), shd);
       if (threadIdx.x == 0) P[0] = mx;
       if (!(mx > 0)) {  // no energy: every value exact, then the FP64 rules as written
+#pragma unroll 4
         for (long i = threadIdx.x; i < n; i += 1024)
           if (D[i] != 0.0f) mark(i);
-        continue;
+        break;
       }
       if (f == 0) {
         const double lvl = mx * 0.4;
+#pragma unroll 4
         for (long i = threadIdx.x; i < n; i += 1024) {
           const double d = D[i];
           if (d != 0.0 && E[i] - d <= lvl && E[i] + d > lvl) mark(i);
         }
-        continue;
+        break;
       }
       double hi_min = 1e308;
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) {
         const double v = (E[i] + (double)D[i]) / mx;
         if (moments_ok(v)) hi_min = fmin(hi_min, v);
       }
       hi_min = block_reduce(hi_min, MinOp(), shd);
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024)
         if (D[i] != 0.0f && (E[i] - (double)D[i]) / mx <= hi_min) mark(i);
-      continue;
+      break;
     }
     const double mx = P[0];
     if (phase == 3) {
       if (f == 0) {
         const double lvl = mx * 0.4;
         long long c = 0;
+#pragma unroll 4
         for (long i = threadIdx.x; i < n; i += 1024) {
           const double d = D[i];
           c += d == 0.0 ? E[i] > lvl : E[i] - d > lvl;  // open pixels were recomputed
         }
         c = block_reduce(c, SumOp(), shl);
         if (threadIdx.x == 0) P[3] = (double)c;
-        continue;
+        break;
       }
       double a = 1e308, b = -1e308;
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) {
         const double v = E[i] / mx;
         if (D[i] == 0.0f && moments_ok(v)) { a = fmin(a, v); b = fmax(b, v); }
@@ -607,6 +710,7 @@ __global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img 
       b = block_reduce(b, MaxOp(), shd);
       if (threadIdx.x == 0) { P[1] = a; P[2] = b; }
       const double sc = hist_scale(a, b, 1024);
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) {
         const double d = D[i];
         if (d == 0.0) continue;
@@ -614,13 +718,14 @@ __global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img 
         const double vl = fmax((E[i] - d) / mx, a), vh = fmin((E[i] + d) / mx, b);
         if (hist_bin(vl, a, sc, 1024) != hist_bin(vh, a, sc, 1024)) mark(i);
       }
-      continue;
+      break;
     }
     const double a = P[1], b = P[2];
     if (phase == 4) {
       for (int k = threadIdx.x; k < 1024; k += 1024) hist[k] = 0;
       __syncthreads();
       const double sc = hist_scale(a, b, 1024);
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) {
         const double e = E[i], d = D[i];
         // d > 0: the bin was decided by the interval (both ends give it)
@@ -632,39 +737,58 @@ __global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img 
       __syncthreads();
       const double thr = thr_s;
       if (threadIdx.x == 0) P[3] = thr;
+#pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) {
         const double d = D[i];
         if (d != 0.0 && (E[i] - d) / mx <= thr && (E[i] + d) / mx > thr) mark(i);
       }
       __syncthreads();  // hist is reused by the next filter
-      continue;
+      break;
     }
     // phase 5
     const double thr = P[3];
     long long c = 0;
+#pragma unroll 4
     for (long i = threadIdx.x; i < n; i += 1024) {
       const double d = D[i];
       c += d == 0.0 ? (E[i] / mx) > thr : (E[i] - d) / mx > thr;
     }
     c = block_reduce(c, SumOp(), shl);
     if (threadIdx.x == 0) o.rows[blockIdx.x][f - 1] = (double)c / prm[3];
+  } while (0);
+  __syncthreads();
+  int *fl = flags + (long)(blockIdx.x * 8 + f) * GR_STRIDE;
+  for (int k = threadIdx.x; k < GR_STRIDE; k += blockDim.x) {
+    if (k == 0) fl[0] = marks[f];
+    else if (k - 1 < marks[f] && k - 1 < GR_CAP) fl[k] = mlist[f][k - 1];
   }
 }
 
 void launch_gabor_post_exact(const Img *imgs, double *const *energy, float *const *bound, const double *taps,
-                             double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s) {
-  const int tiles = ((maxw + GB_TC - 1) / GB_TC) * ((maxh + GB_TR - 1) / GB_TR);
+                             double *params, int *flags, bool loop_tiles, OutRef o, int nimg, int maxh, int maxw,
+                             hipStream_t s) {
+  const int tiles = loop_tiles ? 1 : ((maxw + GB_TC - 1) / GB_TC) * ((maxh + GB_TR - 1) / GB_TR);
   for (int phase = 1; phase <= 5; phase++) {
-    k_gabor_post_exact<<<nimg, 1024, 0, s>>>(phase, imgs, energy, bound, params, o);
-    if (phase < 5) k_gabor_repair<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, taps, energy, bound);
+    k_gabor_post_exact<<<dim3(nimg, 8), 1024, 0, s>>>(phase, imgs, energy, bound, params, o, flags);
+    if (phase < 5) {
+      k_gabor_repair_list<<<dim3(1, 8, nimg), 64, 0, s>>>(imgs, taps, energy, bound, flags);
+      k_gabor_repair<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, taps, energy, bound, flags, loop_tiles);
+    }
   }
 }
 
 void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
                   int maxw, hipStream_t s) {
   const int tiles = ((maxw + GB_TC - 1) / GB_TC) * ((maxh + GB_TR - 1) / GB_TR);
-  k_gabor_conv<<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, kernels, energy);
+  k_gabor_conv_t<false><<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, kernels, energy, nullptr, nullptr, 0.0);
   k_gabor_post<<<nimg, 1024, 0, s>>>(imgs, energy, o);
+}
+void launch_gabor_certified(const Img *imgs, const double *kernels, const double *wsum, double slack,
+                            double *const *energy, float *const *bound, double *params, int *flags, OutRef o,
+                            int nimg, int maxh, int maxw, hipStream_t s) {
+  const int tiles = ((maxw + GB_TC - 1) / GB_TC) * ((maxh + GB_TR - 1) / GB_TR);
+  k_gabor_conv_t<true><<<dim3(tiles, 8, nimg), 256, 0, s>>>(imgs, kernels, energy, wsum, bound, 4e-13 * slack);
+  launch_gabor_post_exact(imgs, energy, bound, kernels, params, flags, slack <= 1.0, o, nimg, maxh, maxw, s);
 }
 void launch_gabor_post(const Img *imgs, double *const *energy, OutRef o, int nimg, hipStream_t s) {
   k_gabor_post<<<nimg, 1024, 0, s>>>(imgs, energy, o);

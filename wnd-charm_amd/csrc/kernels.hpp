@@ -185,13 +185,23 @@ void launch_gabor_post(const Img *imgs, double *const *energy, OutRef o, int nim
 // Gabor: FP32 takes float kernels (k0); BF16x3 takes the hi / mid / lo bf16
 // planes (k0, k1, k2); both write per-pixel error bounds and finish with the
 // exact post-processing (FP64 recomputation of the undecided pixels, taps64)
+// flags: 8 x 64 ints per image (per filter: the energies the last
+// post-processing phase left open, the first 63 of their pixels);
+// loop_tiles: one repair block per (image, filter) taking its tiles in turn,
+// for bounds that leave few energies open
 void launch_gabor_mixed(int precision, const Img *imgs, const void *k0, const void *k1, const void *k2,
                         const double *wsum, const double *taps64, double *const *energy, float *const *bound,
-                        double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
+                        double *params, int *flags, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
 void launch_gabor_post_exact(const Img *imgs, double *const *energy, float *const *bound, const double *taps,
-                             double *params, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
+                             double *params, int *flags, bool loop_tiles, OutRef o, int nimg, int maxh, int maxw,
+                             hipStream_t s);
 void launch_gabor(const Img *imgs, const double *kernels, double *const *energy, OutRef o, int nimg, int maxh,
                   int maxw, hipStream_t s);
+// the plain form's default: FP64 fused multiply-adds with per-energy bounds
+// (scaled by slack >= 1), every count certified by the exact post-processing
+void launch_gabor_certified(const Img *imgs, const double *kernels, const double *wsum, double slack,
+                            double *const *energy, float *const *bound, double *params, int *flags, OutRef o,
+                            int nimg, int maxh, int maxw, hipStream_t s);
 void launch_zernike(const Img *imgs, double *partial, int nchunk, const double *htab, OutRef o, int nimg,
                     hipStream_t s, int precision = 0);
 // Chebyshev-Fourier histograms: matrix-core sums certified against an
