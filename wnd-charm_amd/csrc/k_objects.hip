@@ -9,32 +9,54 @@
 namespace wnd {
 
 // Otsu threshold of a plane (ImageMatrix::Otsu(dynamic), cmatrix.cpp:1520-1570):
-// 1024-bin histogram over [mn, mx], then the reference's sequential
-// omega / myu / sigma recurrence in one thread.  hist must hold 1024 ints.
-// The threshold of a filled 1024-bin histogram (one thread).
+// 1024-bin histogram over [mn, mx], then the reference's omega / myu / sigma
+// recurrence.  The threshold of a filled 1024-bin histogram, by the whole
+// block (every thread calls it; uniform result): the bin fractions
+// hist[i] / wh and, once thread 0 has run the sequential prefix sums om / my
+// (and myu[L-1]) with the reference's operations, each bin's between-class
+// variance are formed in parallel; the threshold is the first bin of the
+// largest variance above 0 (the reference's strict comparison from 0).
 __device__ double otsu_threshold(const int *hist, long n, double mn, double mx) {
-  {
-    const double wh = (double)(unsigned)n;
-    // myu[L-1] first (the recurrence is replayed below with identical operations)
+  __shared__ double pf[1024], om_s[1024], my_s[1024];
+  __shared__ double red_d[32];
+  __shared__ long long red_l[32];
+  __shared__ double my_last_s;
+  const double wh = (double)(unsigned)n;
+  for (int i = threadIdx.x; i < 1024; i += blockDim.x) pf[i] = hist[i] / wh;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // myu[L-1] first (the recurrence below repeats the same operations)
     double my_last = 0.0;
-    for (int i = 1; i < 1024; i++) my_last = my_last + i * (hist[i] / wh);
-    double om = hist[0] / wh, my = 0.0;
-    int thr = 0;
-    double maxs = 0.0;
+    for (int i = 1; i < 1024; i++) my_last = my_last + i * pf[i];
+    my_last_s = my_last;
+    double om = pf[0], my = 0.0;
     for (int i = 0; i < 1023; i++) {
       if (i > 0) {
-        om = om + (hist[i] / wh);
-        my = my + i * (hist[i] / wh);
+        om = om + pf[i];
+        my = my + i * pf[i];
       }
-      double sg;
-      if (om != 0.0 && om != 1.0) {
-        const double d = my_last * om - my;
-        sg = (d * d) / (om * (1.0 - om));
-      } else sg = 0.0;
-      if (sg > maxs) { maxs = sg; thr = i; }
+      om_s[i] = om;
+      my_s[i] = my;
     }
-    return ((((double)thr / (double)(1023)) * (mx - mn)) + mn);
   }
+  __syncthreads();
+  const double my_last = my_last_s;
+  double best = 0.0;
+  long long bi = 1023;  // no variance above 0: threshold bin 0 (below)
+  for (int i = threadIdx.x; i < 1023; i += blockDim.x) {
+    const double om = om_s[i], my = my_s[i];
+    double sg;
+    if (om != 0.0 && om != 1.0) {
+      const double d = my_last * om - my;
+      sg = (d * d) / (om * (1.0 - om));
+    } else sg = 0.0;
+    if (sg > best) { best = sg; bi = i; }  // this thread's bins ascend: its first maximum
+  }
+  const double top = block_reduce(best, MaxOp(), red_d);
+  long long cand = (top > 0.0 && best == top) ? bi : 1023;
+  cand = block_reduce(cand, MinOp(), red_l);
+  const int thr = top > 0.0 ? (int)cand : 0;
+  return ((((double)thr / (double)(1023)) * (mx - mn)) + mn);
 }
 
 template <typename ValF>
@@ -45,10 +67,7 @@ __device__ double block_otsu(long n, double mn, double mx, ValF val, int *hist) 
 #pragma unroll 4
   for (long i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&hist[hist_bin(val(i), mn, sc, 1024)], 1);
   __syncthreads();
-  __shared__ double thr_s;
-  if (threadIdx.x == 0) thr_s = otsu_threshold(hist, n, mn, mx);
-  __syncthreads();
-  return thr_s;
+  return otsu_threshold(hist, n, mn, mx);
 }
 
 // ------------------------------------------------------------- union-find
@@ -561,8 +580,13 @@ __device__ void gabor_repair_tile(const Img &im, int tile, const double *K, doub
 // (loop_tiles: for bounds that leave few energies open).
 constexpr int GR_CAP = 63;
 constexpr int GR_STRIDE = GR_CAP + 1;
+// One wavefront per (image, filter); per listed energy the lanes form the
+// 1444 tap products (each rounded once, as in the sum) into LDS, and lane 0
+// adds them in k_gabor_conv_t<false>'s (and the tile repair's) order: tap
+// (j, i) reads input (y + 19 - j, x + 19 - i), +0 outside the image.
 __global__ void __launch_bounds__(64) k_gabor_repair_list(const Img *imgs, const double *kernels, double *const *energy,
                                                           float *const *bound, const int *flags) {
+  __shared__ double pre[GB_N * GB_N], pim[GB_N * GB_N];
   const int f = blockIdx.y;
   const int *fl = flags + (long)(blockIdx.z * 8 + f) * GR_STRIDE;
   const int cnt = fl[0];
@@ -573,25 +597,42 @@ __global__ void __launch_bounds__(64) k_gabor_repair_list(const Img *imgs, const
   float *D = bound[blockIdx.z] + f * plane;
   const gptr<const double> K = to_global(kernels + (long)f * GB_N * GB_N * 2);
   const gptr<const double> px = to_global((const double *)im.p);
-  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+  const int lane = threadIdx.x;
+  for (int k = 0; k < cnt; k++) {
     const int i = fl[1 + k];
     const int y = i / im.w, x = i - y * im.w;
-    // the taps in k_gabor_conv_t<false>'s (and the tile repair's) order: input
-    // (y + 19 - j, x + 19 - i), +0 outside the image
-    double re = 0.0, imv = 0.0;
-    for (int j = 0; j < GB_N; j++) {
-      const int yy = y + 19 - j;
-      const bool rowin = yy >= 0 && yy < im.h;
-#pragma unroll 2
-      for (int c = 0; c < GB_N; c++) {
-        const int xx = x + 19 - c;
-        const double v = (rowin && xx >= 0 && xx < im.w) ? px[(long)yy * im.w + xx] : 0.0;
-        re += v * K[2 * (j * GB_N + c)];
-        imv += v * K[2 * (j * GB_N + c) + 1];
-      }
+    for (int tp = lane; tp < GB_N * GB_N; tp += 64) {
+      const int j = tp / GB_N, c = tp - j * GB_N;
+      const int yy = y + 19 - j, xx = x + 19 - c;
+      const double v = (yy >= 0 && yy < im.h && xx >= 0 && xx < im.w) ? px[(long)yy * im.w + xx] : 0.0;
+      pre[tp] = v * K[2 * tp];
+      pim[tp] = v * K[2 * tp + 1];
     }
-    E[i] = sqrt(re * re + imv * imv);
-    D[i] = 0.0f;
+    __syncthreads();
+    if (lane == 0) {
+      double re = 0.0, imv = 0.0;
+      int tp = 0;
+      for (; tp + 8 <= GB_N * GB_N; tp += 8) {  // eight products read at a time, added in order
+        double a[8], b[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          a[u] = pre[tp + u];
+          b[u] = pim[tp + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          re += a[u];
+          imv += b[u];
+        }
+      }
+      for (; tp < GB_N * GB_N; tp++) {
+        re += pre[tp];
+        imv += pim[tp];
+      }
+      E[i] = sqrt(re * re + imv * imv);
+      D[i] = 0.0f;
+    }
+    __syncthreads();  // the products of the next energy overwrite these
   }
 }
 
@@ -622,7 +663,6 @@ __global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img 
   __shared__ int hist[1024];
   __shared__ double shd[16];
   __shared__ long long shl[16];
-  __shared__ double thr_s;
   __shared__ int marks[8];            // energies marked per filter in this phase
   __shared__ int mlist[8][GR_CAP];    // the first GR_CAP of them
   if (threadIdx.x < 8) marks[threadIdx.x] = 0;
@@ -733,9 +773,7 @@ __global__ void __launch_bounds__(1024) k_gabor_post_exact(int phase, const Img 
         atomicAdd(&hist[hist_bin(v, a, sc, 1024)], 1);
       }
       __syncthreads();
-      if (threadIdx.x == 0) thr_s = otsu_threshold(hist, n, a, b);
-      __syncthreads();
-      const double thr = thr_s;
+      const double thr = otsu_threshold(hist, n, a, b);
       if (threadIdx.x == 0) P[3] = thr;
 #pragma unroll 4
       for (long i = threadIdx.x; i < n; i += 1024) {
