@@ -231,3 +231,30 @@ def exact_mask_for_names(names):
         alg = g[:g.index(' (')]
         m[c] = k in EXACT[idx[alg]]
     return m
+
+
+def features_in_context(env, imgs, plan):
+    """wndcharm.compute_features(imgs, plan) in a fresh engine context created
+    with the environment variables env set (the engine reads its switches at
+    creation); the context is destroyed afterwards"""
+    import wndcharm
+    lib = wndcharm._load()
+    old = wndcharm._context()
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    c = ctypes.c_void_p()
+    try:
+        rc = lib.wndgpu_create(0, ctypes.byref(c))
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k)
+            else:
+                os.environ[k] = v
+    assert rc == 0
+    wndcharm._ctx = c
+    try:
+        return wndcharm.compute_features(imgs, plan)
+    finally:
+        wndcharm._ctx = old
+        lib.wndgpu_destroy(c)

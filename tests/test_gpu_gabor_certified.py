@@ -7,15 +7,12 @@ The ratios must therefore equal those of the direct path (the reference's
 rounding throughout, WNDGPU_GABOR_DIRECT=1) bit for bit, and stay equal when
 WNDGPU_GABOR_SLACK widens the bounds so that many energies are recomputed,
 by the per-pixel list repair or, past 63 per filter, tile by tile."""
-import ctypes
-import os
-
 import numpy as np
 import pytest
 
 import wndcharm
 from wndcharm import synthetic
-from helpers import ALG_NAMES, load_npz, orc_algorithm, orc_transform
+from helpers import ALG_NAMES, features_in_context, load_npz, orc_algorithm, orc_transform
 
 pytestmark = pytest.mark.gpu
 GABOR = ALG_NAMES.index('Gabor Textures')
@@ -40,26 +37,7 @@ def _images():
 
 def _features(env):
     """features of _images() under _plan() in a fresh context created with env"""
-    lib = wndcharm._load()
-    old = wndcharm._context()
-    saved = {k: os.environ.get(k) for k in env}
-    os.environ.update(env)
-    c = ctypes.c_void_p()
-    try:
-        rc = lib.wndgpu_create(0, ctypes.byref(c))
-    finally:
-        for k, v in saved.items():
-            if v is None:
-                os.environ.pop(k)
-            else:
-                os.environ[k] = v
-    assert rc == 0
-    wndcharm._ctx = c
-    try:
-        return wndcharm.compute_features(_images(), _plan())
-    finally:
-        wndcharm._ctx = old
-        lib.wndgpu_destroy(c)
+    return features_in_context(env, _images(), _plan())
 
 
 @pytest.fixture(scope='module')

@@ -369,6 +369,12 @@ struct wndgpu_ctx {
   // gabor_slack (WNDGPU_GABOR_SLACK, >= 1) scales the certification bound.
   bool gabor_direct = false;
   double gabor_slack = 1;
+  // Radon projections: order-free sums whose histogram counts are
+  // certified (the default), or the reference's sequential sum per bin
+  // (radon_direct, WNDGPU_RADON_DIRECT=1 at creation).  radon_slack
+  // (WNDGPU_RADON_SLACK, >= 1) scales the certification bound.
+  bool radon_direct = false;
+  double radon_slack = 1;
   bool sized = false;
 };
 
@@ -1120,12 +1126,17 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
           const int W = E[k].N->w[E[k].i], H = E[k].N->h[E[k].i];
           const int a = W - 1 - (W - 1) / 2, b = H - 1 - (H - 1) / 2;
           const int rl = (int)ceil(sqrt((double)(a * a) + (double)(b * b))) + 1;
-          pr[k] = A.dalloc_n<double>(4 * (size_t)(2 * rl + 1));
+          pr[k] = A.dalloc_n<double>(8 * (size_t)(2 * rl + 1));  // sums, then bounds (certified form)
           maxbins = std::max(maxbins, 2 * rl + 1);
         }
         double **prd = A.stage(pr);
         const double *trig = (const double *)c->rtrig.p;
-        push(gn, [=](hipStream_t s) { launch_radon(ni, trig, prd, o, ne, maxbins, mh, mw, s); });
+        if (!c->radon_direct && radon_certified_fits(maxbins)) {
+          const double rslack = c->radon_slack;
+          push(gn, [=](hipStream_t s) { launch_radon_certified(ni, trig, prd, o, ne, maxbins, rslack, s); });
+        } else {
+          push(gn, [=](hipStream_t s) { launch_radon(ni, trig, prd, o, ne, maxbins, mh, mw, s); });
+        }
       } break;
       case A_TAMURA: {
         std::vector<double *> sc(ne);
@@ -1812,6 +1823,8 @@ int wndgpu_create(int device, wndgpu_ctx **out) {
   if (const char *cs = getenv("WNDGPU_CF_SLACK")) c->cf_slack = std::max(1.0, atof(cs));
   if (const char *gd = getenv("WNDGPU_GABOR_DIRECT")) c->gabor_direct = atoi(gd) != 0;
   if (const char *gs = getenv("WNDGPU_GABOR_SLACK")) c->gabor_slack = std::max(1.0, atof(gs));
+  if (const char *rd = getenv("WNDGPU_RADON_DIRECT")) c->radon_direct = atoi(rd) != 0;
+  if (const char *rs = getenv("WNDGPU_RADON_SLACK")) c->radon_slack = std::max(1.0, atof(rs));
   if (const char *gm = getenv("WNDGPU_GRAPH")) c->graph_mode = std::min(2, std::max(0, atoi(gm)));
   if (const char *mb = getenv("WNDGPU_TABLE_CACHE_MB")) c->table_budget = (size_t)std::max(0L, atol(mb)) << 20;
   // the co-occurrence kernel keeps a 128.5 KiB triangle in LDS

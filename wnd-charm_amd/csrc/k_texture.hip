@@ -1959,6 +1959,421 @@ __global__ void k_radon_hist(const Img *imgs, double *const *proj, OutRef o) {
   for (int a = 0; a < 12; a++) out[a] = v[ord[a]];
 }
 
+// --------------------------------------------------- certified (order-free) form
+// A projection bin of the reference is the sequential sum of its terms.  Any
+// other summation of the same terms lies within gamma_K * sum|terms| of their
+// exact sum, as does the reference's (K bounds the depth of both summation
+// trees), so the reference's bin lies within 2 gamma_K sum|terms| of an
+// order-free sum.  The 12 features only need each bin's 3-bin histogram slot
+// and the extreme bins of each angle: k_radon_sum forms the order-free sums
+// with a bound per bin, and k_radon_cert settles the extremes and every slot
+// from them, recomputing in the reference's order the bins a bound leaves
+// open.  The counts equal the reference's bit for bit.
+
+// LDS of one wave's recomputation: the terms of one window, and per
+// sub-column of a chunk the offset of its first term and its first sub-row
+constexpr int RADON_RTB = 512;  // terms per window (8 per lane)
+constexpr int RADON_RCH = 256;  // sub-columns per chunk (4 per lane)
+struct RadonWaveLds {
+  double tb[RADON_RTB];
+  int off[RADON_RCH];
+  int m0[RADON_RCH];
+};
+
+// Bin r of the projection at (cs, sn) in the reference's order (radon.cpp:
+// 66-81: sub-columns ascending, sub-rows ascending within one), by the 64
+// lanes of a wave (r uniform).  Per chunk of 256 sub-columns each lane finds
+// the sub-rows of four (those whose floor(rIdx) is r or r-1, a range, as
+// floor(rIdx) is non-increasing in the sub-row), a wave scan turns the counts
+// into offsets, the lanes form the terms 512 at a time into LDS in the
+// reference's order with its operations (8 pixel loads in flight per lane),
+// and lane 0 adds them in that order.  A zero pixel, which the reference
+// skips, gives a +0 term (the running sum is never -0, so adding +0 leaves it
+// unchanged).  Every lane returns the sum.
+__device__ double radon_bin_wave(gptr<const double> P, int M, int N, double cs, double sn, int rFirst, int r,
+                                 RadonWaveLds &W) {
+  const int lane = threadIdx.x & 63;
+  const int xO = (N - 1) / 2, yO = (M - 1) / 2, M2 = 2 * M, N2 = 2 * N;
+  auto yS = [&](int mm) { return ((-yO - 0.25) + 0.5 * (M2 - 1 - mm)) * sn; };
+  const double yS0 = yS(0), ySL = yS(M2 - 1);
+  auto predA = [&](int nn) { return (int)(radon_xc(nn, xO, cs, rFirst) + yS0) < r - 1; };
+  auto predB = [&](int nn) { return (int)(radon_xc(nn, xO, cs, rFirst) + ySL) > r; };
+  int lo = 0, hi = N2;
+  if (cs > 0) {
+    int l = 0, h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (!predA(mid)) h = mid; else l = mid + 1; }
+    lo = l;
+    l = lo; h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (predB(mid)) h = mid; else l = mid + 1; }
+    hi = l;
+  } else if (cs < 0) {
+    int l = 0, h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (!predB(mid)) h = mid; else l = mid + 1; }
+    lo = l;
+    l = lo; h = N2;
+    while (l < h) { const int mid = (l + h) >> 1; if (predA(mid)) h = mid; else l = mid + 1; }
+    hi = l;
+  } else if (predA(0) || predB(0)) {
+    hi = 0;
+  }
+  double acc = 0.0;
+  for (int cb = lo; cb < hi; cb += RADON_RCH) {
+    int cnt[4], mlo[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int nn = cb + 4 * lane + k;
+      int m0 = 0, m1 = 0;
+      if (nn < hi) {
+        const double xc = radon_xc(nn, xO, cs, rFirst);
+        int l = 0, h = M2;
+        while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS(mid)) <= r) h = mid; else l = mid + 1; }
+        m0 = l;
+        h = M2;
+        while (l < h) { const int mid = (l + h) >> 1; if ((int)(xc + yS(mid)) < r - 1) h = mid; else l = mid + 1; }
+        m1 = l;
+      }
+      cnt[k] = m1 - m0;
+      mlo[k] = m0;
+    }
+    const int own = cnt[0] + cnt[1] + cnt[2] + cnt[3];
+    int inc = own;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += t;
+    }
+    const int total = __shfl(inc, 63, 64);
+    int ofs = inc - own;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      W.off[4 * lane + k] = ofs;
+      W.m0[4 * lane + k] = mlo[k];
+      ofs += cnt[k];
+    }
+    wave_sync();
+    for (int w0 = 0; w0 < total; w0 += RADON_RTB) {
+      const int wn = min(RADON_RTB, total - w0);
+      int nj[8], mj[8];
+      double px[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int g = w0 + min(lane + 64 * i, wn - 1);
+        int j = 0;  // the sub-column holding term g: the last j with off[j] <= g
+        for (int st = RADON_RCH / 2; st > 0; st >>= 1)
+          if (W.off[j + st] <= g) j += st;
+        nj[i] = cb + j;
+        mj[i] = W.m0[j] + (g - W.off[j]);
+        px[i] = P[(long)(mj[i] >> 1) * N + (nj[i] >> 1)];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        const int t = lane + 64 * i;
+        if (t < wn) {
+          double pixel = px[i], term = 0.0;
+          if (pixel) {
+            pixel *= 0.25;
+            const double rIdx = (radon_xc(nj[i], xO, cs, rFirst) + yS(mj[i]));
+            const int rLow = (int)rIdx;
+            const double pixelLow = pixel * (1 - rIdx + rLow);
+            term = (rLow == r) ? pixelLow : pixel - pixelLow;
+          }
+          W.tb[t] = term;
+        }
+      }
+      wave_sync();
+      if (lane == 0) {
+        int i = 0;
+        for (; i + 8 <= wn; i += 8) {
+          double t8[8];
+#pragma unroll
+          for (int k = 0; k < 8; k++) t8[k] = W.tb[i + k];
+#pragma unroll
+          for (int k = 0; k < 8; k++) acc += t8[k];
+        }
+        for (; i < wn; i++) acc += W.tb[i];
+      }
+      wave_sync();  // the window is consumed
+    }
+  }
+  return __shfl(acc, 0, 64);
+}
+
+template <typename T, typename Op>
+__device__ __forceinline__ T wave_all_reduce(T v, Op op) {
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Order-free projection sums.  Block = (group of NA angles, image plane); a
+// thread walks pixel columns top to bottom (the lanes of a wave read adjacent
+// pixels of a row).  Each sub-pixel's two terms are the reference's
+// (radon.cpp:66-81: the same rIdx, rLow, pixelLow and pixel - pixelLow).  Down
+// a column rIdx is non-increasing, and the four sub-pixels of a pixel span
+// less than one unit of it, so a pixel's terms fall in bins pb..pb+2 (pb =
+// its smallest rLow) and pb never rises: the thread keeps the sums of those
+// three bins in registers and adds a bin into the block's LDS copy of the
+// projection (a double atomic) once the column has left it.  The terms go
+// into the window with fused multiply-adds by 0 or 1, exact products, so
+// each term is added by one rounding.  Beside each sum, W accumulates |pixel|
+// over every pixel whose window holds the bin: each term is at most
+// |pixel| / 4 (1 + 2^-39), so W bounds the bin's sum|terms|.
+// Output per plane: sums at [a][r], bounds at [4 + a][r].
+template <int NA>
+__global__ void __launch_bounds__(256) k_radon_sum(const Img *imgs, const double *trig, double *const *proj, int ngrp,
+                                                   int nimg) {
+  extern __shared__ double lds[];
+  int ent, grp;
+  if (!xcd_item(ngrp, nimg, ent, grp)) return;
+  const Img im = imgs[ent];
+  const int M = im.h, N = im.w;
+  const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
+                                   (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
+  const int rFirst = -rLast, rSize = 2 * rLast + 1;
+  const int a0 = grp * NA;
+  double *S = lds, *W = lds + NA * rSize;
+  for (int i = threadIdx.x; i < 2 * NA * rSize; i += blockDim.x) lds[i] = 0.0;
+  __syncthreads();
+  const int xO = (N - 1) / 2, yO = (M - 1) / 2, M2 = 2 * M;
+  double cs[NA], sn[NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++) {
+    cs[a] = trig[2 * (a0 + a)];
+    sn[a] = trig[2 * (a0 + a) + 1];
+  }
+  const gptr<const double> P = to_global((const double *)im.p);
+  const double c0 = -yO - 0.25;
+  // a bin's sum into LDS; w == 0 means no nonzero pixel reached it (s == +0)
+  auto flush = [&](int a, int b, double sv, double wv) {
+    if (wv != 0.0) {
+      b = min(max(b, 0), rSize - 1);
+      atomicAdd(&S[a * rSize + b], sv);
+      atomicAdd(&W[a * rSize + b], wv);
+    }
+  };
+  for (int c = threadIdx.x; c < N; c += blockDim.x) {
+    double xa[NA], xb[NA], R0[NA], R1[NA], R2[NA], A0[NA], A1[NA], A2[NA];
+    double base[NA];  // the window's first bin
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      xa[a] = radon_xc(2 * c, xO, cs[a], rFirst);
+      xb[a] = radon_xc(2 * c + 1, xO, cs[a], rFirst);
+      R0[a] = R1[a] = R2[a] = A0[a] = A1[a] = A2[a] = 0.0;
+      base[a] = 1e9;  // above every bin: the first pixel opens the window
+    }
+    double pn = P[c];
+    for (int y = 0; y < M; y++) {
+      const double pix = pn;
+      if (y + 1 < M) pn = P[(long)(y + 1) * N + c];
+      const double q = pix * 0.25, ap = fabs(pix);
+      // yTable of sub-rows 2y and 2y+1 (exact multiples of 0.25)
+      const double ya = c0 + 0.5 * (M2 - 1 - 2 * y), yb = ya - 0.5;
+#pragma unroll
+      for (int a = 0; a < NA; a++) {
+        const double ysa = ya * sn[a], ysb = yb * sn[a];
+        const double r00 = xa[a] + ysa, r01 = xa[a] + ysb, r10 = xb[a] + ysa, r11 = xb[a] + ysb;
+        // rLow = (int)rIdx as a double (rIdx >= 0 is far below 2^31)
+        const double l00 = trunc(r00), l01 = trunc(r01), l10 = trunc(r10), l11 = trunc(r11);
+        const double pb = fmin(fmin(l00, l01), fmin(l10, l11));
+        if (pb < base[a]) {
+          if (base[a] - pb >= 3.0) {
+            flush(a, (int)base[a], R0[a], A0[a]);
+            flush(a, (int)base[a] + 1, R1[a], A1[a]);
+            flush(a, (int)base[a] + 2, R2[a], A2[a]);
+            R0[a] = R1[a] = R2[a] = A0[a] = A1[a] = A2[a] = 0.0;
+          } else {
+            do {
+              flush(a, (int)base[a] + 2, R2[a], A2[a]);
+              R2[a] = R1[a]; R1[a] = R0[a]; R0[a] = 0.0;
+              A2[a] = A1[a]; A1[a] = A0[a]; A0[a] = 0.0;
+              base[a] -= 1.0;
+            } while (pb < base[a]);
+          }
+          base[a] = pb;
+        }
+        auto add = [&](double rIdx, double rLow) {
+          const double pixelLow = q * (1 - rIdx + rLow), pixelHigh = q - pixelLow;
+          const double d = rLow - pb, e = 1.0 - d;  // rLow - pb is 0 or 1
+          R0[a] = fma(e, pixelLow, R0[a]);
+          R1[a] = fma(d, pixelLow, R1[a]);
+          R1[a] = fma(e, pixelHigh, R1[a]);
+          R2[a] = fma(d, pixelHigh, R2[a]);
+        };
+        add(r00, l00);
+        add(r01, l01);
+        add(r10, l10);
+        add(r11, l11);
+        A0[a] += ap;
+        A1[a] += ap;
+        A2[a] += ap;
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < NA; a++) {
+      flush(a, (int)base[a], R0[a], A0[a]);
+      flush(a, (int)base[a] + 1, R1[a], A1[a]);
+      flush(a, (int)base[a] + 2, R2[a], A2[a]);
+    }
+  }
+  __syncthreads();
+  double *out = proj[ent];
+  for (int i = threadIdx.x; i < NA * rSize; i += blockDim.x) {
+    out[(long)a0 * rSize + i] = S[i];
+    out[(long)(4 + a0) * rSize + i] = W[i];
+  }
+}
+
+// Per plane (block of 256): wave a takes angle a.  The bins whose interval
+// [sum - d, sum + d], d = 4 gamma_K W slack, reaches the smallest upper end
+// or the largest lower end are the only ones that can be the minimum or the
+// maximum; they are recomputed in the reference's order, which gives the
+// reference's hmin and hmax exactly.  A bin's slot is then decided from its
+// interval (hist_bin is monotone in the value) or, when the interval straddles
+// a slot boundary, from its recomputed value.  K = 4MN + 4M + N + 8 bounds the
+// number of additions on any path of either summation tree.  Non-finite sums
+// take the reference's path for that angle: every bin recomputed, the
+// histogram formed serially as radon.cpp does.
+__global__ void __launch_bounds__(256) k_radon_cert(const Img *imgs, const double *trig, double *const *proj,
+                                                    OutRef o, double slack) {
+  __shared__ RadonWaveLds wl[4];
+  __shared__ double v[12];  // the 12 counts, angle-major
+  const int a = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const Img im = imgs[blockIdx.x];
+  const int M = im.h, N = im.w;
+  const int rLast = (int)ceil(sqrt((double)((N - 1 - (N - 1) / 2) * (N - 1 - (N - 1) / 2)) +
+                                   (double)((M - 1 - (M - 1) / 2) * (M - 1 - (M - 1) / 2)))) + 1;
+  const int rFirst = -rLast, rSize = 2 * rLast + 1;
+  const double K = 4.0 * M * N + 4.0 * M + N + 8.0, u = 0x1p-53;
+  const double cb = 4.0 * (K * u / (1.0 - K * u)) * slack;
+  // sums of subnormal-sized terms: a floor keeps d from underflowing to 0
+  auto dlt = [&](double wv) { return wv > 0.0 ? fmax(cb * wv, 0x1p-1000) : 0.0; };
+  const gptr<const double> P = to_global((const double *)im.p);
+  double *const p = proj[blockIdx.x] + (long)a * rSize, *const w = proj[blockIdx.x] + (long)(4 + a) * rSize;
+  const double cs = trig[2 * a], sn = trig[2 * a + 1];
+  double L = __builtin_inf(), U = -__builtin_inf();
+  bool nonfinite = false;
+  for (int r = lane; r < rSize; r += 64) {
+    const double d = dlt(w[r]), lo = p[r] - d, hi = p[r] + d;
+    if (!(fabs(lo) <= 1.7976931348623157e308 && fabs(hi) <= 1.7976931348623157e308)) nonfinite = true;
+    L = fmin(L, hi);
+    U = fmax(U, lo);
+  }
+  L = wave_all_reduce(L, MinOp());
+  U = wave_all_reduce(U, MaxOp());
+  const bool bad = __ballot(nonfinite) != 0;
+  // candidates (every bin when bad), each recomputed by the whole wave
+  for (int c0 = 0; c0 < rSize; c0 += 64) {
+    const int r = c0 + lane;
+    bool need = false;
+    if (r < rSize) {
+      const double wv = w[r];
+      if (bad ? wv != 0.0 : wv > 0.0) {
+        const double d = dlt(wv), lo = p[r] - d, hi = p[r] + d;
+        need = bad || lo <= L || hi >= U;
+      }
+    }
+    for (unsigned long long m = __ballot(need); m; m &= m - 1) {
+      const int k = __ffsll((long long)m) - 1;
+      const double e = radon_bin_wave(P, M, N, cs, sn, rFirst, c0 + k, wl[a]);
+      if (lane == k) {
+        p[r] = e;
+        w[r] = 0.0;
+      }
+    }
+  }
+  wave_sync();
+  if (bad) {  // every bin is the reference's: its serial histogram
+    if (lane == 0) {
+      double hmin = kInf, hmax = -kInf;
+      for (int i = 0; i < rSize; i++) { if (p[i] > hmax) hmax = p[i]; if (p[i] < hmin) hmin = p[i]; }
+      const double hs = hist_scale(hmin, hmax, 3);
+      double b0 = 0, b1 = 0, b2 = 0;
+      for (int i = 0; i < rSize; i++) {
+        const int b = hist_bin(p[i], hmin, hs, 3);
+        b0 += b == 0 ? 1.0 : 0.0;
+        b1 += b == 1 ? 1.0 : 0.0;
+        b2 += b == 2 ? 1.0 : 0.0;
+      }
+      v[3 * a] = b0;
+      v[3 * a + 1] = b1;
+      v[3 * a + 2] = b2;
+    }
+  } else {
+    // the minimum and maximum bins are among the recomputed (or exact) ones
+    double mn = __builtin_inf(), mx = -__builtin_inf();
+    for (int r = lane; r < rSize; r += 64) {
+      if (w[r] == 0.0) {
+        mn = fmin(mn, p[r]);
+        mx = fmax(mx, p[r]);
+      }
+    }
+    mn = wave_all_reduce(mn, MinOp());
+    mx = wave_all_reduce(mx, MaxOp());
+    const double hs = hist_scale(mn, mx, 3);
+    int n0 = 0, n1 = 0, n2 = 0;
+    for (int c0 = 0; c0 < rSize; c0 += 64) {
+      const int r = c0 + lane;
+      int b = -1;
+      bool open = false;
+      if (r < rSize) {
+        const double wv = w[r], pv = p[r];
+        if (wv == 0.0) {
+          b = hist_bin(pv, mn, hs, 3);
+        } else {
+          const double d = dlt(wv);
+          b = hist_bin(fmax(pv - d, mn), mn, hs, 3);
+          open = hist_bin(fmin(pv + d, mx), mn, hs, 3) != b;
+        }
+      }
+      for (unsigned long long m = __ballot(open); m; m &= m - 1) {
+        const int k = __ffsll((long long)m) - 1;
+        const double e = radon_bin_wave(P, M, N, cs, sn, rFirst, c0 + k, wl[a]);
+        if (lane == k) b = hist_bin(e, mn, hs, 3);
+      }
+      n0 += b == 0;
+      n1 += b == 1;
+      n2 += b == 2;
+    }
+    n0 = wave_all_reduce(n0, SumOp());
+    n1 = wave_all_reduce(n1, SumOp());
+    n2 = wave_all_reduce(n2, SumOp());
+    if (lane == 0) {
+      v[3 * a] = n0;
+      v[3 * a + 1] = n1;
+      v[3 * a + 2] = n2;
+    }
+  }
+  __syncthreads();
+  // output order of vd_RadonTextures (radon.cpp:98-115): angles 0, 135, 45, 90
+  if (threadIdx.x < 12) {
+    const int t = threadIdx.x, src = t < 3 ? t : t < 6 ? t + 6 : t - 3;
+    o.rows[blockIdx.x][t] = v[src];
+  }
+}
+
+// rSize above which the certified form's LDS (16 bytes per bin and angle)
+// does not fit one angle per block
+constexpr int RADON_CERT_MAXBINS = 10240;
+
+bool radon_certified_fits(int maxbins) { return maxbins <= RADON_CERT_MAXBINS; }
+
+void launch_radon_certified(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
+                            double slack, hipStream_t s) {
+  // angles per block: as many as keep the block's LDS near 48 KB
+  const int na = (64 * maxbins <= 49152) ? 4 : (32 * maxbins <= 49152) ? 2 : 1;
+  const size_t lds = (size_t)2 * na * maxbins * sizeof(double);
+  static const bool lds_ok =
+      hipFuncSetAttribute((const void *)k_radon_sum<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          2 * RADON_CERT_MAXBINS * (int)sizeof(double)) == hipSuccess;
+  (void)lds_ok;
+  const int ngrp = 4 / na;
+  if (na == 4)
+    k_radon_sum<4><<<xcd_grid(ngrp, nimg), 256, lds, s>>>(imgs, trig, proj, ngrp, nimg);
+  else if (na == 2)
+    k_radon_sum<2><<<xcd_grid(ngrp, nimg), 256, lds, s>>>(imgs, trig, proj, ngrp, nimg);
+  else
+    k_radon_sum<1><<<xcd_grid(ngrp, nimg), 256, lds, s>>>(imgs, trig, proj, ngrp, nimg);
+  k_radon_cert<<<nimg, 256, 0, s>>>(imgs, trig, proj, o, slack);
+}
+
 void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
                   int maxh, int maxw, hipStream_t s) {
   (void)maxw;

@@ -165,6 +165,11 @@ void launch_tamura(const Img *imgs, const Stats *st, double *const *scratch, Out
 void launch_comb(const Img *imgs, const double *iis, OutRef o, int nimg, hipStream_t s);
 void launch_radon(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
                   int maxh, int maxw, hipStream_t s);
+// order-free projections with bounds, counts certified (proj: 8 rSize doubles
+// per plane); radon_certified_fits says whether maxbins allows it
+bool radon_certified_fits(int maxbins);
+void launch_radon_certified(const Img *imgs, const double *trig, double *const *proj, OutRef o, int nimg, int maxbins,
+                            double slack, hipStream_t s);
 
 // objects / edges / gabor / zernike / chebyshev-fourier (k_objects.hip, k_spectral.hip)
 struct ObjDesc {
