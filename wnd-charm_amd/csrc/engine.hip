@@ -1102,7 +1102,7 @@ static int build_batch(wndgpu_ctx *c, const wndgpu_plan *plan, const Batch &B, A
     auto gather = [=](hipStream_t gs) { launch_gather_stats(sts, nst, ne, gs); };
     switch (alg) {
       case A_MSH: push(gn, [=](hipStream_t s) { gather(s); launch_msh(ni, nst, o, ne, s); }); break;
-      case A_FRACTAL: push(gn, [=](hipStream_t s) { launch_fractal(ni, o, ne, s); }); break;
+      case A_FRACTAL: push(gn, [=](hipStream_t s) { launch_fractal(ni, o, ne, mh, mw, s); }); break;
       case A_COLOR_HIST: {  // root entries only
         std::vector<unsigned char *> hv(ne);
         for (int k = 0; k < ne; k++) hv[k] = E[k].N->hsv[E[k].i];

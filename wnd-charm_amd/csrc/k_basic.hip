@@ -484,6 +484,80 @@ __global__ void __launch_bounds__(256) k_fractal(const Img *imgs, OutRef o, int 
   if (threadIdx.x == 0) out[bin] = s / (W * (W - k) + H * (H - k));
 }
 
+// The same 20 sums with one block per plane and each pixel read from global
+// memory about once: the block takes 64-column bands in turn and walks their
+// rows in chunks of FRACTAL_CH, keeping rows y .. y + kmax + FRACTAL_CH - 1
+// of the band (plus kmax columns on its right) in an LDS ring, so the
+// vertical and horizontal differences of every row of a chunk at every offset
+// come from LDS, with one load of FRACTAL_CH rows and two barriers per chunk.
+// Thread t takes column t & 63 of the band and the offsets t >> 6, +4, +8,
+// +12, +16; the per-offset partial sums are added in a fixed order at the
+// end.  The ring has a power-of-two number of rows (slot = row & (R - 1)).
+// Used when it fits FRACTAL_LDS bytes.
+constexpr int FRACTAL_LDS = 98304, FRACTAL_CH = 16;
+__host__ __device__ inline int fractal_ring_rows(int kmax) {
+  int r = 16;
+  while (r < kmax + FRACTAL_CH) r *= 2;
+  return r;
+}
+__device__ __forceinline__ void fractal_geometry(int W, int H, int &K, int &step, int &nk) {
+  K = (W > H ? H : W) / 5;
+  step = K / 20;
+  if (step < 1) step = 1;
+  nk = K > 1 ? min(20, (K - 2) / step + 1) : 0;  // offsets 1 + j step < K, j < 20
+}
+__global__ void __launch_bounds__(256) k_fractal_ring(const Img *imgs, OutRef o) {
+  extern __shared__ double ring[];
+  __shared__ double part[20 * 64];
+  const Img im = imgs[blockIdx.x];
+  const int W = im.w, H = im.h;
+  int K, step, nk;
+  fractal_geometry(W, H, K, step, nk);
+  const int kmax = nk > 0 ? 1 + (nk - 1) * step : 0, RM = fractal_ring_rows(kmax) - 1, LW = 64 + kmax;
+  const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const gptr<const double> P = to_global((const double *)im.p);
+  double acc[5] = {0, 0, 0, 0, 0};
+  for (int x0 = 0; x0 < W && nk > 0; x0 += 64) {
+    const int lw = min(LW, W - x0);  // columns of the band and its right margin
+    const int x = x0 + c;
+    for (int y0 = 0; y0 < H; y0 += FRACTAL_CH) {
+      // rows the chunk reads that are not in the ring yet: all of them for
+      // the first chunk, else the last FRACTAL_CH (their slots held rows at
+      // least R rows earlier, below y0: consumed)
+      const int r0 = y0 == 0 ? 0 : y0 + kmax, r1 = min(H, y0 + FRACTAL_CH + kmax);
+      for (int e = threadIdx.x; e < max(0, r1 - r0) * lw; e += blockDim.x) {
+        const int yy = r0 + e / lw, xx = e % lw;
+        ring[(yy & RM) * LW + xx] = P[(long)yy * W + x0 + xx];
+      }
+      __syncthreads();
+      if (x < W) {
+        for (int y = y0; y < min(H, y0 + FRACTAL_CH); y++) {
+          const double *row = ring + (y & RM) * LW;
+          const double v = row[c];
+#pragma unroll
+          for (int i = 0; i < 5; i++) {
+            const int j = g + 4 * i, k = 1 + j * step;
+            if (j < nk) {
+              if (y + k < H) acc[i] += fabs(v - ring[((y + k) & RM) * LW + c]);
+              if (x + k < W) acc[i] += fabs(v - row[c + k]);
+            }
+          }
+        }
+      }
+      __syncthreads();  // the chunk's rows are consumed before their slots refill
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 5; i++) part[(g + 4 * i) * 64 + c] = acc[i];
+  __syncthreads();
+  if (threadIdx.x < 20) {
+    const int j = threadIdx.x, k = 1 + j * step;
+    double sum = 0;
+    for (int l = 0; l < 64; l++) sum += part[j * 64 + l];
+    o.rows[blockIdx.x][j] = j < nk ? sum / (W * (W - k) + H * (H - k)) : 0.0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Small dense histogram of a value list with its own min/max (used for the
 // Chebyshev-20 coefficient block, ImageMatrix::ChebyshevStatistics2D,
@@ -576,8 +650,20 @@ void launch_median_gini(const Img *imgs, uint64_t *const *buf, const Stats *st, 
 void launch_msh(const Img *imgs, const Stats *st, OutRef o, int nimg, hipStream_t s) {
   k_msh<<<nimg, 1024, 0, s>>>(imgs, st, o);
 }
-void launch_fractal(const Img *imgs, OutRef o, int nimg, hipStream_t s) {
-  k_fractal<<<xcd_grid(20, nimg), 256, 0, s>>>(imgs, o, nimg);
+void launch_fractal(const Img *imgs, OutRef o, int nimg, int maxh, int maxw, hipStream_t s) {
+  // the ring's size for the largest offset any image of the batch can have
+  // (the offsets grow with the smaller side)
+  const int side = std::min(maxh, maxw), K = side / 5, step = std::max(1, K / 20);
+  const int nk = K > 1 ? std::min(20, (K - 2) / step + 1) : 0, kmax = nk > 0 ? 1 + (nk - 1) * step : 0;
+  const size_t ring = (size_t)fractal_ring_rows(kmax) * (64 + kmax) * sizeof(double);
+  if (ring <= (size_t)FRACTAL_LDS) {
+    static const bool attr = hipFuncSetAttribute((const void *)k_fractal_ring,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, FRACTAL_LDS) == hipSuccess;
+    (void)attr;
+    k_fractal_ring<<<nimg, 256, ring, s>>>(imgs, o);
+  } else {
+    k_fractal<<<xcd_grid(20, nimg), 256, 0, s>>>(imgs, o, nimg);
+  }
 }
 void launch_hist_values(const Img *vals, int nbins, OutRef o, int nimg, hipStream_t s) {
   k_hist_values<<<nimg, 256, 0, s>>>(vals, nbins, o);

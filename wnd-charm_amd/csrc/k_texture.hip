@@ -1522,19 +1522,28 @@ __global__ void __launch_bounds__(64) k_tam_sat(const Img *imgs, double *const *
   }
 }
 
-__device__ __forceinline__ double local_mean(long x, long y, long k, const double *S, long dimx, long dimy) {
-  const long k2 = k / 2;
-  long starty = y - k2, startx = x - k2, stopy = y + k2 - 1, stopx = x + k2 - 1;
+// The window's pixel count is a power of two 2^e away from the borders;
+// there the quotient is formed as a product with 2^-e, which is exact, so
+// both are the correctly rounded value of the same real number.
+__device__ __forceinline__ double local_mean(int x, int y, int k, const double *S, int dimx, int dimy) {
+  const int k2 = k / 2;
+  int starty = y - k2, startx = x - k2, stopy = y + k2 - 1, stopx = x + k2 - 1;
   if (starty < 0) starty = 0;
   if (startx < 0) startx = 0;
   if (stopx > dimx - 1) stopx = dimx - 1;
   if (stopy > dimy - 1) stopy = dimy - 1;
-  const double links = (startx - 1 < 0) ? 0 : S[stopy * dimx + startx - 1];
-  const double oben = (starty - 1 < 0) ? 0 : S[(stopy - 1) * dimx + startx];
-  const double obenlinks = ((starty - 1 < 0) || (startx - 1 < 0)) ? 0 : S[(stopy - 1) * dimx + startx - 1];
-  const double unten = S[stopy * dimx + startx];
-  const long counter = (stopy - starty + 1) * (stopx - startx + 1);
-  return (unten - links - oben + obenlinks) / counter;
+  const long r1 = (long)stopy * dimx, r0 = r1 - dimx;
+  const double links = (startx - 1 < 0) ? 0 : S[r1 + startx - 1];
+  const double oben = (starty - 1 < 0) ? 0 : S[r0 + startx];
+  const double obenlinks = ((starty - 1 < 0) || (startx - 1 < 0)) ? 0 : S[r0 + startx - 1];
+  const double unten = S[r1 + startx];
+  const int counter = (stopy - starty + 1) * (stopx - startx + 1);
+  const double num = unten - links - oben + obenlinks;
+  if ((counter & (counter - 1)) == 0) {
+    const int e = __builtin_ctz((unsigned)counter);
+    return num * __longlong_as_double((long long)(1023 - e) << 52);  // 2^-e
+  }
+  return num / counter;
 }
 
 // coarseness per pixel (best scale), directionality phi and the sums

@@ -133,7 +133,7 @@ void launch_sort(const Img *imgs, uint64_t *const *buf, const long *cap, int nim
 void launch_median_gini(const Img *imgs, uint64_t *const *buf, const Stats *st, double *median_out, OutRef pix,
                         OutRef gini, int nimg, hipStream_t s);
 void launch_msh(const Img *imgs, const Stats *st, OutRef o, int nimg, hipStream_t s);
-void launch_fractal(const Img *imgs, OutRef o, int nimg, hipStream_t s);
+void launch_fractal(const Img *imgs, OutRef o, int nimg, int maxh, int maxw, hipStream_t s);
 void launch_hist_values(const Img *vals, int nbins, OutRef o, int nimg, hipStream_t s);
 
 // textures (k_texture.hip)
