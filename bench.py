@@ -515,12 +515,12 @@ def grad_main(args):
         'generator_s': round(t_gen, 1),
         'grad_finite': bool(torch.isfinite(g).all().item()),
         'roofline': {'kernel': 'k_ad_gabor_fwd (one 38 x 38 complex Gabor filter over 64 planes 512x512)',
-                     'bound': 'mfma', 'achieved': round(gabor_tf, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                     'bound': 'fp64-valu', 'achieved': round(gabor_tf, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                      'frac': round(gabor_tf / FP64_PEAK_TFLOPS, 4), 'traffic': None,
                      'launch_ms': round(gabor_ms, 4), 'flops_per_launch': gabor_flops,
                      'note': 'FP64 vector FMA; the FP64 matrix and vector peaks of MI355X are equal'},
         'roofline_gemm': {'kernel': 'k_ad_gemm (512 x 512 x 512 Chebyshev GEMM, 64 planes per launch)',
-                          'achieved': round(achieved, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
+                          'bound': 'mfma', 'achieved': round(achieved, 3), 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                           'frac': round(achieved / FP64_PEAK_TFLOPS, 4), 'launch_ms': round(gemm_ms, 4),
                           'flops_per_launch': gemm_flops},
     }
