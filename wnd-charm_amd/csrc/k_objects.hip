@@ -123,27 +123,45 @@ __global__ void __launch_bounds__(1024) k_objects(const Img *imgs, const Stats *
   }
   const double thr = block_otsu(n, mn, mx, val, hist);
   int *lab = OD.label;
-  // mask, labels, global centroid (exact integer sums)
+  // mask, labels, global centroid (exact integer sums).  A wave takes 64
+  // consecutive pixels (raster order); each mask pixel starts at the first
+  // pixel of its run inside that segment (a ballot), so runs need no unions
+  // within a segment and the trees stay shallow.
   long long xm = 0, ym = 0, mass = 0, c1 = 0, c3 = 0;
-  for (long i = t; i < n; i += 1024) {
-    const bool m = val(i) > thr;
-    lab[i] = m ? (int)i : -1;
+  const int lane = t & 63;
+  for (long c0 = 0; c0 < n; c0 += 1024) {
+    const long i = c0 + t;
+    const bool m = i < n && val(i) > thr;
+    const int y = (int)(i / W), x = (int)(i - (long)y * W);
+    // bit l: pixel l of the segment continues the run of pixel l - 1 (both
+    // in the mask, the same row)
+    const unsigned long long mb = __ballot(m);
+    const unsigned long long cont = __ballot(m && x > 0) & (mb << 1);
+    const unsigned long long below = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+    const int head = 63 - __clzll(~cont & below);
+    if (i < n) lab[i] = m ? (int)(i - (lane - head)) : -1;
     if (m) {
-      const int y = i / W, x = i - (long)y * W;
       xm += x + 1; ym += y + 1; mass++;
     }
   }
   __syncthreads();
-  // 8-connected union-find; every root ends at the component's smallest index
+  // 8-connected union-find; every root ends at the component's smallest
+  // index.  Of a horizontal run [s, e] the upper neighbours are s-1 .. e+1:
+  // each pixel joins its upper-right one, the run's first pixel also its
+  // upper-left and upper ones; a segment's first pixel joins its left
+  // neighbour when the run crosses the segment boundary.
   for (long i = t; i < n; i += 1024) {
     if (lab[i] < 0) continue;
     const int y = i / W, x = i - (long)y * W;
-    if (x > 0 && lab[i - 1] >= 0) uf_union(lab, (int)i, (int)(i - 1));
+    const bool left = x > 0 && lab[i - 1] >= 0;
+    if (left && (i & 63) == 0) uf_union(lab, (int)i, (int)(i - 1));
     if (y > 0) {
       const long u = i - W;
-      if (x > 0 && lab[u - 1] >= 0) uf_union(lab, (int)i, (int)(u - 1));
-      if (lab[u] >= 0) uf_union(lab, (int)i, (int)u);
       if (x < W - 1 && lab[u + 1] >= 0) uf_union(lab, (int)i, (int)(u + 1));
+      if (!left) {
+        if (x > 0 && lab[u - 1] >= 0) uf_union(lab, (int)i, (int)(u - 1));
+        if (lab[u] >= 0) uf_union(lab, (int)i, (int)u);
+      }
     }
   }
   __syncthreads();
@@ -172,7 +190,7 @@ __global__ void __launch_bounds__(1024) k_objects(const Img *imgs, const Stats *
     const long i = c0 + t;
     const int isroot = (i < n && lab[i] == (int)i) ? 1 : 0;
     // block exclusive scan of isroot
-    const int lane = t & 63, wv = t >> 6;
+    const int wv = t >> 6;
     const uint64_t bal = __ballot(isroot);
     const int pre = __popcll(bal & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
     if (lane == 0) scan_w[wv] = __popcll(bal);
@@ -189,14 +207,38 @@ __global__ void __launch_bounds__(1024) k_objects(const Img *imgs, const Stats *
   long long *axm = OD.xm, *aym = OD.ym;
   for (long l = t; l <= count; l += 1024) { area[l] = 0; axm[l] = 0; aym[l] = 0; }
   __syncthreads();
-  for (long i = t; i < n; i += 1024) {
-    const int r = lab[i];
-    if (r < 0) continue;
-    const int l = rid[r];
-    const int y = i / W, x = i - (long)y * W;
-    atomicAdd(&area[l], 1);
-    atomicAdd((unsigned long long *)&axm[l], (unsigned long long)(x + 1));
-    atomicAdd((unsigned long long *)&aym[l], (unsigned long long)(y + 1));
+  // areas and coordinate sums per object: the lanes of a wave holding one
+  // label add theirs first, then one lane adds the totals (exact integers)
+  for (long c0 = 0; c0 < n; c0 += 1024) {
+    const long i = c0 + t;
+    int l = -1;
+    long long px = 0, py = 0;
+    if (i < n) {
+      const int r = lab[i];
+      if (r >= 0) {
+        l = rid[r];
+        const int y = i / W, x = i - (long)y * W;
+        px = x + 1;
+        py = y + 1;
+      }
+    }
+    for (unsigned long long pend = __ballot(l >= 0); pend;) {
+      const int lead = __ffsll((long long)pend) - 1;
+      const int ll = __shfl(l, lead, 64);
+      const bool mine = l == ll;
+      const unsigned long long mm = __ballot(mine);
+      long long sx = mine ? px : 0, sy = mine ? py : 0;
+      for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off, 64);
+        sy += __shfl_xor(sy, off, 64);
+      }
+      if (lane == lead) {
+        atomicAdd(&area[ll], (int)__popcll(mm));
+        atomicAdd((unsigned long long *)&axm[ll], (unsigned long long)sx);
+        atomicAdd((unsigned long long *)&aym[ll], (unsigned long long)sy);
+      }
+      pend &= ~mm;
+    }
   }
   __syncthreads();
   // per-object distance to the global centroid
