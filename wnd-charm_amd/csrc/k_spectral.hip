@@ -51,10 +51,14 @@ __global__ void __launch_bounds__(1024) k_zern_moments(const Img *imgs, double *
 // reduced-precision forms, where the radial polynomials stay FP64).
 // Zernike coefficients are floating-point features (held to 1e-6 of the
 // reference), so this function, unlike the exact kernels, lets the compiler
-// contract multiply-adds into FMAs and divides by r^2 once per pixel.
+// contract multiply-adds into FMAs and replaces the reference's divisions
+// by products with reciprocals: of the radius and of the plane's sum once
+// per block, of r once per pixel (1/r^2 and cos, sin of the angle from it),
+// and of pi once.
 template <int N0, int N1, class T>
 __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int lane, double cxm, double cym, double sum,
-                                           const double *H1, const double *H2, const double *H3, double *dst) {
+                                           const double *__restrict__ H1, const double *__restrict__ H2,
+                                           const double *__restrict__ H3, double *__restrict__ dst) {
 #pragma clang fp contract(fast)
   constexpr int NP = [] {
     int c = 0;
@@ -62,35 +66,36 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
     return c;
   }();
   const double rad = (double)(im.w < im.h ? im.w : im.h);
-  const double PI = 3.14159265358979323846264338328;
+  const double irad = 1.0 / rad, isum = 1.0 / sum;
+  const double IPI = 1.0 / 3.14159265358979323846264338328;
   T AR[NP], AI[NP];
 #pragma unroll
   for (int k = 0; k < NP; k++) AR[k] = AI[k] = (T)0;
   for (long p = p0 + lane; p < p1; p += 64) {
     const int j = p / im.w, i = p - (long)j * im.w;  // i: column, j: row
-    const double x = (i + 1 - cxm) / rad;
-    const double y = (j + 1 - cym) / rad;
+    const double x = (i + 1 - cxm) * irad;
+    const double y = (j + 1 - cym) * irad;
     const double r2 = x * x + y * y;
     const double r = sqrt(r2);
     if (r < kDblEps || r > 1.0) continue;
-    const double ir2 = 1.0 / r2;
+    const double ir = 1.0 / r, ir2 = ir * ir;
     double R[N1 + 1];
     R[0] = 1;
 #pragma unroll
     for (int k = 1; k <= N1; k++) R[k] = r * R[k - 1];
     double COST[N1 + 1], SINT[N1 + 1];
-    const double a = COST[0] = x / r;
-    const double b = SINT[0] = y / r;
+    const double a = COST[0] = x * ir;
+    const double b = SINT[0] = y * ir;
 #pragma unroll
     for (int m = 1; m <= N1; m++) {
       COST[m] = a * COST[m - 1] - b * SINT[m - 1];
       SINT[m] = a * SINT[m - 1] + b * COST[m - 1];
     }
-    const double f = im.p[p] / sum;
+    const double f = im.p[p] * isum;
     int base = 0;
 #pragma unroll
     for (int n = N0; n <= N1; n++) {
-      const double const_t = (n + 1) * f / PI;
+      const double const_t = (n + 1) * f * IPI;
       const double Rn = R[n];
       const double Rnm2 = n >= 2 ? R[n >= 2 ? n - 2 : 0] : 0.0;
       double Rnmp2 = 0, Rnmp4 = 0, Rnm = 0;
@@ -127,7 +132,8 @@ __device__ __forceinline__ void zern_group(const Img im, long p0, long p1, int l
 
 // block = one pixel chunk of one image; wave w handles order group w
 template <class T>
-__global__ void __launch_bounds__(256) k_zern_accum(const Img *imgs, double *partial, int nchunk, const double *htab) {
+__global__ void __launch_bounds__(256) k_zern_accum(const Img *__restrict__ imgs, double *__restrict__ partial, int nchunk,
+                                                    const double *__restrict__ htab) {
   const Img im = imgs[blockIdx.y];
   const long P = (long)im.h * im.w;
   const long csz = (P + nchunk - 1) / nchunk;
