@@ -1473,13 +1473,15 @@ __global__ void __launch_bounds__(64) k_tam_sat(const Img *imgs, double *const *
   const Img im = imgs[blockIdx.x];
   const int H = im.h, W = im.w;
   const long P = (long)H * W;
-  const double *I = scr[blockIdx.x];
-  double *S = scr[blockIdx.x] + P;
+  // global, not FLAT: the LDS waits of each step leave the prefetched
+  // inputs in flight
+  const gptr<const double> I = to_global((const double *)scr[blockIdx.x]);
+  const gptr<double> S = to_global(scr[blockIdx.x] + P);
   const int l = threadIdx.x;
   constexpr int PF = 16;  // inputs loaded ahead per lane
   for (int y0 = 0; y0 < H; y0 += 64) {
     const int y = y0 + l;
-    const double *Irow = I + (long)y * W;
+    const gptr<const double> Irow = I + (long)y * W;
     double cur = 0, up_prev = 0;
     double nx[PF];
 #pragma unroll
@@ -1525,7 +1527,7 @@ __global__ void __launch_bounds__(64) k_tam_sat(const Img *imgs, double *const *
 // The window's pixel count is a power of two 2^e away from the borders;
 // there the quotient is formed as a product with 2^-e, which is exact, so
 // both are the correctly rounded value of the same real number.
-__device__ __forceinline__ double local_mean(int x, int y, int k, const double *S, int dimx, int dimy) {
+__device__ __forceinline__ double local_mean(int x, int y, int k, gptr<const double> S, int dimx, int dimy) {
   const int k2 = k / 2;
   int starty = y - k2, startx = x - k2, stopy = y + k2 - 1, stopx = x + k2 - 1;
   if (starty < 0) starty = 0;
@@ -1561,9 +1563,9 @@ __global__ void __launch_bounds__(256) k_tam_pix(const Img *imgs, double *const 
   const long P = (long)H * W;
   const long i = blk * (long)blockDim.x + threadIdx.x;
   if (i >= P) return;
-  double *N = scr[ent];
-  const double *S = N + P;
-  double *phi = N + 2 * P;
+  const gptr<double> N = to_global(scr[ent]);  // global, not FLAT: vector-memory waits only
+  const gptr<const double> S = (gptr<const double>)(N + P);
+  const gptr<double> phi = N + 2 * P;
   const int y = i / W, x = i - (long)y * W;
   // coarseness: argmax over scales of E_h / E_v (tamura.cpp:183-238)
   double maxE = 0;
@@ -1580,7 +1582,7 @@ __global__ void __launch_bounds__(256) k_tam_pix(const Img *imgs, double *const 
     if (eh > maxE) { maxE = eh; maxk = k; }
     if (ev > maxE) { maxE = ev; maxk = k; }
   }
-  ((int *)(N + 4 * P))[i] = maxk;
+  ((gptr<int>)(N + 4 * P))[i] = maxk;
   // directionality filters after the reference's assignments (tamura.cpp:50-57):
   // H = [-1 2 1; -2 0 2; -1 -2 -1], V = [1 0 0; 0 0 0; -1 0 0], correlation order
   // (column offset outer, row offset inner), zero weights skipped (exact).
