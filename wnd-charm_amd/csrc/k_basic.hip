@@ -433,13 +433,29 @@ __global__ void __launch_bounds__(1024) k_msh(const Img *imgs, const Stats *st, 
   const long n = (long)im.h * im.w;
   const double s3 = hist_scale(S.mn, S.mx, 3), s5 = hist_scale(S.mn, S.mx, 5), s7 = hist_scale(S.mn, S.mx, 7),
                s9 = hist_scale(S.mn, S.mx, 9);
-  for (long i = threadIdx.x; i < n; i += blockDim.x) {
-    const double v = im.p[i];
-    atomicAdd(&cnt[hist_bin(v, S.mn, s3, 3)], 1);
-    atomicAdd(&cnt[3 + hist_bin(v, S.mn, s5, 5)], 1);
-    atomicAdd(&cnt[8 + hist_bin(v, S.mn, s7, 7)], 1);
-    atomicAdd(&cnt[15 + hist_bin(v, S.mn, s9, 9)], 1);
+  // each wave counts its 64 pixels per bin by ballot (wave-uniform counts,
+  // kept in scalar registers) and adds its 24 totals into LDS once at the end
+  int wc[24];
+#pragma unroll
+  for (int b = 0; b < 24; b++) wc[b] = 0;
+  for (long i0 = 0; i0 < n; i0 += blockDim.x) {
+    const long i = i0 + threadIdx.x;
+    const bool ok = i < n;
+    const double v = ok ? im.p[i] : S.mn;
+    const int b3 = hist_bin(v, S.mn, s3, 3), b5 = hist_bin(v, S.mn, s5, 5), b7 = hist_bin(v, S.mn, s7, 7),
+              b9 = hist_bin(v, S.mn, s9, 9);
+#pragma unroll
+    for (int b = 0; b < 3; b++) wc[b] += __popcll(__ballot(ok && b3 == b));
+#pragma unroll
+    for (int b = 0; b < 5; b++) wc[3 + b] += __popcll(__ballot(ok && b5 == b));
+#pragma unroll
+    for (int b = 0; b < 7; b++) wc[8 + b] += __popcll(__ballot(ok && b7 == b));
+#pragma unroll
+    for (int b = 0; b < 9; b++) wc[15 + b] += __popcll(__ballot(ok && b9 == b));
   }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int b = 0; b < 24; b++) atomicAdd(&cnt[b], wc[b]);
   __syncthreads();
   if (threadIdx.x == 0) {
     double mx = 0;
