@@ -1656,9 +1656,25 @@ __global__ void __launch_bounds__(1024) k_tam_final(const Img *imgs, double *con
   if (t < 125) hd[t] = 0;
   __syncthreads();
   const double ksc = hist_scale((double)kmn, (double)kmx, 3), psc = hist_scale(pmn, pmx, 125);
-  for (long i = t; i < P; i += blockDim.x) {
-    atomicAdd(&hk[hist_bin((double)kb[i], (double)kmn, ksc, 3)], 1);
-    atomicAdd(&hd[hist_bin(phi[i], pmn, psc, 125)], 1);
+  // the 3-bin coarseness histogram by ballot per wave (counts in scalar
+  // registers, one LDS add per bin and wave); the 125-bin one by LDS atomics
+  int wk0 = 0, wk1 = 0, wk2 = 0;
+  for (long i0 = 0; i0 < P; i0 += blockDim.x) {
+    const long i = i0 + t;
+    const bool ok = i < P;
+    int b = -1;
+    if (ok) {
+      b = hist_bin((double)kb[i], (double)kmn, ksc, 3);
+      atomicAdd(&hd[hist_bin(phi[i], pmn, psc, 125)], 1);
+    }
+    wk0 += __popcll(__ballot(b == 0));
+    wk1 += __popcll(__ballot(b == 1));
+    wk2 += __popcll(__ballot(b == 2));
+  }
+  if ((t & 63) == 0) {
+    atomicAdd(&hk[0], wk0);
+    atomicAdd(&hk[1], wk1);
+    atomicAdd(&hk[2], wk2);
   }
   __syncthreads();
   if (t == 0) {
