@@ -252,24 +252,47 @@ __global__ void __launch_bounds__(256) k_cheb_gemm(const GemmDesc *ds) {
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
   for (int k0 = 0; k0 < d.K; k0 += 16) {
+    int odd = 0;  // a nonzero operand outside [2^-500, 2^500] (or not finite)
     for (int e = threadIdx.x; e < 16 * 64; e += 256) {
       const int kk = e >> 6, rr = e & 63;
       const int k = k0 + kk, r = r0 + rr, c = c0 + rr;
-      As[kk][rr] = (k < d.K && r < d.M) ? d.A[(long)r * d.sar + (long)k * d.sak] : 0.0;
-      Bs[kk][rr] = (k < d.K && c < d.N) ? d.B[(long)k * d.N + c] : 0.0;
+      const double a = (k < d.K && r < d.M) ? d.A[(long)r * d.sar + (long)k * d.sak] : 0.0;
+      const double b = (k < d.K && c < d.N) ? d.B[(long)k * d.N + c] : 0.0;
+      As[kk][rr] = a;
+      Bs[kk][rr] = b;
+      odd |= (a != 0.0 && !(fabs(a) >= 0x1p-500 && fabs(a) <= 0x1p500)) ||
+             (b != 0.0 && !(fabs(b) >= 0x1p-500 && fabs(b) <= 0x1p500));
     }
-    __syncthreads();
+    // With every nonzero operand in [2^-500, 2^500], (a / 2) b rounds to the
+    // same value as (a b) / 2: both are the product's correctly rounded value
+    // scaled by 2^-1, no step underflows, and a / 2 is exact.  A tile with any
+    // other operand takes the reference's expression.
+    const bool scaled = !__syncthreads_or(odd);
     const int kn = d.K - k0 < 16 ? d.K - k0 : 16;
-    for (int kk = 0; kk < kn; kk++) {
-      double av[4], bv[4];
+    if (scaled) {
+      for (int kk = 0; kk < kn; kk++) {
+        double av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+        for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i] * 0.5;
 #pragma unroll
-      for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
+        for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] += (av[i] * bv[j]) / 2;
+          for (int j = 0; j < 4; j++) acc[i][j] += av[i] * bv[j];
+      }
+    } else {
+      for (int kk = 0; kk < kn; kk++) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) av[i] = As[kk][ty * 4 + i];
+#pragma unroll
+        for (int j = 0; j < 4; j++) bv[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) acc[i][j] += (av[i] * bv[j]) / 2;
+      }
     }
     __syncthreads();
   }
