@@ -62,11 +62,13 @@ def test_certified_equals_direct(size):
     assert np.array_equal(got, want, equal_nan=True)
 
 
-@pytest.mark.parametrize('slack', [1e6, 1e15])
-def test_widened_bounds_equal_direct(slack):
-    want = direct('small')
-    print('radon: certified path, slack', slack, flush=True)
-    got = features_in_context({'WNDGPU_RADON_SLACK': repr(slack)}, _images('small'), _plan())
+@pytest.mark.parametrize('size,slack', [('small', 1e6), ('small', 1e15), ('mid', 1e15), ('large', 1e15)])
+def test_widened_bounds_equal_direct(size, slack):
+    """at 1e15 every bin is recomputed by the wave routine, on the larger
+    planes over many 256-sub-column chunks and 512-term windows per bin"""
+    want = direct(size)
+    print('radon: certified path, slack', slack, size, flush=True)
+    got = features_in_context({'WNDGPU_RADON_SLACK': repr(slack)}, _images(size), _plan())
     assert np.array_equal(got, want, equal_nan=True)
 
 
