@@ -528,7 +528,8 @@ __global__ void __launch_bounds__(256) k_fractal(const Img *imgs, OutRef o, int 
 // rows in chunks of FRACTAL_CH, keeping rows y .. y + kmax + FRACTAL_CH - 1
 // of the band (plus kmax columns on its right) in an LDS ring, so the
 // vertical and horizontal differences of every row of a chunk at every offset
-// come from LDS, with one load of FRACTAL_CH rows and two barriers per chunk.
+// come from LDS; the next chunk's rows are loaded (one column per thread)
+// while a chunk is computed, with two barriers per chunk.
 // Thread t takes column t & 63 of the band and the offsets t >> 6, +4, +8,
 // +12, +16; the per-offset partial sums are added in a fixed order at the
 // end.  The ring has a power-of-two number of rows (slot = row & (R - 1)).
@@ -557,18 +558,22 @@ __global__ void __launch_bounds__(256) k_fractal_ring(const Img *imgs, OutRef o)
   const gptr<const double> P = to_global((const double *)im.p);
   double acc[5] = {0, 0, 0, 0, 0};
   for (int x0 = 0; x0 < W && nk > 0; x0 += 64) {
-    const int lw = min(LW, W - x0);  // columns of the band and its right margin
+    const int lw = min(LW, W - x0);  // columns of the band and its right margin (lw <= 256)
     const int x = x0 + c;
+    const bool ld = (int)threadIdx.x < lw;  // thread t loads column x0 + t of the band
+    const gptr<const double> Pc = P + x0 + (ld ? (int)threadIdx.x : 0);
+    // the first chunk's rows 0 .. FRACTAL_CH + kmax - 1
+    if (ld)
+      for (int yy = 0; yy < min(H, FRACTAL_CH + kmax); yy++) ring[(yy & RM) * LW + threadIdx.x] = Pc[(long)yy * W];
+    __syncthreads();
     for (int y0 = 0; y0 < H; y0 += FRACTAL_CH) {
-      // rows the chunk reads that are not in the ring yet: all of them for
-      // the first chunk, else the last FRACTAL_CH (their slots held rows at
-      // least R rows earlier, below y0: consumed)
-      const int r0 = y0 == 0 ? 0 : y0 + kmax, r1 = min(H, y0 + FRACTAL_CH + kmax);
-      for (int e = threadIdx.x; e < max(0, r1 - r0) * lw; e += blockDim.x) {
-        const int yy = r0 + e / lw, xx = e % lw;
-        ring[(yy & RM) * LW + xx] = P[(long)yy * W + x0 + xx];
-      }
-      __syncthreads();
+      // the next chunk's new rows nr .. nr + FRACTAL_CH - 1, loaded while this
+      // chunk is computed and written once its rows are consumed (their slots
+      // held rows at least R rows earlier, below y0 + FRACTAL_CH)
+      const int nr = y0 + FRACTAL_CH + kmax;
+      double pre[FRACTAL_CH];
+#pragma unroll
+      for (int j = 0; j < FRACTAL_CH; j++) pre[j] = (ld && nr + j < H) ? Pc[(long)(nr + j) * W] : 0.0;
       if (x < W) {
         for (int y = y0; y < min(H, y0 + FRACTAL_CH); y++) {
           const double *row = ring + (y & RM) * LW;
@@ -584,6 +589,11 @@ __global__ void __launch_bounds__(256) k_fractal_ring(const Img *imgs, OutRef o)
         }
       }
       __syncthreads();  // the chunk's rows are consumed before their slots refill
+      if (ld)
+#pragma unroll
+        for (int j = 0; j < FRACTAL_CH; j++)
+          if (nr + j < H) ring[((nr + j) & RM) * LW + threadIdx.x] = pre[j];
+      __syncthreads();
     }
   }
 #pragma unroll
