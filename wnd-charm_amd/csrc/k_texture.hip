@@ -100,21 +100,34 @@ __global__ void __launch_bounds__(1024) k_har_cooc(const Img *imgs, const HarDes
   const int n = rows * cols;  // planes hold fewer than 2^31 pixels: 32-bit index arithmetic
   const gptr<const uint8_t> gray = to_global((const uint8_t *)H.gray);
   int local = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    const int row = i / cols, col = i - row * cols;
-    const int g = gray[i];
-    if (!g) continue;
-    int r2 = row, c2 = col;
-    if (angle == 0) { c2 = col + 1; if (!(c2 < cols)) continue; }
-    else if (angle == 1) { r2 = row + 1; c2 = col - 1; if (!(r2 < rows && c2 >= 0)) continue; }
-    else if (angle == 2) { r2 = row + 1; if (!(r2 < rows)) continue; }
-    else { r2 = row + 1; c2 = col + 1; if (!(r2 < rows && c2 < cols)) continue; }
-    const int g2 = gray[r2 * cols + c2];
-    if (!g2) continue;
-    int x = lut[g], y = lut[g2];
-    if (x == y) atomicAdd(&cnt[tri(x, x)], 2u);
-    else atomicAdd(&cnt[x < y ? tri(x, y) : tri(y, x)], 1u);
-    local += 2;
+  // four pixels per thread and pass: the pixel and neighbour loads are clamped
+  // into the plane and issued together, the pairs outside it are not counted
+  for (int i0 = threadIdx.x; i0 < n; i0 += 4 * blockDim.x) {
+    int gv[4], g2v[4];
+    bool okv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      bool ok = i < n;
+      const int ii = ok ? i : 0;
+      const int row = ii / cols, col = ii - row * cols;
+      int r2 = row, c2 = col;
+      if (angle == 0) { c2 = col + 1; ok = ok && c2 < cols; }
+      else if (angle == 1) { r2 = row + 1; c2 = col - 1; ok = ok && r2 < rows && c2 >= 0; }
+      else if (angle == 2) { r2 = row + 1; ok = ok && r2 < rows; }
+      else { r2 = row + 1; c2 = col + 1; ok = ok && r2 < rows && c2 < cols; }
+      gv[u] = gray[ii];
+      g2v[u] = gray[ok ? r2 * cols + c2 : ii];
+      okv[u] = ok;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (!okv[u] || !gv[u] || !g2v[u]) continue;
+      const int x = lut[gv[u]], y = lut[g2v[u]];
+      if (x == y) atomicAdd(&cnt[tri(x, x)], 2u);
+      else atomicAdd(&cnt[x < y ? tri(x, y) : tri(y, x)], 1u);
+      local += 2;
+    }
   }
   atomicAdd(&total, local);
   __syncthreads();
